@@ -1,0 +1,169 @@
+"""Headline benchmark: CG solves/s on D^dag D to a true FP64 relative residual of 1e-10.
+
+Workload (default): simplicial refined icosahedron n=128 (163,842 sites) x Lt=64,
+antiperiodic in time, r=1, M5=m=0.05, von Mises(kappa=1.4) link angles, Gaussian RHS.
+A step is one complete solve (distinct seeded RHS per step).  `value` is the solves/s of
+the fastest method (mixed FP32 CG with FP64 reliable updates); the plain FP64 CG on the
+same GPU is timed beside it and the ratio reported.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=128)
+    p.add_argument("--Lt", type=int, default=64)
+    p.add_argument("--lattice", type=int, default=0)
+    p.add_argument("--m", type=float, default=0.05)
+    p.add_argument("--beta", type=float, default=1.4)
+    p.add_argument("--tol", type=float, default=1e-10)
+    p.add_argument("--cpu-lt", type=int, default=4)
+    p.add_argument("--cpu-iters", type=int, default=12)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-fp64", action="store_true")
+    return p.parse_args()
+
+
+def mean_ell(g, lattice):
+    if lattice == 0:
+        x, y = g.sites[g.links[:, 0]], g.sites[g.links[:, 1]]
+    else:
+        x, y = g.dual_sites[g.dual_links[:, 0]], g.dual_sites[g.dual_links[:, 1]]
+    return float(np.mean(np.arccos(np.clip(np.einsum("ij,ij->i", x, y), -1, 1))))
+
+
+def algorithmic_bytes_D(V, E, Lt, real_bytes):
+    """B_D = 2 B_vec + B_link: one spinor field read and written once, one angle per link."""
+    b_vec = 2 * V * Lt * 2 * real_bytes
+    b_link = (E + V) * Lt * real_bytes
+    return 2 * b_vec + b_link
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(1)
+    import qed3_amd as q
+
+    t_setup = time.time()
+    g = q.Geometry(a.n)
+    at = 0.5 * mean_ell(g, a.lattice)
+    D = q.WilsonDirac(a.lattice, a.n, a.Lt, r=1.0, M5=a.m, at=at)
+    sp, tp = q.random_gauge(a.lattice, a.n, a.Lt, seed=1234, kappa=a.beta)
+    D.update(sp, tp)
+    D.autotune()
+    print(f"# setup {time.time() - t_setup:.1f}s V={D.V} E={D.E} Lt={a.Lt} at={at:.5f}",
+          file=sys.stderr, flush=True)
+
+    nbytes = D.N * 16
+    d_x = q.DeviceArray(nbytes)
+    rhs = [q.DeviceArray.from_numpy(q.random_source(a.lattice, a.n, a.Lt, seed=1000 + k))
+           for k in range(a.warmup + a.steps)]
+
+    def run(solver, k):
+        return D.solve_device(d_x, rhs[k], tol=a.tol, solver=solver)
+
+    results = {}
+    for name, solver in (("mixed", q.CG_MIXED), ("fp64", q.CG_F64)):
+        if name == "fp64" and a.no_fp64:
+            continue
+        for k in range(a.warmup):
+            run(solver, k)
+        q.load().qed3_device_synchronize()
+        t0 = time.time()
+        reps = []
+        for k in range(a.warmup, a.warmup + a.steps):
+            r = run(solver, k)
+            reps.append(r)
+            print(f"# {name} solve {k}: iters={r.iterations} ru={r.reliable_updates} "
+                  f"resid={r.true_resid:.2e} {r.seconds:.3f}s", file=sys.stderr, flush=True)
+        q.load().qed3_device_synchronize()
+        dt = time.time() - t0
+        results[name] = dict(solves_per_s=a.steps / dt, ms_per_solve=1e3 * dt / a.steps,
+                             iters=float(np.mean([r.iterations for r in reps])),
+                             reliable_updates=float(np.mean([r.reliable_updates for r in reps])),
+                             max_true_resid=float(max(r.true_resid for r in reps)))
+
+    # roofline of the dominant kernel: the FP32 stencil (D) inside the mixed solver
+    ms32 = D.time_apply(q.OP_D, q.F32, warmup=5, reps=50)
+    ms64 = D.time_apply(q.OP_D, q.F64, warmup=5, reps=50)
+    b32 = algorithmic_bytes_D(D.V, D.E, a.Lt, 4)
+    b64 = algorithmic_bytes_D(D.V, D.E, a.Lt, 8)
+    gbs32 = b32 / (ms32 * 1e-3) / 1e9
+    gbs64 = b64 / (ms64 * 1e-3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(a, g, at, results.get("fp64", results["mixed"])["iters"], sp, tp)
+
+    best = results["mixed"]
+    line = {
+        "metric": "CG solves/s to 1e-10 true resid, n=128 Lt=64: learned+FP32 vs plain FP64 CG",
+        "value": round(best["solves_per_s"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(best["ms_per_solve"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 reliable updates",
+        "data": "synthetic: von Mises(kappa=%.2f) links, Gaussian RHS, seeded" % a.beta,
+        "config": {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
+                               f"m={a.m} tol={a.tol} D^dag D CG", "n_refine": a.n, "Lt": a.Lt,
+                   "sites": D.V, "links": D.E, "solver": "FP32 CG + FP64 reliable updates"},
+        "methods": results,
+        "speedup_vs_fp64": (round(best["solves_per_s"] / results["fp64"]["solves_per_s"], 3)
+                            if "fp64" in results else None),
+        "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_stencil<float,6,false,EpiStore> (D, FP32)",
+                     "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5),
+                     "fp64_D": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBS, 4),
+                                "bytes_per_launch": b64, "ms_per_launch": round(ms64, 5)}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(a, g, at, gpu_iters, sp_full, tp_full):
+    """Oracle CSR CG (the reference's CPU path restated) on a bounded sample: the same
+    lattice at Lt=cpu_lt, `cpu_iters` CG iterations, scaled to Lt and to the GPU FP64
+    iteration count (CG iteration cost is linear in the number of points)."""
+    try:
+        import oracle as O
+        import qed3_amd as q
+    except Exception as e:  # noqa: BLE001
+        return {"error": str(e)}
+    t0 = time.time()
+    L = O.Lattice(g, a.lattice)
+    Do = O.Dirac(L, a.cpu_lt, r=1.0, M5=a.m, at=at)
+    sp, tp = sp_full[: a.cpu_lt].copy(), tp_full[: a.cpu_lt].copy()
+    Do.update(sp, tp)
+    b = q.random_source(a.lattice, a.n, a.cpu_lt, seed=77)
+    setup = time.time() - t0
+    t1 = time.time()
+    _, it, _ = Do.solve(b, tol=1e-30, maxiter=a.cpu_iters)
+    dt = time.time() - t1
+    per_iter = dt / max(a.cpu_iters, 1) * (a.Lt / a.cpu_lt)
+    sec_per_solve = per_iter * gpu_iters
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(1.0 / sec_per_solve, 6), "unit": "solves/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle CSR CG (matpoly.h:204-267 restated), n={a.n} Lt={a.cpu_lt}, "
+                      f"{a.cpu_iters} iterations in {dt:.1f}s (setup {setup:.1f}s), scaled x"
+                      f"{a.Lt // a.cpu_lt} in volume and x{gpu_iters:.0f} GPU FP64 iterations",
+            "s_per_iteration_full_volume": round(per_iter, 4)}
+
+
+if __name__ == "__main__":
+    main()

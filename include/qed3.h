@@ -1,0 +1,160 @@
+/* qed3_amd: MI355X (gfx950) Wilson-Dirac operator on the refined icosahedron S^2 x time,
+ * CG on D^dagger D and the pseudofermion force -- C ABI.
+ *
+ * Conventions shared by every entry point:
+ *  - Return value: 0 = QED3_OK, otherwise a QED3_ERR_* code; qed3_last_error() gives text.
+ *  - Spinor fields use the reference's layout (src/both_3d/dirac_ext.h:63): complex index
+ *    i = Nx*s + 2*x + a with Nx = 2V, s in [0,Lt), x in [0,V), a in {0,1}; FP64 fields
+ *    are interleaved (re,im) doubles, FP32 fields interleaved floats.
+ *  - Gauge angles use the reference's GaugeExt layout (src/both_3d/gauge_ext.h:13-14):
+ *    spatial[Lt][E] along the stored link orientation (i<j), temporal[Lt][V].
+ *  - Pointers named d_* are device pointers; everything runs on the current HIP device
+ *    on a stream owned by the context.  A context is not thread safe.
+ */
+#ifndef QED3_H
+#define QED3_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  QED3_OK = 0,
+  QED3_ERR_ARG = 1,    /* invalid argument or size */
+  QED3_ERR_HIP = 2,    /* HIP runtime error */
+  QED3_ERR_NOCONV = 3, /* solver hit maxiter before reaching the tolerance */
+  QED3_ERR_NAN = 4,    /* non-finite value / breakdown in a solver */
+  QED3_ERR_STATE = 5   /* call order violated (e.g. no gauge field set) */
+};
+
+enum { QED3_LATTICE_SIMPLICIAL = 0, QED3_LATTICE_DUAL = 1 };
+enum { QED3_BC_ANTIPERIODIC = 0, QED3_BC_PERIODIC = 1 };
+enum { QED3_OP_D = 0, QED3_OP_DDAG = 1, QED3_OP_DDAGD = 2, QED3_OP_DDDAG = 3 };
+enum { QED3_PREC_F64 = 0, QED3_PREC_F32 = 1 };
+enum { QED3_SOLVER_CG_F64 = 0, QED3_SOLVER_CG_MIXED = 1 };
+enum { QED3_HOST = 0, QED3_DEVICE = 1 };
+
+const char* qed3_last_error(void);
+
+/* ---- device memory plumbing (hipMalloc/hipMemcpy on the current device) ---- */
+int qed3_malloc(void** d_ptr, size_t bytes);
+int qed3_free(void* d_ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int qed3_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int qed3_device_synchronize(void);
+
+/* ---- geometry: refined icosahedron, its dual, spin structure ----------------------
+ * Replaces the offline table generator and its .dat files
+ * (integrator/get_geometry.cc:41-435, integrator/test_sorting_simp.cc:66-470,
+ *  integrator/test_sorting.cc:57-450) that src/both_3d/s2n_simp.h:43-269 reads. */
+typedef struct qed3_geom qed3_geom;
+int qed3_geom_create(int n_refine, qed3_geom** out);
+void qed3_geom_destroy(qed3_geom* g);
+/* V = 10n^2+2 sites, E = 30n^2 links, F = 20n^2 triangles */
+int qed3_geom_counts(const qed3_geom* g, int* V, int* E, int* F);
+/* Export every table (host arrays sized from qed3_geom_counts):
+ *  sites[3V], faces[3F] (ccw), links[2E] (i<j), nn_offsets[V+1] + nns[2E],
+ *  dual_sites[3F], dual_links[2E] (faces sharing links[k]), dual_nns[3F],
+ *  dual_face_offsets[V+1] + dual_faces[3F] (ccw ring of faces around each site),
+ *  alpha[2E] (alpha_ij, alpha_ji per simplicial link), omega[E] (omega_ij),
+ *  dual_alpha[2E], dual_omega[E] (same for dual links). */
+int qed3_geom_export(const qed3_geom* g, double* sites, int* faces, int* links,
+                     int* nn_offsets, int* nns, double* dual_sites, int* dual_links,
+                     int* dual_nns, int* dual_face_offsets, int* dual_faces, double* alpha,
+                     double* omega, double* dual_alpha, double* dual_omega);
+
+/* ---- seeded synthetic inputs (host) ------------------------------------------------
+ * The reference's RNG scheme (src/both_3d/rng.h:91-128): a master std::mt19937_64
+ * seeded with `seed` reseeds one stream per (t,link), then one per (t,site).
+ * qed3_random_gauge: spatial[Lt][E] from the link streams, temporal[Lt][V] from the
+ *   site streams (the order of GaugeExt::gaussian, gauge_ext.h:146-149), each angle
+ *   von Mises(0, kappa) (Best & Fisher 1979); kappa <= 0 gives uniform [-pi, pi).
+ * qed3_random_source: xi[2*V*Lt complex] = (g1 + i g2)/sqrt2 per component from the
+ *   site streams (ParallelRngExt::fill_gaussian, rng.h:122-128), real part first. */
+int qed3_random_gauge(int Lt, int E, int V, int seed, double kappa, double* spatial,
+                      double* temporal);
+int qed3_random_source(int Lt, int E, int V, int seed, double* xi);
+
+/* ---- operator context --------------------------------------------------------------
+ * Replaces DiracExt<Base,BaseDirac>(lattice, m, r, M5, at) (src/both_3d/dirac_ext.h:27-46)
+ * plus DWDevice/DiracPf construction (src/both_3d/sparse_dirac.h:52-129,
+ * src/both_3d/dirac_pf.h:24-36).  The bare mass of the reference is unused there; the
+ * diagonal mass term is M5 (dirac_ext.h:201). */
+typedef struct qed3_params {
+  int lattice;  /* QED3_LATTICE_* */
+  int n_refine; /* refinement level n */
+  int Lt;       /* time slices (1 = the 2D operator) */
+  int bc_time;  /* QED3_BC_*; the reference is antiperiodic (dirac_ext.h:162-165) */
+  double r;     /* Wilson parameter */
+  double M5;    /* diagonal mass */
+  double at;    /* temporal lattice spacing */
+} qed3_params;
+
+typedef struct qed3_ctx qed3_ctx;
+int qed3_ctx_create(const qed3_params* p, qed3_ctx** out);
+void qed3_ctx_destroy(qed3_ctx* ctx);
+/* sizes of the chosen lattice (dual: V=20n^2, F=10n^2+2) and <ell> */
+int qed3_ctx_sizes(const qed3_ctx* ctx, int* V, int* E, int* F, double* mean_ell);
+/* Host copies of the derived per-link / per-site quantities: kappa[E], kappa_t[V],
+ * link_volume[E], ell[E], dual_area[V] (s2n_simp.h:301-402, dirac_simp.h:270-277,
+ * dirac_ext.h:307-320).  Any pointer may be NULL. */
+int qed3_ctx_geometry(const qed3_ctx* ctx, double* kappa, double* kappa_t,
+                      double* link_volume, double* ell, double* dual_area);
+
+/* Set the U(1) gauge field (replaces DiracPf::update -> DWDevice::update,
+ * src/both_3d/dirac_pf.h:61-64, sparse_dirac.h:145-151).  where = QED3_HOST/DEVICE. */
+int qed3_set_gauge(qed3_ctx* ctx, const double* spatial, const double* temporal, int where);
+
+/* out = op(in) (replaces DiracPf::{mult,adj,DHD,DDH}_deviceAsyncLaunch,
+ * src/both_3d/dirac_pf.h:66-92).  prec selects complex double or complex float fields. */
+int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in);
+
+/* ---- CG on D^dagger D (replaces MatPoly::solve, src/both_3d/matpoly.h:204-267) --- */
+typedef struct qed3_cg_opts {
+  double tol;      /* stop when ||b - D^dag D x|| <= tol ||b|| (true FP64 residual) */
+  int maxiter;     /* inner iteration cap */
+  int solver;      /* QED3_SOLVER_* */
+  double delta;    /* reliable-update threshold on ||r|| / max ||r|| (mixed solver) */
+  int check_every; /* iterations per device-side chunk between host polls (0 = auto) */
+} qed3_cg_opts;
+
+typedef struct qed3_cg_report {
+  int status;           /* QED3_OK / QED3_ERR_NOCONV / QED3_ERR_NAN */
+  int iterations;       /* inner CG iterations */
+  int reliable_updates; /* FP64 residual recomputations */
+  double true_resid;    /* ||b - D^dag D x|| / ||b|| in FP64 */
+  double seconds;       /* wall time of the solve */
+} qed3_cg_report;
+
+/* Solve D^dag D x = b; d_x, d_b complex double device fields of length 2*V*Lt. x starts
+ * from zero, as in the reference. */
+int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+            qed3_cg_report* rep);
+
+/* ---- BLAS-1 on complex double device fields ---------------------------------------
+ * <x,y> = sum conj(x_i) y_i (the role of cublasZdotc in MatPoly::dot, matpoly.h:130-144) */
+int qed3_dot(qed3_ctx* ctx, const void* d_x, const void* d_y, double* re, double* im);
+
+/* ---- pseudofermion force -----------------------------------------------------------
+ * F_l = -2 Re <D eta, dD/dtheta_l eta> for every spatial link (s,l) and temporal link
+ * (s,x) at once (replaces GaugeExt::compute's per-link loop, gauge_ext.h:204-222, over
+ * DiracPf::precalc_grad/grad_deviceAsyncLaunch, dirac_pf.h:95-121).  Outputs are device
+ * arrays in the gauge layout: d_fs[Lt][E], d_ft[Lt][V]. */
+int qed3_pf_force(qed3_ctx* ctx, const void* d_eta, double* d_fs, double* d_ft);
+
+/* Pick the stencil launch shape (threads per block, time slices per thread) for this
+ * lattice by timing every candidate on the device (project-written autotuner). */
+int qed3_autotune(qed3_ctx* ctx);
+
+/* ---- timing of the operator kernels (HIP events on the context stream) ------------- */
+/* Average device time in ms per launch of op (QED3_OP_D / QED3_OP_DDAG) at prec over
+ * `reps` launches, after `warmup` untimed ones. */
+int qed3_time_apply(qed3_ctx* ctx, int op, int prec, int warmup, int reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QED3_H */

@@ -1,0 +1,1009 @@
+// ORACLE — test infrastructure only.  A CPU FP64 restatement of the reference's
+// fermion path, used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg as the checker.  Nothing in the product links, loads or calls this file.
+//
+// What each part restates (paths relative to the reference repository):
+//   Lattice      src/both_3d/s2n_simp.h:43-402 (S2Simp: reads the .dat tables, computes
+//                ell, link volumes, dual areas, face areas and asserts their sums),
+//                src/both_3d/s2n_dual.h:49-160 (S2Trivalent: role swap of S2Simp)
+//   Spin struct. src/both_3d/dirac_simp.h:12-59,142-151,270-277 and
+//                src/both_3d/dirac_dual.h:15-117,208-215 (alpha/omega maps, gamma, Omega,
+//                kappa; the alpha/omega consistency assert of dirac_dual.h:92-101)
+//   Deficits     integrator/test_sorting_simp.cc:397-425, integrator/test_sorting.cc
+//                (sum of omega around each face mod 4pi)
+//   Operator     src/both_3d/dirac_ext.h:49-116 (coo_structure), :121-213 (coo_format),
+//                :307-320 (kappa_t), CSR and CSR^H in the row order of
+//                src/both_3d/sparse_dirac.h:52-143, host SpMV of
+//                src/both_3d/sparse_matrix.h:257-270
+//   CG           src/both_3d/matpoly.h:204-267 (MatPoly::solve on D^dag D)
+//   Force        src/both_3d/dirac_pf.h:95-121 with dirac_ext.h:233-302 (per link: COO of
+//                dD, SpMV over all N rows, dot with D eta), gauge_ext.h:204-222 (loop)
+//   Gauge action src/both_3d/action_ext.h:33-159 (non-compact), plus the compact form
+//   RNG / HMC    src/both_3d/rng.h:6-128, gauge_ext.h:146-149, pseudofermion.h:36-58,
+//                integrator.h:122-168 (ExplicitLeapfrogML), hmc.h:117-200 (HMC2)
+//
+// Build: oracle/Makefile -> oracle/liboracle.so (g++ -O2 -fopenmp).
+#include <algorithm>
+#include <array>
+#include <cassert>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+namespace {
+
+using Complex = std::complex<double>;
+using Link = std::array<int, 2>;
+const Complex I(0.0, 1.0);
+
+struct V3 {
+  double x, y, z;
+};
+inline double dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double nrm(const V3& a) { return std::sqrt(dot(a, a)); }
+inline V3 sub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 add(const V3& a, const V3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 cross(const V3& a, const V3& b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// 2x2 complex matrix (the role of Eigen::Matrix2cd in the reference).
+struct M2 {
+  Complex a[2][2];
+  M2() { a[0][0] = a[0][1] = a[1][0] = a[1][1] = 0.0; }
+  M2(Complex p, Complex q, Complex r, Complex s) {
+    a[0][0] = p;
+    a[0][1] = q;
+    a[1][0] = r;
+    a[1][1] = s;
+  }
+  Complex& operator()(int i, int j) { return a[i][j]; }
+  Complex operator()(int i, int j) const { return a[i][j]; }
+};
+inline M2 operator*(const M2& A, const M2& B) {
+  M2 C;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) C.a[i][j] = A.a[i][0] * B.a[0][j] + A.a[i][1] * B.a[1][j];
+  return C;
+}
+inline M2 operator*(Complex s, const M2& A) {
+  M2 C;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) C.a[i][j] = s * A.a[i][j];
+  return C;
+}
+inline M2 operator*(const M2& A, Complex s) { return s * A; }
+inline M2 operator+(const M2& A, const M2& B) {
+  M2 C;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) C.a[i][j] = A.a[i][j] + B.a[i][j];
+  return C;
+}
+const M2 sigma0(1, 0, 0, 1), sigma1(0, 1, 1, 0), sigma2(0, -I, I, 0), sigma3(1, 0, 0, -1);
+
+// L'Huilier area (s2n_simp.h:236-249)
+double lhuilier(const V3& x0, const V3& x1, const V3& x2) {
+  const double a_ = std::acos(dot(x0, x1) / (nrm(x0) * nrm(x1)));
+  const double b_ = std::acos(dot(x1, x2) / (nrm(x1) * nrm(x2)));
+  const double c_ = std::acos(dot(x2, x0) / (nrm(x2) * nrm(x0)));
+  const double s_ = 0.5 * (a_ + b_ + c_);
+  const double t = std::tan(0.5 * s_) * std::tan(0.5 * (s_ - a_)) * std::tan(0.5 * (s_ - b_)) *
+                   std::tan(0.5 * (s_ - c_));
+  return 4.0 * std::atan(std::sqrt(t));
+}
+
+// ---------------------------------------------------------------------------- lattice
+struct Lattice {
+  int n_sites = 0, n_links = 0, n_faces = 0;
+  std::vector<V3> sites, dual_sites;
+  std::vector<std::vector<int>> nns, faces;
+  std::vector<Link> links, dual_links;
+  std::map<Link, int> map2il;
+  std::map<Link, int> map2sign;
+  std::vector<int> counter_accum;
+  std::vector<double> ell, link_volume, dual_areas, vols;
+  double mean_ell = 0, mean_vol = 0, mean_link_volume = 0, mean_dual_area = 0;
+  // checks recorded instead of asserted
+  double sum_vols = 0, sum_link_volume = 0, sum_dual_areas = 0, max_ell_diff = 0;
+  // spin structure
+  std::map<Link, double> omega, alpha;
+  std::vector<double> kappa;
+
+  void finish_maps() {
+    map2il.clear();
+    map2sign.clear();
+    for (int il = 0; il < n_links; il++) {
+      const int i = links[il][0], j = links[il][1];
+      map2il[{i, j}] = il;
+      map2il[{j, i}] = il;
+      map2sign[{i, j}] = +1;
+      map2sign[{j, i}] = -1;
+    }
+    counter_accum.clear();
+    int c = 0;
+    for (int ix = 0; ix < n_sites; ix++) {
+      counter_accum.push_back(c);
+      c += int(nns[ix].size());
+    }
+    counter_accum.push_back(c);
+  }
+};
+
+// S2Simp (s2n_simp.h:43-402) from tables.
+std::unique_ptr<Lattice> make_simp(int V, int E, int F, const double* pts, const double* pts_dual,
+                                   const int* faces, const int* links, const int* nn_off,
+                                   const int* nns, const int* dual_links, const int* dface_off,
+                                   const int* dfaces, std::vector<std::vector<int>>& dual_faces_out,
+                                   std::vector<double>& vols_out, double& mean_vol_out) {
+  auto L = std::make_unique<Lattice>();
+  L->n_sites = V;
+  L->n_links = E;
+  L->n_faces = F;
+  for (int i = 0; i < V; i++) L->sites.push_back({pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+  for (int i = 0; i < F; i++)
+    L->dual_sites.push_back({pts_dual[3 * i], pts_dual[3 * i + 1], pts_dual[3 * i + 2]});
+  for (int l = 0; l < E; l++) {
+    L->links.push_back({links[2 * l], links[2 * l + 1]});
+    L->dual_links.push_back({dual_links[2 * l], dual_links[2 * l + 1]});
+  }
+  for (int f = 0; f < F; f++) L->faces.push_back({faces[3 * f], faces[3 * f + 1], faces[3 * f + 2]});
+  for (int x = 0; x < V; x++) L->nns.push_back(std::vector<int>(nns + nn_off[x], nns + nn_off[x + 1]));
+  dual_faces_out.clear();
+  for (int x = 0; x < V; x++)
+    dual_faces_out.push_back(std::vector<int>(dfaces + dface_off[x], dfaces + dface_off[x + 1]));
+  L->finish_maps();
+  // face areas (s2n_simp.h:226-266)
+  L->sum_vols = 0;
+  for (const auto& f : L->faces) {
+    const double a = lhuilier(L->sites[f[0]], L->sites[f[1]], L->sites[f[2]]);
+    L->vols.push_back(a);
+    L->sum_vols += a;
+  }
+  L->mean_vol = L->sum_vols / F;
+  // ell, link volumes (s2n_simp.h:301-364)
+  L->ell.resize(E);
+  L->link_volume.resize(E);
+  for (int il = 0; il < E; il++) {
+    const V3 x = L->sites[L->links[il][0]], y = L->sites[L->links[il][1]];
+    const int iA = std::min(L->dual_links[il][0], L->dual_links[il][1]);
+    const int iB = std::max(L->dual_links[il][0], L->dual_links[il][1]);
+    double ellA, ellB, areaA, areaB;
+    {
+      const V3 p = L->dual_sites[iA];
+      ellA = std::acos(dot(x, y) / (nrm(x) * nrm(y)));
+      areaA = lhuilier(x, y, p);
+    }
+    {
+      const V3 p = L->dual_sites[iB];
+      ellB = std::acos(dot(x, y) / (nrm(x) * nrm(y)));
+      areaB = lhuilier(x, y, p);
+    }
+    L->max_ell_diff = std::max(L->max_ell_diff, std::abs(ellA - ellB));
+    L->ell[il] = ellA;
+    L->link_volume[il] = areaA + areaB;
+  }
+  L->sum_link_volume = 0;
+  for (double v : L->link_volume) L->sum_link_volume += v;
+  L->mean_link_volume = L->sum_link_volume / E;
+  L->mean_ell = 0;
+  for (double v : L->ell) L->mean_ell += v;
+  L->mean_ell /= E;
+  // dual areas (s2n_simp.h:366-402)
+  L->dual_areas.resize(V);
+  for (int ip = 0; ip < V; ip++) {
+    double area = 0;
+    const auto& face = dual_faces_out[ip];
+    for (size_t i = 0; i < face.size(); i++)
+      area += lhuilier(L->dual_sites[face[i]], L->dual_sites[face[(i + 1) % face.size()]],
+                       L->sites[ip]);
+    L->dual_areas[ip] = area;
+  }
+  L->sum_dual_areas = 0;
+  for (double v : L->dual_areas) L->sum_dual_areas += v;
+  L->mean_dual_area = L->sum_dual_areas / V;
+  vols_out = L->vols;
+  mean_vol_out = L->mean_vol;
+  return L;
+}
+
+// S2Trivalent (s2n_dual.h:49-160): role swap of the simplicial tables.
+std::unique_ptr<Lattice> make_dual(const Lattice& simp, const std::vector<std::vector<int>>& dual_faces,
+                                   const int* dual_nns) {
+  auto L = std::make_unique<Lattice>();
+  L->sites = simp.dual_sites;
+  L->n_sites = int(L->sites.size());
+  for (int f = 0; f < L->n_sites; f++)
+    L->nns.push_back({dual_nns[3 * f], dual_nns[3 * f + 1], dual_nns[3 * f + 2]});
+  L->links = simp.dual_links;
+  L->n_links = int(L->links.size());
+  L->faces = dual_faces;
+  L->n_faces = int(L->faces.size());
+  L->vols = simp.dual_areas;
+  L->mean_vol = simp.mean_dual_area;
+  L->dual_sites = simp.sites;
+  L->dual_links = simp.links;
+  L->ell.resize(L->n_links);
+  L->link_volume.resize(L->n_links);
+  for (int il = 0; il < L->n_links; il++) {
+    const V3 x = L->sites[L->links[il][0]], y = L->sites[L->links[il][1]];
+    const int iA = L->dual_links[il][0], iB = L->dual_links[il][1];
+    const double ellA = std::acos(dot(x, y) / (nrm(x) * nrm(y)));
+    const double ellB = ellA;
+    const double areaA = lhuilier(x, y, L->dual_sites[iA]);
+    const double areaB = lhuilier(x, y, L->dual_sites[iB]);
+    L->max_ell_diff = std::max(L->max_ell_diff, std::abs(ellA - ellB));
+    L->ell[il] = ellA;
+    L->link_volume[il] = areaA + areaB;
+  }
+  L->sum_link_volume = 0;
+  for (double v : L->link_volume) L->sum_link_volume += v;
+  L->mean_link_volume = L->sum_link_volume / L->n_links;
+  L->mean_ell = 0;
+  for (double v : L->ell) L->mean_ell += v;
+  L->mean_ell /= L->n_links;
+  L->dual_areas = simp.vols;
+  L->mean_dual_area = simp.mean_vol;
+  L->sum_dual_areas = 0;
+  for (double v : L->dual_areas) L->sum_dual_areas += v;
+  L->sum_vols = 0;
+  for (double v : L->vols) L->sum_vols += v;
+  L->finish_maps();
+  return L;
+}
+
+// Spin structure maps (dirac_simp.h:12-59): omega (i,j) and (j,i) = -omega; alpha per
+// directed link.  Then kappa (dirac_simp.h:270-277 / dirac_dual.h:208-215).
+void set_spin(Lattice& L, const double* alpha2, const double* omega1) {
+  L.omega.clear();
+  L.alpha.clear();
+  for (int il = 0; il < L.n_links; il++) {
+    const int i = L.links[il][0], j = L.links[il][1];
+    L.omega[{i, j}] = omega1[il];
+    L.omega[{j, i}] = -omega1[il];
+    L.alpha[{i, j}] = alpha2[2 * il];
+    L.alpha[{j, i}] = alpha2[2 * il + 1];
+  }
+  L.kappa.resize(L.n_links);
+  for (int il = 0; il < L.n_links; il++)
+    L.kappa[il] = 2.0 * L.link_volume[il] / L.ell[il] / L.mean_ell;
+}
+
+M2 gamma_of(const Lattice& L, int ix, int iy) {
+  const double al = L.alpha.at({ix, iy});
+  return std::cos(al) * sigma1 + Complex(std::sin(al)) * sigma2;
+}
+M2 Omega_of(const Lattice& L, int ix, int iy) {
+  const double om = L.omega.at({ix, iy});
+  return Complex(std::cos(0.5 * om)) * sigma0 + (-I * std::sin(0.5 * om)) * sigma3;
+}
+
+double mod_round(double a, double b) {  // dirac_dual.h:9-13
+  const int p = int(std::round(a / b));
+  return a - p * b;
+}
+double mod_floor(double a, double b) {  // geodesic.h:36-41
+  const int p = int(std::floor(a / b));
+  return a - p * b;
+}
+
+// ---------------------------------------------------------------------------- operator
+struct Dirac {
+  const Lattice* lat;
+  int Nt, bc;
+  double r, M5, at;
+  long Nx, N;
+  std::vector<double> kappa_t;
+  std::vector<int> is, js;
+  std::vector<Complex> v_coo;
+  // CSR / CSR^H (sparse_dirac.h:52-143)
+  std::vector<long> rows, rowsT;
+  std::vector<int> cols, colsT;
+  std::vector<long> ell2em, ell2emT;
+  std::vector<Complex> val, valH;
+  std::vector<double> spatial, temporal;  // [Nt][E], [Nt][V]
+
+  double sp(int s, int ix, int iy) const {  // gauge_ext.h:45-50
+    const int il = lat->map2il.at({ix, iy});
+    const int sign = lat->map2sign.at({ix, iy});
+    return sign * spatial[size_t((s + Nt) % Nt) * lat->n_links + il];
+  }
+  double tp(int s, int ix) const { return temporal[size_t((s + Nt) % Nt) * lat->n_sites + ix]; }
+
+  void coo_structure() {  // dirac_ext.h:49-116
+    const int V = lat->n_sites;
+    const long nsp = 4L * lat->counter_accum.back();
+    const long len = nsp * Nt + 8L * V * Nt + 4L * V * Nt;
+    is.assign(len, 0);
+    js.assign(len, 0);
+    for (int s = 0; s < Nt; s++)
+      for (int ix = 0; ix < V; ix++) {
+        long c = nsp * s + 4L * lat->counter_accum[ix];
+        for (int iy : lat->nns[ix]) {
+          is[c] = int(Nx * s + 2 * ix); js[c] = int(Nx * s + 2 * iy); c++;
+          is[c] = int(Nx * s + 2 * ix); js[c] = int(Nx * s + 2 * iy + 1); c++;
+          is[c] = int(Nx * s + 2 * ix + 1); js[c] = int(Nx * s + 2 * iy); c++;
+          is[c] = int(Nx * s + 2 * ix + 1); js[c] = int(Nx * s + 2 * iy + 1); c++;
+        }
+      }
+    for (int s = 0; s < Nt; s++)
+      for (int ix = 0; ix < V; ix++) {
+        long c = nsp * Nt + 8L * (long(V) * s + ix);
+        is[c] = int((Nx * (s + 1) + 2 * ix) % N); js[c] = int(Nx * s + 2 * ix); c++;
+        is[c] = int((Nx * (s + 1) + 2 * ix) % N); js[c] = int(Nx * s + 2 * ix + 1); c++;
+        is[c] = int((Nx * (s - 1) + 2 * ix + N) % N); js[c] = int(Nx * s + 2 * ix); c++;
+        is[c] = int((Nx * (s - 1) + 2 * ix + N) % N); js[c] = int(Nx * s + 2 * ix + 1); c++;
+        is[c] = int((Nx * (s + 1) + 2 * ix + 1) % N); js[c] = int(Nx * s + 2 * ix); c++;
+        is[c] = int((Nx * (s + 1) + 2 * ix + 1) % N); js[c] = int(Nx * s + 2 * ix + 1); c++;
+        is[c] = int((Nx * (s - 1) + 2 * ix + 1 + N) % N); js[c] = int(Nx * s + 2 * ix); c++;
+        is[c] = int((Nx * (s - 1) + 2 * ix + 1 + N) % N); js[c] = int(Nx * s + 2 * ix + 1); c++;
+      }
+    for (int s = 0; s < Nt; s++)
+      for (int ix = 0; ix < V; ix++) {
+        long c = nsp * Nt + 8L * V * Nt + 4L * (long(V) * s + ix);
+        is[c] = int(Nx * s + 2 * ix); js[c] = int(Nx * s + 2 * ix); c++;
+        is[c] = int(Nx * s + 2 * ix); js[c] = int(Nx * s + 2 * ix + 1); c++;
+        is[c] = int(Nx * s + 2 * ix + 1); js[c] = int(Nx * s + 2 * ix); c++;
+        is[c] = int(Nx * s + 2 * ix + 1); js[c] = int(Nx * s + 2 * ix + 1); c++;
+      }
+  }
+
+  // DWDevice::initialize (sparse_dirac.h:52-129): for every row i the entries with
+  // is==i (resp. js==i for CSR^H) in increasing COO position.  Same order via a stable
+  // counting sort instead of the O(N*nnz) scan.
+  void build_csr() {
+    const long len = long(is.size());
+    rows.assign(N + 1, 0);
+    rowsT.assign(N + 1, 0);
+    for (long e = 0; e < len; e++) {
+      rows[is[e] + 1]++;
+      rowsT[js[e] + 1]++;
+    }
+    for (long i = 0; i < N; i++) {
+      rows[i + 1] += rows[i];
+      rowsT[i + 1] += rowsT[i];
+    }
+    std::vector<long> fill(rows.begin(), rows.end() - 1), fillT(rowsT.begin(), rowsT.end() - 1);
+    cols.assign(len, 0);
+    colsT.assign(len, 0);
+    ell2em.assign(len, 0);
+    ell2emT.assign(len, 0);
+    for (long e = 0; e < len; e++) {
+      const long em = fill[is[e]]++;
+      ell2em[e] = em;
+      cols[em] = js[e];
+      const long emT = fillT[js[e]]++;
+      ell2emT[e] = emT;
+      colsT[emT] = is[e];
+    }
+    val.assign(len, 0.0);
+    valH.assign(len, 0.0);
+  }
+
+  // DiracExt::coo_format (dirac_ext.h:121-213) + coo2csr_csrH (sparse_dirac.h:132-143)
+  void update(const double* sp_in, const double* tp_in) {
+    const int V = lat->n_sites, E = lat->n_links;
+    spatial.assign(sp_in, sp_in + size_t(Nt) * E);
+    temporal.assign(tp_in, tp_in + size_t(Nt) * V);
+    v_coo.assign(is.size(), 0.0);
+    const long nsp = 4L * lat->counter_accum.back();
+    // per directed link: the time independent part 1/2 kappa (-r + gamma) ... Omega
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < Nt; s++) {
+      for (int ix = 0; ix < V; ix++) {
+        long c = nsp * s + 4L * lat->counter_accum[ix];
+        for (int iy : lat->nns[ix]) {
+          const int il = lat->map2il.at({ix, iy});
+          const M2 tmp = Complex(0.5 * lat->kappa[il]) * (Complex(-r) * sigma0 + gamma_of(*lat, ix, iy)) *
+                         std::exp(I * sp(s, ix, iy)) * Omega_of(*lat, ix, iy);
+          v_coo[c++] = tmp(0, 0);
+          v_coo[c++] = tmp(0, 1);
+          v_coo[c++] = tmp(1, 0);
+          v_coo[c++] = tmp(1, 1);
+        }
+      }
+    }
+    const double sgn = bc == 1 ? 1.0 : -1.0;
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < Nt; s++) {
+      const double signP = (s == Nt - 1) ? sgn : 1.0;
+      const double signM = (s == 0) ? sgn : 1.0;
+      for (int ix = 0; ix < V; ix++) {
+        long c = nsp * Nt + 8L * (long(V) * s + ix);
+        const M2 tmpP = Complex(0.5 * signP * kappa_t[ix]) * (Complex(-r) * sigma0 + Complex(-1.0) * sigma3) *
+                        std::exp(-I * tp(s, ix));
+        const M2 tmpM = Complex(0.5 * signM * kappa_t[ix]) * (Complex(-r) * sigma0 + sigma3) *
+                        std::exp(I * tp(s - 1, ix));
+        v_coo[c++] = tmpP(0, 0);
+        v_coo[c++] = tmpP(0, 1);
+        v_coo[c++] = tmpM(0, 0);
+        v_coo[c++] = tmpM(0, 1);
+        v_coo[c++] = tmpP(1, 0);
+        v_coo[c++] = tmpP(1, 1);
+        v_coo[c++] = tmpM(1, 0);
+        v_coo[c++] = tmpM(1, 1);
+      }
+    }
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < Nt; s++)
+      for (int ix = 0; ix < V; ix++) {
+        double coeff = 0.0;
+        for (int iy : lat->nns[ix]) coeff += 0.5 * r * lat->kappa[lat->map2il.at({ix, iy})];
+        coeff += r * kappa_t[ix];
+        coeff += M5;
+        long c = nsp * Nt + 8L * V * Nt + 4L * (long(V) * s + ix);
+        v_coo[c++] = coeff;
+        v_coo[c++] = 0.0;
+        v_coo[c++] = 0.0;
+        v_coo[c++] = coeff;
+      }
+    const long len = long(is.size());
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < len; e++) {
+      val[ell2em[e]] = v_coo[e];
+      valH[ell2emT[e]] = std::conj(v_coo[e]);
+    }
+  }
+
+  // host SpMV (sparse_matrix.h:257-270), rows in parallel
+  void matmul(Complex* res, const Complex* v, bool H) const {
+    const std::vector<long>& R = H ? rowsT : rows;
+    const std::vector<int>& Cc = H ? colsT : cols;
+    const std::vector<Complex>& Vv = H ? valH : val;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; i++) {
+      Complex acc = 0.0;
+      for (long jj = R[i]; jj < R[i + 1]; jj++) acc = acc + Vv[jj] * v[Cc[jj]];
+      res[i] = acc;
+    }
+  }
+};
+
+std::unique_ptr<Dirac> make_dirac(const Lattice* L, int Nt, int bc, double r, double M5, double at) {
+  auto D = std::make_unique<Dirac>();
+  D->lat = L;
+  D->Nt = Nt;
+  D->bc = bc;
+  D->r = r;
+  D->M5 = M5;
+  D->at = at;
+  D->Nx = 2L * L->n_sites;
+  D->N = D->Nx * Nt;
+  D->kappa_t.resize(L->n_sites);
+  for (int ix = 0; ix < L->n_sites; ix++)  // dirac_ext.h:307-320
+    D->kappa_t[ix] = (Nt != 1) ? L->dual_areas[ix] / L->mean_ell / at : 0.0;
+  D->coo_structure();
+  D->build_csr();
+  return D;
+}
+
+// Zdotc: sum conj(x) y, sequential (the reduction order of cuBLAS is not restated).
+Complex zdotc(const Complex* x, const Complex* y, long n) {
+  Complex s = 0.0;
+  for (long i = 0; i < n; i++) s += std::conj(x[i]) * y[i];
+  return s;
+}
+
+// MatPoly::solve (matpoly.h:204-267) on A = D^dag D, x0 = 0, stop on |r|^2 < tol^2 |b|^2.
+int cg(const Dirac& D, Complex* x, const Complex* b, double tol, int maxiter, double* mu_out) {
+  const long N = D.N;
+  std::vector<Complex> p(N), q(N), r(N), tmp(N);
+  for (long i = 0; i < N; i++) {
+    x[i] = 0.0;
+    r[i] = b[i];
+    p[i] = r[i];
+  }
+  double mu = std::real(zdotc(r.data(), r.data(), N));
+  double mu_old = mu;
+  const double b_norm_sq = std::real(zdotc(r.data(), r.data(), N));
+  const double mu_crit = tol * tol * b_norm_sq;
+  int k = 0;
+  if (!(mu < mu_crit)) {
+    for (; k < maxiter; ++k) {
+      D.matmul(tmp.data(), p.data(), false);
+      D.matmul(q.data(), tmp.data(), true);
+      const Complex gam = zdotc(p.data(), q.data(), N);
+      const Complex al = mu / gam;
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < N; i++) {
+        x[i] = al * p[i] + x[i];
+        r[i] = -al * q[i] + r[i];
+      }
+      mu = std::real(zdotc(r.data(), r.data(), N));
+      if (mu < mu_crit || std::isnan(mu)) break;
+      const Complex bet = mu / mu_old;
+      mu_old = mu;
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < N; i++) p[i] = bet * p[i] + r[i];
+    }
+  }
+  if (mu_out) *mu_out = mu;
+  return k;
+}
+
+// d_coo_format for a spatial link (dirac_ext.h:233-269) and a temporal link (:272-302)
+struct COOE {
+  Complex v;
+  long i, j;
+};
+void d_coo_spatial(const Dirac& D, int s, int ix, int iy, std::vector<COOE>& e) {
+  const Lattice& L = *D.lat;
+  const long Nx = D.Nx;
+  e.clear();
+  {
+    const int il = L.map2il.at({ix, iy});
+    const M2 tmp = Complex(0.5 * L.kappa[il]) * (Complex(-D.r) * sigma0 + gamma_of(L, ix, iy)) *
+                   (I * std::exp(I * D.sp(s, ix, iy))) * Omega_of(L, ix, iy);
+    e.push_back({tmp(0, 0), Nx * s + 2 * ix, Nx * s + 2 * iy});
+    e.push_back({tmp(0, 1), Nx * s + 2 * ix, Nx * s + 2 * iy + 1});
+    e.push_back({tmp(1, 0), Nx * s + 2 * ix + 1, Nx * s + 2 * iy});
+    e.push_back({tmp(1, 1), Nx * s + 2 * ix + 1, Nx * s + 2 * iy + 1});
+  }
+  {
+    const int il = L.map2il.at({ix, iy});
+    const M2 tmp = Complex(-0.5 * L.kappa[il]) * (Complex(-D.r) * sigma0 + gamma_of(L, iy, ix)) *
+                   (I * std::exp(I * D.sp(s, iy, ix))) * Omega_of(L, iy, ix);
+    e.push_back({tmp(0, 0), Nx * s + 2 * iy, Nx * s + 2 * ix});
+    e.push_back({tmp(0, 1), Nx * s + 2 * iy, Nx * s + 2 * ix + 1});
+    e.push_back({tmp(1, 0), Nx * s + 2 * iy + 1, Nx * s + 2 * ix});
+    e.push_back({tmp(1, 1), Nx * s + 2 * iy + 1, Nx * s + 2 * ix + 1});
+  }
+}
+void d_coo_temporal(const Dirac& D, int s, int ix, std::vector<COOE>& e) {
+  const long Nx = D.Nx, N = D.N;
+  e.clear();
+  const double sign = (s == D.Nt - 1) ? (D.bc == 1 ? 1.0 : -1.0) : 1.0;
+  const double kt = D.kappa_t[ix];
+  const M2 tmpP = Complex(-0.5 * sign * kt) * (Complex(-D.r) * sigma0 + Complex(-1.0) * sigma3) *
+                  (I * std::exp(-I * D.tp(s, ix)));
+  const M2 tmpM = Complex(0.5 * sign * kt) * (Complex(-D.r) * sigma0 + sigma3) *
+                  (I * std::exp(I * D.tp(s, ix)));
+  e.push_back({tmpP(0, 0), (Nx * (s + 1) + 2 * ix) % N, Nx * s + 2 * ix});
+  e.push_back({tmpP(0, 1), (Nx * (s + 1) + 2 * ix) % N, Nx * s + 2 * ix + 1});
+  e.push_back({tmpM(0, 0), (Nx * s + 2 * ix + N) % N, (Nx * (s + 1) + 2 * ix) % N});
+  e.push_back({tmpM(0, 1), (Nx * s + 2 * ix + N) % N, (Nx * (s + 1) + 2 * ix + 1) % N});
+  e.push_back({tmpP(1, 0), (Nx * (s + 1) + 2 * ix + 1) % N, Nx * s + 2 * ix});
+  e.push_back({tmpP(1, 1), (Nx * (s + 1) + 2 * ix + 1) % N, Nx * s + 2 * ix + 1});
+  e.push_back({tmpM(1, 0), (Nx * s + 2 * ix + 1 + N) % N, (Nx * (s + 1) + 2 * ix) % N});
+  e.push_back({tmpM(1, 1), (Nx * s + 2 * ix + 1 + N) % N, (Nx * (s + 1) + 2 * ix + 1) % N});
+}
+
+// grad for one link, full route: SpMV of dD over all N rows, Zdotc with D eta
+double grad_full(const Dirac& D, const std::vector<COOE>& e, const Complex* Deta,
+                 const Complex* eta, std::vector<Complex>& buf) {
+  std::fill(buf.begin(), buf.end(), Complex(0.0));
+  for (const auto& c : e) buf[c.i] += c.v * eta[c.j];  // matmulcoo (sparse_matrix.h:250-254)
+  return -2.0 * std::real(zdotc(Deta, buf.data(), D.N));
+}
+// the same value summed over the rows dD touches only
+double grad_local(const std::vector<COOE>& e, const Complex* Deta, const Complex* eta) {
+  std::map<long, Complex> rowsum;
+  for (const auto& c : e) rowsum[c.i] += c.v * eta[c.j];
+  Complex s = 0.0;
+  for (auto& kv : rowsum) s += std::conj(Deta[kv.first]) * kv.second;
+  return -2.0 * std::real(s);
+}
+
+// ---------------------------------------------------------------------------- gauge action
+// U1WilsonExt (action_ext.h:33-159) on the faces of lattice L; compact adds the cosine form.
+struct GaugeAction {
+  const Lattice* L;
+  double gsq, at;
+  bool compact;
+  std::vector<double> beta_s, beta_t;
+  int Nt;
+  void set_beta() {  // action_ext.h:150-159
+    beta_s.resize(L->n_faces);
+    beta_t.resize(L->n_links);
+    for (int i = 0; i < L->n_faces; i++) beta_s[i] = at / L->vols[i] / gsq;
+    for (int il = 0; il < L->n_links; il++)
+      beta_t[il] = 2.0 / at * L->link_volume[il] / std::pow(L->ell[il], 2) / gsq;
+  }
+  double sp(const double* u, int s, int ix, int iy) const {
+    const int il = L->map2il.at({ix, iy});
+    return L->map2sign.at({ix, iy}) * u[size_t((s + Nt) % Nt) * L->n_links + il];
+  }
+  double face_angle(const double* u, int s, const std::vector<int>& f) const {  // gauge_ext.h:124-133
+    double sum = 0;
+    for (size_t i = 0; i < f.size(); i++) sum += sp(u, s, f[i], f[(i + 1) % f.size()]);
+    return sum;
+  }
+  double link_angle(const double* u, const double* t, int s, const Link& l) const {  // :135-143
+    const int V = L->n_sites;
+    return sp(u, s, l[0], l[1]) + t[size_t((s + Nt) % Nt) * V + l[1]] - sp(u, s + 1, l[0], l[1]) -
+           t[size_t((s + Nt) % Nt) * V + l[0]];
+  }
+  double S(const double* u, const double* t) const {
+    double res = 0;
+    for (int s = 0; s < Nt; s++) {
+      double tmp = 0;
+      for (int i = 0; i < L->n_faces; i++) {
+        const double P = face_angle(u, s, L->faces[i]);
+        tmp += compact ? beta_s[i] * (1.0 - std::cos(P)) : 0.5 * beta_s[i] * P * P;
+      }
+      res += tmp;
+    }
+    for (int s = 0; s < Nt; s++) {
+      double tmp = 0;
+      for (const Link& l : L->links) {
+        const int il = L->map2il.at(l);
+        const double P = link_angle(u, t, s, l);
+        tmp += compact ? beta_t[il] * (1.0 - std::cos(P)) : 0.5 * beta_t[il] * P * P;
+      }
+      res += tmp;
+    }
+    return res;
+  }
+  void force(const double* u, const double* t, double* fs, double* ft) const {
+    const int V = L->n_sites, E = L->n_links;
+    std::fill(fs, fs + size_t(Nt) * E, 0.0);
+    std::fill(ft, ft + size_t(Nt) * V, 0.0);
+    auto g = [&](double b, double P) { return compact ? b * std::sin(P) : b * P; };
+    for (int s = 0; s < Nt; s++)
+      for (int i = 0; i < L->n_faces; i++) {
+        const auto& f = L->faces[i];
+        const double grad = g(beta_s[i], face_angle(u, s, f));
+        for (size_t k = 0; k < f.size(); k++) {
+          const Link e{f[k], f[(k + 1) % f.size()]};
+          fs[size_t(s) * E + L->map2il.at(e)] += grad * L->map2sign.at(e);
+        }
+      }
+    for (int s = 0; s < Nt; s++)
+      for (const Link& l : L->links) {
+        const int il = L->map2il.at(l);
+        const double grad = g(beta_t[il], link_angle(u, t, s, l));
+        const double grad2 = g(beta_t[il], link_angle(u, t, s - 1, l));
+        fs[size_t(s) * E + il] += grad;
+        ft[size_t(s) * V + l[1]] += grad;
+        fs[size_t(s) * E + il] -= grad2;
+        ft[size_t(s) * V + l[0]] -= grad;
+      }
+  }
+};
+
+// ---------------------------------------------------------------------------- RNG
+// SingleRng / ParallelRngExt (rng.h:6-120).  reseed(int) narrows the master draw to int
+// exactly as the reference's signature does.
+struct SingleRng {
+  std::mt19937_64 mt;
+  std::normal_distribution<double> dist_gaussian;
+  std::uniform_real_distribution<double> dist_01;
+  double gaussian() { return dist_gaussian(mt); }
+  double uniform() { return dist_01(mt); }
+  void reseed(const int seed) { mt.seed(seed); }
+};
+struct ParallelRng {
+  int Nt, E, V;
+  SingleRng master;
+  std::vector<SingleRng> links, sites;  // [t][il], [t][ix]
+  ParallelRng(int Nt_, int E_, int V_, int seed) : Nt(Nt_), E(E_), V(V_) {
+    links.resize(size_t(Nt) * E);
+    sites.resize(size_t(Nt) * V);
+    master.reseed(seed);
+    for (auto& r : links) r.reseed(int(master.mt()));
+    for (auto& r : sites) r.reseed(int(master.mt()));
+  }
+  double gaussian_link(int s, int il) { return links[size_t(s) * E + il].gaussian(); }
+  double gaussian_site(int s, int ix) { return sites[size_t(s) * V + ix].gaussian(); }
+  double uniform() { return master.uniform(); }
+};
+
+// ---------------------------------------------------------------------------- HMC
+// HMC2::run (hmc.h:172-199) with ExplicitLeapfrogML::integrate (integrator.h:122-168),
+// PseudoFermion (pseudofermion.h:36-58) and the per-link force of gauge_ext.h:204-222
+// evaluated on the rows dD touches.
+struct HMCState {
+  Dirac* D;
+  GaugeAction* Sg;
+  ParallelRng* rng;
+  int npf;
+  double tol;
+  std::vector<std::vector<Complex>> phi, eta;
+  std::vector<double> u, t, pu, pt;
+  long cg_iters = 0;
+
+  void update_eta(int k) {
+    cg(*D, eta[k].data(), phi[k].data(), tol, 1000000, nullptr);
+    cg_iters++;
+  }
+  void pf_gen(int k) {
+    const int V = D->lat->n_sites, Nt = D->Nt;
+    std::vector<Complex> xi(D->N);
+    for (int s = 0; s < Nt; s++)
+      for (int ix = 0; ix < V; ix++)
+        for (int a = 0; a < 2; a++) {
+          const double re = rng->gaussian_site(s, ix);
+          const double im = rng->gaussian_site(s, ix);
+          xi[D->Nx * s + 2 * ix + a] = Complex(re, im) / std::sqrt(2.0);
+        }
+    phi[k].assign(D->N, 0.0);
+    D->matmul(phi[k].data(), xi.data(), true);
+    eta[k].assign(D->N, 0.0);
+    update_eta(k);
+  }
+  double Sf(int k) const { return std::real(zdotc(phi[k].data(), eta[k].data(), D->N)); }
+  void fermion_force(int k, std::vector<double>& fs, std::vector<double>& ft) {
+    const Lattice& L = *D->lat;
+    const int Nt = D->Nt, V = L.n_sites, E = L.n_links;
+    std::vector<Complex> Deta(D->N);
+    D->matmul(Deta.data(), eta[k].data(), false);
+    fs.assign(size_t(Nt) * E, 0.0);
+    ft.assign(size_t(Nt) * V, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < Nt; s++) {
+      std::vector<COOE> e;
+      for (int il = 0; il < E; il++) {
+        d_coo_spatial(*D, s, L.links[il][0], L.links[il][1], e);
+        fs[size_t(s) * E + il] = grad_local(e, Deta.data(), eta[k].data());
+      }
+      for (int ix = 0; ix < V; ix++) {
+        d_coo_temporal(*D, s, ix, e);
+        ft[size_t(s) * V + ix] = grad_local(e, Deta.data(), eta[k].data());
+      }
+    }
+  }
+  double H() {
+    double res = 0;
+    for (double v : pu) res += v * v;
+    for (double v : pt) res += v * v;
+    res *= 0.5;
+    res += Sg->S(u.data(), t.data());
+    for (int k = 0; k < npf; k++) res += Sf(k);
+    return res;
+  }
+  void integrate(double stot, int nsteps, int nsteps_inner) {
+    const double tau = stot / nsteps, tau_in = tau / nsteps_inner;
+    std::vector<double> gs, gt, fs, ft;
+    auto kick = [&](const std::vector<double>& a, const std::vector<double>& b, double c) {
+      for (size_t i = 0; i < pu.size(); i++) pu[i] += c * a[i];
+      for (size_t i = 0; i < pt.size(); i++) pt[i] += c * b[i];
+    };
+    gs.resize(u.size());
+    gt.resize(t.size());
+    for (int k = 0; k < npf; k++) {
+      fermion_force(k, fs, ft);
+      kick(fs, ft, -0.5 * tau);
+    }
+    for (int n = 0; n < nsteps; n++) {
+      Sg->force(u.data(), t.data(), gs.data(), gt.data());
+      kick(gs, gt, -0.5 * tau_in);
+      for (int ni = 0; ni < nsteps_inner; ni++) {
+        for (size_t i = 0; i < u.size(); i++) u[i] += tau_in * pu[i];
+        for (size_t i = 0; i < t.size(); i++) t[i] += tau_in * pt[i];
+        Sg->force(u.data(), t.data(), gs.data(), gt.data());
+        kick(gs, gt, (ni != nsteps_inner - 1) ? -1.0 * tau_in : -0.5 * tau_in);
+      }
+      D->update(u.data(), t.data());
+      for (int k = 0; k < npf; k++) {
+        update_eta(k);
+        fermion_force(k, fs, ft);
+        kick(fs, ft, (n != nsteps - 1) ? -1.0 * tau : -0.5 * tau);
+      }
+    }
+  }
+};
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+struct oracle_lat {
+  std::unique_ptr<Lattice> L;
+  std::unique_ptr<Lattice> simp_keep;
+};
+
+// Build the simplicial (lattice=0) or dual (lattice=1) lattice from the tables.
+void* oracle_lattice_create(int lattice, int V, int E, int F, const double* pts,
+                            const double* pts_dual, const int* faces, const int* links,
+                            const int* nn_off, const int* nns, const int* dual_links,
+                            const int* dual_nns, const int* dface_off, const int* dfaces,
+                            const double* alpha, const double* omega, const double* dual_alpha,
+                            const double* dual_omega) {
+  auto* h = new oracle_lat;
+  std::vector<std::vector<int>> dual_faces;
+  std::vector<double> vols;
+  double mv;
+  auto S = make_simp(V, E, F, pts, pts_dual, faces, links, nn_off, nns, dual_links, dface_off,
+                     dfaces, dual_faces, vols, mv);
+  if (lattice == 0) {
+    set_spin(*S, alpha, omega);
+    h->L = std::move(S);
+  } else {
+    auto D = make_dual(*S, dual_faces, dual_nns);
+    set_spin(*D, dual_alpha, dual_omega);
+    h->L = std::move(D);
+    h->simp_keep = std::move(S);
+  }
+  return h;
+}
+void oracle_lattice_destroy(void* h) { delete static_cast<oracle_lat*>(h); }
+
+// out[0..6]: sum face areas, sum link volumes, sum dual areas, max |ellA-ellB|,
+// max spin-structure deviation, number of faces failing the deficit range, mean_ell
+void oracle_lattice_checks(void* hh, double* out) {
+  const Lattice& L = *static_cast<oracle_lat*>(hh)->L;
+  out[0] = L.sum_vols;
+  out[1] = L.sum_link_volume;
+  out[2] = L.sum_dual_areas;
+  out[3] = L.max_ell_diff;
+  // alpha_xy - (alpha_yx + pi + omega_xy) == 0 mod 2pi (dirac_dual.h:92-101)
+  double dev = 0;
+  for (int ix = 0; ix < L.n_sites; ix++)
+    for (int iy : L.nns[ix]) {
+      const double diff = (L.alpha.at({iy, ix}) + M_PI + L.omega.at({ix, iy})) - L.alpha.at({ix, iy});
+      dev = std::max(dev, std::abs(mod_round(diff, 2.0 * M_PI)));
+    }
+  out[4] = dev;
+  // deficits: sign-oriented sum of omega around each face mod 4pi in (-1.5*4pi/nf, 0)
+  // (test_sorting_simp.cc:397-425; the dual check of test_sorting.cc uses face.size()).
+  int bad = 0;
+  const bool simp = !L.faces.empty() && L.faces[0].size() == 3 && L.n_faces == 2 * (L.n_sites - 2);
+  for (int ia = 0; ia < L.n_faces; ia++) {
+    const auto& f = L.faces[ia];
+    int sign = 1;
+    if (simp) {
+      const V3 x0 = L.sites[f[0]], x1 = L.sites[f[1]], x2 = L.sites[f[2]];
+      const V3 sum = add(add(x0, x1), x2);
+      if (dot(cross(sub(x1, x0), sub(x2, x0)), sum) < 0) sign = -1;
+    } else {
+      const V3 x0 = L.dual_sites[ia], x1 = L.sites[f[0]], x2 = L.sites[f[1]];
+      if (dot(cross(sub(x1, x0), sub(x2, x1)), x0) < 0) sign = -1;
+    }
+    double sum = 0;
+    for (size_t i = 0; i < f.size(); i++) sum += L.omega.at({f[i], f[(i + 1) % f.size()]});
+    sum *= sign;
+    double mod = mod_floor(sum, 4.0 * M_PI);
+    if (mod > 2.0 * M_PI) mod -= 4.0 * M_PI;
+    const double lo = simp ? -1.5 * 4.0 * M_PI / L.n_faces : -1.5 * 4.0 * M_PI / f.size();
+    if (!(lo < mod && mod < 0.0)) bad++;
+  }
+  out[5] = bad;
+  out[6] = L.mean_ell;
+}
+
+void oracle_lattice_kappa(void* hh, double* kappa, double* ell, double* link_volume,
+                          double* dual_area) {
+  const Lattice& L = *static_cast<oracle_lat*>(hh)->L;
+  for (int l = 0; l < L.n_links; l++) {
+    if (kappa) kappa[l] = L.kappa[l];
+    if (ell) ell[l] = L.ell[l];
+    if (link_volume) link_volume[l] = L.link_volume[l];
+  }
+  if (dual_area)
+    for (int x = 0; x < L.n_sites; x++) dual_area[x] = L.dual_areas[x];
+}
+
+void* oracle_dirac_create(void* hl, int Nt, int bc, double r, double M5, double at) {
+  return make_dirac(static_cast<oracle_lat*>(hl)->L.get(), Nt, bc, r, M5, at).release();
+}
+void oracle_dirac_destroy(void* h) { delete static_cast<Dirac*>(h); }
+long oracle_dirac_nnz(void* h) { return long(static_cast<Dirac*>(h)->is.size()); }
+void oracle_dirac_update(void* h, const double* sp, const double* tp) {
+  static_cast<Dirac*>(h)->update(sp, tp);
+}
+// op: 0 D, 1 D^dag, 2 D^dag D ; in/out interleaved complex of length N
+void oracle_apply(void* h, int op, double* out, const double* in) {
+  const Dirac& D = *static_cast<Dirac*>(h);
+  auto* o = reinterpret_cast<Complex*>(out);
+  const auto* i = reinterpret_cast<const Complex*>(in);
+  if (op == 0) D.matmul(o, i, false);
+  if (op == 1) D.matmul(o, i, true);
+  if (op == 2) {
+    std::vector<Complex> t(D.N);
+    D.matmul(t.data(), i, false);
+    D.matmul(o, t.data(), true);
+  }
+}
+int oracle_cg(void* h, double* x, const double* b, double tol, int maxiter, double* mu) {
+  return cg(*static_cast<Dirac*>(h), reinterpret_cast<Complex*>(x),
+            reinterpret_cast<const Complex*>(b), tol, maxiter, mu);
+}
+// Dense D (N x N, row major, interleaved complex) for small lattices.
+void oracle_dense(void* h, double* out) {
+  const Dirac& D = *static_cast<Dirac*>(h);
+  auto* o = reinterpret_cast<Complex*>(out);
+  for (long i = 0; i < D.N * D.N; i++) o[i] = 0.0;
+  for (long i = 0; i < D.N; i++)
+    for (long jj = D.rows[i]; jj < D.rows[i + 1]; jj++) o[i * D.N + D.cols[jj]] += D.val[jj];
+}
+// Force for all links. full = 1: the per-link route over all N rows (O(N^2)); else local.
+void oracle_force(void* h, const double* eta_in, int full, double* fs, double* ft) {
+  const Dirac& D = *static_cast<Dirac*>(h);
+  const Lattice& L = *D.lat;
+  const auto* eta = reinterpret_cast<const Complex*>(eta_in);
+  std::vector<Complex> Deta(D.N);
+  D.matmul(Deta.data(), eta, false);  // precalc_grad (dirac_pf.h:95-103)
+  const int Nt = D.Nt, V = L.n_sites, E = L.n_links;
+#pragma omp parallel for schedule(static)
+  for (int s = 0; s < Nt; s++) {
+    std::vector<COOE> e;
+    std::vector<Complex> buf(full ? D.N : 0);
+    for (int il = 0; il < E; il++) {
+      d_coo_spatial(D, s, L.links[il][0], L.links[il][1], e);
+      fs[size_t(s) * E + il] = full ? grad_full(D, e, Deta.data(), eta, buf) : grad_local(e, Deta.data(), eta);
+    }
+    for (int ix = 0; ix < V; ix++) {
+      d_coo_temporal(D, s, ix, e);
+      ft[size_t(s) * V + ix] = full ? grad_full(D, e, Deta.data(), eta, buf) : grad_local(e, Deta.data(), eta);
+    }
+  }
+}
+
+void* oracle_gauge_action_create(void* hl, int Nt, double gsq, double at, int compact) {
+  auto* g = new GaugeAction;
+  g->L = static_cast<oracle_lat*>(hl)->L.get();
+  g->gsq = gsq;
+  g->at = at;
+  g->compact = compact != 0;
+  g->Nt = Nt;
+  g->set_beta();
+  return g;
+}
+void oracle_gauge_action_destroy(void* h) { delete static_cast<GaugeAction*>(h); }
+double oracle_gauge_action(void* h, const double* u, const double* t) {
+  return static_cast<GaugeAction*>(h)->S(u, t);
+}
+void oracle_gauge_force(void* h, const double* u, const double* t, double* fs, double* ft) {
+  static_cast<GaugeAction*>(h)->force(u, t, fs, ft);
+}
+
+// The k-th output of std::mt19937_64 seeded with `seed` (k >= 1).
+uint64_t oracle_mt64(uint64_t seed, long k) {
+  std::mt19937_64 m(seed);
+  uint64_t v = 0;
+  for (long i = 0; i < k; i++) v = m();
+  return v;
+}
+
+// One HMC2 trajectory (hmc.h:172-199) from gauge (u,t) in place; returns dH, writes
+// h0/h1, accept flag and the total number of CG solves.
+double oracle_hmc_trajectory(void* hd, void* hg, int seed, int npf, double tau, int nsteps,
+                             int nsteps_inner, double tol, double* u, double* t, double* h0h1,
+                             int* accept, long* ncg) {
+  Dirac* D = static_cast<Dirac*>(hd);
+  const Lattice& L = *D->lat;
+  ParallelRng rng(D->Nt, L.n_links, L.n_sites, seed);
+  HMCState st;
+  st.D = D;
+  st.Sg = static_cast<GaugeAction*>(hg);
+  st.rng = &rng;
+  st.npf = npf;
+  st.tol = tol;
+  st.phi.resize(npf);
+  st.eta.resize(npf);
+  const size_t nS = size_t(D->Nt) * L.n_links, nT = size_t(D->Nt) * L.n_sites;
+  st.u.assign(u, u + nS);
+  st.t.assign(t, t + nT);
+  st.pu.resize(nS);
+  st.pt.resize(nT);
+  // pi.gaussian(rng) (gauge_ext.h:146-149)
+  for (int s = 0; s < D->Nt; s++)
+    for (int il = 0; il < L.n_links; il++) st.pu[size_t(s) * L.n_links + il] = rng.gaussian_link(s, il);
+  for (int s = 0; s < D->Nt; s++)
+    for (int ix = 0; ix < L.n_sites; ix++) st.pt[size_t(s) * L.n_sites + ix] = rng.gaussian_site(s, ix);
+  D->update(st.u.data(), st.t.data());
+  for (int k = 0; k < npf; k++) st.pf_gen(k);
+  const double h0 = st.H();
+  st.integrate(tau, nsteps, nsteps_inner);
+  const double h1 = st.H();
+  const double dH = h1 - h0;
+  const double rr = std::min(1.0, std::exp(-dH));
+  const double a = rng.uniform();
+  *accept = (a < rr) ? 1 : 0;
+  if (*accept) {
+    std::copy(st.u.begin(), st.u.end(), u);
+    std::copy(st.t.begin(), st.t.end(), t);
+  }
+  h0h1[0] = h0;
+  h0h1[1] = h1;
+  *ncg = st.cg_iters;
+  return dH;
+}
+
+}  // extern "C"

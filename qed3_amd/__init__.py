@@ -1,0 +1,233 @@
+"""qed3_amd: MI355X-native fermion path of qed3 (Wilson-Dirac on the refined icosahedron,
+CG on D^dagger D, pseudofermion force) behind the C ABI of include/qed3.h.
+
+The Python layer is a thin host mirror of the reference's roles for tests and scripts:
+  Geometry       -> the .dat tables the reference reads (s2n_simp.h)
+  WilsonDirac    -> DiracExt + DiracPf (update / mult / adj / DHD / force)
+  WilsonDirac.solve -> MatPoly::solve on D^dag D
+All compute goes through libqed3.so; there is no fallback path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import CGOpts, CGReport, Params, Qed3Error, check, load
+
+SIMPLICIAL, DUAL = 0, 1
+ANTIPERIODIC, PERIODIC = 0, 1
+OP_D, OP_DDAG, OP_DDAGD, OP_DDDAG = 0, 1, 2, 3
+F64, F32 = 0, 1
+CG_F64, CG_MIXED = 0, 1
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Geometry:
+    """Refined icosahedron tables (sites, links, faces, dual, spin structure)."""
+
+    def __init__(self, n_refine: int):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.qed3_geom_create(n_refine, C.byref(h)))
+        try:
+            V, E, F = C.c_int(), C.c_int(), C.c_int()
+            check(lib.qed3_geom_counts(h, C.byref(V), C.byref(E), C.byref(F)))
+            V, E, F = V.value, E.value, F.value
+            self.n, self.V, self.E, self.F = n_refine, V, E, F
+            self.sites = np.zeros((V, 3))
+            self.faces = np.zeros((F, 3), np.int32)
+            self.links = np.zeros((E, 2), np.int32)
+            self.nn_offsets = np.zeros(V + 1, np.int32)
+            self.nns = np.zeros(2 * E, np.int32)
+            self.dual_sites = np.zeros((F, 3))
+            self.dual_links = np.zeros((E, 2), np.int32)
+            self.dual_nns = np.zeros((F, 3), np.int32)
+            self.dual_face_offsets = np.zeros(V + 1, np.int32)
+            self.dual_faces = np.zeros(3 * F, np.int32)
+            self.alpha = np.zeros((E, 2))
+            self.omega = np.zeros(E)
+            self.dual_alpha = np.zeros((E, 2))
+            self.dual_omega = np.zeros(E)
+            check(lib.qed3_geom_export(
+                h, _dp(self.sites), _ip(self.faces), _ip(self.links), _ip(self.nn_offsets),
+                _ip(self.nns), _dp(self.dual_sites), _ip(self.dual_links), _ip(self.dual_nns),
+                _ip(self.dual_face_offsets), _ip(self.dual_faces), _dp(self.alpha),
+                _dp(self.omega), _dp(self.dual_alpha), _dp(self.dual_omega)))
+        finally:
+            lib.qed3_geom_destroy(h)
+
+    def neighbours(self, x):
+        return self.nns[self.nn_offsets[x]:self.nn_offsets[x + 1]]
+
+    def dual_ring(self, x):
+        return self.dual_faces[self.dual_face_offsets[x]:self.dual_face_offsets[x + 1]]
+
+
+def lattice_sizes(lattice: int, n: int):
+    """(V, E, F) of the simplicial or dual lattice at refinement n."""
+    if lattice == SIMPLICIAL:
+        return 10 * n * n + 2, 30 * n * n, 20 * n * n
+    return 20 * n * n, 30 * n * n, 10 * n * n + 2
+
+
+def random_gauge(lattice: int, n: int, Lt: int, seed: int, kappa: float):
+    """Seeded angles (spatial[Lt][E], temporal[Lt][V]); kappa<=0: uniform [-pi,pi)."""
+    V, E, _ = lattice_sizes(lattice, n)
+    sp = np.zeros((Lt, E))
+    tp = np.zeros((Lt, V))
+    check(load().qed3_random_gauge(Lt, E, V, seed, float(kappa), _dp(sp), _dp(tp)))
+    return sp, tp
+
+
+def random_source(lattice: int, n: int, Lt: int, seed: int):
+    """Seeded complex Gaussian field of 2*V*Lt components (reference layout)."""
+    V, E, _ = lattice_sizes(lattice, n)
+    xi = np.zeros(2 * V * Lt, np.complex128)
+    check(load().qed3_random_source(Lt, E, V, seed, _dp(xi.view(np.float64))))
+    return xi
+
+
+class DeviceArray:
+    """Raw device allocation owned by Python (hipMalloc through the C ABI)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        check(load().qed3_malloc(C.byref(self.ptr), self.nbytes))
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        check(load().qed3_memcpy(d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 0))
+        return d
+
+    def to_numpy(self, dtype, count=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize if count is None else count, dtype)
+        check(load().qed3_memcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, 1))
+        return out
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(load().qed3_memcpy(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 0))
+
+    def free(self):
+        if self.ptr:
+            load().qed3_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class WilsonDirac:
+    """Wilson-Dirac operator on S^2 x time (DiracExt + DiracPf roles) on one GPU."""
+
+    def __init__(self, lattice: int, n_refine: int, Lt: int, r: float = 1.0, M5: float = 0.0,
+                 at: float = None, bc: int = ANTIPERIODIC):
+        lib = load()
+        V, E, F = lattice_sizes(lattice, n_refine)
+        if at is None:
+            at = 1.0
+        self.params = Params(lattice, n_refine, Lt, bc, r, M5, at)
+        self._h = C.c_void_p()
+        check(lib.qed3_ctx_create(C.byref(self.params), C.byref(self._h)))
+        Vc, Ec, Fc, ml = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib.qed3_ctx_sizes(self._h, C.byref(Vc), C.byref(Ec), C.byref(Fc), C.byref(ml)))
+        self.V, self.E, self.F, self.mean_ell = Vc.value, Ec.value, Fc.value, ml.value
+        self.Lt, self.lattice, self.n = Lt, lattice, n_refine
+        self.N = 2 * self.V * Lt  # complex components
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().qed3_ctx_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def geometry(self):
+        k, kt = np.zeros(self.E), np.zeros(self.V)
+        lv, ell, da = np.zeros(self.E), np.zeros(self.E), np.zeros(self.V)
+        check(load().qed3_ctx_geometry(self._h, _dp(k), _dp(kt), _dp(lv), _dp(ell), _dp(da)))
+        return dict(kappa=k, kappa_t=kt, link_volume=lv, ell=ell, dual_area=da)
+
+    def update(self, spatial, temporal):
+        sp = np.ascontiguousarray(spatial, np.float64)
+        tp = np.ascontiguousarray(temporal, np.float64)
+        assert sp.size == self.Lt * self.E and tp.size == self.Lt * self.V
+        check(load().qed3_set_gauge(self._h, sp.ctypes.data_as(C.c_void_p),
+                                    tp.ctypes.data_as(C.c_void_p), 0))
+
+    # device-pointer level
+    def apply_device(self, op, prec, d_out, d_in):
+        check(load().qed3_apply(self._h, op, prec, d_out.ptr, d_in.ptr))
+
+    def apply(self, op, v: np.ndarray, prec=F64):
+        """op(v) for a host complex vector (round trip through the device)."""
+        dt = np.complex128 if prec == F64 else np.complex64
+        v = np.ascontiguousarray(v, dt)
+        assert v.size == self.N
+        d_in = DeviceArray.from_numpy(v)
+        d_out = DeviceArray(v.nbytes)
+        self.apply_device(op, prec, d_out, d_in)
+        return d_out.to_numpy(dt)
+
+    def mult(self, v, prec=F64):
+        return self.apply(OP_D, v, prec)
+
+    def adj(self, v, prec=F64):
+        return self.apply(OP_DDAG, v, prec)
+
+    def DHD(self, v, prec=F64):
+        return self.apply(OP_DDAGD, v, prec)
+
+    def solve_device(self, d_x, d_b, tol=1e-10, maxiter=100000, solver=CG_F64, delta=0.1,
+                     check_every=0, raise_on_fail=True):
+        opts = CGOpts(tol, maxiter, solver, delta, check_every)
+        rep = CGReport()
+        code = load().qed3_cg(self._h, d_x.ptr, d_b.ptr, C.byref(opts), C.byref(rep))
+        if code != 0 and raise_on_fail:
+            check(code)
+        return rep
+
+    def solve(self, b: np.ndarray, **kw):
+        """x with D^dag D x = b (host vectors); returns (x, report)."""
+        b = np.ascontiguousarray(b, np.complex128)
+        d_b = DeviceArray.from_numpy(b)
+        d_x = DeviceArray(b.nbytes)
+        rep = self.solve_device(d_x, d_b, **kw)
+        return d_x.to_numpy(np.complex128), rep
+
+    def dot(self, d_x, d_y):
+        re, im = C.c_double(), C.c_double()
+        check(load().qed3_dot(self._h, d_x.ptr, d_y.ptr, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def force(self, eta: np.ndarray):
+        """Pseudofermion force (spatial[Lt][E], temporal[Lt][V]) from eta (host)."""
+        d_eta = DeviceArray.from_numpy(np.ascontiguousarray(eta, np.complex128))
+        d_fs = DeviceArray(8 * self.Lt * self.E)
+        d_ft = DeviceArray(8 * self.Lt * self.V)
+        check(load().qed3_pf_force(self._h, d_eta.ptr, d_fs.ptr, d_ft.ptr))
+        return (d_fs.to_numpy(np.float64).reshape(self.Lt, self.E),
+                d_ft.to_numpy(np.float64).reshape(self.Lt, self.V))
+
+    def autotune(self):
+        check(load().qed3_autotune(self._h))
+
+    def time_apply(self, op=OP_D, prec=F64, warmup=3, reps=20):
+        ms = C.c_double()
+        check(load().qed3_time_apply(self._h, op, prec, warmup, reps, C.byref(ms)))
+        return ms.value

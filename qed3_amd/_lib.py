@@ -1,0 +1,88 @@
+"""ctypes binding of libqed3.so (include/qed3.h)."""
+import ctypes as C
+import os
+import pathlib
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libqed3.so"
+
+c_int_p = C.POINTER(C.c_int)
+c_double_p = C.POINTER(C.c_double)
+
+
+class Params(C.Structure):
+    _fields_ = [("lattice", C.c_int), ("n_refine", C.c_int), ("Lt", C.c_int),
+                ("bc_time", C.c_int), ("r", C.c_double), ("M5", C.c_double),
+                ("at", C.c_double)]
+
+
+class CGOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("solver", C.c_int),
+                ("delta", C.c_double), ("check_every", C.c_int)]
+
+
+class CGReport(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("reliable_updates", C.c_int),
+                ("true_resid", C.c_double), ("seconds", C.c_double)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/qed3.h
+SIGNATURES = {
+    "qed3_last_error": (C.c_char_p, []),
+    "qed3_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "qed3_free": (C.c_int, [C.c_void_p]),
+    "qed3_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "qed3_device_synchronize": (C.c_int, []),
+    "qed3_geom_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "qed3_geom_destroy": (None, [C.c_void_p]),
+    "qed3_geom_counts": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p]),
+    "qed3_geom_export": (C.c_int, [C.c_void_p, c_double_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                   c_double_p, c_int_p, c_int_p, c_int_p, c_int_p, c_double_p,
+                                   c_double_p, c_double_p, c_double_p]),
+    "qed3_random_gauge": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, c_double_p,
+                                    c_double_p]),
+    "qed3_random_source": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]),
+    "qed3_ctx_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "qed3_ctx_destroy": (None, [C.c_void_p]),
+    "qed3_ctx_sizes": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p, c_double_p]),
+    "qed3_ctx_geometry": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                    c_double_p]),
+    "qed3_set_gauge": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "qed3_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "qed3_cg": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
+                          C.POINTER(CGReport)]),
+    "qed3_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
+    "qed3_pf_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qed3_autotune": (C.c_int, [C.c_void_p]),
+    "qed3_time_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libqed3.so; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m qed3_amd.build`")
+    lib = C.CDLL(str(LIB_PATH), mode=C.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class Qed3Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qed3 error {code}: {msg}")
+        self.code = code
+
+
+def check(code):
+    if code != 0:
+        raise Qed3Error(code, load().qed3_last_error().decode())
+    return code

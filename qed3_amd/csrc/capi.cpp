@@ -1,0 +1,346 @@
+// C ABI of libqed3 (declared in include/qed3.h).  Every entry point converts C++
+// exceptions into status codes; the text of the last error is kept per thread.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/qed3.h"
+#include "cg.hpp"
+#include "dirac.hpp"
+#include "geometry.hpp"
+#include "rng.hpp"
+
+struct qed3_geom {
+  qed3::Geometry g;
+};
+
+struct qed3_ctx {
+  qed3_params p;
+  std::unique_ptr<qed3::Geometry> g;
+  std::unique_ptr<qed3::Dirac> D;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const qed3::HipError& e) {
+    return fail(QED3_ERR_HIP, e.what());
+  } catch (const std::invalid_argument& e) {
+    return fail(QED3_ERR_ARG, e.what());
+  } catch (const std::bad_alloc& e) {
+    return fail(QED3_ERR_HIP, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(QED3_ERR_ARG, e.what());
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* qed3_last_error(void) { return g_err.c_str(); }
+
+int qed3_malloc(void** d_ptr, size_t bytes) {
+  if (!d_ptr) return fail(QED3_ERR_ARG, "null pointer");
+  return guarded([&]() -> int {
+    QED3_HIP(hipMalloc(d_ptr, bytes));
+    return QED3_OK;
+  });
+}
+
+int qed3_free(void* d_ptr) {
+  return guarded([&]() -> int {
+    QED3_HIP(hipFree(d_ptr));
+    return QED3_OK;
+  });
+}
+
+int qed3_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  static const hipMemcpyKind kinds[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
+                                         hipMemcpyDeviceToDevice};
+  if (kind < 0 || kind > 2) return fail(QED3_ERR_ARG, "bad memcpy kind");
+  return guarded([&]() -> int {
+    QED3_HIP(hipMemcpy(dst, src, bytes, kinds[kind]));
+    return QED3_OK;
+  });
+}
+
+int qed3_device_synchronize(void) {
+  return guarded([&]() -> int {
+    QED3_HIP(hipDeviceSynchronize());
+    return QED3_OK;
+  });
+}
+
+int qed3_geom_create(int n_refine, qed3_geom** out) {
+  if (!out) return fail(QED3_ERR_ARG, "null out");
+  return guarded([&]() -> int {
+    auto* g = new qed3_geom;
+    try {
+      g->g = qed3::make_geometry(n_refine);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+    return QED3_OK;
+  });
+}
+
+void qed3_geom_destroy(qed3_geom* g) { delete g; }
+
+int qed3_geom_counts(const qed3_geom* g, int* V, int* E, int* F) {
+  if (!g) return fail(QED3_ERR_ARG, "null geometry");
+  if (V) *V = int(g->g.sites.size());
+  if (E) *E = int(g->g.links.size());
+  if (F) *F = int(g->g.faces.size());
+  return QED3_OK;
+}
+
+int qed3_geom_export(const qed3_geom* gg, double* sites, int* faces, int* links, int* nn_offsets,
+                     int* nns, double* dual_sites, int* dual_links, int* dual_nns,
+                     int* dual_face_offsets, int* dual_faces, double* alpha, double* omega,
+                     double* dual_alpha, double* dual_omega) {
+  if (!gg) return fail(QED3_ERR_ARG, "null geometry");
+  const qed3::Geometry& g = gg->g;
+  const size_t V = g.sites.size(), E = g.links.size(), F = g.faces.size();
+  if (sites)
+    for (size_t i = 0; i < V; i++) {
+      sites[3 * i] = g.sites[i].x;
+      sites[3 * i + 1] = g.sites[i].y;
+      sites[3 * i + 2] = g.sites[i].z;
+    }
+  if (faces)
+    for (size_t f = 0; f < F; f++)
+      for (int k = 0; k < 3; k++) faces[3 * f + k] = g.faces[f][k];
+  if (links)
+    for (size_t l = 0; l < E; l++) {
+      links[2 * l] = g.links[l][0];
+      links[2 * l + 1] = g.links[l][1];
+    }
+  if (nn_offsets || nns) {
+    int o = 0;
+    for (size_t x = 0; x < V; x++) {
+      if (nn_offsets) nn_offsets[x] = o;
+      for (int y : g.nns[x]) {
+        if (nns) nns[o] = y;
+        o++;
+      }
+    }
+    if (nn_offsets) nn_offsets[V] = o;
+  }
+  if (dual_sites)
+    for (size_t f = 0; f < F; f++) {
+      dual_sites[3 * f] = g.dual_sites[f].x;
+      dual_sites[3 * f + 1] = g.dual_sites[f].y;
+      dual_sites[3 * f + 2] = g.dual_sites[f].z;
+    }
+  if (dual_links)
+    for (size_t l = 0; l < E; l++) {
+      dual_links[2 * l] = g.dual_links[l][0];
+      dual_links[2 * l + 1] = g.dual_links[l][1];
+    }
+  if (dual_nns)
+    for (size_t f = 0; f < F; f++)
+      for (int k = 0; k < 3; k++) dual_nns[3 * f + k] = g.dual_nns[f][k];
+  if (dual_face_offsets || dual_faces) {
+    int o = 0;
+    for (size_t x = 0; x < V; x++) {
+      if (dual_face_offsets) dual_face_offsets[x] = o;
+      for (int f : g.dual_faces[x]) {
+        if (dual_faces) dual_faces[o] = f;
+        o++;
+      }
+    }
+    if (dual_face_offsets) dual_face_offsets[V] = o;
+  }
+  for (size_t l = 0; l < E; l++) {
+    if (alpha) {
+      alpha[2 * l] = g.alpha_fwd[l];
+      alpha[2 * l + 1] = g.alpha_bwd[l];
+    }
+    if (omega) omega[l] = g.omega[l];
+    if (dual_alpha) {
+      dual_alpha[2 * l] = g.dual_alpha_fwd[l];
+      dual_alpha[2 * l + 1] = g.dual_alpha_bwd[l];
+    }
+    if (dual_omega) dual_omega[l] = g.dual_omega[l];
+  }
+  return QED3_OK;
+}
+
+int qed3_random_gauge(int Lt, int E, int V, int seed, double kappa, double* spatial,
+                      double* temporal) {
+  if (Lt < 1 || E < 1 || V < 1 || !spatial || !temporal) return fail(QED3_ERR_ARG, "bad argument");
+  return guarded([&]() -> int {
+    qed3::draw_gauge(Lt, E, V, seed, kappa, spatial, temporal);
+    return QED3_OK;
+  });
+}
+
+int qed3_random_source(int Lt, int E, int V, int seed, double* xi) {
+  if (Lt < 1 || E < 1 || V < 1 || !xi) return fail(QED3_ERR_ARG, "bad argument");
+  return guarded([&]() -> int {
+    qed3::draw_source(Lt, E, V, seed, xi);
+    return QED3_OK;
+  });
+}
+
+int qed3_ctx_create(const qed3_params* p, qed3_ctx** out) {
+  if (!p || !out) return fail(QED3_ERR_ARG, "null argument");
+  if (p->lattice != 0 && p->lattice != 1) return fail(QED3_ERR_ARG, "lattice must be 0 or 1");
+  if (p->Lt < 1 || p->n_refine < 1) return fail(QED3_ERR_ARG, "Lt and n_refine must be >= 1");
+  if (p->bc_time != 0 && p->bc_time != 1) return fail(QED3_ERR_ARG, "bc_time must be 0 or 1");
+  if (!(p->at > 0)) return fail(QED3_ERR_ARG, "at must be positive");
+  return guarded([&]() -> int {
+    auto c = std::make_unique<qed3_ctx>();
+    c->p = *p;
+    c->g = std::make_unique<qed3::Geometry>(qed3::make_geometry(p->n_refine));
+    c->D = std::make_unique<qed3::Dirac>(*c->g, p->lattice, p->Lt, p->bc_time, p->r, p->M5, p->at);
+    *out = c.release();
+    return QED3_OK;
+  });
+}
+
+void qed3_ctx_destroy(qed3_ctx* ctx) { delete ctx; }
+
+int qed3_ctx_sizes(const qed3_ctx* ctx, int* V, int* E, int* F, double* mean_ell) {
+  if (!ctx) return fail(QED3_ERR_ARG, "null context");
+  const auto& h = ctx->D->tables();
+  if (V) *V = h.V;
+  if (E) *E = h.E;
+  if (F) *F = h.F;
+  if (mean_ell) *mean_ell = h.mean_ell;
+  return QED3_OK;
+}
+
+int qed3_ctx_geometry(const qed3_ctx* ctx, double* kappa, double* kappa_t, double* link_volume,
+                      double* ell, double* dual_area) {
+  if (!ctx) return fail(QED3_ERR_ARG, "null context");
+  const auto& h = ctx->D->tables();
+  if (kappa) std::memcpy(kappa, h.kappa.data(), h.E * sizeof(double));
+  if (kappa_t) std::memcpy(kappa_t, h.kappa_t.data(), h.V * sizeof(double));
+  if (link_volume) std::memcpy(link_volume, h.link_volume.data(), h.E * sizeof(double));
+  if (ell) std::memcpy(ell, h.ell.data(), h.E * sizeof(double));
+  if (dual_area) std::memcpy(dual_area, h.dual_area.data(), h.V * sizeof(double));
+  return QED3_OK;
+}
+
+int qed3_set_gauge(qed3_ctx* ctx, const double* spatial, const double* temporal, int where) {
+  if (!ctx || !spatial || !temporal) return fail(QED3_ERR_ARG, "null argument");
+  if (where != QED3_HOST && where != QED3_DEVICE) return fail(QED3_ERR_ARG, "bad where");
+  return guarded([&]() -> int {
+    ctx->D->set_gauge(spatial, temporal, where == QED3_DEVICE);
+    return QED3_OK;
+  });
+}
+
+int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
+  if (!ctx || !d_out || !d_in) return fail(QED3_ERR_ARG, "null argument");
+  if (op < 0 || op > 3) return fail(QED3_ERR_ARG, "bad op");
+  if (prec != QED3_PREC_F64 && prec != QED3_PREC_F32) return fail(QED3_ERR_ARG, "bad precision");
+  if (d_out == d_in) return fail(QED3_ERR_ARG, "in-place apply is not supported");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    auto& D = *ctx->D;
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      auto* o = static_cast<qed3::Spin<T>*>(d_out);
+      auto* i = static_cast<const qed3::Spin<T>*>(d_in);
+      if (op == QED3_OP_D || op == QED3_OP_DDAG) {
+        D.apply<T>(op == QED3_OP_DDAG, o, i, true);
+        return;
+      }
+      qed3::Spin<T>* tmp = nullptr;
+      QED3_HIP(hipMalloc(&tmp, D.points() * sizeof(qed3::Spin<T>)));
+      if (op == QED3_OP_DDAGD)
+        D.apply_dagd<T>(o, i, tmp, true);
+      else
+        D.apply_ddag<T>(o, i, tmp, true);
+      QED3_HIP(hipFree(tmp));
+    };
+    if (prec == QED3_PREC_F64)
+      go(double{});
+    else
+      go(float{});
+    return QED3_OK;
+  });
+}
+
+int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+            qed3_cg_report* rep) {
+  if (!ctx || !d_x || !d_b || !opts) return fail(QED3_ERR_ARG, "null argument");
+  if (d_x == d_b) return fail(QED3_ERR_ARG, "x and b must differ");
+  if (!(opts->tol > 0) || opts->maxiter < 1) return fail(QED3_ERR_ARG, "bad tolerance/maxiter");
+  if (opts->solver != QED3_SOLVER_CG_F64 && opts->solver != QED3_SOLVER_CG_MIXED)
+    return fail(QED3_ERR_ARG, "bad solver");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    qed3::CGOptions o;
+    o.tol = opts->tol;
+    o.maxiter = opts->maxiter;
+    o.mixed = opts->solver == QED3_SOLVER_CG_MIXED;
+    o.delta = opts->delta > 0 ? opts->delta : 0.1;
+    o.chunk = opts->check_every;
+    const qed3::CGResult r = qed3::cg_solve(*ctx->D, static_cast<qed3::Spin<double>*>(d_x),
+                                            static_cast<const qed3::Spin<double>*>(d_b), o);
+    if (rep) {
+      rep->status = r.status == 0 ? QED3_OK : (r.status == 4 ? QED3_ERR_NAN : QED3_ERR_NOCONV);
+      rep->iterations = r.iterations;
+      rep->reliable_updates = r.reliable_updates;
+      rep->true_resid = r.true_resid;
+      rep->seconds = r.seconds;
+    }
+    if (r.status == 4) return fail(QED3_ERR_NAN, "CG breakdown (non-finite or zero curvature)");
+    if (r.status != 0) return fail(QED3_ERR_NOCONV, "CG did not reach the tolerance");
+    return QED3_OK;
+  });
+}
+
+int qed3_dot(qed3_ctx* ctx, const void* d_x, const void* d_y, double* re, double* im) {
+  if (!ctx || !d_x || !d_y) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    ctx->D->dot(static_cast<const qed3::Spin<double>*>(d_x),
+                static_cast<const qed3::Spin<double>*>(d_y), re, im);
+    return QED3_OK;
+  });
+}
+
+int qed3_pf_force(qed3_ctx* ctx, const void* d_eta, double* d_fs, double* d_ft) {
+  if (!ctx || !d_eta || !d_fs || !d_ft) return fail(QED3_ERR_ARG, "null argument");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    ctx->D->force(static_cast<const qed3::Spin<double>*>(d_eta), d_fs, d_ft);
+    return QED3_OK;
+  });
+}
+
+int qed3_autotune(qed3_ctx* ctx) {
+  if (!ctx) return fail(QED3_ERR_ARG, "null context");
+  return guarded([&]() -> int {
+    ctx->D->autotune();
+    return QED3_OK;
+  });
+}
+
+int qed3_time_apply(qed3_ctx* ctx, int op, int prec, int warmup, int reps, double* ms) {
+  if (!ctx || !ms) return fail(QED3_ERR_ARG, "null argument");
+  if (op != QED3_OP_D && op != QED3_OP_DDAG) return fail(QED3_ERR_ARG, "op must be D or DDAG");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    *ms = ctx->D->time_apply(op == QED3_OP_DDAG, prec == QED3_PREC_F32, warmup, reps);
+    return QED3_OK;
+  });
+}
+
+}  // extern "C"

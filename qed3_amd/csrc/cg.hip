@@ -1,0 +1,356 @@
+// Device-driven FP64 and mixed FP32/FP64 CG on D^dagger D (see cg.hpp).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "cg.hpp"
+#include "stencil.hpp"
+
+namespace qed3 {
+
+namespace {
+
+__device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < 1e300; }
+
+// K1: w = D p, gam = |w|^2, alpha = mu / gam
+template <typename T>
+struct EpiK1 {
+  static constexpr bool kReduce = true;
+  Spin<T>* w;
+  CGScalars* sc;
+  __device__ bool skip() const { return sc->done != 0; }
+  __device__ void begin() {}
+  __device__ void operator()(size_t i, const Spin<T>& v, double2& acc) const {
+    st(w + i, v);
+    acc.x += norm2(v);
+  }
+  __device__ void operator()(double2 t) const {
+    sc->gam = t.x;
+    if (!finite_pos(t.x)) {
+      sc->done = 2;
+      return;
+    }
+    sc->alpha = sc->mu / t.x;
+  }
+};
+
+// K2: q = D^dag w; x += alpha p; r -= alpha q; |r|^2 and the reliable-update decision
+template <typename T>
+struct EpiK2 {
+  static constexpr bool kReduce = true;
+  Spin<T>* x;
+  Spin<T>* r;
+  const Spin<T>* p;
+  CGScalars* sc;
+  T alpha;
+  __device__ bool skip() const { return sc->done != 0; }
+  __device__ void begin() { alpha = T(sc->alpha); }
+  __device__ void operator()(size_t i, const Spin<T>& q, double2& acc) const {
+    const Spin<T> pp = ld(p + i);
+    Spin<T> xx = ld(x + i), rr = ld(r + i);
+    xx.a.x += alpha * pp.a.x;
+    xx.a.y += alpha * pp.a.y;
+    xx.b.x += alpha * pp.b.x;
+    xx.b.y += alpha * pp.b.y;
+    rr.a.x -= alpha * q.a.x;
+    rr.a.y -= alpha * q.a.y;
+    rr.b.x -= alpha * q.b.x;
+    rr.b.y -= alpha * q.b.y;
+    st(x + i, xx);
+    st(r + i, rr);
+    acc.x += norm2(rr);
+  }
+  __device__ void operator()(double2 t) const {
+    const double m = t.x;
+    sc->iter += 1;
+    if (!(m >= 0.0 && m < 1e300)) {
+      sc->done = 2;
+      return;
+    }
+    if (m < sc->delta2 * sc->maxr2 || m <= sc->tol2 * sc->bnorm2) {
+      sc->need_ru = 1;
+    } else {
+      sc->beta = m / sc->mu;
+      sc->mu = m;
+      sc->maxr2 = fmax(sc->maxr2, m);
+    }
+    if (sc->iter >= sc->maxiter && !sc->need_ru) sc->done = 3;
+  }
+};
+
+// U2: t = D x64 (only when an update is due)
+struct EpiStoreIfRU {
+  static constexpr bool kReduce = false;
+  Spin<double>* out;
+  const CGScalars* sc;
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru == 0; }
+  __device__ void begin() {}
+  __device__ void operator()(size_t i, const Spin<double>& v, double2&) const { st(out + i, v); }
+  __device__ void operator()(double2) const {}
+};
+
+// U3: r64 = b - D^dag t; r_inner = r64; |r64|^2 decides convergence and beta
+template <typename T>
+struct EpiResid {
+  static constexpr bool kReduce = true;
+  const Spin<double>* b;
+  Spin<T>* r;
+  CGScalars* sc;
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru == 0; }
+  __device__ void begin() {}
+  __device__ void operator()(size_t i, const Spin<double>& v, double2& acc) const {
+    const Spin<double> bb = ld(b + i);
+    Spin<double> rr;
+    rr.a = {bb.a.x - v.a.x, bb.a.y - v.a.y};
+    rr.b = {bb.b.x - v.b.x, bb.b.y - v.b.y};
+    Spin<T> ri;
+    ri.a = {T(rr.a.x), T(rr.a.y)};
+    ri.b = {T(rr.b.x), T(rr.b.y)};
+    st(r + i, ri);
+    acc.x += norm2(rr);
+  }
+  __device__ void operator()(double2 t) const {
+    const double m = t.x;
+    sc->need_ru = 0;
+    sc->n_ru += 1;
+    sc->true_r2 = m;
+    if (!(m >= 0.0 && m < 1e300)) {
+      sc->done = 2;
+      return;
+    }
+    if (m <= sc->tol2 * sc->bnorm2) {
+      sc->done = 1;
+      return;
+    }
+    sc->beta = m / sc->mu;
+    sc->mu = m;
+    sc->maxr2 = m;
+    if (sc->iter >= sc->maxiter) sc->done = 3;
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_init(const Spin<double>* __restrict__ b, Spin<double>* x64,
+                                              Spin<T>* xin, Spin<T>* r, Spin<T>* p, size_t n,
+                                              CGScalars* sc, GridReduce gr, double tol2,
+                                              double delta2, int maxiter) {
+  double2 acc = {0.0, 0.0};
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<double> bb = b[i];
+    Spin<T> t;
+    t.a = {T(bb.a.x), T(bb.a.y)};
+    t.b = {T(bb.b.x), T(bb.b.y)};
+    r[i] = t;
+    p[i] = t;
+    Spin<double> z;
+    z.a = {0.0, 0.0};
+    z.b = {0.0, 0.0};
+    x64[i] = z;
+    if ((void*)xin != (void*)x64) {
+      Spin<T> zt;
+      zt.a = {T(0), T(0)};
+      zt.b = {T(0), T(0)};
+      xin[i] = zt;
+    }
+    acc.x += norm2(bb);
+  }
+  grid_reduce(acc, gr, [=](double2 t) {
+    sc->mu = t.x;
+    sc->bnorm2 = t.x;
+    sc->maxr2 = t.x;
+    sc->true_r2 = t.x;
+    sc->gam = 0;
+    sc->alpha = 0;
+    sc->beta = 0;
+    sc->tol2 = tol2;
+    sc->delta2 = delta2;
+    sc->iter = 0;
+    sc->n_ru = 0;
+    sc->maxiter = maxiter;
+    sc->need_ru = 0;
+    sc->done = (t.x == 0.0) ? 1 : (t.x < 1e300 ? 0 : 2);
+  });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_flush(Spin<double>* x64, Spin<T>* xin, size_t n,
+                                               const CGScalars* sc, int always) {
+  if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<T> a = xin[i];
+    Spin<double> xx = x64[i];
+    xx.a.x += a.a.x;
+    xx.a.y += a.a.y;
+    xx.b.x += a.b.x;
+    xx.b.y += a.b.y;
+    x64[i] = xx;
+    Spin<T> z;
+    z.a = {T(0), T(0)};
+    z.b = {T(0), T(0)};
+    xin[i] = z;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pupdate(Spin<T>* p, const Spin<T>* __restrict__ r,
+                                                 size_t n, const CGScalars* sc) {
+  if (sc->done != 0) return;
+  const T beta = T(sc->beta);
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<T> rr = r[i];
+    Spin<T> pp = p[i];
+    pp.a.x = rr.a.x + beta * pp.a.x;
+    pp.a.y = rr.a.y + beta * pp.a.y;
+    pp.b.x = rr.b.x + beta * pp.b.x;
+    pp.b.y = rr.b.y + beta * pp.b.y;
+    p[i] = pp;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_resid_norms(const Spin<double>* __restrict__ b,
+                                                     const Spin<double>* __restrict__ ax, size_t n,
+                                                     GridReduce gr, double2* out) {
+  double2 acc = {0.0, 0.0};
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<double> bb = b[i], a = ax[i];
+    const double d0 = bb.a.x - a.a.x, d1 = bb.a.y - a.a.y, d2 = bb.b.x - a.b.x, d3 = bb.b.y - a.b.y;
+    acc.x += d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
+    acc.y += norm2(bb);
+  }
+  grid_reduce(acc, gr, [out](double2 t) { *out = t; });
+}
+
+template <typename T>
+struct Work {
+  Spin<T>* r = nullptr;
+  Spin<T>* p = nullptr;
+  Spin<T>* w = nullptr;
+  Spin<T>* xin = nullptr;
+  Spin<double>* t = nullptr;
+};
+
+size_t stream_blocks(size_t n) { return std::min<size_t>((n + 255) / 256, 8192); }
+
+template <typename T>
+CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
+  constexpr bool kMixed = std::is_same<T, float>::value;
+  const size_t n = D.points();
+  hipStream_t st = D.stream();
+  auto t0 = std::chrono::steady_clock::now();
+  Work<T> W;
+  QED3_HIP(hipMallocAsync((void**)&W.r, n * sizeof(Spin<T>), st));
+  QED3_HIP(hipMallocAsync((void**)&W.p, n * sizeof(Spin<T>), st));
+  QED3_HIP(hipMallocAsync((void**)&W.w, n * sizeof(Spin<T>), st));
+  if (kMixed) {
+    QED3_HIP(hipMallocAsync((void**)&W.xin, n * sizeof(Spin<T>), st));
+    QED3_HIP(hipMallocAsync((void**)&W.t, n * sizeof(Spin<double>), st));
+  } else {
+    W.xin = reinterpret_cast<Spin<T>*>(x);
+    W.t = reinterpret_cast<Spin<double>*>(W.w);
+  }
+  CGScalars* sc = nullptr;
+  QED3_HIP(hipMallocAsync((void**)&sc, sizeof(CGScalars), st));
+  CGScalars* hsc = nullptr;
+  QED3_HIP(hipHostMalloc((void**)&hsc, sizeof(CGScalars), hipHostMallocDefault));
+
+  const GridReduce gr = D.reducer();
+  const int nbk = int(stream_blocks(n));
+  const double tol2 = o.tol * o.tol;
+  const double delta2 = kMixed ? o.delta * o.delta : 0.0;
+  hipLaunchKernelGGL(k_init<T>, dim3(nbk), dim3(256), 0, st, b, x, W.xin, W.r, W.p, n, sc, gr,
+                     tol2, delta2, o.maxiter);
+  QED3_HIP(hipGetLastError());
+
+  const DiracView<T> vin = D.view<T>();
+  const DiracView<double> v64 = D.view<double>();
+  const int nn = D.tables().max_nn;
+  auto one_iter = [&]() {
+    launch_stencil<T, false>(vin, nn, W.p, EpiK1<T>{W.w, sc}, D.tile<T>(false), gr, st);
+    launch_stencil<T, true>(vin, nn, W.w, EpiK2<T>{W.xin, W.r, W.p, sc, T(0)}, D.tile<T>(true), gr, st);
+    if (kMixed)
+      hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 0);
+    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{W.t, sc}, D.tile<double>(false), gr, st);
+    launch_stencil<double, true>(v64, nn, W.t, EpiResid<T>{b, W.r, sc}, D.tile<double>(true), gr, st);
+    hipLaunchKernelGGL(k_pupdate<T>, dim3(nbk), dim3(256), 0, st, W.p, W.r, n, sc);
+  };
+
+  int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (o.use_graph) {
+    QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk; i++) one_iter();
+    QED3_HIP(hipStreamEndCapture(st, &graph));
+    QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  const int max_chunks = o.maxiter / chunk + 4;
+  for (int c = 0; c < max_chunks; c++) {
+    if (exec)
+      QED3_HIP(hipGraphLaunch(exec, st));
+    else
+      for (int i = 0; i < chunk; i++) one_iter();
+    QED3_HIP(hipGetLastError());
+    QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+    QED3_HIP(hipStreamSynchronize(st));
+    if (hsc->done != 0) break;
+  }
+  if (kMixed) hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 1);
+  QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  QED3_HIP(hipStreamSynchronize(st));
+  CGResult res;
+  res.iterations = hsc->iter;
+  res.reliable_updates = hsc->n_ru;
+  res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipFreeAsync(W.r, st);
+  (void)hipFreeAsync(W.p, st);
+  (void)hipFreeAsync(W.w, st);
+  if (kMixed) {
+    (void)hipFreeAsync(W.xin, st);
+    (void)hipFreeAsync(W.t, st);
+  }
+  (void)hipFreeAsync(sc, st);
+  QED3_HIP(hipStreamSynchronize(st));
+  (void)hipHostFree(hsc);
+  res.true_resid = true_residual(D, x, b);
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
+  return res;
+}
+
+}  // namespace
+
+double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b) {
+  const size_t n = D.points();
+  hipStream_t st = D.stream();
+  Spin<double>*t1 = nullptr, *t2 = nullptr;
+  double2* out = nullptr;
+  QED3_HIP(hipMallocAsync((void**)&t1, n * sizeof(Spin<double>), st));
+  QED3_HIP(hipMallocAsync((void**)&t2, n * sizeof(Spin<double>), st));
+  QED3_HIP(hipMallocAsync((void**)&out, sizeof(double2), st));
+  D.apply_dagd<double>(t2, x, t1, false);
+  hipLaunchKernelGGL(k_resid_norms, dim3(int(stream_blocks(n))), dim3(256), 0, st, b, t2, n,
+                     D.reducer(), out);
+  QED3_HIP(hipGetLastError());
+  double2 h;
+  QED3_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, st));
+  QED3_HIP(hipStreamSynchronize(st));
+  (void)hipFreeAsync(t1, st);
+  (void)hipFreeAsync(t2, st);
+  (void)hipFreeAsync(out, st);
+  QED3_HIP(hipStreamSynchronize(st));
+  return h.y > 0 ? std::sqrt(h.x / h.y) : 0.0;
+}
+
+CGResult cg_solve(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
+  if (o.mixed) return run_cg<float>(D, x, b, o);
+  return run_cg<double>(D, x, b, o);
+}
+
+}  // namespace qed3

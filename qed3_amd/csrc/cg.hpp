@@ -1,0 +1,48 @@
+// Device-driven CG on A = D^dagger D (the role of MatPoly::solve,
+// src/both_3d/matpoly.h:204-267).
+//
+// Each iteration is six kernels whose control flow lives on the device:
+//   K1  w = D p,           gam = |w|^2           (stencil + grid reduction)
+//   K2  q = D^dag w,       alpha = mu/gam, x += alpha p, r -= alpha q, |r|^2
+//   U1  x64 += x_inner                           (only when a reliable update is due)
+//   U2  t = D x64                                (FP64, only when due)
+//   U3  r64 = b - D^dag t, r_inner = r64, |r64|^2 (FP64, only when due)
+//   K3  p = r + beta p
+// Kernels read the scalars (alpha, beta, flags) from device memory and exit at once
+// when the solve is finished or no update is due, so a chunk of iterations is
+// captured once in a hipGraph and replayed with one host poll per chunk.
+//
+// Inner precision float gives FP32 CG with FP64 reliable updates (Sleijpen & van der
+// Vorst 1996; Clark et al., CPC 181 (2010) 1517): the FP64 residual is recomputed
+// whenever |r| has dropped by `delta` since the last update, and convergence is only
+// declared on the recomputed FP64 residual.  Inner precision double is plain FP64 CG
+// whose stop test is likewise confirmed on the true residual.
+#pragma once
+#include "dirac.hpp"
+
+namespace qed3 {
+
+struct CGOptions {
+  double tol = 1e-10;
+  int maxiter = 100000;
+  bool mixed = false;
+  double delta = 0.1;
+  int chunk = 0;  // iterations per graph replay (0 = auto)
+  bool use_graph = true;
+};
+
+struct CGResult {
+  int status = 0;  // 0 ok, 3 no convergence, 4 breakdown
+  int iterations = 0;
+  int reliable_updates = 0;
+  double true_resid = 0;
+  double seconds = 0;
+};
+
+// Solve D^dag D x = b for FP64 device fields x, b (x overwritten, starting from 0).
+CGResult cg_solve(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o);
+
+// ||b - D^dag D x|| / ||b|| in FP64
+double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b);
+
+}  // namespace qed3

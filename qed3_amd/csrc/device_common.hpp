@@ -1,0 +1,127 @@
+// Device-side types and helpers shared by the gfx950 kernels: 2-component complex
+// spinors, complex arithmetic, wave64 + grid reductions with a last-block finaliser.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+namespace qed3 {
+
+#define QED3_HIP(call)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::qed3::HipError(std::string(#call) + ": " + hipGetErrorString(e_) + " @" + \
+                             __FILE__ + ":" + std::to_string(__LINE__));               \
+  } while (0)
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+template <typename T>
+using cplx_t = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+template <typename T>
+using real4_t = typename std::conditional<std::is_same<T, double>::value, double4, float4>::type;
+
+// One lattice point: the two spin components at (s, x).
+template <typename T>
+struct alignas(4 * sizeof(T)) Spin {
+  cplx_t<T> a, b;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cmul(cplx_t<T> u, cplx_t<T> v) {
+  return {u.x * v.x - u.y * v.y, u.x * v.y + u.y * v.x};
+}
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cconjmul(cplx_t<T> u, cplx_t<T> v) {  // conj(u) v
+  return {u.x * v.x + u.y * v.y, u.x * v.y - u.y * v.x};
+}
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cadd(cplx_t<T> u, cplx_t<T> v) {
+  return {u.x + v.x, u.y + v.y};
+}
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cscale(T s, cplx_t<T> v) {
+  return {s * v.x, s * v.y};
+}
+
+// Spin loads/stores as whole 16 B / 32 B vectors.
+template <typename T>
+__device__ __forceinline__ Spin<T> ld(const Spin<T>* p) {
+  return *p;
+}
+template <typename T>
+__device__ __forceinline__ void st(Spin<T>* p, const Spin<T>& v) {
+  *p = v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of two doubles; result valid in thread 0.
+__device__ __forceinline__ double2 block_sum2(double2 v) {
+  __shared__ double2 sh[16];
+  v.x = wave_sum(v.x);
+  v.y = wave_sum(v.y);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double2 r = {0.0, 0.0};
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; i++) {
+      r.x += sh[i].x;
+      r.y += sh[i].y;
+    }
+  }
+  return r;
+}
+
+// Scratch for single-pass grid reductions: one partial per block plus a ticket counter.
+struct GridReduce {
+  double2* partials;
+  unsigned int* counter;
+};
+
+// Every block of the grid must call this exactly once (uniformly).  The last block to
+// arrive sums the partials in block order (deterministic) and calls fin(total) from
+// thread 0, then re-arms the counter for the next launch on the stream.
+template <class Fin>
+__device__ __forceinline__ void grid_reduce(double2 v, GridReduce gr, const Fin& fin) {
+  __shared__ bool is_last;
+  const double2 b = block_sum2(v);
+  const unsigned nblocks = gridDim.x * gridDim.y;
+  if (threadIdx.x == 0) {
+    const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
+    __hip_atomic_store(&gr.partials[bid].x, b.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&gr.partials[bid].y, b.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned t = atomicAdd(gr.counter, 1u);
+    is_last = (t == nblocks - 1);
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    double2 s = {0.0, 0.0};
+    for (unsigned i = threadIdx.x; i < nblocks; i += blockDim.x) {
+      s.x += __hip_atomic_load(&gr.partials[i].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s.y += __hip_atomic_load(&gr.partials[i].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const double2 tot = block_sum2(s);
+    if (threadIdx.x == 0) {
+      fin(tot);
+      __hip_atomic_store(gr.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+}  // namespace qed3

@@ -1,0 +1,367 @@
+// Dirac operator host class, gauge-phase preparation, FP64 dot and the all-links
+// pseudofermion force kernel (gfx950).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "dirac.hpp"
+#include "stencil.hpp"
+
+namespace qed3 {
+
+namespace {
+constexpr size_t kReduceCap = kReduceCapacity;
+
+template <typename T>
+__global__ void k_phases(const double* __restrict__ ang, cplx_t<T>* __restrict__ u, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double sn, cs;
+    sincos(ang[i], &sn, &cs);
+    u[i] = {T(cs), T(sn)};
+  }
+}
+
+// <x,y> = sum conj(x) y over complex double fields of n points.
+__global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
+                                             const Spin<double>* __restrict__ y, size_t n,
+                                             GridReduce gr, double2* out) {
+  double2 acc = {0.0, 0.0};
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<double> a = x[i], b = y[i];
+    const double2 p = cconjmul<double>(a.a, b.a), q = cconjmul<double>(a.b, b.b);
+    acc.x += p.x + q.x;
+    acc.y += p.y + q.y;
+  }
+  grid_reduce(acc, gr, [out](double2 t) { *out = t; });
+}
+
+// Pseudofermion force, one thread per spatial link (s,l) followed by one per temporal
+// link (s,x):  F = -2 Re <w, dD/dtheta eta> with w = D eta  (dirac_pf.h:107-121,
+// with the two derivative blocks of dirac_ext.h:233-269 and :272-302).
+__global__ __launch_bounds__(256) void k_force(DiracView<double> d, const int4* __restrict__ ends,
+                                               const Spin<double>* __restrict__ eta,
+                                               const Spin<double>* __restrict__ w,
+                                               double* __restrict__ fs, double* __restrict__ ft) {
+  using C = double2;
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t nS = size_t(d.Lt) * d.E, nT = size_t(d.Lt) * d.V;
+  const double r = d.r;
+  if (i < nS) {
+    const int s = int(i / d.E), l = int(i % d.E);
+    const int4 e = ends[l];  // x, y, slot of y at x, slot of x at y
+    const int x = e.x, y = e.y;
+    const C u = d.uS[i];
+    const size_t ox = size_t(s) * d.V + x, oy = size_t(s) * d.V + y;
+    const Spin<double> ex = eta[ox], ey = eta[oy], wx = w[ox], wy = w[oy];
+    auto hmul = [&](const double4 h, const C p0, const C p1, C& o0, C& o1) {
+      const double rC = r * h.x, rS = r * h.y, A = h.z, B = h.w;
+      o0 = {-rC * p0.x - rS * p0.y + A * p1.x + B * p1.y, -rC * p0.y + rS * p0.x + A * p1.y - B * p1.x};
+      o1 = {A * p0.x - B * p0.y - rC * p1.x + rS * p1.y, A * p0.y + B * p0.x - rC * p1.y - rS * p1.x};
+    };
+    // row x: i e^{i theta} H_xy eta(y)
+    C a0, a1;
+    hmul(d.hop[size_t(e.z) * d.V + x], cmul<double>(u, ey.a), cmul<double>(u, ey.b), a0, a1);
+    // row y: -i e^{-i theta} H_yx eta(x)
+    const C uc = {u.x, -u.y};
+    C b0, b1;
+    hmul(d.hop[size_t(e.w) * d.V + y], cmul<double>(uc, ex.a), cmul<double>(uc, ex.b), b0, b1);
+    // Re <w, i v> = -Im <w, v>
+    const C t1 = cadd<double>(cconjmul<double>(wx.a, a0), cconjmul<double>(wx.b, a1));
+    const C t2 = cadd<double>(cconjmul<double>(wy.a, b0), cconjmul<double>(wy.b, b1));
+    const double re = -t1.y + t2.y;
+    fs[i] = -2.0 * re;
+  } else if (i < nS + nT) {
+    const size_t k = i - nS;
+    const int s = int(k / d.V), x = int(k % d.V);
+    const int sp = (s == d.Lt - 1) ? 0 : s + 1;
+    const double sg = (s == d.Lt - 1) ? double(d.bc) : 1.0;
+    const double kh = d.kth[x];
+    const double m0 = kh * (-r - 1.0), m1 = kh * (-r + 1.0);  // T- = diag(m0,m1), T+ = diag(m1,m0)
+    const C u = d.uT[k];
+    const size_t o0 = size_t(s) * d.V + x, o1 = size_t(sp) * d.V + x;
+    const Spin<double> e0 = eta[o0], e1 = eta[o1], w0 = w[o0], w1 = w[o1];
+    // dD[s+1 <- s] = -i sg T- e^{-i theta};  dD[s <- s+1] = +i sg T+ e^{+i theta}
+    const C uc = {u.x, -u.y};
+    const C p0 = cmul<double>(uc, e0.a), p1 = cmul<double>(uc, e0.b);
+    const C q0 = cmul<double>(u, e1.a), q1 = cmul<double>(u, e1.b);
+    const C tP = cadd<double>(cconjmul<double>(w1.a, cscale<double>(m0, p0)),
+                              cconjmul<double>(w1.b, cscale<double>(m1, p1)));
+    const C tM = cadd<double>(cconjmul<double>(w0.a, cscale<double>(m1, q0)),
+                              cconjmul<double>(w0.b, cscale<double>(m0, q1)));
+    // Re<w, -i sg tP> = sg Im(tP);  Re<w, i sg tM> = -sg Im(tM)
+    const double re = sg * (tP.y - tM.y);
+    ft[k] = -2.0 * re;
+  }
+}
+
+template <typename T>
+T* dalloc(size_t n) {
+  T* p = nullptr;
+  if (n) QED3_HIP(hipMalloc(&p, n * sizeof(T)));
+  return p;
+}
+
+template <typename T, typename S>
+T* dupload(const std::vector<S>& v, hipStream_t st) {
+  std::vector<T> tmp(v.begin(), v.end());
+  T* p = dalloc<T>(tmp.size());
+  if (!tmp.empty()) {
+    QED3_HIP(hipMemcpyAsync(p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    QED3_HIP(hipStreamSynchronize(st));
+  }
+  return p;
+}
+
+}  // namespace
+
+size_t reduce_capacity() { return kReduceCap; }
+
+Dirac::Dirac(const Geometry& g, int lattice, int Lt, int bc, double r, double M5, double at)
+    : h_(make_hop_tables(g, lattice, Lt, r, M5, at)), bc_(bc) {
+  QED3_HIP(hipStreamCreate(&stream_));
+  upload_tables(d64_);
+  upload_tables(d32_);
+  // link ends for the force kernel
+  std::vector<int> ends(size_t(h_.E) * 4, -1);
+  for (int x = 0; x < h_.V; x++)
+    for (int j = 0; j < h_.nn_count[x]; j++) {
+      const int sl = h_.lnk[size_t(j) * h_.V + x];
+      if (sl >= 0) {
+        ends[size_t(sl) * 4 + 0] = x;
+        ends[size_t(sl) * 4 + 1] = h_.nbr[size_t(j) * h_.V + x];
+        ends[size_t(sl) * 4 + 2] = j;
+      } else {
+        ends[size_t(~sl) * 4 + 3] = j;
+      }
+    }
+  link_ends_ = dupload<int>(ends, stream_);
+  angS_ = dalloc<double>(size_t(h_.Lt) * h_.E);
+  angT_ = dalloc<double>(size_t(h_.Lt) * h_.V);
+  partials_ = dalloc<double2>(kReduceCap);
+  counter_ = dalloc<unsigned int>(1);
+  dot_out_ = dalloc<double2>(1);
+  QED3_HIP(hipMemsetAsync(counter_, 0, sizeof(unsigned int), stream_));
+  QED3_HIP(hipStreamSynchronize(stream_));
+  for (auto& a : tile_)
+    for (auto& b : a) b = StencilTile{256, std::min(4, h_.Lt)};
+}
+
+template <typename T>
+void Dirac::upload_tables(DeviceTables<T>& d) {
+  d.nbr = dupload<int>(h_.nbr, stream_);
+  d.lnk = dupload<int>(h_.lnk, stream_);
+  const size_t ns = size_t(h_.max_nn) * h_.V;
+  std::vector<real4_t<T>> hd(ns), hdd(ns);
+  for (size_t i = 0; i < ns; i++) {
+    hd[i] = {T(h_.hopD[4 * i]), T(h_.hopD[4 * i + 1]), T(h_.hopD[4 * i + 2]), T(h_.hopD[4 * i + 3])};
+    hdd[i] = {T(h_.hopDdag[4 * i]), T(h_.hopDdag[4 * i + 1]), T(h_.hopDdag[4 * i + 2]),
+              T(h_.hopDdag[4 * i + 3])};
+  }
+  d.hop = dupload<real4_t<T>>(hd, stream_);
+  d.hopd = dupload<real4_t<T>>(hdd, stream_);
+  d.diag = dupload<T>(h_.diag, stream_);
+  std::vector<double> kth(h_.V);
+  for (int x = 0; x < h_.V; x++) kth[x] = 0.5 * h_.kappa_t[x];
+  d.kth = dupload<T>(kth, stream_);
+  d.uS = dalloc<cplx_t<T>>(size_t(h_.Lt) * h_.E);
+  d.uT = dalloc<cplx_t<T>>(size_t(h_.Lt) * h_.V);
+}
+
+Dirac::~Dirac() {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  for (void* p : {(void*)d64_.nbr, (void*)d64_.lnk, (void*)d64_.hop, (void*)d64_.hopd,
+                  (void*)d64_.diag, (void*)d64_.kth, (void*)d64_.uS, (void*)d64_.uT,
+                  (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
+                  (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
+                  (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
+                  (void*)counter_, (void*)dot_out_, (void*)tmp64_})
+    f(p);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Dirac::set_gauge(const double* spatial, const double* temporal, bool on_device) {
+  const size_t nS = size_t(h_.Lt) * h_.E, nT = size_t(h_.Lt) * h_.V;
+  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  QED3_HIP(hipMemcpyAsync(angS_, spatial, nS * sizeof(double), k, stream_));
+  QED3_HIP(hipMemcpyAsync(angT_, temporal, nT * sizeof(double), k, stream_));
+  const int B = 256;
+  hipLaunchKernelGGL(k_phases<double>, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, d64_.uS, nS);
+  hipLaunchKernelGGL(k_phases<double>, dim3((nT + B - 1) / B), dim3(B), 0, stream_, angT_, d64_.uT, nT);
+  hipLaunchKernelGGL(k_phases<float>, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, d32_.uS, nS);
+  hipLaunchKernelGGL(k_phases<float>, dim3((nT + B - 1) / B), dim3(B), 0, stream_, angT_, d32_.uT, nT);
+  QED3_HIP(hipGetLastError());
+  QED3_HIP(hipStreamSynchronize(stream_));
+  has_gauge_ = true;
+}
+
+template <typename T>
+DiracView<T> Dirac::view() const {
+  const DeviceTables<T>& d = [this]() -> const DeviceTables<T>& {
+    if constexpr (std::is_same<T, double>::value) return d64_;
+    else return d32_;
+  }();
+  DiracView<T> v;
+  v.nbr = d.nbr;
+  v.lnk = d.lnk;
+  v.hop = d.hop;
+  v.hopd = d.hopd;
+  v.diag = d.diag;
+  v.kth = d.kth;
+  v.uS = d.uS;
+  v.uT = d.uT;
+  v.V = h_.V;
+  v.E = h_.E;
+  v.Lt = h_.Lt;
+  v.r = T(h_.r);
+  v.bc = bc_ == 1 ? T(1) : T(-1);
+  return v;
+}
+
+template <typename T>
+StencilTile Dirac::tile(bool dag) const {
+  return tile_[std::is_same<T, float>::value ? 1 : 0][dag ? 1 : 0];
+}
+
+template <typename T>
+void Dirac::apply(bool dag, Spin<T>* out, const Spin<T>* in, bool sync) {
+  const DiracView<T> v = view<T>();
+  EpiStore<T> e{out};
+  if (dag)
+    launch_stencil<T, true>(v, h_.max_nn, in, e, tile<T>(true), reducer(), stream_);
+  else
+    launch_stencil<T, false>(v, h_.max_nn, in, e, tile<T>(false), reducer(), stream_);
+  QED3_HIP(hipGetLastError());
+  if (sync) QED3_HIP(hipStreamSynchronize(stream_));
+}
+
+template <typename T>
+void Dirac::apply_dagd(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync) {
+  apply<T>(false, tmp, in, false);
+  apply<T>(true, out, tmp, sync);
+}
+
+template <typename T>
+void Dirac::apply_ddag(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync) {
+  apply<T>(true, tmp, in, false);
+  apply<T>(false, out, tmp, sync);
+}
+
+void Dirac::dot(const Spin<double>* x, const Spin<double>* y, double* re, double* im) {
+  const size_t n = points();
+  const int B = 256;
+  const size_t nb = std::min<size_t>((n + B - 1) / B, 4096);
+  hipLaunchKernelGGL(k_dot, dim3(std::max<size_t>(nb, 1)), dim3(B), 0, stream_, x, y, n, reducer(), dot_out_);
+  QED3_HIP(hipGetLastError());
+  double2 h;
+  QED3_HIP(hipMemcpyAsync(&h, dot_out_, sizeof(h), hipMemcpyDeviceToHost, stream_));
+  QED3_HIP(hipStreamSynchronize(stream_));
+  if (re) *re = h.x;
+  if (im) *im = h.y;
+}
+
+void Dirac::force(const Spin<double>* eta, double* fs, double* ft) {
+  if (!tmp64_) tmp64_ = dalloc<Spin<double>>(points());
+  apply<double>(false, tmp64_, eta, false);  // precalc: w = D eta
+  const size_t n = size_t(h_.Lt) * (h_.E + h_.V);
+  const int B = 256;
+  hipLaunchKernelGGL(k_force, dim3((n + B - 1) / B), dim3(B), 0, stream_, view<double>(),
+                     reinterpret_cast<const int4*>(link_ends_), eta, tmp64_, fs, ft);
+  QED3_HIP(hipGetLastError());
+  QED3_HIP(hipStreamSynchronize(stream_));
+}
+
+double Dirac::time_apply(bool dag, bool f32, int warmup, int reps) {
+  const size_t n = points();
+  hipEvent_t e0, e1;
+  QED3_HIP(hipEventCreate(&e0));
+  QED3_HIP(hipEventCreate(&e1));
+  float ms = 0;
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    Spin<T>* a = dalloc<Spin<T>>(n);
+    Spin<T>* b = dalloc<Spin<T>>(n);
+    QED3_HIP(hipMemsetAsync(a, 0, n * sizeof(Spin<T>), stream_));
+    for (int i = 0; i < warmup; i++) apply<T>(dag, b, a, false);
+    QED3_HIP(hipEventRecord(e0, stream_));
+    for (int i = 0; i < reps; i++) apply<T>(dag, b, a, false);
+    QED3_HIP(hipEventRecord(e1, stream_));
+    QED3_HIP(hipEventSynchronize(e1));
+    QED3_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipFree(a);
+    (void)hipFree(b);
+  };
+  if (f32)
+    run(float{});
+  else
+    run(double{});
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return double(ms) / std::max(reps, 1);
+}
+
+// Project-written autotuner: times each candidate (block size, time-tile) of the
+// stencil on this lattice shape with HIP events and keeps the fastest.
+template <typename T>
+void Dirac::tune_one(bool dag) {
+  const size_t n = points();
+  Spin<T>* a = dalloc<Spin<T>>(n);
+  Spin<T>* b = dalloc<Spin<T>>(n);
+  QED3_HIP(hipMemsetAsync(a, 0, n * sizeof(Spin<T>), stream_));
+  hipEvent_t e0, e1;
+  QED3_HIP(hipEventCreate(&e0));
+  QED3_HIP(hipEventCreate(&e1));
+  StencilTile best = tile<T>(dag);
+  float best_ms = 1e30f;
+  const int f = std::is_same<T, float>::value ? 1 : 0;
+  for (int block : {128, 256}) {
+    for (int ts : {1, 2, 4, 8, 16}) {
+      if (ts > h_.Lt) continue;
+      tile_[f][dag] = StencilTile{block, ts};
+      for (int i = 0; i < 2; i++) apply<T>(dag, b, a, false);
+      QED3_HIP(hipEventRecord(e0, stream_));
+      const int reps = 5;
+      for (int i = 0; i < reps; i++) apply<T>(dag, b, a, false);
+      QED3_HIP(hipEventRecord(e1, stream_));
+      QED3_HIP(hipEventSynchronize(e1));
+      float ms;
+      QED3_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = tile_[f][dag];
+      }
+    }
+  }
+  tile_[f][dag] = best;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+}
+
+void Dirac::autotune() {
+  if (tuned_) return;
+  tune_one<double>(false);
+  tune_one<double>(true);
+  tune_one<float>(false);
+  tune_one<float>(true);
+  tuned_ = true;
+}
+
+template DiracView<double> Dirac::view<double>() const;
+template DiracView<float> Dirac::view<float>() const;
+template StencilTile Dirac::tile<double>(bool) const;
+template StencilTile Dirac::tile<float>(bool) const;
+template void Dirac::apply<double>(bool, Spin<double>*, const Spin<double>*, bool);
+template void Dirac::apply<float>(bool, Spin<float>*, const Spin<float>*, bool);
+template void Dirac::apply_dagd<double>(Spin<double>*, const Spin<double>*, Spin<double>*, bool);
+template void Dirac::apply_dagd<float>(Spin<float>*, const Spin<float>*, Spin<float>*, bool);
+template void Dirac::apply_ddag<double>(Spin<double>*, const Spin<double>*, Spin<double>*, bool);
+template void Dirac::apply_ddag<float>(Spin<float>*, const Spin<float>*, Spin<float>*, bool);
+
+}  // namespace qed3

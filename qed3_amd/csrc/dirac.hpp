@@ -1,0 +1,135 @@
+// Host-side Wilson-Dirac operator on S^2 x time for one GPU: owns the hop tables, the
+// gauge phases and the kernels' scratch.  Plays the role of the reference's
+// DiracExt + DWDevice + DiracPf trio (src/both_3d/dirac_ext.h, sparse_dirac.h,
+// dirac_pf.h) with a matrix-free stencil instead of CSR.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "device_common.hpp"
+#include "geometry.hpp"
+
+namespace qed3 {
+
+// Device copy of the time-independent tables in one precision.
+template <typename T>
+struct DiracView {
+  const int* nbr = nullptr;          // [nslot][V]
+  const int* lnk = nullptr;          // [nslot][V] link id or ~id
+  const real4_t<T>* hop = nullptr;   // [nslot][V] (C,S,a,b) for D
+  const real4_t<T>* hopd = nullptr;  // [nslot][V] for D^dagger
+  const T* diag = nullptr;           // [V]
+  const T* kth = nullptr;            // [V] kappa_t/2
+  const cplx_t<T>* uS = nullptr;     // [Lt][E] exp(i theta_s(l))
+  const cplx_t<T>* uT = nullptr;     // [Lt][V] exp(i theta^t_s(x))
+  int V = 0, E = 0, Lt = 1;
+  T r = 1;
+  T bc = -1;  // sign on the time wrap: -1 antiperiodic, +1 periodic
+};
+
+// Launch shape of one stencil kernel: threads per block and time slices per thread.
+struct StencilTile {
+  int block = 256;
+  int ts = 4;
+};
+
+template <typename T>
+struct DeviceTables {
+  int* nbr = nullptr;
+  int* lnk = nullptr;
+  real4_t<T>* hop = nullptr;
+  real4_t<T>* hopd = nullptr;
+  T* diag = nullptr;
+  T* kth = nullptr;
+  cplx_t<T>* uS = nullptr;
+  cplx_t<T>* uT = nullptr;
+};
+
+// Scalars of a device-driven CG; read and written only by kernels.
+struct CGScalars {
+  double mu;       // |r|^2 of the iterated residual
+  double gam;      // |D p|^2 = <p, D^dag D p>
+  double alpha, beta;
+  double maxr2;    // max |r|^2 since the last reliable update
+  double bnorm2, tol2, delta2;
+  double true_r2;  // last true FP64 |b - A x|^2
+  int done;        // 0 running, 1 converged, 2 breakdown / NaN, 3 maxiter
+  int need_ru;     // a reliable update is due in this iteration
+  int iter, n_ru, maxiter;
+  int pad;
+};
+
+class Dirac {
+ public:
+  Dirac(const Geometry& g, int lattice, int Lt, int bc, double r, double M5, double at);
+  ~Dirac();
+  Dirac(const Dirac&) = delete;
+  Dirac& operator=(const Dirac&) = delete;
+
+  const HopTables& tables() const { return h_; }
+  int V() const { return h_.V; }
+  int E() const { return h_.E; }
+  int Lt() const { return h_.Lt; }
+  size_t points() const { return size_t(h_.V) * h_.Lt; }
+  hipStream_t stream() const { return stream_; }
+  bool has_gauge() const { return has_gauge_; }
+
+  // angles in the reference's GaugeExt layout; on host or device
+  void set_gauge(const double* spatial, const double* temporal, bool on_device);
+
+  template <typename T>
+  DiracView<T> view() const;
+  template <typename T>
+  StencilTile tile(bool dag) const;
+
+  // out = D in / D^dag in (no reduction); blocking on the stream when sync
+  template <typename T>
+  void apply(bool dag, Spin<T>* out, const Spin<T>* in, bool sync = true);
+  template <typename T>
+  void apply_dagd(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync = true);
+  template <typename T>
+  void apply_ddag(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync = true);
+
+  // complex <x,y> in FP64
+  void dot(const Spin<double>* x, const Spin<double>* y, double* re, double* im);
+
+  // pseudofermion force for all links from eta (FP64)
+  void force(const Spin<double>* eta, double* fs, double* ft);
+
+  double time_apply(bool dag, bool f32, int warmup, int reps);
+
+  // scratch for reductions
+  GridReduce reducer() const { return {partials_, counter_}; }
+  // autotune the stencil launch shapes for this lattice (persisted in memory per shape)
+  void autotune();
+
+ private:
+  template <typename T>
+  void upload_tables(DeviceTables<T>& d);
+  template <typename T>
+  void tune_one(bool dag);
+
+  HopTables h_;
+  int bc_ = 0;
+  hipStream_t stream_ = nullptr;
+  DeviceTables<double> d64_;
+  DeviceTables<float> d32_;
+  double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)
+  double* angT_ = nullptr;  // [Lt][V]
+  int* link_ends_ = nullptr;  // [E][4]: x, y, slot of y in x, slot of x in y
+  double2* partials_ = nullptr;
+  unsigned int* counter_ = nullptr;
+  double2* dot_out_ = nullptr;
+  Spin<double>* tmp64_ = nullptr;  // D eta for the force
+  bool has_gauge_ = false;
+  StencilTile tile_[2][2];  // [f32][dag]
+  bool tuned_ = false;
+};
+
+size_t reduce_capacity();  // max blocks a reducing kernel may use
+constexpr size_t kReduceCapacity = size_t(1) << 20;
+
+}  // namespace qed3

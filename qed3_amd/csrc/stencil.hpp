@@ -1,0 +1,136 @@
+// Matrix-free Wilson-Dirac stencil on S^2 x time for gfx950 (wave64).
+//
+// One thread owns one site x for a tile of `ts` consecutive time slices: the site's
+// neighbour ids, link ids and 2x2 hop parameters are loaded once into registers and
+// reused for every slice of the tile, and the temporal neighbours psi(s+-1,x) roll
+// through registers.  Sites are numbered along a Hilbert curve, so the spatial gathers
+// psi(s,y) of a 256-thread block mostly hit lines its own block just loaded.
+//
+// Operator (src/both_3d/dirac_ext.h:121-213):
+//   (D psi)(s,x) = c_x psi(s,x) + sum_y H_xy e^{i theta_s(x,y)} psi(s,y)
+//                + b_- 1/2 kt_x (-r - sigma3) e^{-i theta^t_{s-1}(x)} psi(s-1,x)
+//                + b_+ 1/2 kt_x (-r + sigma3) e^{+i theta^t_s(x)}   psi(s+1,x)
+// with b_- = bc at s = 0, b_+ = bc at s = Lt-1 (bc = -1 antiperiodic), 1 otherwise.
+// D^dagger (the explicit conjugate transpose of sparse_dirac.h:132-143) is the same
+// stencil with H_xy -> H_yx^dagger and the two temporal projectors swapped.
+//
+// The epilogue functor receives each output point and may fuse BLAS-1 work and a
+// FP64 partial sum; kernels with a reducing epilogue finish with a single-pass grid
+// reduction whose last block runs the epilogue's finaliser.
+#pragma once
+#include "device_common.hpp"
+#include "dirac.hpp"
+
+namespace qed3 {
+
+template <typename T, int NS, bool DAG, class Epi>
+__global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* __restrict__ in,
+                                                  Epi epi, int ts, GridReduce gr) {
+  if (epi.skip()) return;  // uniform: solver already finished
+  using C = cplx_t<T>;
+  const int V = d.V, Lt = d.Lt;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s0 = blockIdx.y * ts;
+  double2 acc = {0.0, 0.0};
+  epi.begin();
+  if (x < V && s0 < Lt) {
+    int nb[NS], lk[NS];
+    real4_t<T> hp[NS];
+    const real4_t<T>* hop = DAG ? d.hopd : d.hop;
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+      nb[j] = d.nbr[j * V + x];
+      lk[j] = d.lnk[j * V + x];
+      hp[j] = hop[j * V + x];
+    }
+    const T c = d.diag[x];
+    const T kh = d.kth[x];
+    const T m0 = kh * (-d.r - T(1)), m1 = kh * (-d.r + T(1));
+    // T- = diag(m0, m1), T+ = diag(m1, m0); D uses (T- back, T+ fwd), D^dag swaps them.
+    const T bk0 = DAG ? m1 : m0, bk1 = DAG ? m0 : m1;
+    const T fw0 = DAG ? m0 : m1, fw1 = DAG ? m1 : m0;
+    const int s1 = min(s0 + ts, Lt);
+    const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
+    Spin<T> prev = ld(in + size_t(sm) * V + x);
+    Spin<T> cur = ld(in + size_t(s0) * V + x);
+    C uprev = d.uT[size_t(sm) * V + x];
+    for (int s = s0; s < s1; s++) {
+      const int sp = (s == Lt - 1) ? 0 : s + 1;
+      const Spin<T> next = ld(in + size_t(sp) * V + x);
+      const C ucur = d.uT[size_t(s) * V + x];
+      C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
+      const C* uS = d.uS + size_t(s) * d.E;
+      const Spin<T>* ins = in + size_t(s) * V;
+#pragma unroll
+      for (int j = 0; j < NS; j++) {
+        const int l = lk[j];
+        C u = uS[l >= 0 ? l : ~l];
+        if (l < 0) u.y = -u.y;
+        const Spin<T> py = ld(ins + nb[j]);
+        const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
+        const T Cc = hp[j].x, Ss = hp[j].y, A = hp[j].z, B = hp[j].w;
+        // H = [[-r(C - iS), a - ib], [a + ib, -r(C + iS)]]
+        const T rC = d.r * Cc, rS = d.r * Ss;
+        o0.x += -rC * p0.x - rS * p0.y + A * p1.x + B * p1.y;
+        o0.y += -rC * p0.y + rS * p0.x + A * p1.y - B * p1.x;
+        o1.x += A * p0.x - B * p0.y - rC * p1.x + rS * p1.y;
+        o1.y += A * p0.y + B * p0.x - rC * p1.y - rS * p1.x;
+      }
+      {  // backward temporal hop from s-1 with phase exp(-i theta^t_{s-1})
+        const T sg = (s == 0) ? d.bc : T(1);
+        const C q0 = cconjmul<T>(uprev, prev.a), q1 = cconjmul<T>(uprev, prev.b);
+        o0.x += sg * bk0 * q0.x;
+        o0.y += sg * bk0 * q0.y;
+        o1.x += sg * bk1 * q1.x;
+        o1.y += sg * bk1 * q1.y;
+      }
+      {  // forward temporal hop from s+1 with phase exp(+i theta^t_s)
+        const T sg = (s == Lt - 1) ? d.bc : T(1);
+        const C q0 = cmul<T>(ucur, next.a), q1 = cmul<T>(ucur, next.b);
+        o0.x += sg * fw0 * q0.x;
+        o0.y += sg * fw0 * q0.y;
+        o1.x += sg * fw1 * q1.x;
+        o1.y += sg * fw1 * q1.y;
+      }
+      Spin<T> o;
+      o.a = o0;
+      o.b = o1;
+      epi(size_t(s) * V + x, o, acc);
+      prev = cur;
+      cur = next;
+      uprev = ucur;
+    }
+  }
+  if constexpr (Epi::kReduce) grid_reduce(acc, gr, epi);
+}
+
+// Epilogue: plain store.
+template <typename T>
+struct EpiStore {
+  static constexpr bool kReduce = false;
+  Spin<T>* out;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ void operator()(size_t i, const Spin<T>& v, double2&) const { st(out + i, v); }
+  __device__ void operator()(double2) const {}
+};
+
+template <typename T>
+__device__ __forceinline__ double norm2(const Spin<T>& v) {
+  return double(v.a.x) * v.a.x + double(v.a.y) * v.a.y + double(v.b.x) * v.b.x +
+         double(v.b.y) * v.b.y;
+}
+
+template <typename T, bool DAG, class Epi>
+inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in, const Epi& epi,
+                           StencilTile t, GridReduce gr, hipStream_t st) {
+  const dim3 grid((v.V + t.block - 1) / t.block, (v.Lt + t.ts - 1) / t.ts);
+  if (Epi::kReduce && size_t(grid.x) * grid.y > kReduceCapacity)
+    throw std::invalid_argument("stencil grid exceeds the reduction scratch");
+  if (max_nn == 6)
+    hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, gr);
+  else
+    hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, gr);
+}
+
+}  // namespace qed3

@@ -1,0 +1,67 @@
+"""GPU parity of the stencil operator (D, D^dag, D^dag D) against the oracle's CSR SpMV."""
+import numpy as np
+import pytest
+
+import qed3_amd as q
+from conftest import rel
+from pairs import links, oracle_setup, source
+
+CASES = [(lat, n, Lt) for lat in (0, 1) for n in (1, 2, 4) for Lt in (1, 2, 3, 8)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice,n,Lt", CASES)
+def test_dirac_f64_matches_oracle(gpu, lattice, n, Lt):
+    print(f"case lattice={lattice} n={n} Lt={Lt}", flush=True)
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.1)
+    sp, tp = links(lattice, n, Lt, seed=7 + Lt, kappa=1.0)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.1, at=at)
+    Dq.update(sp, tp)
+    v = source(lattice, n, Lt, seed=3)
+    for op in (q.OP_D, q.OP_DDAG, q.OP_DDAGD):
+        ref = Do.apply(op, v)
+        got = Dq.apply(op, v, q.F64)
+        assert rel(got, ref) < 1e-14, (op, rel(got, ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_dirac_f32_matches_oracle(gpu, lattice):
+    n, Lt = 4, 4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.05)
+    sp, tp = links(lattice, n, Lt, seed=5, kappa=1.4)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.05, at=at)
+    Dq.update(sp, tp)
+    v = source(lattice, n, Lt, seed=9)
+    for op in (q.OP_D, q.OP_DDAG):
+        ref = Do.apply(op, v)
+        got = Dq.apply(op, v.astype(np.complex64), q.F32)
+        assert rel(got, ref) < 2e-6, (op, rel(got, ref))
+
+
+@pytest.mark.gpu
+def test_periodic_bc_matches_oracle(gpu):
+    lattice, n, Lt = 0, 2, 4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.2, bc=1)
+    sp, tp = links(lattice, n, Lt, seed=2)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.2, at=at, bc=q.PERIODIC)
+    Dq.update(sp, tp)
+    v = source(lattice, n, Lt, seed=4)
+    assert rel(Dq.mult(v), Do.apply(0, v)) < 1e-14
+
+
+@pytest.mark.gpu
+def test_adjoint_identity_large(gpu):
+    """<u, D v> = <D^dag u, v> at a lattice too large for the dense oracle."""
+    lattice, n, Lt = 0, 16, 16
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.05, at=0.05)
+    sp, tp = links(lattice, n, Lt, seed=21)
+    Dq.update(sp, tp)
+    u = source(lattice, n, Lt, seed=1)
+    v = source(lattice, n, Lt, seed=2)
+    lhs = np.vdot(u, Dq.mult(v))
+    rhs = np.vdot(Dq.adj(u), v)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-13
