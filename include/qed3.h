@@ -145,6 +145,45 @@ int qed3_dot(qed3_ctx* ctx, const void* d_x, const void* d_y, double* re, double
  * arrays in the gauge layout: d_fs[Lt][E], d_ft[Lt][V]. */
 int qed3_pf_force(qed3_ctx* ctx, const void* d_eta, double* d_fs, double* d_ft);
 
+/* ---- gauge action and HMC (host FP64 over the GPU fermion path) --------------------
+ * U(1) plaquette action on the context's lattice with beta_s = at/(vol_f g^2) per face
+ * and beta_t = 2 vol_l/(at ell_l^2 g^2) per temporal plaquette (action_ext.h:150-159):
+ * compact = 0 is the reference's 1/2 beta P^2 (action_ext.h:33-80), compact = 1 the
+ * Wilson form beta (1 - cos P).  Host arrays; fs/ft (gauge layout) may be NULL. */
+int qed3_gauge_action(qed3_ctx* ctx, double gsq, int compact, const double* spatial,
+                      const double* temporal, double* S, double* fs, double* ft);
+
+typedef struct qed3_hmc_params {
+  double gsq;       /* gauge coupling g^2 = 1/beta */
+  int compact;      /* gauge action form (see qed3_gauge_action) */
+  int npf;          /* pseudofermions (Nf = 2 npf) */
+  double tau;       /* trajectory length */
+  int nsteps;       /* outer (fermion) leapfrog steps */
+  int nsteps_inner; /* gauge steps per outer step (integrator.h:58-73) */
+  double tol;       /* true-residual tolerance of every pseudofermion solve */
+  int solver;       /* QED3_SOLVER_* */
+  int seed;         /* ParallelRngExt master seed (rng.h:91-120) */
+} qed3_hmc_params;
+
+typedef struct qed3_hmc_report {
+  double dH, h0, h1;
+  int accept;
+  int cg_solves;
+  long long cg_iterations;
+  double max_resid;
+  double seconds;
+} qed3_hmc_report;
+
+/* HMC2 (hmc.h:103-201) with ExplicitLeapfrogML (integrator.h:58-170); the RNG streams
+ * persist across trajectories of one handle. */
+typedef struct qed3_hmc qed3_hmc;
+int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out);
+void qed3_hmc_destroy(qed3_hmc* h);
+/* One trajectory from the host gauge field (spatial[Lt][E], temporal[Lt][V]), replaced
+ * in place by the new field when accepted (or always when no_reject != 0). */
+int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_reject,
+                        qed3_hmc_report* rep);
+
 /* Pick the stencil launch shape (threads per block, time slices per thread) for this
  * lattice by timing every candidate on the device (project-written autotuner). */
 int qed3_autotune(qed3_ctx* ctx);

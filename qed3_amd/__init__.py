@@ -224,6 +224,16 @@ class WilsonDirac:
         return (d_fs.to_numpy(np.float64).reshape(self.Lt, self.E),
                 d_ft.to_numpy(np.float64).reshape(self.Lt, self.V))
 
+    def gauge_action(self, spatial, temporal, gsq, compact=True, force=False):
+        sp = np.ascontiguousarray(spatial, np.float64)
+        tp = np.ascontiguousarray(temporal, np.float64)
+        S = C.c_double()
+        fs = np.zeros((self.Lt, self.E)) if force else None
+        ft = np.zeros((self.Lt, self.V)) if force else None
+        check(load().qed3_gauge_action(self._h, gsq, int(compact), _dp(sp), _dp(tp), C.byref(S),
+                                       _dp(fs) if force else None, _dp(ft) if force else None))
+        return (S.value, fs, ft) if force else S.value
+
     def autotune(self):
         check(load().qed3_autotune(self._h))
 
@@ -231,3 +241,32 @@ class WilsonDirac:
         ms = C.c_double()
         check(load().qed3_time_apply(self._h, op, prec, warmup, reps, C.byref(ms)))
         return ms.value
+
+
+class HMC:
+    """HMC2 + ExplicitLeapfrogML over a WilsonDirac (host FP64 H and accept step)."""
+
+    def __init__(self, D: WilsonDirac, gsq, compact=True, npf=1, tau=1.0, nsteps=20,
+                 nsteps_inner=10, tol=1e-10, solver=CG_MIXED, seed=1):
+        self.D = D
+        self.p = _lib.HMCParams(gsq, int(compact), npf, tau, nsteps, nsteps_inner, tol, solver,
+                                seed)
+        self._h = C.c_void_p()
+        check(load().qed3_hmc_create(D._h, C.byref(self.p), C.byref(self._h)))
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().qed3_hmc_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def trajectory(self, spatial, temporal, no_reject=False):
+        """Runs one trajectory; spatial/temporal (float64 arrays) are updated in place."""
+        assert spatial.dtype == np.float64 and spatial.flags.c_contiguous
+        assert temporal.dtype == np.float64 and temporal.flags.c_contiguous
+        rep = _lib.HMCReport()
+        check(load().qed3_hmc_trajectory(self._h, _dp(spatial), _dp(temporal), int(no_reject),
+                                         C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in rep._fields_}

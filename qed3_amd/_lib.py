@@ -26,6 +26,18 @@ class CGReport(C.Structure):
                 ("true_resid", C.c_double), ("seconds", C.c_double)]
 
 
+class HMCParams(C.Structure):
+    _fields_ = [("gsq", C.c_double), ("compact", C.c_int), ("npf", C.c_int), ("tau", C.c_double),
+                ("nsteps", C.c_int), ("nsteps_inner", C.c_int), ("tol", C.c_double),
+                ("solver", C.c_int), ("seed", C.c_int)]
+
+
+class HMCReport(C.Structure):
+    _fields_ = [("dH", C.c_double), ("h0", C.c_double), ("h1", C.c_double), ("accept", C.c_int),
+                ("cg_solves", C.c_int), ("cg_iterations", C.c_longlong),
+                ("max_resid", C.c_double), ("seconds", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/qed3.h
 SIGNATURES = {
     "qed3_last_error": (C.c_char_p, []),
@@ -53,6 +65,11 @@ SIGNATURES = {
                           C.POINTER(CGReport)]),
     "qed3_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
     "qed3_pf_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qed3_gauge_action": (C.c_int, [C.c_void_p, C.c_double, C.c_int, c_double_p, c_double_p,
+                                    c_double_p, c_double_p, c_double_p]),
+    "qed3_hmc_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "qed3_hmc_destroy": (None, [C.c_void_p]),
+    "qed3_hmc_trajectory": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_int, C.c_void_p]),
     "qed3_autotune": (C.c_int, [C.c_void_p]),
     "qed3_time_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]),
 }

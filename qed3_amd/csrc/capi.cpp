@@ -10,6 +10,7 @@
 #include "cg.hpp"
 #include "dirac.hpp"
 #include "geometry.hpp"
+#include "hmc.hpp"
 #include "rng.hpp"
 
 struct qed3_geom {
@@ -20,6 +21,11 @@ struct qed3_ctx {
   qed3_params p;
   std::unique_ptr<qed3::Geometry> g;
   std::unique_ptr<qed3::Dirac> D;
+};
+
+struct qed3_hmc {
+  qed3_ctx* ctx;
+  std::unique_ptr<qed3::HMC> hmc;
 };
 
 namespace {
@@ -321,6 +327,61 @@ int qed3_pf_force(qed3_ctx* ctx, const void* d_eta, double* d_fs, double* d_ft) 
   if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
   return guarded([&]() -> int {
     ctx->D->force(static_cast<const qed3::Spin<double>*>(d_eta), d_fs, d_ft);
+    return QED3_OK;
+  });
+}
+
+int qed3_gauge_action(qed3_ctx* ctx, double gsq, int compact, const double* spatial,
+                      const double* temporal, double* S, double* fs, double* ft) {
+  if (!ctx || !spatial || !temporal) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    const qed3::GaugeAction A(ctx->D->tables(), gsq, ctx->p.at, compact != 0);
+    if (S) *S = A.S(spatial, temporal);
+    if (fs && ft) A.force(spatial, temporal, fs, ft);
+    return QED3_OK;
+  });
+}
+
+int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
+  if (!ctx || !p || !out) return fail(QED3_ERR_ARG, "null argument");
+  if (p->solver != QED3_SOLVER_CG_F64 && p->solver != QED3_SOLVER_CG_MIXED)
+    return fail(QED3_ERR_ARG, "bad solver");
+  return guarded([&]() -> int {
+    qed3::HMCParams hp;
+    hp.gsq = p->gsq;
+    hp.compact = p->compact != 0;
+    hp.npf = p->npf;
+    hp.tau = p->tau;
+    hp.nsteps = p->nsteps;
+    hp.nsteps_inner = p->nsteps_inner;
+    hp.tol = p->tol;
+    hp.mixed = p->solver == QED3_SOLVER_CG_MIXED;
+    hp.seed = p->seed;
+    auto h = std::make_unique<qed3_hmc>();
+    h->ctx = ctx;
+    h->hmc = std::make_unique<qed3::HMC>(*ctx->D, hp);
+    *out = h.release();
+    return QED3_OK;
+  });
+}
+
+void qed3_hmc_destroy(qed3_hmc* h) { delete h; }
+
+int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_reject,
+                        qed3_hmc_report* rep) {
+  if (!h || !spatial || !temporal) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    const qed3::HMCReport r = h->hmc->trajectory(spatial, temporal, no_reject != 0);
+    if (rep) {
+      rep->dH = r.dH;
+      rep->h0 = r.h0;
+      rep->h1 = r.h1;
+      rep->accept = r.accept;
+      rep->cg_solves = r.cg_solves;
+      rep->cg_iterations = r.cg_iterations;
+      rep->max_resid = r.max_resid;
+      rep->seconds = r.seconds;
+    }
     return QED3_OK;
   });
 }

@@ -1,0 +1,247 @@
+#include "hmc.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <stdexcept>
+
+#include "host_parallel.hpp"
+
+namespace qed3 {
+
+GaugeAction::GaugeAction(const HopTables& h, double gsq, double at, bool compact)
+    : h_(h), compact_(compact), Lt_(h.Lt) {
+  if (!(gsq > 0)) throw std::invalid_argument("gsq must be positive");
+  beta_s_.resize(h.F);
+  beta_t_.resize(h.E);
+  for (int f = 0; f < h.F; f++) beta_s_[f] = at / h.face_area[f] / gsq;   // action_ext.h:152-154
+  for (int l = 0; l < h.E; l++)
+    beta_t_[l] = 2.0 / at * h.link_volume[l] / (h.ell[l] * h.ell[l]) / gsq;  // :155-157
+  std::map<std::pair<int, int>, int> id;
+  for (int l = 0; l < h.E; l++) id[{h.links[l][0], h.links[l][1]}] = l;
+  fe_off_.push_back(0);
+  for (const auto& ring : h.face_rings) {
+    for (size_t k = 0; k < ring.size(); k++) {
+      const int a = ring[k], b = ring[(k + 1) % ring.size()];
+      const bool fwd = a < b;
+      fe_link_.push_back(id.at(fwd ? std::make_pair(a, b) : std::make_pair(b, a)));
+      fe_sign_.push_back(fwd ? 1 : -1);
+    }
+    fe_off_.push_back(int(fe_link_.size()));
+  }
+}
+
+double GaugeAction::face_angle(const double* u, int s, int f) const {
+  const double* us = u + size_t((s + Lt_) % Lt_) * h_.E;
+  double sum = 0;
+  for (int k = fe_off_[f]; k < fe_off_[f + 1]; k++) sum += fe_sign_[k] * us[fe_link_[k]];
+  return sum;
+}
+
+double GaugeAction::link_angle(const double* u, const double* t, int s, int l) const {
+  const int s0 = (s + Lt_) % Lt_, s1 = (s + 1) % Lt_;
+  const int x = h_.links[l][0], y = h_.links[l][1];
+  return u[size_t(s0) * h_.E + l] + t[size_t(s0) * h_.V + y] - u[size_t(s1) * h_.E + l] -
+         t[size_t(s0) * h_.V + x];
+}
+
+double GaugeAction::S(const double* u, const double* t) const {
+  const int F = h_.F, E = h_.E;
+  std::vector<long double> part(size_t(Lt_) * 2, 0.0L);
+  parallel_ranges(size_t(Lt_) * 2, [&](size_t a, size_t b) {
+    for (size_t k = a; k < b; k++) {
+      const int s = int(k / 2);
+      long double acc = 0;
+      if (k % 2 == 0) {
+        for (int f = 0; f < F; f++) {
+          const double P = face_angle(u, s, f);
+          acc += compact_ ? beta_s_[f] * (1.0 - std::cos(P)) : 0.5 * beta_s_[f] * P * P;
+        }
+      } else {
+        for (int l = 0; l < E; l++) {
+          const double P = link_angle(u, t, s, l);
+          acc += compact_ ? beta_t_[l] * (1.0 - std::cos(P)) : 0.5 * beta_t_[l] * P * P;
+        }
+      }
+      part[k] = acc;
+    }
+  });
+  long double tot = 0;
+  for (auto v : part) tot += v;
+  return double(tot);
+}
+
+void GaugeAction::force(const double* u, const double* t, double* fs, double* ft) const {
+  const int F = h_.F, E = h_.E, V = h_.V;
+  auto g = [&](double b, double P) { return compact_ ? b * std::sin(P) : b * P; };
+  // one slice per task: every write lands in slice s (spatial faces), or in slice s of
+  // fs/ft from the temporal plaquettes at s and s-1, so slices are independent.
+  parallel_ranges(size_t(Lt_), [&](size_t a, size_t b) {
+    for (size_t ss = a; ss < b; ss++) {
+      const int s = int(ss);
+      double* fss = fs + size_t(s) * E;
+      double* fts = ft + size_t(s) * V;
+      for (int l = 0; l < E; l++) fss[l] = 0;
+      for (int x = 0; x < V; x++) fts[x] = 0;
+      for (int f = 0; f < F; f++) {
+        const double grad = g(beta_s_[f], face_angle(u, s, f));
+        for (int k = fe_off_[f]; k < fe_off_[f + 1]; k++) fss[fe_link_[k]] += grad * fe_sign_[k];
+      }
+      for (int l = 0; l < E; l++) {
+        const double grad = g(beta_t_[l], link_angle(u, t, s, l));
+        const double grad2 = g(beta_t_[l], link_angle(u, t, s - 1, l));
+        fss[l] += grad - grad2;
+        fts[h_.links[l][1]] += grad;
+        fts[h_.links[l][0]] -= grad;
+      }
+    }
+  });
+}
+
+HMC::HMC(Dirac& D, const HMCParams& p)
+    : D_(D), p_(p), Sg_(D.tables(), p.gsq, D.tables().at, p.compact) {
+  if (p.npf < 1 || p.nsteps < 1 || p.nsteps_inner < 1) throw std::invalid_argument("bad HMC parameters");
+  const HopTables& h = D.tables();
+  rng_ = std::make_unique<ParallelRng>(h.Lt, h.E, h.V, p.seed);
+  const size_t n = D.points();
+  phi_.resize(p.npf);
+  eta_.resize(p.npf);
+  for (int k = 0; k < p.npf; k++) {
+    QED3_HIP(hipMalloc(&phi_[k], n * sizeof(Spin<double>)));
+    QED3_HIP(hipMalloc(&eta_[k], n * sizeof(Spin<double>)));
+  }
+  QED3_HIP(hipMalloc(&xi_, n * sizeof(Spin<double>)));
+  QED3_HIP(hipMalloc(&dfs_, size_t(h.Lt) * h.E * sizeof(double)));
+  QED3_HIP(hipMalloc(&dft_, size_t(h.Lt) * h.V * sizeof(double)));
+}
+
+HMC::~HMC() {
+  for (auto* p : phi_) (void)hipFree(p);
+  for (auto* p : eta_) (void)hipFree(p);
+  (void)hipFree(xi_);
+  (void)hipFree(dfs_);
+  (void)hipFree(dft_);
+}
+
+void HMC::set_gauge(const std::vector<double>& u, const std::vector<double>& t) {
+  D_.set_gauge(u.data(), t.data(), false);
+}
+
+void HMC::update_eta(int k) {
+  CGOptions o;
+  o.tol = p_.tol;
+  o.mixed = p_.mixed;
+  const CGResult r = cg_solve(D_, eta_[k], phi_[k], o);
+  rep_.cg_solves++;
+  rep_.cg_iterations += r.iterations;
+  rep_.max_resid = std::max(rep_.max_resid, r.true_resid);
+  if (r.status != 0) throw std::runtime_error("HMC: pseudofermion solve failed");
+}
+
+// PseudoFermion::gen (pseudofermion.h:36-49): xi from the site streams, phi = D^dag xi
+void HMC::pf_gen(int k) {
+  const HopTables& h = D_.tables();
+  const size_t n = D_.points();
+  std::vector<double> xi(4 * n);
+  const double inv = 1.0 / std::sqrt(2.0);
+  for (int s = 0; s < h.Lt; s++)
+    for (int x = 0; x < h.V; x++)
+      for (int a = 0; a < 2; a++) {
+        const double re = rng_->gaussian_site(s, x);
+        const double im = rng_->gaussian_site(s, x);
+        const size_t o = 4 * (size_t(s) * h.V + x) + 2 * a;
+        xi[o] = re * inv;
+        xi[o + 1] = im * inv;
+      }
+  QED3_HIP(hipMemcpy(xi_, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice));
+  D_.apply<double>(true, phi_[k], xi_, true);
+  update_eta(k);
+}
+
+double HMC::Sf(int k) {  // Re <phi, eta> (pseudofermion.h:53-58)
+  double re, im;
+  D_.dot(phi_[k], eta_[k], &re, &im);
+  return re;
+}
+
+void HMC::fermion_force(int k, std::vector<double>& fs, std::vector<double>& ft) {
+  D_.force(eta_[k], dfs_, dft_);
+  QED3_HIP(hipMemcpy(fs.data(), dfs_, fs.size() * sizeof(double), hipMemcpyDeviceToHost));
+  QED3_HIP(hipMemcpy(ft.data(), dft_, ft.size() * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+double HMC::H() {  // hmc.h:159-166
+  long double kin = 0;
+  for (double v : pu_) kin += (long double)v * v;
+  for (double v : pt_) kin += (long double)v * v;
+  long double res = 0.5L * kin;
+  res += Sg_.S(u_.data(), t_.data());
+  for (int k = 0; k < p_.npf; k++) res += Sf(k);
+  return double(res);
+}
+
+HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const HopTables& h = D_.tables();
+  const size_t nS = size_t(h.Lt) * h.E, nT = size_t(h.Lt) * h.V;
+  rep_ = HMCReport();
+  u_.assign(u, u + nS);
+  t_.assign(t, t + nT);
+  pu_.resize(nS);
+  pt_.resize(nT);
+  // pi.gaussian(rng) (gauge_ext.h:146-149)
+  for (int s = 0; s < h.Lt; s++)
+    for (int l = 0; l < h.E; l++) pu_[size_t(s) * h.E + l] = rng_->gaussian_link(s, l);
+  for (int s = 0; s < h.Lt; s++)
+    for (int x = 0; x < h.V; x++) pt_[size_t(s) * h.V + x] = rng_->gaussian_site(s, x);
+  set_gauge(u_, t_);
+  for (int k = 0; k < p_.npf; k++) pf_gen(k);
+  const double h0 = H();
+
+  // ExplicitLeapfrogML::integrate, multi-pseudofermion form (integrator.h:132-168)
+  const double tau = p_.tau / p_.nsteps, tau_in = tau / p_.nsteps_inner;
+  std::vector<double> gs(nS), gt(nT), fs(nS), ft(nT);
+  auto kick = [&](const std::vector<double>& a, const std::vector<double>& b, double c) {
+    for (size_t i = 0; i < nS; i++) pu_[i] += c * a[i];
+    for (size_t i = 0; i < nT; i++) pt_[i] += c * b[i];
+  };
+  for (int k = 0; k < p_.npf; k++) {
+    fermion_force(k, fs, ft);
+    kick(fs, ft, -0.5 * tau);
+  }
+  for (int n = 0; n < p_.nsteps; n++) {
+    Sg_.force(u_.data(), t_.data(), gs.data(), gt.data());
+    kick(gs, gt, -0.5 * tau_in);
+    for (int ni = 0; ni < p_.nsteps_inner; ni++) {
+      for (size_t i = 0; i < nS; i++) u_[i] += tau_in * pu_[i];
+      for (size_t i = 0; i < nT; i++) t_[i] += tau_in * pt_[i];
+      Sg_.force(u_.data(), t_.data(), gs.data(), gt.data());
+      kick(gs, gt, (ni != p_.nsteps_inner - 1) ? -1.0 * tau_in : -0.5 * tau_in);
+    }
+    set_gauge(u_, t_);
+    for (int k = 0; k < p_.npf; k++) {
+      update_eta(k);
+      fermion_force(k, fs, ft);
+      kick(fs, ft, (n != p_.nsteps - 1) ? -1.0 * tau : -0.5 * tau);
+    }
+  }
+  const double h1 = H();
+  rep_.h0 = h0;
+  rep_.h1 = h1;
+  rep_.dH = h1 - h0;
+  const double r = std::min(1.0, std::exp(-rep_.dH));
+  const double a = rng_->uniform();
+  rep_.accept = (a < r || no_reject) ? 1 : 0;
+  if (rep_.accept) {
+    std::copy(u_.begin(), u_.end(), u);
+    std::copy(t_.begin(), t_.end(), t);
+  } else {
+    D_.set_gauge(u, t, false);
+  }
+  rep_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rep_;
+}
+
+}  // namespace qed3

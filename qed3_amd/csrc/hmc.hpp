@@ -1,0 +1,83 @@
+// Host-side HMC over the GPU fermion path: pseudofermions, U(1) gauge action (the
+// reference's non-compact form and the compact Wilson form), the multi-level leapfrog
+// and the accept/reject step.  Roles of the reference's PseudoFermion
+// (src/both_3d/pseudofermion.h:6-61), U1WilsonExt (action_ext.h:4-161),
+// ExplicitLeapfrogML (integrator.h:58-170) and HMC2 (hmc.h:103-201).
+// The Hamiltonian and accept step are FP64 on the host; the fermion solves and forces
+// run on the GPU.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "cg.hpp"
+#include "dirac.hpp"
+#include "rng.hpp"
+
+namespace qed3 {
+
+// U(1) plaquette action on the lattice of a Dirac operator.
+class GaugeAction {
+ public:
+  GaugeAction(const HopTables& h, double gsq, double at, bool compact);
+  double S(const double* u, const double* t) const;
+  void force(const double* u, const double* t, double* fs, double* ft) const;
+
+ private:
+  double face_angle(const double* u, int s, int f) const;
+  double link_angle(const double* u, const double* t, int s, int l) const;
+  const HopTables& h_;
+  bool compact_;
+  int Lt_;
+  std::vector<double> beta_s_, beta_t_;
+  std::vector<int> fe_off_, fe_link_, fe_sign_;  // oriented edges of each face
+};
+
+struct HMCParams {
+  double gsq = 1.0 / 1.4;
+  bool compact = true;
+  int npf = 1;
+  double tau = 1.0;
+  int nsteps = 20;
+  int nsteps_inner = 10;
+  double tol = 1e-10;
+  bool mixed = true;
+  int seed = 1;
+};
+
+struct HMCReport {
+  double dH = 0, h0 = 0, h1 = 0;
+  int accept = 0;
+  int cg_solves = 0;
+  long cg_iterations = 0;
+  double seconds = 0;
+  double max_resid = 0;
+};
+
+class HMC {
+ public:
+  HMC(Dirac& D, const HMCParams& p);
+  ~HMC();
+  // one trajectory; u/t (host, gauge layout) are replaced by the new field on accept
+  HMCReport trajectory(double* u, double* t, bool no_reject = false);
+
+ private:
+  void set_gauge(const std::vector<double>& u, const std::vector<double>& t);
+  void pf_gen(int k);
+  void update_eta(int k);
+  double Sf(int k);
+  void fermion_force(int k, std::vector<double>& fs, std::vector<double>& ft);
+  double H();
+
+  Dirac& D_;
+  HMCParams p_;
+  GaugeAction Sg_;
+  std::unique_ptr<ParallelRng> rng_;
+  std::vector<Spin<double>*> phi_, eta_;
+  Spin<double>* xi_ = nullptr;
+  double* dfs_ = nullptr;
+  double* dft_ = nullptr;
+  std::vector<double> u_, t_, pu_, pt_;
+  HMCReport rep_;
+};
+
+}  // namespace qed3

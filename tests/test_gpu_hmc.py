@@ -1,0 +1,60 @@
+"""HMC trajectory (HMC2 + ExplicitLeapfrogML) against the oracle on the same seeds."""
+import numpy as np
+import pytest
+
+import oracle as O
+import qed3_amd as q
+from conftest import rel
+from pairs import links, oracle_setup
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice,compact", [(0, False), (0, True), (1, True)])
+def test_gauge_action_matches_oracle(gpu, lattice, compact):
+    n, Lt = 2, 4
+    L, Do, at = oracle_setup(lattice, n, Lt)
+    sp, tp = links(lattice, n, Lt, seed=4, kappa=0.7)
+    Sg = O.GaugeAction(L, Lt, gsq=1 / 1.4, at=at, compact=compact)
+    Dq = q.WilsonDirac(lattice, n, Lt, M5=0.1, at=at)
+    S, fs, ft = Dq.gauge_action(sp, tp, 1 / 1.4, compact=compact, force=True)
+    fs_r, ft_r = Sg.force(sp, tp)
+    assert abs(S - Sg.S(sp, tp)) < 1e-12 * abs(S)
+    assert rel(fs, fs_r) < 1e-13 and rel(ft, ft_r) < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice,compact,npf", [(0, True, 1), (1, False, 2)])
+def test_hmc_trajectory_matches_oracle(gpu, lattice, compact, npf):
+    n, Lt, M5, gsq = 1, 4, 0.1, 1 / 1.4
+    print(f"case lattice={lattice} compact={compact} npf={npf}", flush=True)
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    sp, tp = links(lattice, n, Lt, seed=3, kappa=1.4)
+    Sg = O.GaugeAction(L, Lt, gsq=gsq, at=at, compact=compact)
+    ref = O.hmc_trajectory(Do, Sg, seed=12, npf=npf, tau=0.5, nsteps=4, nsteps_inner=3,
+                           tol=1e-13, sp=sp, tp=tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, M5=M5, at=at)
+    H = q.HMC(Dq, gsq, compact=compact, npf=npf, tau=0.5, nsteps=4, nsteps_inner=3, tol=1e-12,
+              solver=q.CG_F64, seed=12)
+    u, t = sp.copy(), tp.copy()
+    rep = H.trajectory(u, t, no_reject=True)
+    assert abs(rep["h0"] - ref["h0"]) < 1e-9 * abs(ref["h0"])
+    assert abs(rep["dH"] - ref["dH"]) < 1e-9 * max(1.0, abs(ref["dH"]))
+    assert rel(u, ref["u"]) < 1e-10 and rel(t, ref["t"]) < 1e-10
+    assert rep["cg_solves"] == npf * (1 + 4)
+
+
+@pytest.mark.gpu
+def test_hmc_dH_scales_as_tau_squared(gpu):
+    """Leapfrog: dH = O(eps^2) (grad.cu:344-369; figs/scaling.plt guide 100 tau^2)."""
+    lattice, n, Lt, M5 = 0, 2, 4, 0.2
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    sp, tp = links(lattice, n, Lt, seed=5, kappa=2.0)
+    Dq = q.WilsonDirac(lattice, n, Lt, M5=M5, at=at)
+    dh = []
+    for nsteps in (4, 8):
+        H = q.HMC(Dq, 1 / 2.0, compact=True, npf=1, tau=0.2, nsteps=nsteps, nsteps_inner=8,
+                  tol=1e-12, solver=q.CG_F64, seed=7)
+        rep = H.trajectory(sp.copy(), tp.copy(), no_reject=True)
+        dh.append(abs(rep["dH"]))
+    ratio = dh[0] / dh[1]
+    assert 2.5 < ratio < 6.0, (dh, ratio)
