@@ -32,7 +32,24 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=12)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--kernels-only", action="store_true",
+                   help="only launch the D stencils and the calibration copy (for profilers)")
     return p.parse_args()
+
+
+def copy_bandwidth(q, nbytes=1 << 30, reps=20):
+    """GB/s of a plain device copy (read + write of nbytes) and its ms per launch."""
+    import ctypes as C
+    ms = C.c_double()
+    q.check(q.load().qed3_time_copy(nbytes, reps, C.byref(ms)))
+    return 2 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
+
+
+def tile_info(q, D, op, prec):
+    import ctypes as C
+    out = (C.c_int * 3)()
+    q.check(q.load().qed3_tile_info(D._h, op, prec, out))
+    return {"block": out[0], "ts": out[1], "xcd_remap": out[2]}
 
 
 def mean_ell(g, lattice):
@@ -66,6 +83,18 @@ def main():
     D.autotune()
     print(f"# setup {time.time() - t_setup:.1f}s V={D.V} E={D.E} Lt={a.Lt} at={at:.5f}",
           file=sys.stderr, flush=True)
+
+    if a.kernels_only:
+        for prec, nm in ((q.F32, "f32"), (q.F64, "f64")):
+            for op in (q.OP_D, q.OP_DDAG):
+                ms = D.time_apply(op, prec, warmup=2, reps=10)
+                rb = 4 if prec == q.F32 else 8
+                gbs = algorithmic_bytes_D(D.V, D.E, a.Lt, rb) / (ms * 1e-3) / 1e9
+                print(f"# D op={op} {nm}: {ms:.4f} ms  {gbs:.0f} GB/s algorithmic  "
+                      f"tile={tile_info(q, D, op, prec)}", file=sys.stderr, flush=True)
+        gbs, ms = copy_bandwidth(q, 1 << 30, reps=10)
+        print(f"# copy 1 GiB: {ms:.4f} ms  {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+        return
 
     nbytes = D.N * 16
     d_x = q.DeviceArray(nbytes)
@@ -103,6 +132,7 @@ def main():
     b64 = algorithmic_bytes_D(D.V, D.E, a.Lt, 8)
     gbs32 = b32 / (ms32 * 1e-3) / 1e9
     gbs64 = b64 / (ms64 * 1e-3) / 1e9
+    copy_gbs, _ = copy_bandwidth(q)
 
     cpu = None
     if not a.no_cpu:
@@ -128,6 +158,8 @@ def main():
                      "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_stencil<float,6,false,EpiStore> (D, FP32)",
                      "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5),
+                     "tile": tile_info(q, D, q.OP_D, q.F32),
+                     "copy_kernel_GBs": round(copy_gbs, 1),
                      "fp64_D": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBS, 4),
                                 "bytes_per_launch": b64, "ms_per_launch": round(ms64, 5)}},
         "cpu_baseline": cpu,

@@ -192,6 +192,12 @@ int qed3_autotune(qed3_ctx* ctx);
 /* Average device time in ms per launch of op (QED3_OP_D / QED3_OP_DDAG) at prec over
  * `reps` launches, after `warmup` untimed ones. */
 int qed3_time_apply(qed3_ctx* ctx, int op, int prec, int warmup, int reps, double* ms);
+/* Average ms per launch of a plain device copy of `bytes` (read + write): the reference
+ * point for HBM bandwidth and for calibrating the FETCH_SIZE/WRITE_SIZE counters. */
+int qed3_time_copy(size_t bytes, int reps, double* ms);
+/* Launch shape the autotuner chose for op (QED3_OP_D / QED3_OP_DDAG) at prec:
+ * out[0] threads per block, out[1] time slices per thread, out[2] XCD-aware order. */
+int qed3_tile_info(const qed3_ctx* ctx, int op, int prec, int* out);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,8 @@ SIGNATURES = {
     "qed3_hmc_trajectory": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_int, C.c_void_p]),
     "qed3_autotune": (C.c_int, [C.c_void_p]),
     "qed3_time_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]),
+    "qed3_time_copy": (C.c_int, [C.c_size_t, C.c_int, c_double_p]),
+    "qed3_tile_info": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p]),
 }
 
 _lib = None

@@ -386,6 +386,24 @@ int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_r
   });
 }
 
+int qed3_time_copy(size_t bytes, int reps, double* ms) {
+  if (!ms || bytes < 16) return fail(QED3_ERR_ARG, "bad argument");
+  return guarded([&]() -> int {
+    *ms = qed3::time_copy(bytes, reps);
+    return QED3_OK;
+  });
+}
+
+int qed3_tile_info(const qed3_ctx* ctx, int op, int prec, int* out) {
+  if (!ctx || !out) return fail(QED3_ERR_ARG, "null argument");
+  const bool dag = op == QED3_OP_DDAG;
+  const qed3::StencilTile t = prec == QED3_PREC_F32 ? ctx->D->tile<float>(dag) : ctx->D->tile<double>(dag);
+  out[0] = t.block;
+  out[1] = t.ts;
+  out[2] = t.remap;
+  return QED3_OK;
+}
+
 int qed3_autotune(qed3_ctx* ctx) {
   if (!ctx) return fail(QED3_ERR_ARG, "null context");
   return guarded([&]() -> int {

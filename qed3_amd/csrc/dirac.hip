@@ -120,6 +120,40 @@ T* dupload(const std::vector<S>& v, hipStream_t st) {
 
 size_t reduce_capacity() { return kReduceCap; }
 
+namespace {
+// Plain streaming copy: the calibration kernel for HBM counters and achievable bandwidth.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b,
+                                              size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+}  // namespace
+
+double time_copy(size_t bytes, int reps) {
+  const size_t n = bytes / sizeof(float4);
+  float4 *a = nullptr, *b = nullptr;
+  QED3_HIP(hipMalloc(&a, n * sizeof(float4)));
+  QED3_HIP(hipMalloc(&b, n * sizeof(float4)));
+  QED3_HIP(hipMemset(a, 0, n * sizeof(float4)));
+  hipEvent_t e0, e1;
+  QED3_HIP(hipEventCreate(&e0));
+  QED3_HIP(hipEventCreate(&e1));
+  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 256 * 1024));
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+  QED3_HIP(hipEventRecord(e0, 0));
+  for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+  QED3_HIP(hipEventRecord(e1, 0));
+  QED3_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QED3_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return double(ms) / std::max(reps, 1);
+}
+
 Dirac::Dirac(const Geometry& g, int lattice, int Lt, int bc, double r, double M5, double at)
     : h_(make_hop_tables(g, lattice, Lt, r, M5, at)), bc_(bc) {
   QED3_HIP(hipStreamCreate(&stream_));
@@ -319,10 +353,11 @@ void Dirac::tune_one(bool dag) {
   StencilTile best = tile<T>(dag);
   float best_ms = 1e30f;
   const int f = std::is_same<T, float>::value ? 1 : 0;
-  for (int block : {128, 256}) {
+  for (int remap : {1, 0})
+  for (int block : {64, 128, 256}) {
     for (int ts : {1, 2, 4, 8, 16}) {
       if (ts > h_.Lt) continue;
-      tile_[f][dag] = StencilTile{block, ts};
+      tile_[f][dag] = StencilTile{block, ts, remap};
       for (int i = 0; i < 2; i++) apply<T>(dag, b, a, false);
       QED3_HIP(hipEventRecord(e0, stream_));
       const int reps = 5;

@@ -34,6 +34,7 @@ struct DiracView {
 struct StencilTile {
   int block = 256;
   int ts = 4;
+  int remap = 1;  // XCD-aware block order
 };
 
 template <typename T>
@@ -130,6 +131,7 @@ class Dirac {
 };
 
 size_t reduce_capacity();  // max blocks a reducing kernel may use
+double time_copy(size_t bytes, int reps);  // ms per launch of a plain device copy
 constexpr size_t kReduceCapacity = size_t(1) << 20;
 
 }  // namespace qed3

@@ -23,14 +23,39 @@
 
 namespace qed3 {
 
+// Block -> (site range, time tile).  Workgroups are dispatched round-robin over the 8
+// XCDs; with `remap` the blocks landing on one XCD cover one contiguous eighth of the
+// (Hilbert-ordered) site ranges for every time tile, so that XCD's L2 holds that eighth
+// of the hop tables and the spatial halos stay inside it except at its two ends.
+struct TileMap {
+  int nxr;      // site ranges of blockDim.x sites
+  int nxr_pad;  // nxr rounded up to a multiple of 8
+  int nst;      // time tiles
+  int remap;
+};
+
+__device__ __forceinline__ void tile_of(const TileMap& m, int b, int& xr, int& st) {
+  if (m.remap) {
+    const int per = m.nxr_pad >> 3;
+    const int j = b >> 3;
+    st = j / per;
+    xr = (b & 7) * per + (j - st * per);
+  } else {
+    st = b / m.nxr_pad;
+    xr = b - st * m.nxr_pad;
+  }
+}
+
 template <typename T, int NS, bool DAG, class Epi>
 __global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* __restrict__ in,
-                                                  Epi epi, int ts, GridReduce gr) {
+                                                  Epi epi, int ts, TileMap tm, GridReduce gr) {
   if (epi.skip()) return;  // uniform: solver already finished
   using C = cplx_t<T>;
   const int V = d.V, Lt = d.Lt;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s0 = blockIdx.y * ts;
+  int xr, st;
+  tile_of(tm, blockIdx.x, xr, st);
+  const int x = (xr < tm.nxr) ? xr * blockDim.x + threadIdx.x : V;
+  const int s0 = st * ts;
   double2 acc = {0.0, 0.0};
   epi.begin();
   if (x < V && s0 < Lt) {
@@ -124,13 +149,19 @@ __device__ __forceinline__ double norm2(const Spin<T>& v) {
 template <typename T, bool DAG, class Epi>
 inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in, const Epi& epi,
                            StencilTile t, GridReduce gr, hipStream_t st) {
-  const dim3 grid((v.V + t.block - 1) / t.block, (v.Lt + t.ts - 1) / t.ts);
-  if (Epi::kReduce && size_t(grid.x) * grid.y > kReduceCapacity)
+  TileMap tm;
+  tm.nxr = (v.V + t.block - 1) / t.block;
+  tm.nxr_pad = (tm.nxr + 7) / 8 * 8;
+  tm.nst = (v.Lt + t.ts - 1) / t.ts;
+  tm.remap = t.remap;
+  const size_t nblk = size_t(tm.nxr_pad) * tm.nst;
+  if (Epi::kReduce && nblk > kReduceCapacity)
     throw std::invalid_argument("stencil grid exceeds the reduction scratch");
+  const dim3 grid{static_cast<unsigned>(nblk), 1u, 1u};
   if (max_nn == 6)
-    hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, gr);
+    hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
   else
-    hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, gr);
+    hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
 }
 
 }  // namespace qed3

@@ -41,6 +41,11 @@ def test_hmc_trajectory_matches_oracle(gpu, lattice, compact, npf):
     assert abs(rep["dH"] - ref["dH"]) < 1e-9 * max(1.0, abs(ref["dH"]))
     assert rel(u, ref["u"]) < 1e-10 and rel(t, ref["t"]) < 1e-10
     assert rep["cg_solves"] == npf * (1 + 4)
+    # same accept decision from the same master stream (no_reject only forces the field)
+    H2 = q.HMC(Dq, gsq, compact=compact, npf=npf, tau=0.5, nsteps=4, nsteps_inner=3, tol=1e-12,
+               solver=q.CG_F64, seed=12)
+    rep2 = H2.trajectory(sp.copy(), tp.copy())
+    assert bool(rep2["accept"]) == ref["accept"]
 
 
 @pytest.mark.gpu
