@@ -47,9 +47,9 @@ def copy_bandwidth(q, nbytes=1 << 30, reps=20):
 
 def tile_info(q, D, op, prec):
     import ctypes as C
-    out = (C.c_int * 3)()
+    out = (C.c_int * 4)()
     q.check(q.load().qed3_tile_info(D._h, op, prec, out))
-    return {"block": out[0], "ts": out[1], "xcd_remap": out[2]}
+    return {"block": out[0], "ts": out[1], "xcd_remap": out[2], "lds": out[3]}
 
 
 def mean_ell(g, lattice):

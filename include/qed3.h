@@ -196,8 +196,12 @@ int qed3_time_apply(qed3_ctx* ctx, int op, int prec, int warmup, int reps, doubl
  * point for HBM bandwidth and for calibrating the FETCH_SIZE/WRITE_SIZE counters. */
 int qed3_time_copy(size_t bytes, int reps, double* ms);
 /* Launch shape the autotuner chose for op (QED3_OP_D / QED3_OP_DDAG) at prec:
- * out[0] threads per block, out[1] time slices per thread, out[2] XCD-aware order. */
+ * out[0] threads per block, out[1] time slices per thread, out[2] XCD-aware order,
+ * out[3] LDS-staged tile. */
 int qed3_tile_info(const qed3_ctx* ctx, int op, int prec, int* out);
+/* Fix the launch shape of op at prec: in[0] threads per block (64/128/256), in[1] time
+ * slices per thread, in[2] XCD-aware order (0/1), in[3] LDS-staged tile (0/1). */
+int qed3_set_tile(qed3_ctx* ctx, int op, int prec, const int* in);
 
 #ifdef __cplusplus
 }

@@ -788,6 +788,195 @@ struct HMCState {
   }
 };
 
+// ---------------------------------------------------------------------------- learned net
+// FP64 restatement of the gauge-covariant linear message-passing preconditioner (a new
+// component, not in the reference).  Parameters (flat, interleaved complex):
+//   v[5][C]                 input head:   h1_c   = sum_k v_k[c] F_k(r)
+//   W[L-2][5][C][C]         middle layers: h^l_c = sum_k sum_d W^l_k[c][d] F_k(h^{l-1})_d
+//   u[5][C]                 output head:  out    = sum_k sum_c u_k[c] F_k(h^{L-1})_c
+// Fields h[B][Lt][V][2][Ch] complex.  The five feature maps of a field h at (b,s,x), spin a:
+//   F0 = h(x)_a
+//   F1 = sum_y tau_xy,a h(y)_a,  tau_xy,a = e^{i theta_s(x,y)} kappa_xy/2 e^{-+ i omega_xy/2}
+//   F2: F2_0 = sum_y e^{-i alpha_xy} tau_xy,1 h(y)_1,  F2_1 = sum_y e^{+i alpha_xy} tau_xy,0 h(y)_0
+//   F3 = b+ e^{i theta^t_s(x)} h(s+1,x),  F4 = b- e^{-i theta^t_{s-1}(x)} h(s-1,x)
+// (b+ / b- carry the time boundary sign at s = Lt-1 / s = 0.)
+struct Net {
+  const Dirac* D;
+  int C, L, B;
+  std::vector<Complex> v, W, u;
+  size_t V, Lt;
+  Complex tau(int s, int x, int y, int a) const {
+    const Lattice& Lx = *D->lat;
+    const int il = Lx.map2il.at({x, y});
+    const double om = Lx.omega.at({x, y});
+    return std::exp(I * D->sp(s, x, y)) * (0.5 * Lx.kappa[il]) *
+           std::exp((a == 0 ? -1.0 : 1.0) * I * 0.5 * om);
+  }
+  double bcs() const { return D->bc == 1 ? 1.0 : -1.0; }
+  size_t idx(int ch, int b, int s, int x, int a, int c) const {
+    return (((size_t(b) * Lt + s) * V + x) * 2 + a) * ch + c;
+  }
+  void features(const std::vector<Complex>& h, int ch, std::vector<Complex> F[5]) const {
+    for (int k = 0; k < 5; k++) F[k].assign(h.size(), 0.0);
+    const Lattice& Lx = *D->lat;
+    for (int b = 0; b < B; b++)
+      for (int s = 0; s < int(Lt); s++)
+        for (int x = 0; x < int(V); x++) {
+          const int sp = (s + 1) % int(Lt), sm = (s - 1 + int(Lt)) % int(Lt);
+          const double bp = (s == int(Lt) - 1) ? bcs() : 1.0, bm = (s == 0) ? bcs() : 1.0;
+          const Complex up = std::exp(I * D->tp(s, x)), um = std::conj(std::exp(I * D->tp(s - 1, x)));
+          for (int a = 0; a < 2; a++)
+            for (int c = 0; c < ch; c++) {
+              F[0][idx(ch, b, s, x, a, c)] = h[idx(ch, b, s, x, a, c)];
+              F[3][idx(ch, b, s, x, a, c)] = bp * up * h[idx(ch, b, sp, x, a, c)];
+              F[4][idx(ch, b, s, x, a, c)] = bm * um * h[idx(ch, b, sm, x, a, c)];
+            }
+          for (int y : Lx.nns[x]) {
+            const double al = Lx.alpha.at({x, y});
+            const Complex t0 = tau(s, x, y, 0), t1 = tau(s, x, y, 1);
+            for (int c = 0; c < ch; c++) {
+              F[1][idx(ch, b, s, x, 0, c)] += t0 * h[idx(ch, b, s, y, 0, c)];
+              F[1][idx(ch, b, s, x, 1, c)] += t1 * h[idx(ch, b, s, y, 1, c)];
+              F[2][idx(ch, b, s, x, 0, c)] += std::exp(-I * al) * t1 * h[idx(ch, b, s, y, 1, c)];
+              F[2][idx(ch, b, s, x, 1, c)] += std::exp(I * al) * t0 * h[idx(ch, b, s, y, 0, c)];
+            }
+          }
+        }
+  }
+  // g += sum_k F_k^dagger G[k]
+  void features_adjoint(const std::vector<Complex> G[5], int ch, std::vector<Complex>& g) const {
+    const Lattice& Lx = *D->lat;
+    for (int b = 0; b < B; b++)
+      for (int s = 0; s < int(Lt); s++)
+        for (int x = 0; x < int(V); x++) {
+          const int sp = (s + 1) % int(Lt), sm = (s - 1 + int(Lt)) % int(Lt);
+          const double bp = (s == int(Lt) - 1) ? bcs() : 1.0, bm = (s == 0) ? bcs() : 1.0;
+          const Complex up = std::exp(I * D->tp(s, x)), um = std::conj(std::exp(I * D->tp(s - 1, x)));
+          for (int a = 0; a < 2; a++)
+            for (int c = 0; c < ch; c++) {
+              g[idx(ch, b, s, x, a, c)] += G[0][idx(ch, b, s, x, a, c)];
+              g[idx(ch, b, sp, x, a, c)] += bp * std::conj(up) * G[3][idx(ch, b, s, x, a, c)];
+              g[idx(ch, b, sm, x, a, c)] += bm * std::conj(um) * G[4][idx(ch, b, s, x, a, c)];
+            }
+          for (int y : Lx.nns[x]) {
+            const double al = Lx.alpha.at({x, y});
+            const Complex t0 = tau(s, x, y, 0), t1 = tau(s, x, y, 1);
+            for (int c = 0; c < ch; c++) {
+              g[idx(ch, b, s, y, 0, c)] += std::conj(t0) * G[1][idx(ch, b, s, x, 0, c)];
+              g[idx(ch, b, s, y, 1, c)] += std::conj(t1) * G[1][idx(ch, b, s, x, 1, c)];
+              g[idx(ch, b, s, y, 1, c)] += std::conj(std::exp(-I * al) * t1) * G[2][idx(ch, b, s, x, 0, c)];
+              g[idx(ch, b, s, y, 0, c)] += std::conj(std::exp(I * al) * t0) * G[2][idx(ch, b, s, x, 1, c)];
+            }
+          }
+        }
+  }
+  Complex Wk(int l, int k, int c, int d) const { return W[(((size_t(l) * 5 + k) * C) + c) * C + d]; }
+
+  void forward(const Complex* r, Complex* out, std::vector<std::vector<Complex>>& hs) const {
+    const size_t nq = size_t(B) * Lt * V * 2;
+    std::vector<Complex> rr(r, r + nq);
+    std::vector<Complex> F[5];
+    features(rr, 1, F);
+    hs.assign(L - 1, std::vector<Complex>(nq * C, 0.0));  // h^1 .. h^{L-1}
+    for (size_t q = 0; q < nq; q++)
+      for (int c = 0; c < C; c++) {
+        Complex acc = 0.0;
+        for (int k = 0; k < 5; k++) acc += v[k * C + c] * F[k][q];
+        hs[0][q * C + c] = acc;
+      }
+    for (int l = 0; l < L - 2; l++) {
+      features(hs[l], C, F);
+      for (size_t q = 0; q < nq; q++)
+        for (int c = 0; c < C; c++) {
+          Complex acc = 0.0;
+          for (int k = 0; k < 5; k++)
+            for (int d = 0; d < C; d++) acc += Wk(l, k, c, d) * F[k][q * C + d];
+          hs[l + 1][q * C + c] = acc;
+        }
+    }
+    features(hs[L - 2], C, F);
+    for (size_t q = 0; q < nq; q++) {
+      Complex acc = 0.0;
+      for (int k = 0; k < 5; k++)
+        for (int c = 0; c < C; c++) acc += u[k * C + c] * F[k][q * C + c];
+      out[q] = acc;
+    }
+  }
+
+  // loss = (1/B) sum_b |out_b - e_b|^2 / |e_b|^2; gradients as d/dRe + i d/dIm
+  double loss_grad(const Complex* r, const Complex* e, std::vector<Complex>& gv,
+                   std::vector<Complex>& gW, std::vector<Complex>& gu) const {
+    const size_t nq = size_t(B) * Lt * V * 2, nb = size_t(Lt) * V * 2;
+    std::vector<std::vector<Complex>> hs;
+    std::vector<Complex> out(nq);
+    forward(r, out.data(), hs);
+    double loss = 0;
+    std::vector<Complex> gout(nq);
+    for (int b = 0; b < B; b++) {
+      double en = 0, dn = 0;
+      for (size_t i = 0; i < nb; i++) {
+        en += std::norm(e[b * nb + i]);
+        dn += std::norm(out[b * nb + i] - e[b * nb + i]);
+      }
+      loss += dn / en / B;
+      for (size_t i = 0; i < nb; i++) gout[b * nb + i] = 2.0 * (out[b * nb + i] - e[b * nb + i]) / en / double(B);
+    }
+    gv.assign(v.size(), 0.0);
+    gW.assign(W.size(), 0.0);
+    gu.assign(u.size(), 0.0);
+    std::vector<Complex> F[5], G[5];
+    // output head
+    features(hs[L - 2], C, F);
+    std::vector<Complex> g(nq * C, 0.0);
+    for (int k = 0; k < 5; k++) G[k].assign(nq * C, 0.0);
+    for (size_t q = 0; q < nq; q++)
+      for (int k = 0; k < 5; k++)
+        for (int c = 0; c < C; c++) {
+          gu[k * C + c] += gout[q] * std::conj(F[k][q * C + c]);
+          G[k][q * C + c] = std::conj(u[k * C + c]) * gout[q];
+        }
+    features_adjoint(G, C, g);
+    for (int l = L - 3; l >= 0; l--) {
+      features(hs[l], C, F);
+      for (int k = 0; k < 5; k++) G[k].assign(nq * C, 0.0);
+      for (size_t q = 0; q < nq; q++)
+        for (int k = 0; k < 5; k++)
+          for (int c = 0; c < C; c++)
+            for (int d = 0; d < C; d++) {
+              gW[(((size_t(l) * 5 + k) * C) + c) * C + d] += g[q * C + c] * std::conj(F[k][q * C + d]);
+              G[k][q * C + d] += std::conj(Wk(l, k, c, d)) * g[q * C + c];
+            }
+      std::vector<Complex> gin(nq * C, 0.0);
+      features_adjoint(G, C, gin);
+      g.swap(gin);
+    }
+    std::vector<Complex> rr(r, r + nq);
+    features(rr, 1, F);
+    for (size_t q = 0; q < nq; q++)
+      for (int k = 0; k < 5; k++)
+        for (int c = 0; c < C; c++) gv[k * C + c] += g[q * C + c] * std::conj(F[k][q]);
+    return loss;
+  }
+};
+
+size_t net_params(int C, int L) { return 10 * size_t(C) + size_t(L - 2) * 5 * C * C; }
+
+Net make_net(const Dirac* D, int C, int L, int B, const double* params) {
+  Net n;
+  n.D = D;
+  n.C = C;
+  n.L = L;
+  n.B = B;
+  n.V = D->lat->n_sites;
+  n.Lt = D->Nt;
+  const Complex* p = reinterpret_cast<const Complex*>(params);
+  const size_t nw = size_t(L - 2) * 5 * C * C;
+  n.v.assign(p, p + 5 * C);
+  n.W.assign(p + 5 * C, p + 5 * C + nw);
+  n.u.assign(p + 5 * C + nw, p + 10 * C + nw);
+  return n;
+}
+
 }  // namespace
 
 // ============================================================================ C ABI
@@ -951,6 +1140,27 @@ double oracle_gauge_action(void* h, const double* u, const double* t) {
 }
 void oracle_gauge_force(void* h, const double* u, const double* t, double* fs, double* ft) {
   static_cast<GaugeAction*>(h)->force(u, t, fs, ft);
+}
+
+// Learned preconditioner: forward on B fields (interleaved complex) with the Dirac's
+// current gauge field, and loss + parameter gradient against targets e.
+void oracle_mpnn_forward(void* hd, int C, int L, int B, const double* params, const double* r,
+                         double* out) {
+  const Net n = make_net(static_cast<Dirac*>(hd), C, L, B, params);
+  std::vector<std::vector<Complex>> hs;
+  n.forward(reinterpret_cast<const Complex*>(r), reinterpret_cast<Complex*>(out), hs);
+}
+double oracle_mpnn_loss_grad(void* hd, int C, int L, int B, const double* params, const double* r,
+                             const double* e, double* grad) {
+  const Net n = make_net(static_cast<Dirac*>(hd), C, L, B, params);
+  std::vector<Complex> gv, gW, gu;
+  const double loss = n.loss_grad(reinterpret_cast<const Complex*>(r),
+                                  reinterpret_cast<const Complex*>(e), gv, gW, gu);
+  Complex* g = reinterpret_cast<Complex*>(grad);
+  std::copy(gv.begin(), gv.end(), g);
+  std::copy(gW.begin(), gW.end(), g + gv.size());
+  std::copy(gu.begin(), gu.end(), g + gv.size() + gW.size());
+  return loss;
 }
 
 // The k-th output of std::mt19937_64 seeded with `seed` (k >= 1).

@@ -237,6 +237,17 @@ class WilsonDirac:
     def autotune(self):
         check(load().qed3_autotune(self._h))
 
+    def set_tile(self, block=256, ts=4, remap=1, lds=0, ops=(OP_D, OP_DDAG), precs=(F64, F32)):
+        arr = (C.c_int * 4)(block, ts, remap, lds)
+        for op in ops:
+            for prec in precs:
+                check(load().qed3_set_tile(self._h, op, prec, arr))
+
+    def tile(self, op=OP_D, prec=F32):
+        out = (C.c_int * 4)()
+        check(load().qed3_tile_info(self._h, op, prec, out))
+        return dict(block=out[0], ts=out[1], remap=out[2], lds=out[3])
+
     def time_apply(self, op=OP_D, prec=F64, warmup=3, reps=20):
         ms = C.c_double()
         check(load().qed3_time_apply(self._h, op, prec, warmup, reps, C.byref(ms)))

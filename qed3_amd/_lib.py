@@ -74,6 +74,7 @@ SIGNATURES = {
     "qed3_time_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]),
     "qed3_time_copy": (C.c_int, [C.c_size_t, C.c_int, c_double_p]),
     "qed3_tile_info": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p]),
+    "qed3_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p]),
 }
 
 _lib = None

@@ -401,7 +401,20 @@ int qed3_tile_info(const qed3_ctx* ctx, int op, int prec, int* out) {
   out[0] = t.block;
   out[1] = t.ts;
   out[2] = t.remap;
+  out[3] = t.lds;
   return QED3_OK;
+}
+
+int qed3_set_tile(qed3_ctx* ctx, int op, int prec, const int* in) {
+  if (!ctx || !in) return fail(QED3_ERR_ARG, "null argument");
+  if (op != QED3_OP_D && op != QED3_OP_DDAG) return fail(QED3_ERR_ARG, "op must be D or DDAG");
+  if (in[0] != 64 && in[0] != 128 && in[0] != 256) return fail(QED3_ERR_ARG, "block must be 64/128/256");
+  if (in[1] < 1 || in[1] > 64) return fail(QED3_ERR_ARG, "time tile must be in [1,64]");
+  return guarded([&]() -> int {
+    ctx->D->set_tile(prec == QED3_PREC_F32, op == QED3_OP_DDAG,
+                     qed3::StencilTile{in[0], in[1], in[2] ? 1 : 0, in[3] ? 1 : 0});
+    return QED3_OK;
+  });
 }
 
 int qed3_autotune(qed3_ctx* ctx) {

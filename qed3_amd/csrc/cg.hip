@@ -269,13 +269,15 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   const DiracView<T> vin = D.view<T>();
   const DiracView<double> v64 = D.view<double>();
   const int nn = D.tables().max_nn;
+  const HaloView hvf = D.halo_for<T>(false), hvt = D.halo_for<T>(true);
+  const HaloView hv64f = D.halo_for<double>(false), hv64t = D.halo_for<double>(true);
   auto one_iter = [&]() {
-    launch_stencil<T, false>(vin, nn, W.p, EpiK1<T>{W.w, sc}, D.tile<T>(false), gr, st);
-    launch_stencil<T, true>(vin, nn, W.w, EpiK2<T>{W.xin, W.r, W.p, sc, T(0)}, D.tile<T>(true), gr, st);
+    launch_stencil<T, false>(vin, nn, W.p, EpiK1<T>{W.w, sc}, D.tile<T>(false), gr, st, hvf);
+    launch_stencil<T, true>(vin, nn, W.w, EpiK2<T>{W.xin, W.r, W.p, sc, T(0)}, D.tile<T>(true), gr, st, hvt);
     if (kMixed)
       hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 0);
-    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{W.t, sc}, D.tile<double>(false), gr, st);
-    launch_stencil<double, true>(v64, nn, W.t, EpiResid<T>{b, W.r, sc}, D.tile<double>(true), gr, st);
+    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{W.t, sc}, D.tile<double>(false), gr, st, hv64f);
+    launch_stencil<double, true>(v64, nn, W.t, EpiResid<T>{b, W.r, sc}, D.tile<double>(true), gr, st, hv64t);
     hipLaunchKernelGGL(k_pupdate<T>, dim3(nbk), dim3(256), 0, st, W.p, W.r, n, sc);
   };
 

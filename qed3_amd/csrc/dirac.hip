@@ -216,7 +216,60 @@ Dirac::~Dirac() {
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
+  for (auto& kv : halos_) {
+    f(kv.second.nbl);
+    f(kv.second.off);
+    f(kv.second.sites);
+  }
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+HaloView Dirac::halo(int R) {
+  auto it = halos_.find(R);
+  if (it == halos_.end()) {
+    const int V = h_.V, M = h_.max_nn, nr = (V + R - 1) / R;
+    std::vector<unsigned short> nbl(size_t(M) * V);
+    std::vector<int> off(nr + 1, 0), sites;
+    int hmax = 0;
+    for (int r = 0; r < nr; r++) {
+      const int x0 = r * R, x1 = std::min(V, x0 + R);
+      std::vector<int> ext;
+      for (int x = x0; x < x1; x++)
+        for (int j = 0; j < M; j++) {
+          const int y = h_.nbr[size_t(j) * V + x];
+          if (y < x0 || y >= x1) ext.push_back(y);
+        }
+      std::sort(ext.begin(), ext.end());
+      ext.erase(std::unique(ext.begin(), ext.end()), ext.end());
+      off[r] = int(sites.size());
+      for (int x = x0; x < x1; x++)
+        for (int j = 0; j < M; j++) {
+          const int y = h_.nbr[size_t(j) * V + x];
+          int loc;
+          if (y >= x0 && y < x1)
+            loc = y - x0;
+          else
+            loc = R + int(std::lower_bound(ext.begin(), ext.end(), y) - ext.begin());
+          if (loc > 65535) throw std::runtime_error("halo index exceeds 16 bits");
+          nbl[size_t(j) * V + x] = (unsigned short)loc;
+        }
+      sites.insert(sites.end(), ext.begin(), ext.end());
+      hmax = std::max(hmax, int(ext.size()));
+    }
+    off[nr] = int(sites.size());
+    HaloTables t;
+    t.nbl = dupload<unsigned short>(nbl, stream_);
+    t.off = dupload<int>(off, stream_);
+    t.sites = dupload<int>(sites.empty() ? std::vector<int>{0} : sites, stream_);
+    t.hmax = hmax;
+    it = halos_.emplace(R, t).first;
+  }
+  HaloView v;
+  v.nbl = it->second.nbl;
+  v.halo_off = it->second.off;
+  v.halo = it->second.sites;
+  v.hmax = it->second.hmax;
+  return v;
 }
 
 void Dirac::set_gauge(const double* spatial, const double* temporal, bool on_device) {
@@ -267,9 +320,9 @@ void Dirac::apply(bool dag, Spin<T>* out, const Spin<T>* in, bool sync) {
   const DiracView<T> v = view<T>();
   EpiStore<T> e{out};
   if (dag)
-    launch_stencil<T, true>(v, h_.max_nn, in, e, tile<T>(true), reducer(), stream_);
+    launch_stencil<T, true>(v, h_.max_nn, in, e, tile<T>(true), reducer(), stream_, halo_for<T>(true));
   else
-    launch_stencil<T, false>(v, h_.max_nn, in, e, tile<T>(false), reducer(), stream_);
+    launch_stencil<T, false>(v, h_.max_nn, in, e, tile<T>(false), reducer(), stream_, halo_for<T>(false));
   QED3_HIP(hipGetLastError());
   if (sync) QED3_HIP(hipStreamSynchronize(stream_));
 }
@@ -353,11 +406,12 @@ void Dirac::tune_one(bool dag) {
   StencilTile best = tile<T>(dag);
   float best_ms = 1e30f;
   const int f = std::is_same<T, float>::value ? 1 : 0;
+  for (int lds : {0, 1})
   for (int remap : {1, 0})
   for (int block : {64, 128, 256}) {
     for (int ts : {1, 2, 4, 8, 16}) {
       if (ts > h_.Lt) continue;
-      tile_[f][dag] = StencilTile{block, ts, remap};
+      tile_[f][dag] = StencilTile{block, ts, remap, lds};
       for (int i = 0; i < 2; i++) apply<T>(dag, b, a, false);
       QED3_HIP(hipEventRecord(e0, stream_));
       const int reps = 5;

@@ -35,6 +35,17 @@ struct StencilTile {
   int block = 256;
   int ts = 4;
   int remap = 1;  // XCD-aware block order
+  int lds = 0;    // stage the time-slice spinor tile (range + halo) in LDS
+};
+
+// Halo tables of the LDS stencil for one range size R (see k_stencil_lds): per slot the
+// 16-bit index of the neighbour in the block's LDS tile (range sites first, then the
+// range's halo), and the halo site lists of all ranges.
+struct HaloView {
+  const unsigned short* nbl = nullptr;  // [nslot][V]
+  const int* halo_off = nullptr;        // [nranges+1]
+  const int* halo = nullptr;            // halo site ids, grouped by range
+  int hmax = 0;
 };
 
 template <typename T>
@@ -104,8 +115,19 @@ class Dirac {
 
   // scratch for reductions
   GridReduce reducer() const { return {partials_, counter_}; }
+  // LDS halo tables for ranges of R sites (built on first use)
+  HaloView halo(int R);
+  template <typename T>
+  HaloView halo_for(bool dag) {
+    const StencilTile t = tile<T>(dag);
+    return t.lds ? halo(t.block) : HaloView{};
+  }
   // autotune the stencil launch shapes for this lattice (persisted in memory per shape)
   void autotune();
+  void set_tile(bool f32, bool dag, StencilTile t) {
+    tile_[f32 ? 1 : 0][dag ? 1 : 0] = t;
+    tuned_ = true;
+  }
 
  private:
   template <typename T>
@@ -128,6 +150,13 @@ class Dirac {
   bool has_gauge_ = false;
   StencilTile tile_[2][2];  // [f32][dag]
   bool tuned_ = false;
+  struct HaloTables {
+    unsigned short* nbl = nullptr;
+    int* off = nullptr;
+    int* sites = nullptr;
+    int hmax = 0;
+  };
+  std::map<int, HaloTables> halos_;
 };
 
 size_t reduce_capacity();  // max blocks a reducing kernel may use

@@ -435,6 +435,8 @@ HopTables make_hop_tables(const Geometry& g, int lattice, int Lt, double r, doub
   h.lnk.assign(size_t(M) * V, 0);
   h.hopD.assign(size_t(M) * V * 4, 0.0);
   h.hopDdag.assign(size_t(M) * V * 4, 0.0);
+  h.slot_alpha.assign(size_t(M) * V, 0.0);
+  h.slot_alpha_nb.assign(size_t(M) * V, 0.0);
   h.diag.resize(V);
   for (int x = 0; x < V; x++) {
     auto& a = adj[x];
@@ -458,6 +460,8 @@ HopTables make_hop_tables(const Geometry& g, int lattice, int Lt, double r, doub
       const double w = fwd ? om[l] : -om[l];            // omega_xy
       const double axy = fwd ? afwd[l] : abwd[l];       // alpha_xy
       const double ayx = fwd ? abwd[l] : afwd[l];       // alpha_yx
+      h.slot_alpha[sidx] = axy;
+      h.slot_alpha_nb[sidx] = ayx;
       const double C = 0.5 * kap * std::cos(0.5 * w), Sg = 0.5 * kap * std::sin(0.5 * w);
       double* hd = &h.hopD[sidx * 4];
       hd[0] = C;

@@ -69,6 +69,8 @@ struct HopTables {
   // The D^dagger table uses H_yx^dagger, which has the same form with (C,S,a_yx,b_yx).
   std::vector<double> hopD;                // [max_nn][V][4] (C,S,a,b)
   std::vector<double> hopDdag;             // [max_nn][V][4]
+  std::vector<double> slot_alpha;          // [max_nn][V] alpha_xy of slot x->y
+  std::vector<double> slot_alpha_nb;       // [max_nn][V] alpha_yx (the angle at the neighbour)
   std::vector<double> diag;                // c_x = sum_y r kappa_xy/2 + r kappa_t(x) + M5
   std::vector<double> kappa_t;             // V
   std::vector<double> kappa;               // E

@@ -1,0 +1,74 @@
+// Learned preconditioner: a gauge-covariant linear message-passing network on the
+// lattice graph of a Dirac operator (new component; the reference has no ML part).
+//
+// Feature maps of a field h (any number of complex channels, two spin components) at
+// (s,x), spin a -- each commutes with U(1) gauge and local spin-frame rotations:
+//   F0 = h(x)_a
+//   F1 = sum_y tau_xy,a h(y)_a,      tau_xy,a = e^{i theta_s(x,y)} kappa_xy/2 e^{-+i omega_xy/2}
+//   F2 = sum_y (gamma_xy tau_xy h(y))_a  (gamma_xy = cos(alpha_xy) s1 + sin(alpha_xy) s2)
+//   F3 = b+ e^{+i theta^t_s(x)} h(s+1,x),   F4 = b- e^{-i theta^t_{s-1}(x)} h(s-1,x)
+// Network (C = 16 complex channels, L >= 2 layers):
+//   h1   = sum_k v_k (x) F_k(r)                     input head, 1 -> C
+//   h^l  = sum_k W^l_k F_k(h^{l-1}), l = 2..L-1     C -> C channel mixing on BF16 MFMA
+//   out  = sum_k sum_c u_k[c] F_k(h^{L-1})_c        output head, C -> 1
+// Linear in r (so scale-equivariant); used inside flexible CG.  Parameters are stored
+// as FP32 complex masters; the mixing matrices are rounded to BF16 for the MFMA tiles.
+#pragma once
+#include <vector>
+
+#include "dirac.hpp"
+
+namespace qed3 {
+
+constexpr int kNetC = 16;
+
+struct NetParamsView;
+
+class Net {
+ public:
+  Net(Dirac& D, int layers, int seed);
+  ~Net();
+  Net(const Net&) = delete;
+  Net& operator=(const Net&) = delete;
+
+  int layers() const { return L_; }
+  size_t nparams() const { return 10 * kNetC + size_t(L_ - 2) * 5 * kNetC * kNetC; }  // complex
+  void get_params(double* p) const;  // interleaved complex
+  void set_params(const double* p);
+
+  // out = N(r) for B fields of D.points() FP32 spinors each (device)
+  void apply(Spin<float>* out, const Spin<float>* r, int B);
+  // loss = (1/B) sum_b |N(r_b) - e_b|^2/|e_b|^2 and its gradient (d/dRe + i d/dIm)
+  double loss_grad(const Spin<float>* r, const Spin<float>* e, int B, std::vector<double>& grad);
+  // Adam over `npairs` device pairs in mini-batches of B; returns the last epoch's mean loss
+  double train(const Spin<float>* r, const Spin<float>* e, int npairs, int B, int epochs,
+               double lr, std::vector<double>* history);
+
+ private:
+  void upload_params();
+  void ensure(int B, bool training);
+  Dirac& D_;
+  int L_;
+  std::vector<double> p_;  // interleaved complex master parameters
+  // device
+  float* slot_ = nullptr;      // [M][V] float4 (C, S, cos a_xy, sin a_xy)
+  float* slot_nb_ = nullptr;   // [M][V] float2 (cos a_yx, sin a_yx)
+  float* heads_ = nullptr;     // v[5][C], u[5][C] interleaved complex FP32
+  void* mixA_ = nullptr;       // per middle layer: R (32 x 160) bf16
+  void* mixAT_ = nullptr;      // per middle layer: R^T (160 x 32) bf16
+  void* act_ = nullptr;        // activations h1..h^{L-1}: [L-1][B*points][2][32] bf16
+  float* gfield_ = nullptr;    // gradient field [B*points][2][32] FP32 (two buffers)
+  float* gfield2_ = nullptr;
+  float* G_ = nullptr;         // [B*points][5][2][32] FP32
+  Spin<float>* out_ = nullptr;
+  Spin<float>* gout_ = nullptr;
+  double* partial_ = nullptr;  // per-block weight-gradient partials
+  double* red_ = nullptr;
+  int capB_ = 0;
+  bool cap_train_ = false;
+  // Adam state
+  std::vector<double> m_, v_;
+  long step_ = 0;
+};
+
+}  // namespace qed3

@@ -129,6 +129,111 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* 
   if constexpr (Epi::kReduce) grid_reduce(acc, gr, epi);
 }
 
+// LDS-staged variant.  A block owns the site range [x0, x0+R) of one tile of time
+// slices; per slice it stages the range's spinors plus its precomputed spatial halo
+// (the outside neighbours of the range) in LDS with one coalesced load of the range
+// and one gather of the halo, then reads all spatial neighbours from LDS through a
+// 16-bit local index table.  Every spinor is fetched once per block and slice.
+template <typename T, int NS, bool DAG, class Epi>
+__global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, const Spin<T>* __restrict__ in,
+                                                      Epi epi, int ts, TileMap tm, HaloView hv,
+                                                      GridReduce gr) {
+  if (epi.skip()) return;
+  extern __shared__ unsigned char smem_raw[];
+  Spin<T>* tileS = reinterpret_cast<Spin<T>*>(smem_raw);
+  using C = cplx_t<T>;
+  const int V = d.V, Lt = d.Lt, R = blockDim.x;
+  int xr, st;
+  tile_of(tm, blockIdx.x, xr, st);
+  const bool live = xr < tm.nxr;
+  const int x0 = xr * R;
+  const int x = live ? x0 + threadIdx.x : V;
+  const bool act = x < V;
+  const int s0 = st * ts;
+  const int s1 = min(s0 + ts, Lt);
+  const int h0 = live ? hv.halo_off[xr] : 0;
+  const int nh = live ? hv.halo_off[xr + 1] - h0 : 0;
+  double2 acc = {0.0, 0.0};
+  epi.begin();
+  int nb[NS], lk[NS];
+  real4_t<T> hp[NS];
+  T c = 0, bk0 = 0, bk1 = 0, fw0 = 0, fw1 = 0;
+  Spin<T> prev, cur;
+  C uprev;
+  if (act && s0 < Lt) {
+    const real4_t<T>* hop = DAG ? d.hopd : d.hop;
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+      nb[j] = hv.nbl[j * V + x];
+      lk[j] = d.lnk[j * V + x];
+      hp[j] = hop[j * V + x];
+    }
+    c = d.diag[x];
+    const T kh = d.kth[x];
+    const T m0 = kh * (-d.r - T(1)), m1 = kh * (-d.r + T(1));
+    bk0 = DAG ? m1 : m0;
+    bk1 = DAG ? m0 : m1;
+    fw0 = DAG ? m0 : m1;
+    fw1 = DAG ? m1 : m0;
+    const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
+    prev = ld(in + size_t(sm) * V + x);
+    cur = ld(in + size_t(s0) * V + x);
+    uprev = d.uT[size_t(sm) * V + x];
+  }
+  for (int s = s0; s < s1; s++) {
+    const Spin<T>* ins = in + size_t(s) * V;
+    __syncthreads();  // previous slice's LDS reads are done
+    if (act) tileS[threadIdx.x] = cur;
+    for (int k = threadIdx.x; k < nh; k += R) tileS[R + k] = ld(ins + hv.halo[h0 + k]);
+    __syncthreads();
+    if (act) {
+      const int sp = (s == Lt - 1) ? 0 : s + 1;
+      const Spin<T> next = ld(in + size_t(sp) * V + x);
+      const C ucur = d.uT[size_t(s) * V + x];
+      C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
+      const C* uS = d.uS + size_t(s) * d.E;
+#pragma unroll
+      for (int j = 0; j < NS; j++) {
+        const int l = lk[j];
+        C u = uS[l >= 0 ? l : ~l];
+        if (l < 0) u.y = -u.y;
+        const Spin<T> py = tileS[nb[j]];
+        const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
+        const T Cc = hp[j].x, Ss = hp[j].y, A = hp[j].z, B = hp[j].w;
+        const T rC = d.r * Cc, rS = d.r * Ss;
+        o0.x += -rC * p0.x - rS * p0.y + A * p1.x + B * p1.y;
+        o0.y += -rC * p0.y + rS * p0.x + A * p1.y - B * p1.x;
+        o1.x += A * p0.x - B * p0.y - rC * p1.x + rS * p1.y;
+        o1.y += A * p0.y + B * p0.x - rC * p1.y - rS * p1.x;
+      }
+      {
+        const T sg = (s == 0) ? d.bc : T(1);
+        const C q0 = cconjmul<T>(uprev, prev.a), q1 = cconjmul<T>(uprev, prev.b);
+        o0.x += sg * bk0 * q0.x;
+        o0.y += sg * bk0 * q0.y;
+        o1.x += sg * bk1 * q1.x;
+        o1.y += sg * bk1 * q1.y;
+      }
+      {
+        const T sg = (s == Lt - 1) ? d.bc : T(1);
+        const C q0 = cmul<T>(ucur, next.a), q1 = cmul<T>(ucur, next.b);
+        o0.x += sg * fw0 * q0.x;
+        o0.y += sg * fw0 * q0.y;
+        o1.x += sg * fw1 * q1.x;
+        o1.y += sg * fw1 * q1.y;
+      }
+      Spin<T> o;
+      o.a = o0;
+      o.b = o1;
+      epi(size_t(s) * V + x, o, acc);
+      prev = cur;
+      cur = next;
+      uprev = ucur;
+    }
+  }
+  if constexpr (Epi::kReduce) grid_reduce(acc, gr, epi);
+}
+
 // Epilogue: plain store.
 template <typename T>
 struct EpiStore {
@@ -148,7 +253,7 @@ __device__ __forceinline__ double norm2(const Spin<T>& v) {
 
 template <typename T, bool DAG, class Epi>
 inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in, const Epi& epi,
-                           StencilTile t, GridReduce gr, hipStream_t st) {
+                           StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv) {
   TileMap tm;
   tm.nxr = (v.V + t.block - 1) / t.block;
   tm.nxr_pad = (tm.nxr + 7) / 8 * 8;
@@ -158,6 +263,17 @@ inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in,
   if (Epi::kReduce && nblk > kReduceCapacity)
     throw std::invalid_argument("stencil grid exceeds the reduction scratch");
   const dim3 grid{static_cast<unsigned>(nblk), 1u, 1u};
+  if (t.lds) {
+    if (!hv.nbl) throw std::logic_error("LDS stencil launched without halo tables");
+    const size_t smem = size_t(t.block + hv.hmax) * sizeof(Spin<T>);
+    if (max_nn == 6)
+      hipLaunchKernelGGL((k_stencil_lds<T, 6, DAG, Epi>), grid, dim3(t.block), smem, st, v, in, epi,
+                         t.ts, tm, hv, gr);
+    else
+      hipLaunchKernelGGL((k_stencil_lds<T, 3, DAG, Epi>), grid, dim3(t.block), smem, st, v, in, epi,
+                         t.ts, tm, hv, gr);
+    return;
+  }
   if (max_nn == 6)
     hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
   else

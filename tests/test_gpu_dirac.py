@@ -41,6 +41,31 @@ def test_dirac_f32_matches_oracle(gpu, lattice):
         assert rel(got, ref) < 2e-6, (op, rel(got, ref))
 
 
+TILES = [dict(block=b, ts=t, remap=r, lds=l) for (b, t, r, l) in
+         [(64, 1, 0, 0), (128, 3, 1, 0), (256, 8, 1, 0), (64, 1, 0, 1), (128, 2, 1, 1),
+          (256, 5, 0, 1)]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+@pytest.mark.parametrize("tile", TILES, ids=lambda t: "b{block}t{ts}r{remap}l{lds}".format(**t))
+def test_every_launch_shape_matches_oracle(gpu, lattice, tile):
+    """Every autotuner candidate (block size, time tile, XCD order, LDS staging)."""
+    n, Lt = 8, 7
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.1)
+    sp, tp = links(lattice, n, Lt, seed=29, kappa=1.0)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.1, at=at)
+    Dq.update(sp, tp)
+    Dq.set_tile(**tile)
+    v = source(lattice, n, Lt, seed=5)
+    for op in (q.OP_D, q.OP_DDAG):
+        assert rel(Dq.apply(op, v, q.F64), Do.apply(op, v)) < 1e-14
+        assert rel(Dq.apply(op, v.astype(np.complex64), q.F32), Do.apply(op, v)) < 2e-6
+    x, rep = Dq.solve(Do.apply(1, v), tol=1e-11, solver=q.CG_MIXED)
+    assert rep.status == 0 and rep.true_resid <= 1e-11
+
+
 @pytest.mark.gpu
 def test_periodic_bc_matches_oracle(gpu):
     lattice, n, Lt = 0, 2, 4
