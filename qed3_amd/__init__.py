@@ -254,6 +254,90 @@ class WilsonDirac:
         return ms.value
 
 
+def batch_solve(ops, xs, bs, tol=1e-10, maxiter=100000, delta=0.1, check_every=0,
+                raise_on_fail=True):
+    """Batched mixed CG over WilsonDirac systems of any shapes; xs/bs DeviceArrays."""
+    n = len(ops)
+    ctxs = (C.c_void_p * n)(*[o._h for o in ops])
+    dx = (C.c_void_p * n)(*[x.ptr for x in xs])
+    db = (C.c_void_p * n)(*[b.ptr for b in bs])
+    opts = CGOpts(tol, maxiter, CG_MIXED, delta, check_every)
+    reps = (CGReport * n)()
+    code = load().qed3_batch_cg(ctxs, n, dx, db, C.byref(opts), reps)
+    if code != 0 and raise_on_fail:
+        check(code)
+    return list(reps)
+
+
+class Net:
+    """Learned gauge-covariant preconditioner on the lattice of a WilsonDirac."""
+
+    def __init__(self, D: WilsonDirac, layers=4, seed=1):
+        self.D = D
+        self._h = C.c_void_p()
+        check(load().qed3_net_create(D._h, layers, seed, C.byref(self._h)))
+        n = C.c_int()
+        check(load().qed3_net_nparams(self._h, C.byref(n)))
+        self.nparams = n.value
+        self.layers = layers
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().qed3_net_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.zeros(self.nparams, np.complex128)
+        check(load().qed3_net_get_params(self._h, _dp(p.view(np.float64))))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.complex128)
+        assert p.size == self.nparams
+        check(load().qed3_net_set_params(self._h, _dp(p.view(np.float64))))
+
+    def apply(self, r):
+        """N(r) for a host array of shape [B, N] (or [N]) complex."""
+        r = np.ascontiguousarray(r, np.complex64)
+        B = 1 if r.ndim == 1 else r.shape[0]
+        d_in = DeviceArray.from_numpy(r)
+        d_out = DeviceArray(r.nbytes)
+        check(load().qed3_net_apply(self._h, d_out.ptr, d_in.ptr, B))
+        return d_out.to_numpy(np.complex64).reshape(r.shape)
+
+    def loss_grad(self, r, e):
+        r = np.ascontiguousarray(r, np.complex64)
+        e = np.ascontiguousarray(e, np.complex64)
+        B = r.shape[0]
+        d_r, d_e = DeviceArray.from_numpy(r), DeviceArray.from_numpy(e)
+        loss = C.c_double()
+        g = np.zeros(self.nparams, np.complex128)
+        check(load().qed3_net_loss_grad(self._h, d_r.ptr, d_e.ptr, B, C.byref(loss),
+                                        _dp(g.view(np.float64))))
+        return loss.value, g
+
+    def train_device(self, d_r, d_e, npairs, batch=4, epochs=10, lr=1e-2):
+        hist = np.zeros(epochs)
+        check(load().qed3_net_train(self._h, d_r.ptr, d_e.ptr, npairs, batch, epochs, lr,
+                                    _dp(hist)))
+        return hist
+
+    def fcg(self, d_x, d_b, tol=1e-10, maxiter=20000, delta=0.1, raise_on_fail=True):
+        opts = CGOpts(tol, maxiter, CG_MIXED, delta, 0)
+        rep = CGReport()
+        pa, fb = C.c_int(), C.c_int()
+        code = load().qed3_fcg(self.D._h, self._h, d_x.ptr, d_b.ptr, C.byref(opts), C.byref(rep),
+                               C.byref(pa), C.byref(fb))
+        if code != 0 and raise_on_fail:
+            check(code)
+        return rep, pa.value, fb.value
+
+
 class HMC:
     """HMC2 + ExplicitLeapfrogML over a WilsonDirac (host FP64 H and accept step)."""
 

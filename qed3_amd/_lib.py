@@ -64,6 +64,20 @@ SIGNATURES = {
     "qed3_cg": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
                           C.POINTER(CGReport)]),
     "qed3_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
+    "qed3_batch_cg": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p),
+                                C.POINTER(C.c_void_p), C.POINTER(CGOpts), C.POINTER(CGReport)]),
+    "qed3_net_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "qed3_net_destroy": (None, [C.c_void_p]),
+    "qed3_net_nparams": (C.c_int, [C.c_void_p, c_int_p]),
+    "qed3_net_get_params": (C.c_int, [C.c_void_p, c_double_p]),
+    "qed3_net_set_params": (C.c_int, [C.c_void_p, c_double_p]),
+    "qed3_net_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "qed3_net_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p,
+                                     c_double_p]),
+    "qed3_net_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                 C.c_double, c_double_p]),
+    "qed3_fcg": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
+                           C.POINTER(CGReport), c_int_p, c_int_p]),
     "qed3_pf_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "qed3_gauge_action": (C.c_int, [C.c_void_p, C.c_double, C.c_int, c_double_p, c_double_p,
                                     c_double_p, c_double_p, c_double_p]),

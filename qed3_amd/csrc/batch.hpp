@@ -1,0 +1,18 @@
+// Batched CG: many independent systems D_k^dag D_k x_k = b_k of different lattices
+// (simplicial/dual, refinement n, Lt) and masses advanced together, one launch per CG
+// phase for the whole batch.  The blocks of a launch are laid out system after system;
+// each block finds its system, runs that system's stencil tile (time tile from the
+// system's autotuned FP32 shape) and reduces into that system's scalars, so systems
+// converge independently (a finished system's blocks exit at once).  FP32 inner
+// iteration with FP64 reliable updates and true-residual convergence, as cg_solve.
+#pragma once
+#include <vector>
+
+#include "cg.hpp"
+
+namespace qed3 {
+
+std::vector<CGResult> batch_cg(const std::vector<Dirac*>& Ds, const std::vector<Spin<double>*>& xs,
+                               const std::vector<const Spin<double>*>& bs, const CGOptions& o);
+
+}  // namespace qed3

@@ -10,7 +10,10 @@
 #include "cg.hpp"
 #include "dirac.hpp"
 #include "geometry.hpp"
+#include "batch.hpp"
+#include "fcg.hpp"
 #include "hmc.hpp"
+#include "mpnn.hpp"
 #include "rng.hpp"
 
 struct qed3_geom {
@@ -26,6 +29,11 @@ struct qed3_ctx {
 struct qed3_hmc {
   qed3_ctx* ctx;
   std::unique_ptr<qed3::HMC> hmc;
+};
+
+struct qed3_net {
+  qed3_ctx* ctx;
+  std::unique_ptr<qed3::Net> net;
 };
 
 namespace {
@@ -309,6 +317,145 @@ int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
     }
     if (r.status == 4) return fail(QED3_ERR_NAN, "CG breakdown (non-finite or zero curvature)");
     if (r.status != 0) return fail(QED3_ERR_NOCONV, "CG did not reach the tolerance");
+    return QED3_OK;
+  });
+}
+
+int qed3_net_create(qed3_ctx* ctx, int layers, int seed, qed3_net** out) {
+  if (!ctx || !out) return fail(QED3_ERR_ARG, "null argument");
+  if (layers < 2 || layers > 16) return fail(QED3_ERR_ARG, "layers must be in [2,16]");
+  return guarded([&]() -> int {
+    auto n = std::make_unique<qed3_net>();
+    n->ctx = ctx;
+    n->net = std::make_unique<qed3::Net>(*ctx->D, layers, seed);
+    *out = n.release();
+    return QED3_OK;
+  });
+}
+
+void qed3_net_destroy(qed3_net* net) { delete net; }
+
+int qed3_net_nparams(const qed3_net* net, int* n) {
+  if (!net || !n) return fail(QED3_ERR_ARG, "null argument");
+  *n = int(net->net->nparams());
+  return QED3_OK;
+}
+
+int qed3_net_get_params(const qed3_net* net, double* p) {
+  if (!net || !p) return fail(QED3_ERR_ARG, "null argument");
+  net->net->get_params(p);
+  return QED3_OK;
+}
+
+int qed3_net_set_params(qed3_net* net, const double* p) {
+  if (!net || !p) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    net->net->set_params(p);
+    return QED3_OK;
+  });
+}
+
+int qed3_net_apply(qed3_net* net, void* d_out, const void* d_in, int batch) {
+  if (!net || !d_out || !d_in || batch < 1) return fail(QED3_ERR_ARG, "bad argument");
+  if (!net->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    net->net->apply(static_cast<qed3::Spin<float>*>(d_out), static_cast<const qed3::Spin<float>*>(d_in), batch);
+    QED3_HIP(hipStreamSynchronize(net->ctx->D->stream()));
+    return QED3_OK;
+  });
+}
+
+int qed3_net_loss_grad(qed3_net* net, const void* d_r, const void* d_e, int batch, double* loss,
+                       double* grad) {
+  if (!net || !d_r || !d_e || batch < 1) return fail(QED3_ERR_ARG, "bad argument");
+  if (!net->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    std::vector<double> g;
+    const double l = net->net->loss_grad(static_cast<const qed3::Spin<float>*>(d_r),
+                                         static_cast<const qed3::Spin<float>*>(d_e), batch, g);
+    if (loss) *loss = l;
+    if (grad) std::copy(g.begin(), g.end(), grad);
+    return QED3_OK;
+  });
+}
+
+int qed3_net_train(qed3_net* net, const void* d_r, const void* d_e, int npairs, int batch,
+                   int epochs, double lr, double* history) {
+  if (!net || !d_r || !d_e || npairs < 1 || batch < 1 || batch > npairs || epochs < 1)
+    return fail(QED3_ERR_ARG, "bad argument");
+  if (!net->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    std::vector<double> h;
+    net->net->train(static_cast<const qed3::Spin<float>*>(d_r), static_cast<const qed3::Spin<float>*>(d_e),
+                    npairs, batch, epochs, lr, &h);
+    if (history) std::copy(h.begin(), h.end(), history);
+    return QED3_OK;
+  });
+}
+
+int qed3_fcg(qed3_ctx* ctx, qed3_net* net, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+             qed3_cg_report* rep, int* precond_applies, int* used_fallback) {
+  if (!ctx || !net || !d_x || !d_b || !opts) return fail(QED3_ERR_ARG, "null argument");
+  if (net->ctx != ctx) return fail(QED3_ERR_ARG, "net belongs to another context");
+  if (!(opts->tol > 0) || opts->maxiter < 1) return fail(QED3_ERR_ARG, "bad tolerance/maxiter");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    qed3::CGOptions o;
+    o.tol = opts->tol;
+    o.maxiter = opts->maxiter;
+    o.delta = opts->delta > 0 ? opts->delta : 0.1;
+    qed3::FCGStats s;
+    const qed3::CGResult r = qed3::fcg_solve(*ctx->D, *net->net, static_cast<qed3::Spin<double>*>(d_x),
+                                             static_cast<const qed3::Spin<double>*>(d_b), o, &s);
+    if (rep) {
+      rep->status = r.status == 0 ? QED3_OK : QED3_ERR_NOCONV;
+      rep->iterations = r.iterations;
+      rep->reliable_updates = r.reliable_updates;
+      rep->true_resid = r.true_resid;
+      rep->seconds = r.seconds;
+    }
+    if (precond_applies) *precond_applies = s.precond_applies;
+    if (used_fallback) *used_fallback = s.fallback;
+    if (r.status != 0) return fail(QED3_ERR_NOCONV, "FCG did not reach the tolerance");
+    return QED3_OK;
+  });
+}
+
+int qed3_batch_cg(qed3_ctx* const* ctxs, int nsys, void* const* d_x, const void* const* d_b,
+                  const qed3_cg_opts* opts, qed3_cg_report* reps) {
+  if (!ctxs || nsys < 1 || !d_x || !d_b || !opts) return fail(QED3_ERR_ARG, "bad argument");
+  if (!(opts->tol > 0) || opts->maxiter < 1) return fail(QED3_ERR_ARG, "bad tolerance/maxiter");
+  std::vector<qed3::Dirac*> Ds;
+  std::vector<qed3::Spin<double>*> xs;
+  std::vector<const qed3::Spin<double>*> bs;
+  for (int k = 0; k < nsys; k++) {
+    if (!ctxs[k] || !d_x[k] || !d_b[k]) return fail(QED3_ERR_ARG, "null system");
+    if (!ctxs[k]->D->has_gauge()) return fail(QED3_ERR_STATE, "a system has no gauge field");
+    Ds.push_back(ctxs[k]->D.get());
+    xs.push_back(static_cast<qed3::Spin<double>*>(d_x[k]));
+    bs.push_back(static_cast<const qed3::Spin<double>*>(d_b[k]));
+  }
+  return guarded([&]() -> int {
+    qed3::CGOptions o;
+    o.tol = opts->tol;
+    o.maxiter = opts->maxiter;
+    o.mixed = true;
+    o.delta = opts->delta > 0 ? opts->delta : 0.1;
+    o.chunk = opts->check_every;
+    const std::vector<qed3::CGResult> r = qed3::batch_cg(Ds, xs, bs, o);
+    int worst = QED3_OK;
+    for (int k = 0; k < nsys; k++) {
+      const int st = r[k].status == 0 ? QED3_OK : (r[k].status == 4 ? QED3_ERR_NAN : QED3_ERR_NOCONV);
+      if (st != QED3_OK) worst = st;
+      if (reps) {
+        reps[k].status = st;
+        reps[k].iterations = r[k].iterations;
+        reps[k].reliable_updates = r[k].reliable_updates;
+        reps[k].true_resid = r[k].true_resid;
+        reps[k].seconds = r[k].seconds;
+      }
+    }
+    if (worst != QED3_OK) return fail(worst, "a batched system did not converge");
     return QED3_OK;
   });
 }

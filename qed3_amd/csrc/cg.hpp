@@ -29,6 +29,7 @@ struct CGOptions {
   double delta = 0.1;
   int chunk = 0;  // iterations per graph replay (0 = auto)
   bool use_graph = true;
+  int fcg_m = 4;  // directions kept by flexible CG
 };
 
 struct CGResult {

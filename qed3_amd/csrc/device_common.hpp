@@ -124,4 +124,34 @@ __device__ __forceinline__ void grid_reduce(double2 v, GridReduce gr, const Fin&
   }
 }
 
+// Same as grid_reduce for a sub-grid: the caller's block is number lb of nb blocks that
+// share `partials` (nb slots) and `counter`.
+template <class Fin>
+__device__ __forceinline__ void grid_reduce_part(double2 v, double2* partials, unsigned int* counter,
+                                                 unsigned lb, unsigned nb, const Fin& fin) {
+  __shared__ bool is_last_p;
+  const double2 b = block_sum2(v);
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partials[lb].x, b.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&partials[lb].y, b.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    is_last_p = (t == nb - 1);
+  }
+  __syncthreads();
+  if (is_last_p) {
+    __threadfence();
+    double2 s = {0.0, 0.0};
+    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x) {
+      s.x += __hip_atomic_load(&partials[i].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s.y += __hip_atomic_load(&partials[i].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const double2 tot = block_sum2(s);
+    if (threadIdx.x == 0) {
+      fin(tot);
+      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 }  // namespace qed3

@@ -46,18 +46,13 @@ __device__ __forceinline__ void tile_of(const TileMap& m, int b, int& xr, int& s
   }
 }
 
+// The stencil for site x over the time slices [s0, s0+ts), handing every output point
+// to the epilogue (shared by the single-system and the batched kernels).
 template <typename T, int NS, bool DAG, class Epi>
-__global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* __restrict__ in,
-                                                  Epi epi, int ts, TileMap tm, GridReduce gr) {
-  if (epi.skip()) return;  // uniform: solver already finished
+__device__ __forceinline__ void stencil_site(const DiracView<T>& d, const Spin<T>* __restrict__ in,
+                                             const Epi& epi, int x, int s0, int ts, double2& acc) {
   using C = cplx_t<T>;
   const int V = d.V, Lt = d.Lt;
-  int xr, st;
-  tile_of(tm, blockIdx.x, xr, st);
-  const int x = (xr < tm.nxr) ? xr * blockDim.x + threadIdx.x : V;
-  const int s0 = st * ts;
-  double2 acc = {0.0, 0.0};
-  epi.begin();
   if (x < V && s0 < Lt) {
     int nb[NS], lk[NS];
     real4_t<T> hp[NS];
@@ -126,6 +121,18 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* 
       uprev = ucur;
     }
   }
+}
+
+template <typename T, int NS, bool DAG, class Epi>
+__global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* __restrict__ in,
+                                                  Epi epi, int ts, TileMap tm, GridReduce gr) {
+  if (epi.skip()) return;  // uniform: solver already finished
+  int xr, st;
+  tile_of(tm, blockIdx.x, xr, st);
+  const int x = (xr < tm.nxr) ? xr * blockDim.x + threadIdx.x : d.V;
+  double2 acc = {0.0, 0.0};
+  epi.begin();
+  stencil_site<T, NS, DAG>(d, in, epi, x, st * ts, ts, acc);
   if constexpr (Epi::kReduce) grid_reduce(acc, gr, epi);
 }
 
