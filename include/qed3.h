@@ -66,6 +66,17 @@ int qed3_geom_export(const qed3_geom* g, double* sites, int* faces, int* links,
                      int* dual_nns, int* dual_face_offsets, int* dual_faces, double* alpha,
                      double* omega, double* dual_alpha, double* dual_omega);
 
+/* Build a geometry from external tables with the same meaning and sizes as
+ * qed3_geom_export (e.g. the reference's QFE-generated .dat files, read as in
+ * s2n_simp.h:43-200 and dirac_simp.h:12-59).  alpha/omega follow each link's stored
+ * orientation (links[k] = (i,j): alpha[2k] = alpha_ij, alpha[2k+1] = alpha_ji). */
+int qed3_geom_import(int n_refine, int V, int E, int F, const double* sites, const int* faces,
+                     const int* links, const int* nn_offsets, const int* nns,
+                     const double* dual_sites, const int* dual_links, const int* dual_nns,
+                     const int* dual_face_offsets, const int* dual_faces, const double* alpha,
+                     const double* omega, const double* dual_alpha, const double* dual_omega,
+                     qed3_geom** out);
+
 /* ---- seeded synthetic inputs (host) ------------------------------------------------
  * The reference's RNG scheme (src/both_3d/rng.h:91-128): a master std::mt19937_64
  * seeded with `seed` reseeds one stream per (t,link), then one per (t,site).
@@ -95,6 +106,8 @@ typedef struct qed3_params {
 
 typedef struct qed3_ctx qed3_ctx;
 int qed3_ctx_create(const qed3_params* p, qed3_ctx** out);
+/* same, on a given geometry (p->n_refine is ignored; the geometry is copied) */
+int qed3_ctx_create_geom(const qed3_params* p, const qed3_geom* g, qed3_ctx** out);
 void qed3_ctx_destroy(qed3_ctx* ctx);
 /* sizes of the chosen lattice (dual: V=20n^2, F=10n^2+2) and <ell> */
 int qed3_ctx_sizes(const qed3_ctx* ctx, int* V, int* E, int* F, double* mean_ell);

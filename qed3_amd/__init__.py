@@ -70,6 +70,21 @@ class Geometry:
         return self.dual_faces[self.dual_face_offsets[x]:self.dual_face_offsets[x + 1]]
 
 
+def import_geometry(g):
+    """qed3_geom handle built from tables with Geometry's attributes (caller destroys)."""
+    arrs = [np.ascontiguousarray(a, dt) for a, dt in (
+        (g.sites, np.float64), (g.faces, np.int32), (g.links, np.int32),
+        (g.nn_offsets, np.int32), (g.nns, np.int32), (g.dual_sites, np.float64),
+        (g.dual_links, np.int32), (g.dual_nns, np.int32), (g.dual_face_offsets, np.int32),
+        (g.dual_faces, np.int32), (g.alpha, np.float64), (g.omega, np.float64),
+        (g.dual_alpha, np.float64), (g.dual_omega, np.float64))]
+    ptrs = [(_dp if a.dtype == np.float64 else _ip)(a) for a in arrs]
+    h = C.c_void_p()
+    check(load().qed3_geom_import(int(g.n), len(g.sites), len(g.links), len(g.faces), *ptrs,
+                                  C.byref(h)))
+    return h
+
+
 def lattice_sizes(lattice: int, n: int):
     """(V, E, F) of the simplicial or dual lattice at refinement n."""
     if lattice == SIMPLICIAL:
@@ -135,14 +150,22 @@ class WilsonDirac:
     """Wilson-Dirac operator on S^2 x time (DiracExt + DiracPf roles) on one GPU."""
 
     def __init__(self, lattice: int, n_refine: int, Lt: int, r: float = 1.0, M5: float = 0.0,
-                 at: float = None, bc: int = ANTIPERIODIC):
+                 at: float = None, bc: int = ANTIPERIODIC, geometry=None):
+        """geometry: optional tables (a Geometry or dat_io.read_dats() result) to use
+        instead of the generated refined icosahedron."""
         lib = load()
-        V, E, F = lattice_sizes(lattice, n_refine)
         if at is None:
             at = 1.0
         self.params = Params(lattice, n_refine, Lt, bc, r, M5, at)
         self._h = C.c_void_p()
-        check(lib.qed3_ctx_create(C.byref(self.params), C.byref(self._h)))
+        if geometry is None:
+            check(lib.qed3_ctx_create(C.byref(self.params), C.byref(self._h)))
+        else:
+            gh = import_geometry(geometry)
+            try:
+                check(lib.qed3_ctx_create_geom(C.byref(self.params), gh, C.byref(self._h)))
+            finally:
+                lib.qed3_geom_destroy(gh)
         Vc, Ec, Fc, ml = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(lib.qed3_ctx_sizes(self._h, C.byref(Vc), C.byref(Ec), C.byref(Fc), C.byref(ml)))
         self.V, self.E, self.F, self.mean_ell = Vc.value, Ec.value, Fc.value, ml.value

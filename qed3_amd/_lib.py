@@ -51,6 +51,11 @@ SIGNATURES = {
     "qed3_geom_export": (C.c_int, [C.c_void_p, c_double_p, c_int_p, c_int_p, c_int_p, c_int_p,
                                    c_double_p, c_int_p, c_int_p, c_int_p, c_int_p, c_double_p,
                                    c_double_p, c_double_p, c_double_p]),
+    "qed3_geom_import": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, c_int_p,
+                                   c_int_p, c_int_p, c_int_p, c_double_p, c_int_p, c_int_p,
+                                   c_int_p, c_int_p, c_double_p, c_double_p, c_double_p,
+                                   c_double_p, C.POINTER(C.c_void_p)]),
+    "qed3_ctx_create_geom": (C.c_int, [C.POINTER(Params), C.c_void_p, C.POINTER(C.c_void_p)]),
     "qed3_random_gauge": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, c_double_p,
                                     c_double_p]),
     "qed3_random_source": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]),

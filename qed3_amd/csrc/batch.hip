@@ -68,13 +68,36 @@ __global__ __launch_bounds__(256) void k_batch_stencil(const Sys* __restrict__ s
       stencil_site<decltype(T{}.a.x), 6, PH == kK2 || PH == kU3>(v, in, epi, x, st * ts, ts, acc);
     else
       stencil_site<decltype(T{}.a.x), 3, PH == kK2 || PH == kU3>(v, in, epi, x, st * ts, ts, acc);
-    if constexpr (decltype(epi)::kReduce)
-      grid_reduce_part(acc, partials + blk0, S.counter, unsigned(lb), unsigned(nb), epi);
+    if constexpr (decltype(epi)::kReduce) store_partial(acc, partials + blk0, unsigned(lb));
+    (void)nb;
   };
   if constexpr (PH == kK1) run(EpiK1<float>{S.w, S.sc}, S.v32, (const Spin<float>*)S.p);
   if constexpr (PH == kK2) run(EpiK2<float>{S.xin, S.r, S.p, S.sc, 0.f}, S.v32, (const Spin<float>*)S.w);
   if constexpr (PH == kU2) run(EpiStoreIfRU{S.t, S.sc}, S.v64, (const Spin<double>*)S.x);
   if constexpr (PH == kU3) run(EpiResid<float>{S.b, S.r, S.sc}, S.v64, (const Spin<double>*)S.t);
+}
+
+// one block per system: sum that system's partials in block order and run its finaliser
+template <int PH>
+__global__ __launch_bounds__(256) void k_batch_finalize(const Sys* __restrict__ sys, const double2* partials) {
+  const Sys& S = sys[blockIdx.x];
+  constexpr bool F64 = (PH == kU2 || PH == kU3);
+  const int blk0 = F64 ? S.blk64 : S.blk32;
+  const int nb = F64 ? S.nblk64 : S.nblk32;
+  auto fin = [&](auto epi) {
+    if (epi.skip()) return;
+    double2 s = {0.0, 0.0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      const double2 p = partials[blk0 + i];
+      s.x += p.x;
+      s.y += p.y;
+    }
+    const double2 tot = block_sum2(s);
+    if (threadIdx.x == 0) epi(tot);
+  };
+  if constexpr (PH == kK1) fin(EpiK1<float>{S.w, S.sc});
+  if constexpr (PH == kK2) fin(EpiK2<float>{S.xin, S.r, S.p, S.sc, 0.f});
+  if constexpr (PH == kU3) fin(EpiResid<float>{S.b, S.r, S.sc});
 }
 
 // streaming phases: 0 init, 1 flush (x64 += xin when an update is due), 2 p update,
@@ -217,11 +240,14 @@ std::vector<CGResult> batch_cg(const std::vector<Dirac*>& Ds, const std::vector<
   QED3_HIP(hipGetLastError());
   auto one_iter = [&]() {
     hipLaunchKernelGGL(k_batch_stencil<kK1>, dim3(tot32), dim3(kBlock), 0, st, dsys, d32, ns, partials);
+    hipLaunchKernelGGL(k_batch_finalize<kK1>, dim3(ns), dim3(kBlock), 0, st, dsys, partials);
     hipLaunchKernelGGL(k_batch_stencil<kK2>, dim3(tot32), dim3(kBlock), 0, st, dsys, d32, ns, partials);
+    hipLaunchKernelGGL(k_batch_finalize<kK2>, dim3(ns), dim3(kBlock), 0, st, dsys, partials);
     hipLaunchKernelGGL(k_batch_stream<1>, dim3(tots), dim3(kBlock), 0, st, dsys, dss, ns, partials, tol2,
                        delta2, o.maxiter);
     hipLaunchKernelGGL(k_batch_stencil<kU2>, dim3(tot64), dim3(kBlock), 0, st, dsys, d64, ns, partials);
     hipLaunchKernelGGL(k_batch_stencil<kU3>, dim3(tot64), dim3(kBlock), 0, st, dsys, d64, ns, partials);
+    hipLaunchKernelGGL(k_batch_finalize<kU3>, dim3(ns), dim3(kBlock), 0, st, dsys, partials);
     hipLaunchKernelGGL(k_batch_stream<2>, dim3(tots), dim3(kBlock), 0, st, dsys, dss, ns, partials, tol2,
                        delta2, o.maxiter);
   };

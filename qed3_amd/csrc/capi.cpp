@@ -193,6 +193,56 @@ int qed3_geom_export(const qed3_geom* gg, double* sites, int* faces, int* links,
   return QED3_OK;
 }
 
+int qed3_geom_import(int n_refine, int V, int E, int F, const double* sites, const int* faces,
+                     const int* links, const int* nn_offsets, const int* nns,
+                     const double* dual_sites, const int* dual_links, const int* dual_nns,
+                     const int* dual_face_offsets, const int* dual_faces, const double* alpha,
+                     const double* omega, const double* dual_alpha, const double* dual_omega,
+                     qed3_geom** out) {
+  if (!out || !sites || !faces || !links || !nn_offsets || !nns || !dual_sites || !dual_links ||
+      !dual_nns || !dual_face_offsets || !dual_faces || !alpha || !omega || !dual_alpha || !dual_omega)
+    return fail(QED3_ERR_ARG, "null table");
+  if (V < 4 || E < 6 || F < 4 || V - E + F != 2) return fail(QED3_ERR_ARG, "V - E + F must be 2");
+  return guarded([&]() -> int {
+    auto g = std::make_unique<qed3_geom>();
+    qed3::Geometry& G = g->g;
+    G.n = n_refine;
+    for (int i = 0; i < V; i++) G.sites.push_back({sites[3 * i], sites[3 * i + 1], sites[3 * i + 2]});
+    for (int f = 0; f < F; f++) {
+      G.faces.push_back({faces[3 * f], faces[3 * f + 1], faces[3 * f + 2]});
+      G.dual_sites.push_back({dual_sites[3 * f], dual_sites[3 * f + 1], dual_sites[3 * f + 2]});
+      G.dual_nns.push_back({dual_nns[3 * f], dual_nns[3 * f + 1], dual_nns[3 * f + 2]});
+    }
+    auto in = [](int v, int hi) {
+      if (v < 0 || v >= hi) throw std::invalid_argument("table index out of range");
+      return v;
+    };
+    for (int l = 0; l < E; l++) {
+      G.links.push_back({in(links[2 * l], V), in(links[2 * l + 1], V)});
+      G.dual_links.push_back({in(dual_links[2 * l], F), in(dual_links[2 * l + 1], F)});
+      G.alpha_fwd.push_back(alpha[2 * l]);
+      G.alpha_bwd.push_back(alpha[2 * l + 1]);
+      G.omega.push_back(omega[l]);
+      G.dual_alpha_fwd.push_back(dual_alpha[2 * l]);
+      G.dual_alpha_bwd.push_back(dual_alpha[2 * l + 1]);
+      G.dual_omega.push_back(dual_omega[l]);
+    }
+    G.nns.resize(V);
+    G.dual_faces.resize(V);
+    for (int x = 0; x < V; x++) {
+      for (int k = nn_offsets[x]; k < nn_offsets[x + 1]; k++) G.nns[x].push_back(in(nns[k], V));
+      for (int k = dual_face_offsets[x]; k < dual_face_offsets[x + 1]; k++)
+        G.dual_faces[x].push_back(in(dual_faces[k], F));
+    }
+    for (auto& f : G.faces)
+      for (int v : f) in(v, V);
+    for (auto& f : G.dual_nns)
+      for (int v : f) in(v, F);
+    *out = g.release();
+    return QED3_OK;
+  });
+}
+
 int qed3_random_gauge(int Lt, int E, int V, int seed, double kappa, double* spatial,
                       double* temporal) {
   if (Lt < 1 || E < 1 || V < 1 || !spatial || !temporal) return fail(QED3_ERR_ARG, "bad argument");
@@ -220,6 +270,22 @@ int qed3_ctx_create(const qed3_params* p, qed3_ctx** out) {
     auto c = std::make_unique<qed3_ctx>();
     c->p = *p;
     c->g = std::make_unique<qed3::Geometry>(qed3::make_geometry(p->n_refine));
+    c->D = std::make_unique<qed3::Dirac>(*c->g, p->lattice, p->Lt, p->bc_time, p->r, p->M5, p->at);
+    *out = c.release();
+    return QED3_OK;
+  });
+}
+
+int qed3_ctx_create_geom(const qed3_params* p, const qed3_geom* g, qed3_ctx** out) {
+  if (!p || !g || !out) return fail(QED3_ERR_ARG, "null argument");
+  if (p->lattice != 0 && p->lattice != 1) return fail(QED3_ERR_ARG, "lattice must be 0 or 1");
+  if (p->Lt < 1) return fail(QED3_ERR_ARG, "Lt must be >= 1");
+  if (p->bc_time != 0 && p->bc_time != 1) return fail(QED3_ERR_ARG, "bc_time must be 0 or 1");
+  if (!(p->at > 0)) return fail(QED3_ERR_ARG, "at must be positive");
+  return guarded([&]() -> int {
+    auto c = std::make_unique<qed3_ctx>();
+    c->p = *p;
+    c->g = std::make_unique<qed3::Geometry>(g->g);
     c->D = std::make_unique<qed3::Dirac>(*c->g, p->lattice, p->Lt, p->bc_time, p->r, p->M5, p->at);
     *out = c.release();
     return QED3_OK;

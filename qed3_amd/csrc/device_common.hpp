@@ -124,6 +124,30 @@ __device__ __forceinline__ void grid_reduce(double2 v, GridReduce gr, const Fin&
   }
 }
 
+// Two-launch reduction for the hot loops: every block stores its partial with a plain
+// store; a one-block finaliser kernel launched next on the stream sums the partials in
+// block order and runs the epilogue's finaliser.  The kernel boundary orders the
+// stores, so no block issues a device-scope fence (on a multi-XCD part such a fence
+// writes back the XCD's L2).
+__device__ __forceinline__ void store_partial(double2 v, double2* partials, unsigned slot) {
+  const double2 b = block_sum2(v);
+  if (threadIdx.x == 0) partials[slot] = b;
+}
+
+template <class Fin>
+__global__ __launch_bounds__(256) void k_finalize(const double2* __restrict__ partials, unsigned n,
+                                                  Fin fin) {
+  if (fin.skip()) return;
+  double2 s = {0.0, 0.0};
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+    const double2 p = partials[i];
+    s.x += p.x;
+    s.y += p.y;
+  }
+  const double2 tot = block_sum2(s);
+  if (threadIdx.x == 0) fin(tot);
+}
+
 // Same as grid_reduce for a sub-grid: the caller's block is number lb of nb blocks that
 // share `partials` (nb slots) and `counter`.
 template <class Fin>

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* 
   double2 acc = {0.0, 0.0};
   epi.begin();
   stencil_site<T, NS, DAG>(d, in, epi, x, st * ts, ts, acc);
-  if constexpr (Epi::kReduce) grid_reduce(acc, gr, epi);
+  if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
 }
 
 // LDS-staged variant.  A block owns the site range [x0, x0+R) of one tile of time
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, const Spin<
       uprev = ucur;
     }
   }
-  if constexpr (Epi::kReduce) grid_reduce(acc, gr, epi);
+  if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
 }
 
 // Epilogue: plain store.
@@ -279,12 +279,13 @@ inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in,
     else
       hipLaunchKernelGGL((k_stencil_lds<T, 3, DAG, Epi>), grid, dim3(t.block), smem, st, v, in, epi,
                          t.ts, tm, hv, gr);
-    return;
-  }
-  if (max_nn == 6)
+  } else if (max_nn == 6) {
     hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
-  else
+  } else {
     hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
+  }
+  if constexpr (Epi::kReduce)
+    hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, unsigned(nblk), epi);
 }
 
 }  // namespace qed3

@@ -90,3 +90,20 @@ def test_adjoint_identity_large(gpu):
     lhs = np.vdot(u, Dq.mult(v))
     rhs = np.vdot(Dq.adj(u), v)
     assert abs(lhs - rhs) / abs(lhs) < 1e-13
+
+
+@pytest.mark.gpu
+def test_imported_dat_geometry_gives_same_operator(gpu, tmp_path):
+    """A context built from .dat tables (the reference's input format) applies the same D."""
+    from qed3_amd import dat_io
+    lattice, n, Lt = 1, 3, 4
+    g = q.Geometry(n)
+    dat_io.write_dats(g, tmp_path)
+    h = dat_io.read_dats(tmp_path, n)
+    sp, tp = links(lattice, n, Lt, seed=3)
+    A = q.WilsonDirac(lattice, n, Lt, M5=0.1, at=0.2)
+    B = q.WilsonDirac(lattice, n, Lt, M5=0.1, at=0.2, geometry=h)
+    A.update(sp, tp)
+    B.update(sp, tp)
+    v = source(lattice, n, Lt, seed=1)
+    assert rel(B.mult(v), A.mult(v)) < 1e-13

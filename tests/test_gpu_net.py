@@ -92,11 +92,14 @@ def make_pairs(Dq, lattice, n, Lt, cfg_seeds, rhs_seeds, ks, betas):
             b = source(lattice, n, Lt, seed=rsd)
             xs, _ = Dq.solve(b, tol=1e-11, solver=q.CG_F64)
             for k in ks:
-                d_b = q.DeviceArray.from_numpy(b)
-                d_x = q.DeviceArray(b.nbytes)
-                Dq.solve_device(d_x, d_b, tol=1e-14, maxiter=k, solver=q.CG_F64,
-                                raise_on_fail=False)
-                xk = d_x.to_numpy(np.complex128)
+                if k == 0:
+                    xk = np.zeros_like(b)  # x0 = 0: r = b, e = x*
+                else:
+                    d_b = q.DeviceArray.from_numpy(b)
+                    d_x = q.DeviceArray(b.nbytes)
+                    Dq.solve_device(d_x, d_b, tol=1e-14, maxiter=k, solver=q.CG_F64,
+                                    raise_on_fail=False)
+                    xk = d_x.to_numpy(np.complex128)
                 ek = xs - xk
                 rk = b - Dq.DHD(xk)
                 s = 1.0 / np.linalg.norm(rk)
