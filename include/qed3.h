@@ -152,25 +152,35 @@ int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
  * message-passing network on the context's lattice (16 complex channels, `layers` >= 2):
  * input head 1->16, (layers-2) BF16-MFMA channel-mixing layers, output head 16->1; its
  * messages are the link parallel transports exp(i theta) Omega, the gamma-weighted
- * transports and the two temporal transports, so the net commutes with U(1) gauge and
- * local spin-frame transformations.  It reads the context's current gauge field.
+ * transports and the two temporal transports (with kappa_t/2), its weights may differ per
+ * spin component, so the net commutes with U(1) gauge and local spin-frame
+ * transformations and can represent D and D^dagger.  It reads the context's current gauge field.
  * Fields are complex float device arrays of `batch` consecutive spinor fields. */
 typedef struct qed3_net qed3_net;
 int qed3_net_create(qed3_ctx* ctx, int layers, int seed, qed3_net** out);
 void qed3_net_destroy(qed3_net* net);
 int qed3_net_nparams(const qed3_net* net, int* n); /* complex parameters */
-/* parameters as interleaved complex doubles: v[5][16], W[layers-2][5][16][16], u[5][16] */
+/* parameters as interleaved complex doubles, per output spin a:
+ * v[2][5][16], W[layers-2][2][5][16][16], u[2][5][16] */
 int qed3_net_get_params(const qed3_net* net, double* p);
 int qed3_net_set_params(qed3_net* net, const double* p);
+/* the fixed skip coefficient c0 of out = N(r) + c0 r; the default 1/B, B a Gershgorin
+ * bound on |D|_2, is set at creation (c0 = 0 leaves the bare network) */
+int qed3_net_get_skip(const qed3_net* net, double* c0);
+int qed3_net_set_skip(qed3_net* net, double c0);
 int qed3_net_apply(qed3_net* net, void* d_out, const void* d_in, int batch);
-/* loss = mean_b |N(r_b) - e_b|^2/|e_b|^2 and d loss/d Re p + i d loss/d Im p (grad: 2n doubles) */
+/* d_out = N^dagger(d_in), the adjoint network (reverse layers, adjoint feature maps) */
+int qed3_net_apply_adjoint(qed3_net* net, void* d_out, const void* d_in, int batch);
+/* loss = mean_b |N(r_b) + c0 r_b - e_b|^2/|e_b|^2 and d loss/d Re p + i d loss/d Im p (grad: 2n doubles) */
 int qed3_net_loss_grad(qed3_net* net, const void* d_r, const void* d_e, int batch, double* loss,
                        double* grad);
 /* Adam over npairs (r, e) device pairs in mini-batches; history[epochs] = mean loss/epoch */
 int qed3_net_train(qed3_net* net, const void* d_r, const void* d_e, int npairs, int batch,
                    int epochs, double lr, double* history);
-/* Flexible CG (FP32 vectors, FP64 reliable updates) preconditioned by net; plain mixed CG
- * takes over on breakdown or stall and the FP64 true residual decides convergence.
+/* Flexible CG (FP32 vectors, FP64 reliable updates) on D^dag D preconditioned by
+ * M = N N^dagger (net trained as an approximate inverse of D, so M is Hermitian positive
+ * semi-definite); plain mixed CG takes over on breakdown or stall and the FP64 true
+ * residual decides convergence.
  * precond_applies / used_fallback may be NULL. */
 int qed3_fcg(qed3_ctx* ctx, qed3_net* net, void* d_x, const void* d_b, const qed3_cg_opts* opts,
              qed3_cg_report* rep, int* precond_applies, int* used_fallback);

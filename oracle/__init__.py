@@ -48,8 +48,9 @@ def load():
             "oracle_gauge_action": (C.c_double, [vp, dp, dp]),
             "oracle_gauge_force": (None, [vp, dp, dp, dp, dp]),
             "oracle_mt64": (C.c_uint64, [C.c_uint64, C.c_long]),
-            "oracle_mpnn_forward": (None, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
-            "oracle_mpnn_loss_grad": (C.c_double, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]),
+            "oracle_mpnn_forward": (None, [vp, C.c_int, C.c_int, C.c_int, dp, C.c_double, dp, dp]),
+            "oracle_mpnn_loss_grad": (C.c_double, [vp, C.c_int, C.c_int, C.c_int, dp, C.c_double, dp, dp,
+                                                                 dp]),
             "oracle_hmc_trajectory": (C.c_double, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_int,
                                                    C.c_int, C.c_double, dp, dp, dp, ip,
                                                    C.POINTER(C.c_long)]),
@@ -176,24 +177,24 @@ class GaugeAction:
         return fs, ft
 
 
-def mpnn_forward(dirac: "Dirac", C_, L_, params, r):
-    """Net output for fields r (shape [B, N] complex) with the Dirac's current links."""
+def mpnn_forward(dirac: "Dirac", C_, L_, params, r, skip=0.0):
+    """Net output N(r) + skip r for fields r (shape [B, N] complex) with the Dirac's links."""
     r = np.ascontiguousarray(r, np.complex128)
     B = r.shape[0]
     p = np.ascontiguousarray(params, np.complex128)
     out = np.zeros_like(r)
-    load().oracle_mpnn_forward(dirac.h, C_, L_, B, _dp(p.view(np.float64)),
+    load().oracle_mpnn_forward(dirac.h, C_, L_, B, _dp(p.view(np.float64)), float(skip),
                                _dp(r.view(np.float64)), _dp(out.view(np.float64)))
     return out
 
 
-def mpnn_loss_grad(dirac: "Dirac", C_, L_, params, r, e):
+def mpnn_loss_grad(dirac: "Dirac", C_, L_, params, r, e, skip=0.0):
     r = np.ascontiguousarray(r, np.complex128)
     e = np.ascontiguousarray(e, np.complex128)
     B = r.shape[0]
     p = np.ascontiguousarray(params, np.complex128)
     g = np.zeros_like(p)
-    loss = load().oracle_mpnn_loss_grad(dirac.h, C_, L_, B, _dp(p.view(np.float64)),
+    loss = load().oracle_mpnn_loss_grad(dirac.h, C_, L_, B, _dp(p.view(np.float64)), float(skip),
                                         _dp(r.view(np.float64)), _dp(e.view(np.float64)),
                                         _dp(g.view(np.float64)))
     return loss, g

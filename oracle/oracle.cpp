@@ -790,19 +790,21 @@ struct HMCState {
 
 // ---------------------------------------------------------------------------- learned net
 // FP64 restatement of the gauge-covariant linear message-passing preconditioner (a new
-// component, not in the reference).  Parameters (flat, interleaved complex):
-//   v[5][C]                 input head:   h1_c   = sum_k v_k[c] F_k(r)
-//   W[L-2][5][C][C]         middle layers: h^l_c = sum_k sum_d W^l_k[c][d] F_k(h^{l-1})_d
-//   u[5][C]                 output head:  out    = sum_k sum_c u_k[c] F_k(h^{L-1})_c
+// component, not in the reference).  Parameters (flat, interleaved complex), each set per
+// output spin a of the feature maps:
+//   v[2][5][C]              input head:   h1_c,a   = sum_k v_a,k[c] F_k(r)_a
+//   W[L-2][2][5][C][C]      middle layers: h^l_c,a = sum_k sum_d W^l_a,k[c][d] F_k(h^{l-1})_d,a
+//   u[2][5][C]              output head:  out_a    = sum_k sum_c u_a,k[c] F_k(h^{L-1})_c,a + skip r_a
 // Fields h[B][Lt][V][2][Ch] complex.  The five feature maps of a field h at (b,s,x), spin a:
 //   F0 = h(x)_a
 //   F1 = sum_y tau_xy,a h(y)_a,  tau_xy,a = e^{i theta_s(x,y)} kappa_xy/2 e^{-+ i omega_xy/2}
 //   F2: F2_0 = sum_y e^{-i alpha_xy} tau_xy,1 h(y)_1,  F2_1 = sum_y e^{+i alpha_xy} tau_xy,0 h(y)_0
-//   F3 = b+ e^{i theta^t_s(x)} h(s+1,x),  F4 = b- e^{-i theta^t_{s-1}(x)} h(s-1,x)
+//   F3 = b+ kappa_t(x)/2 e^{i theta^t_s(x)} h(s+1,x),  F4 = b- kappa_t(x)/2 e^{-i theta^t_{s-1}(x)} h(s-1,x)
 // (b+ / b- carry the time boundary sign at s = Lt-1 / s = 0.)
 struct Net {
   const Dirac* D;
   int C, L, B;
+  double skip = 0;  // out = N(r) + skip r
   std::vector<Complex> v, W, u;
   size_t V, Lt;
   Complex tau(int s, int x, int y, int a) const {
@@ -823,7 +825,8 @@ struct Net {
       for (int s = 0; s < int(Lt); s++)
         for (int x = 0; x < int(V); x++) {
           const int sp = (s + 1) % int(Lt), sm = (s - 1 + int(Lt)) % int(Lt);
-          const double bp = (s == int(Lt) - 1) ? bcs() : 1.0, bm = (s == 0) ? bcs() : 1.0;
+          const double kh = 0.5 * D->kappa_t[x];
+          const double bp = ((s == int(Lt) - 1) ? bcs() : 1.0) * kh, bm = ((s == 0) ? bcs() : 1.0) * kh;
           const Complex up = std::exp(I * D->tp(s, x)), um = std::conj(std::exp(I * D->tp(s - 1, x)));
           for (int a = 0; a < 2; a++)
             for (int c = 0; c < ch; c++) {
@@ -850,7 +853,8 @@ struct Net {
       for (int s = 0; s < int(Lt); s++)
         for (int x = 0; x < int(V); x++) {
           const int sp = (s + 1) % int(Lt), sm = (s - 1 + int(Lt)) % int(Lt);
-          const double bp = (s == int(Lt) - 1) ? bcs() : 1.0, bm = (s == 0) ? bcs() : 1.0;
+          const double kh = 0.5 * D->kappa_t[x];
+          const double bp = ((s == int(Lt) - 1) ? bcs() : 1.0) * kh, bm = ((s == 0) ? bcs() : 1.0) * kh;
           const Complex up = std::exp(I * D->tp(s, x)), um = std::conj(std::exp(I * D->tp(s - 1, x)));
           for (int a = 0; a < 2; a++)
             for (int c = 0; c < ch; c++) {
@@ -870,7 +874,12 @@ struct Net {
           }
         }
   }
-  Complex Wk(int l, int k, int c, int d) const { return W[(((size_t(l) * 5 + k) * C) + c) * C + d]; }
+  // weights per output spin a (the last index of a point q is its spin: a = q % 2)
+  size_t wi(int l, int a, int k, int c, int d) const {
+    return ((((size_t(l) * 2 + a) * 5 + k) * C) + c) * C + d;
+  }
+  Complex Wk(int l, int a, int k, int c, int d) const { return W[wi(l, a, k, c, d)]; }
+  size_t hi(int a, int k, int c) const { return (size_t(a) * 5 + k) * C + c; }
 
   void forward(const Complex* r, Complex* out, std::vector<std::vector<Complex>>& hs) const {
     const size_t nq = size_t(B) * Lt * V * 2;
@@ -881,7 +890,7 @@ struct Net {
     for (size_t q = 0; q < nq; q++)
       for (int c = 0; c < C; c++) {
         Complex acc = 0.0;
-        for (int k = 0; k < 5; k++) acc += v[k * C + c] * F[k][q];
+        for (int k = 0; k < 5; k++) acc += v[hi(q % 2, k, c)] * F[k][q];
         hs[0][q * C + c] = acc;
       }
     for (int l = 0; l < L - 2; l++) {
@@ -890,7 +899,7 @@ struct Net {
         for (int c = 0; c < C; c++) {
           Complex acc = 0.0;
           for (int k = 0; k < 5; k++)
-            for (int d = 0; d < C; d++) acc += Wk(l, k, c, d) * F[k][q * C + d];
+            for (int d = 0; d < C; d++) acc += Wk(l, q % 2, k, c, d) * F[k][q * C + d];
           hs[l + 1][q * C + c] = acc;
         }
     }
@@ -898,8 +907,8 @@ struct Net {
     for (size_t q = 0; q < nq; q++) {
       Complex acc = 0.0;
       for (int k = 0; k < 5; k++)
-        for (int c = 0; c < C; c++) acc += u[k * C + c] * F[k][q * C + c];
-      out[q] = acc;
+        for (int c = 0; c < C; c++) acc += u[hi(q % 2, k, c)] * F[k][q * C + c];
+      out[q] = acc + skip * rr[q];
     }
   }
 
@@ -932,8 +941,8 @@ struct Net {
     for (size_t q = 0; q < nq; q++)
       for (int k = 0; k < 5; k++)
         for (int c = 0; c < C; c++) {
-          gu[k * C + c] += gout[q] * std::conj(F[k][q * C + c]);
-          G[k][q * C + c] = std::conj(u[k * C + c]) * gout[q];
+          gu[hi(q % 2, k, c)] += gout[q] * std::conj(F[k][q * C + c]);
+          G[k][q * C + c] = std::conj(u[hi(q % 2, k, c)]) * gout[q];
         }
     features_adjoint(G, C, g);
     for (int l = L - 3; l >= 0; l--) {
@@ -943,8 +952,8 @@ struct Net {
         for (int k = 0; k < 5; k++)
           for (int c = 0; c < C; c++)
             for (int d = 0; d < C; d++) {
-              gW[(((size_t(l) * 5 + k) * C) + c) * C + d] += g[q * C + c] * std::conj(F[k][q * C + d]);
-              G[k][q * C + d] += std::conj(Wk(l, k, c, d)) * g[q * C + c];
+              gW[wi(l, q % 2, k, c, d)] += g[q * C + c] * std::conj(F[k][q * C + d]);
+              G[k][q * C + d] += std::conj(Wk(l, q % 2, k, c, d)) * g[q * C + c];
             }
       std::vector<Complex> gin(nq * C, 0.0);
       features_adjoint(G, C, gin);
@@ -954,26 +963,27 @@ struct Net {
     features(rr, 1, F);
     for (size_t q = 0; q < nq; q++)
       for (int k = 0; k < 5; k++)
-        for (int c = 0; c < C; c++) gv[k * C + c] += g[q * C + c] * std::conj(F[k][q]);
+        for (int c = 0; c < C; c++) gv[hi(q % 2, k, c)] += g[q * C + c] * std::conj(F[k][q]);
     return loss;
   }
 };
 
-size_t net_params(int C, int L) { return 10 * size_t(C) + size_t(L - 2) * 5 * C * C; }
+size_t net_params(int C, int L) { return 20 * size_t(C) + size_t(L - 2) * 10 * C * C; }
 
-Net make_net(const Dirac* D, int C, int L, int B, const double* params) {
+Net make_net(const Dirac* D, int C, int L, int B, const double* params, double skip) {
   Net n;
   n.D = D;
+  n.skip = skip;
   n.C = C;
   n.L = L;
   n.B = B;
   n.V = D->lat->n_sites;
   n.Lt = D->Nt;
   const Complex* p = reinterpret_cast<const Complex*>(params);
-  const size_t nw = size_t(L - 2) * 5 * C * C;
-  n.v.assign(p, p + 5 * C);
-  n.W.assign(p + 5 * C, p + 5 * C + nw);
-  n.u.assign(p + 5 * C + nw, p + 10 * C + nw);
+  const size_t nw = size_t(L - 2) * 10 * C * C;
+  n.v.assign(p, p + 10 * C);
+  n.W.assign(p + 10 * C, p + 10 * C + nw);
+  n.u.assign(p + 10 * C + nw, p + 20 * C + nw);
   return n;
 }
 
@@ -1144,15 +1154,15 @@ void oracle_gauge_force(void* h, const double* u, const double* t, double* fs, d
 
 // Learned preconditioner: forward on B fields (interleaved complex) with the Dirac's
 // current gauge field, and loss + parameter gradient against targets e.
-void oracle_mpnn_forward(void* hd, int C, int L, int B, const double* params, const double* r,
-                         double* out) {
-  const Net n = make_net(static_cast<Dirac*>(hd), C, L, B, params);
+void oracle_mpnn_forward(void* hd, int C, int L, int B, const double* params, double skip,
+                         const double* r, double* out) {
+  const Net n = make_net(static_cast<Dirac*>(hd), C, L, B, params, skip);
   std::vector<std::vector<Complex>> hs;
   n.forward(reinterpret_cast<const Complex*>(r), reinterpret_cast<Complex*>(out), hs);
 }
-double oracle_mpnn_loss_grad(void* hd, int C, int L, int B, const double* params, const double* r,
-                             const double* e, double* grad) {
-  const Net n = make_net(static_cast<Dirac*>(hd), C, L, B, params);
+double oracle_mpnn_loss_grad(void* hd, int C, int L, int B, const double* params, double skip,
+                             const double* r, const double* e, double* grad) {
+  const Net n = make_net(static_cast<Dirac*>(hd), C, L, B, params, skip);
   std::vector<Complex> gv, gW, gu;
   const double loss = n.loss_grad(reinterpret_cast<const Complex*>(r),
                                   reinterpret_cast<const Complex*>(e), gv, gW, gu);

@@ -333,6 +333,15 @@ class Net:
         check(load().qed3_net_apply(self._h, d_out.ptr, d_in.ptr, B))
         return d_out.to_numpy(np.complex64).reshape(r.shape)
 
+    def apply_adj(self, r):
+        """N^dagger(r) for a host array of shape [B, N] (or [N]) complex."""
+        r = np.ascontiguousarray(r, np.complex64)
+        B = 1 if r.ndim == 1 else r.shape[0]
+        d_in = DeviceArray.from_numpy(r)
+        d_out = DeviceArray(r.nbytes)
+        check(load().qed3_net_apply_adjoint(self._h, d_out.ptr, d_in.ptr, B))
+        return d_out.to_numpy(np.complex64).reshape(r.shape)
+
     def loss_grad(self, r, e):
         r = np.ascontiguousarray(r, np.complex64)
         e = np.ascontiguousarray(e, np.complex64)
@@ -343,6 +352,16 @@ class Net:
         check(load().qed3_net_loss_grad(self._h, d_r.ptr, d_e.ptr, B, C.byref(loss),
                                         _dp(g.view(np.float64))))
         return loss.value, g
+
+    @property
+    def skip(self):
+        c = C.c_double()
+        check(load().qed3_net_get_skip(self._h, C.byref(c)))
+        return c.value
+
+    @skip.setter
+    def skip(self, c0):
+        check(load().qed3_net_set_skip(self._h, float(c0)))
 
     def train_device(self, d_r, d_e, npairs, batch=4, epochs=10, lr=1e-2):
         hist = np.zeros(epochs)

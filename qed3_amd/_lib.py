@@ -76,6 +76,9 @@ SIGNATURES = {
     "qed3_net_nparams": (C.c_int, [C.c_void_p, c_int_p]),
     "qed3_net_get_params": (C.c_int, [C.c_void_p, c_double_p]),
     "qed3_net_set_params": (C.c_int, [C.c_void_p, c_double_p]),
+    "qed3_net_apply_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "qed3_net_get_skip": (C.c_int, [C.c_void_p, c_double_p]),
+    "qed3_net_set_skip": (C.c_int, [C.c_void_p, C.c_double]),
     "qed3_net_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "qed3_net_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p,
                                      c_double_p]),

@@ -2,6 +2,7 @@
 // exceptions into status codes; the text of the last error is kept per thread.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -421,11 +422,35 @@ int qed3_net_set_params(qed3_net* net, const double* p) {
   });
 }
 
+int qed3_net_get_skip(const qed3_net* net, double* c0) {
+  if (!net || !c0) return fail(QED3_ERR_ARG, "null argument");
+  *c0 = net->net->skip();
+  return QED3_OK;
+}
+
+int qed3_net_set_skip(qed3_net* net, double c0) {
+  if (!net || !std::isfinite(c0) || c0 < 0) return fail(QED3_ERR_ARG, "skip must be finite and >= 0");
+  net->net->set_skip(c0);
+  return QED3_OK;
+}
+
 int qed3_net_apply(qed3_net* net, void* d_out, const void* d_in, int batch) {
   if (!net || !d_out || !d_in || batch < 1) return fail(QED3_ERR_ARG, "bad argument");
   if (!net->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
   return guarded([&]() -> int {
     net->net->apply(static_cast<qed3::Spin<float>*>(d_out), static_cast<const qed3::Spin<float>*>(d_in), batch);
+    QED3_HIP(hipStreamSynchronize(net->ctx->D->stream()));
+    return QED3_OK;
+  });
+}
+
+int qed3_net_apply_adjoint(qed3_net* net, void* d_out, const void* d_in, int batch) {
+  if (!net || !d_out || !d_in || batch < 1) return fail(QED3_ERR_ARG, "bad argument");
+  if (d_out == d_in) return fail(QED3_ERR_ARG, "d_out must not alias d_in");
+  if (!net->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    net->net->apply_adj(static_cast<qed3::Spin<float>*>(d_out), static_cast<const qed3::Spin<float>*>(d_in),
+                        batch);
     QED3_HIP(hipStreamSynchronize(net->ctx->D->stream()));
     return QED3_OK;
   });

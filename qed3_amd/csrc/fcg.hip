@@ -1,8 +1,13 @@
 // Flexible CG on D^dagger D with the learned preconditioner (Notay, SIAM J. Sci. Comput.
 // 22 (2000) 1444, FCG(m): each new direction is A-orthogonalised against the last m),
-// FP32 vectors with FP64 reductions and FP64 reliable updates.  A non-positive curvature, a non-finite value or a stalled preconditioned
-// iteration hands the remaining residual to the plain mixed CG; convergence is judged
-// on the FP64 true residual only.
+// FP32 vectors with FP64 reductions and FP64 reliable updates.  The net K is trained as
+// an approximate inverse of D, and the preconditioner of D^dag D is M = K K^dag, Hermitian
+// positive semi-definite by construction: z = K(K^dag r), so <z,r> = |K^dag r|^2.  Rounding
+// (BF16 activations) makes M vary slightly from step to step, which FCG tolerates.  z is
+// still kept a descent direction: when Re<z,r> < eps |z||r|, z is shifted along r until
+// the angle bound holds.  A non-positive
+// curvature, a non-finite value or a stalled preconditioned iteration hands the remaining
+// residual to the plain mixed CG; convergence is judged on the FP64 true residual only.
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -97,7 +102,7 @@ __global__ __launch_bounds__(256) void k_resid64(const Spin<double>* __restrict_
 }
 
 struct Buf {
-  Spin<float>*r, *z, *w, *x;
+  Spin<float>*r, *z, *w, *x, *y;
   std::vector<Spin<float>*> P, Q;  // stored directions p_j and q_j = A p_j
   Spin<double>*r64, *t64, *t2;
   double2* out;
@@ -117,7 +122,7 @@ CGResult fcg_solve(Dirac& D, Net& net, Spin<double>* x, const Spin<double>* b, c
   Buf B{};
   B.P.resize(m + 1);
   B.Q.resize(m + 1);
-  for (Spin<float>** p : {&B.r, &B.z, &B.w, &B.x})
+  for (Spin<float>** p : {&B.r, &B.z, &B.w, &B.x, &B.y})
     QED3_HIP(hipMallocAsync((void**)p, n * sizeof(Spin<float>), st));
   for (int j = 0; j <= m; j++) {
     QED3_HIP(hipMallocAsync((void**)&B.P[j], n * sizeof(Spin<float>), st));
@@ -158,8 +163,19 @@ CGResult fcg_solve(Dirac& D, Net& net, Spin<double>* x, const Spin<double>* b, c
     const int cur = it % (m + 1);
     Spin<float>* p = B.P[cur];
     Spin<float>* qv = B.Q[cur];
-    net.apply(B.z, B.r, 1);
+    net.apply_adj(B.y, B.r, 1);
+    net.apply(B.z, B.y, 1);
     s.precond_applies++;
+    {
+      constexpr double kEps = 0.05;
+      const double zz = dot32(B.z, B.z).real(), zr = dot32(B.z, B.r).real();
+      const double need = kEps * std::sqrt(zz * rr);
+      if (!(zr >= need) && std::isfinite(zz) && rr > 0) {
+        const double mu = (need - zr) / rr;
+        hipLaunchKernelGGL(k_caxpy32, dim3(nbk), dim3(256), 0, st, B.z, B.r, float(mu), 0.f, n);
+        s.shifted++;
+      }
+    }
     // p = z - sum_j <q_j, z>/<q_j, p_j> p_j over the kept directions
     QED3_HIP(hipMemcpyAsync(p, B.z, n * sizeof(Spin<float>), hipMemcpyDeviceToDevice, st));
     for (int jj = 1; jj <= nstored; jj++) {
@@ -171,7 +187,9 @@ CGResult fcg_solve(Dirac& D, Net& net, Spin<double>* x, const Spin<double>* b, c
     D.apply<float>(true, qv, B.w, false);
     const double pq = dot32(p, qv).real();
     const cd pr = dot32(p, B.r);
-    if (trace) fprintf(stderr, "fcg it %d pq %.3e pr %.3e%+.3ei |r|^2 %.3e\n", it, pq, pr.real(), pr.imag(), rr);
+    if (trace)
+      fprintf(stderr, "fcg it %d pq %.3e pr %.3e%+.3ei |r|^2 %.3e shifted %d\n", it, pq, pr.real(),
+              pr.imag(), rr, s.shifted);
     if (!(pq > 0) || !std::isfinite(pq) || !std::isfinite(pr.real())) {
       s.fallback = 1;
       break;
@@ -222,7 +240,7 @@ CGResult fcg_solve(Dirac& D, Net& net, Spin<double>* x, const Spin<double>* b, c
       hipLaunchKernelGGL(k_add64, dim3(nbk), dim3(256), 0, st, x, (const Spin<double>*)dx, n);
     }
   }
-  for (Spin<float>* p : {B.r, B.z, B.w, B.x}) (void)hipFreeAsync(p, st);
+  for (Spin<float>* p : {B.r, B.z, B.w, B.x, B.y}) (void)hipFreeAsync(p, st);
   for (int j = 0; j <= m; j++) {
     (void)hipFreeAsync(B.P[j], st);
     (void)hipFreeAsync(B.Q[j], st);

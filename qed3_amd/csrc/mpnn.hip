@@ -28,6 +28,7 @@ struct NetGeo {
   const float2* slot_nb;  // (cos a_yx, sin a_yx)
   const float2* uS;
   const float2* uT;
+  const float* kt;  // kappa_t(x)/2 per site
   int V, E, Lt, M;
   float bc;
   size_t np;  // B * Lt * V
@@ -42,7 +43,7 @@ struct PointIdx {
   int b, s, x;
   size_t base;  // (b*Lt + s) * V
   size_t Pp, Pm;
-  float bp, bm;
+  float bp, bm;  // boundary sign times kappa_t(x)/2 of the two time hops
 };
 
 __device__ __forceinline__ PointIdx point_of(const NetGeo& g, size_t P) {
@@ -56,8 +57,9 @@ __device__ __forceinline__ PointIdx point_of(const NetGeo& g, size_t P) {
   const size_t pb = size_t(q.b) * g.Lt;
   q.Pp = (pb + sp) * g.V + q.x;
   q.Pm = (pb + sm) * g.V + q.x;
-  q.bp = (q.s == g.Lt - 1) ? g.bc : 1.f;
-  q.bm = (q.s == 0) ? g.bc : 1.f;
+  const float kh = g.kt[q.x];
+  q.bp = (q.s == g.Lt - 1) ? g.bc * kh : kh;
+  q.bm = (q.s == 0) ? g.bc * kh : kh;
   return q;
 }
 
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(256) void k_net_in(NetGeo g, const Spin<float>* __r
     for (int c = 0; c < C; c++) {
       float2 z = {0.f, 0.f};
 #pragma unroll
-      for (int k = 0; k < 5; k++) cadd_(z, cm(v[k * C + c], F[k][a]));
+      for (int k = 0; k < 5; k++) cadd_(z, cm(v[(a * 5 + k) * C + c], F[k][a]));
       o[2 * c] = bf16(z.x);
       o[2 * c + 1] = bf16(z.y);
     }
@@ -196,8 +198,10 @@ __device__ __forceinline__ void feat4(const NetGeo& g, const bf16* __restrict__ 
 // MFMA operand layouts of v_mfma_f32_32x32x16_bf16 (wave64): A(32x16): lane l holds row
 // l%32, k = 8(l/32)..+8; B(16x32): column l%32, k = 8(l/32)..+8; D(32x32): column l%32,
 // rows 8(i/4) + 4(l/32) + i%4 for accumulator element i.
+// Columns of a block are ordered (spin a, point p): n = a PTS + p, so waves 0-1 hold spin
+// 0 and waves 2-3 spin 1, and each wave multiplies by its spin's weight matrix.
 __global__ __launch_bounds__(256) void k_net_mid(NetGeo g, const bf16* __restrict__ hin,
-                                                 const bf16* __restrict__ A /*[32][160]*/,
+                                                 const bf16* __restrict__ A /*[2][32][160]*/,
                                                  bf16* __restrict__ hout) {
   __shared__ __attribute__((aligned(16))) bf16 Fl[2 * PTS][LDSK];
   const int t = threadIdx.x;
@@ -220,23 +224,24 @@ __global__ __launch_bounds__(256) void k_net_mid(NetGeo g, const bf16* __restric
           w[2 * i] = bf16(F[k][a][i].x);
           w[2 * i + 1] = bf16(F[k][a][i].y);
         }
-        *reinterpret_cast<bf16x8*>(&Fl[2 * p + a][k * RW + 8 * grp]) = w;
+        *reinterpret_cast<bf16x8*>(&Fl[a * PTS + p][k * RW + 8 * grp]) = w;
       }
   }
   __syncthreads();
   const int lane = t & 63, w = t >> 6;
   const int m = lane & 31, hi = lane >> 5;
+  const bf16* Aw = A + size_t(w >> 1) * RW * KF;
   f32x16 acc = {};
 #pragma unroll
   for (int kk = 0; kk < KF / 16; kk++) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(A + m * KF + 16 * kk + 8 * hi);
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(Aw + m * KF + 16 * kk + 8 * hi);
     const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Fl[32 * w + m][16 * kk + 8 * hi]);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
-  const int n = 32 * w + m;  // column = 2 p + a
-  const size_t P = P0 + (n >> 1);
+  const int n = 32 * w + m;  // column = a PTS + p
+  const size_t P = P0 + (n % PTS);
   if (P < g.np) {
-    bf16* dst = hout + (P * 2 + (n & 1)) * RW;
+    bf16* dst = hout + (P * 2 + n / PTS) * RW;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 q4;
@@ -246,29 +251,33 @@ __global__ __launch_bounds__(256) void k_net_mid(NetGeo g, const bf16* __restric
   }
 }
 
-// out = sum_k F_k(hhat_k), hhat_k = sum_c u_k[c] h_c (the maps act channel-wise)
+// out = sum_k F_k(hhat_k) + skip r, hhat_k = sum_c u_k[c] h_c (the maps act channel-wise)
 __global__ __launch_bounds__(256) void k_net_out(NetGeo g, const bf16* __restrict__ h,
-                                                 const float2* __restrict__ u, Spin<float>* __restrict__ out) {
+                                                 const float2* __restrict__ u,
+                                                 const Spin<float>* __restrict__ r, float skip,
+                                                 Spin<float>* __restrict__ out) {
   const size_t P = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (P >= g.np) return;
   const PointIdx q = point_of(g, P);
-  auto proj = [&](size_t Q, int k, int a) {
-    const bf16x8* src = reinterpret_cast<const bf16x8*>(h + (Q * 2 + a) * RW);
+  // sum_c u[ao][k][c] h_c at spin ai of point Q (ao: the output spin the weights belong to)
+  auto proj = [&](size_t Q, int k, int ai, int ao) {
+    const bf16x8* src = reinterpret_cast<const bf16x8*>(h + (Q * 2 + ai) * RW);
+    const float2* uk = u + (ao * 5 + k) * C;
     float2 z = {0.f, 0.f};
 #pragma unroll
     for (int part = 0; part < 4; part++) {
       const bf16x8 w8 = src[part];
 #pragma unroll
-      for (int i = 0; i < 4; i++) cadd_(z, cm(u[k * C + 4 * part + i], float2{float(w8[2 * i]), float(w8[2 * i + 1])}));
+      for (int i = 0; i < 4; i++) cadd_(z, cm(uk[4 * part + i], float2{float(w8[2 * i]), float(w8[2 * i + 1])}));
     }
     return z;
   };
-  float2 o0 = proj(P, 0, 0), o1 = proj(P, 0, 1);
+  float2 o0 = proj(P, 0, 0, 0), o1 = proj(P, 0, 1, 1);
   for (int j = 0; j < g.M; j++) {
     const SlotT t = slot_t(g, j, q.x, q.s, false);
     const size_t Q = q.base + t.y;
-    const float2 a0 = proj(Q, 1, 0), a1 = proj(Q, 1, 1);
-    const float2 b0 = proj(Q, 2, 0), b1 = proj(Q, 2, 1);
+    const float2 a0 = proj(Q, 1, 0, 0), a1 = proj(Q, 1, 1, 1);
+    const float2 b0 = proj(Q, 2, 0, 1), b1 = proj(Q, 2, 1, 0);
     cadd_(o0, cm(t.t0, a0));
     cadd_(o1, cm(t.t1, a1));
     cadd_(o0, cm(cm(t.ea, t.t1), b1));
@@ -277,13 +286,14 @@ __global__ __launch_bounds__(256) void k_net_out(NetGeo g, const bf16* __restric
   const float2 up = g.uT[size_t(q.s) * g.V + q.x];
   const float2 um = cj(g.uT[size_t(q.s == 0 ? g.Lt - 1 : q.s - 1) * g.V + q.x]);
   const float2 wp = {q.bp * up.x, q.bp * up.y}, wm = {q.bm * um.x, q.bm * um.y};
-  cadd_(o0, cm(wp, proj(q.Pp, 3, 0)));
-  cadd_(o1, cm(wp, proj(q.Pp, 3, 1)));
-  cadd_(o0, cm(wm, proj(q.Pm, 4, 0)));
-  cadd_(o1, cm(wm, proj(q.Pm, 4, 1)));
+  cadd_(o0, cm(wp, proj(q.Pp, 3, 0, 0)));
+  cadd_(o1, cm(wp, proj(q.Pp, 3, 1, 1)));
+  cadd_(o0, cm(wm, proj(q.Pm, 4, 0, 0)));
+  cadd_(o1, cm(wm, proj(q.Pm, 4, 1, 1)));
+  const Spin<float> rp = r[P];
   Spin<float> s;
-  s.a = o0;
-  s.b = o1;
+  s.a = {o0.x + skip * rp.a.x, o0.y + skip * rp.a.y};
+  s.b = {o1.x + skip * rp.b.x, o1.y + skip * rp.b.y};
   out[P] = s;
 }
 
@@ -323,8 +333,10 @@ __global__ __launch_bounds__(256) void k_net_gout(const Spin<float>* __restrict_
   gout[P] = gg;
 }
 
-// Output head backward: A_k = F_k^dagger gout;  dh[c,a] = sum_k conj(u_k[c]) A_k,a;
-// per-point contributions to du_k[c] = sum_a A_k,a conj(h_c,a) (summed by k_colsum).
+// Output head backward: A_k = F_k^dagger gout.  Input spin a of F_k feeds output spin
+// o_k(a) = a (k != 2) or 1 - a (k = 2, the gamma map), so dh[c,a] = sum_k conj(u[o_k(a)][k][c])
+// A_k,a and du[o][k][c] = A_k,a conj(h_c,a) with o = o_k(a) (per-point contributions,
+// summed by k_colsum).
 __global__ __launch_bounds__(256) void k_net_out_bwd(NetGeo g, const Spin<float>* __restrict__ gout,
                                                      const bf16* __restrict__ h,
                                                      const float2* __restrict__ u,
@@ -333,29 +345,31 @@ __global__ __launch_bounds__(256) void k_net_out_bwd(NetGeo g, const Spin<float>
   if (P >= g.np) return;
   float2 A[5][2];
   feat1<true>(g, gout, P, A);
-  float* cdst = contrib + P * (10 * C);
   for (int c = 0; c < C; c++) {
+    for (int a = 0; a < 2; a++) {
+      float2 z = {0.f, 0.f};
+      for (int k = 0; k < 5; k++) cadd_(z, cmc(u[((k == 2 ? 1 - a : a) * 5 + k) * C + c], A[k][a]));
+      dh[(P * 2 + a) * RW + 2 * c] = z.x;
+      dh[(P * 2 + a) * RW + 2 * c + 1] = z.y;
+    }
+    if (!contrib) continue;  // adjoint apply: no weight gradient
+    float* cdst = contrib + P * (20 * C);
     float2 hc[2];
     for (int a = 0; a < 2; a++) {
       const bf16* src = h + (P * 2 + a) * RW + 2 * c;
       hc[a] = {float(src[0]), float(src[1])};
     }
-    for (int a = 0; a < 2; a++) {
-      float2 z = {0.f, 0.f};
-      for (int k = 0; k < 5; k++) cadd_(z, cmc(u[k * C + c], A[k][a]));
-      dh[(P * 2 + a) * RW + 2 * c] = z.x;
-      dh[(P * 2 + a) * RW + 2 * c + 1] = z.y;
-    }
-    for (int k = 0; k < 5; k++) {
-      float2 z = cmc(hc[0], A[k][0]);
-      cadd_(z, cmc(hc[1], A[k][1]));
-      cdst[2 * (k * C + c)] = z.x;
-      cdst[2 * (k * C + c) + 1] = z.y;
-    }
+    for (int a = 0; a < 2; a++)
+      for (int k = 0; k < 5; k++) {
+        const int o = k == 2 ? 1 - a : a;
+        const float2 z = cmc(hc[a], A[k][a]);
+        cdst[2 * ((o * 5 + k) * C + c)] = z.x;
+        cdst[2 * ((o * 5 + k) * C + c) + 1] = z.y;
+      }
   }
 }
 
-// Input head backward: contributions to dv_k[c] = sum_a dh1[c,a] conj(F_k(r)_a)
+// Input head backward: contributions to dv[a][k][c] = dh1[c,a] conj(F_k(r)_a)
 __global__ __launch_bounds__(256) void k_net_in_bwd(NetGeo g, const Spin<float>* __restrict__ r,
                                                     const float* __restrict__ dh,
                                                     float* __restrict__ contrib) {
@@ -363,29 +377,29 @@ __global__ __launch_bounds__(256) void k_net_in_bwd(NetGeo g, const Spin<float>*
   if (P >= g.np) return;
   float2 F[5][2];
   feat1<false>(g, r, P, F);
-  float* cdst = contrib + P * (10 * C);
-  for (int c = 0; c < C; c++) {
-    const float2 g0 = {dh[(P * 2) * RW + 2 * c], dh[(P * 2) * RW + 2 * c + 1]};
-    const float2 g1 = {dh[(P * 2 + 1) * RW + 2 * c], dh[(P * 2 + 1) * RW + 2 * c + 1]};
-    for (int k = 0; k < 5; k++) {
-      float2 z = cmc(F[k][0], g0);
-      cadd_(z, cmc(F[k][1], g1));
-      cdst[2 * (k * C + c)] = z.x;
-      cdst[2 * (k * C + c) + 1] = z.y;
+  float* cdst = contrib + P * (20 * C);
+  for (int c = 0; c < C; c++)
+    for (int a = 0; a < 2; a++) {
+      const float2 ga = {dh[(P * 2 + a) * RW + 2 * c], dh[(P * 2 + a) * RW + 2 * c + 1]};
+      for (int k = 0; k < 5; k++) {
+        const float2 z = cmc(F[k][a], ga);
+        cdst[2 * ((a * 5 + k) * C + c)] = z.x;
+        cdst[2 * ((a * 5 + k) * C + c) + 1] = z.y;
+      }
     }
-  }
 }
 
-// Input gradient of a mixing layer, G = R^T dz per column (MFMA, M = 160 in 5 tiles,
-// K = 32): G[P][k][a][32] FP32.
+// Input gradient of a mixing layer, G = R_a^T dz per column of spin a (MFMA, M = 160 in 5
+// tiles, K = 32): G[P][k][a][32] FP32.  Columns ordered (spin, point) as in k_net_mid.
 __global__ __launch_bounds__(256) void k_net_mid_G(size_t np, const float* __restrict__ dz,
-                                                   const bf16* __restrict__ AT /*[160][32]*/,
+                                                   const bf16* __restrict__ AT /*[2][160][32]*/,
                                                    float* __restrict__ G) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int m = lane & 31, hi = lane >> 5;
   const int n = 32 * w + m;
-  const size_t P = size_t(blockIdx.x) * PTS + (n >> 1);
-  const int a = n & 1;
+  const size_t P = size_t(blockIdx.x) * PTS + (n % PTS);
+  const int a = n / PTS;
+  const bf16* ATa = AT + size_t(w >> 1) * KF * RW;
   bf16x8 b[2];
   for (int kk = 0; kk < 2; kk++)
     for (int i = 0; i < 8; i++)
@@ -393,7 +407,7 @@ __global__ __launch_bounds__(256) void k_net_mid_G(size_t np, const float* __res
   for (int tile = 0; tile < 5; tile++) {
     f32x16 acc = {};
     for (int kk = 0; kk < 2; kk++) {
-      const bf16x8 av = *reinterpret_cast<const bf16x8*>(AT + (32 * tile + m) * RW + 16 * kk + 8 * hi);
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(ATa + (32 * tile + m) * RW + 16 * kk + 8 * hi);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[kk], acc, 0, 0, 0);
     }
     if (P < np) {
@@ -403,8 +417,9 @@ __global__ __launch_bounds__(256) void k_net_mid_G(size_t np, const float* __res
   }
 }
 
-// Weight gradient of a mixing layer, dR (32 x 160) = sum_cols dz (32) F^T (160) (MFMA with
-// K over the block's 128 columns), one partial per block.
+// Weight gradient of a mixing layer, dR_a (32 x 160) = sum over the block's 64 columns of
+// spin a of dz (32) F^T (160) (MFMA with K over those columns), one partial per block:
+// partial[block][a][32][160].
 __global__ __launch_bounds__(256) void k_net_mid_dW(NetGeo g, const bf16* __restrict__ hin,
                                                     const float* __restrict__ dz,
                                                     float* __restrict__ partial) {
@@ -419,7 +434,7 @@ __global__ __launch_bounds__(256) void k_net_mid_dW(NetGeo g, const bf16* __rest
     const bool ok = P < g.np;
     if (ok) feat4(g, hin, P, grp, F);
     for (int a = 0; a < 2; a++) {
-      const int col = 2 * p + a;
+      const int col = a * PTS + p;
       for (int k = 0; k < 5; k++)
         for (int i = 0; i < 4; i++) {
           const int f = k * RW + 2 * (4 * grp + i);
@@ -435,15 +450,16 @@ __global__ __launch_bounds__(256) void k_net_mid_dW(NetGeo g, const bf16* __rest
   __syncthreads();
   const int lane = t & 63, w = t >> 6;
   const int m = lane & 31, hi = lane >> 5;
-  // wave w accumulates feature tiles w and w+4 (5 tiles over 4 waves)
-  for (int tile = w; tile < 5; tile += 4) {
+  // 10 jobs (5 feature tiles x 2 spins) over 4 waves: wave w takes jobs w, w+4, w+8
+  for (int job = w; job < 10; job += 4) {
+    const int tile = job % 5, a = job / 5;
     f32x16 acc = {};
-    for (int kk = 0; kk < (2 * PTS) / 16; kk++) {
+    for (int kk = a * (PTS / 16); kk < (a + 1) * (PTS / 16); kk++) {
       const bf16x8 av = *reinterpret_cast<const bf16x8*>(&ZT[m][16 * kk + 8 * hi]);
       const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&FT[32 * tile + m][16 * kk + 8 * hi]);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
     }
-    float* dst = partial + size_t(blockIdx.x) * (RW * KF);
+    float* dst = partial + (size_t(blockIdx.x) * 2 + a) * (RW * KF);
     for (int i = 0; i < 16; i++) {
       const int row = 8 * (i / 4) + 4 * hi + (i % 4);
       dst[row * KF + 32 * tile + m] = acc[i];
@@ -507,6 +523,51 @@ __global__ __launch_bounds__(256) void k_net_adj(NetGeo g, const float* __restri
   }
 }
 
+// Input head adjoint: out_a = sum_k [F_k^dagger w_k]_a + skip in_a with the single-channel
+// fields w_k(Q)_a = sum_c conj(v[a][k][c]) dh(Q)_c,a, formed on the fly at each gathered point
+// (the adjoint transports as in k_net_adj).
+__global__ __launch_bounds__(256) void k_net_in_adj(NetGeo g, const float* __restrict__ dh,
+                                                    const float2* __restrict__ v,
+                                                    const Spin<float>* __restrict__ in, float skip,
+                                                    Spin<float>* __restrict__ out) {
+  const size_t P = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (P >= g.np) return;
+  const PointIdx q = point_of(g, P);
+  auto w = [&](size_t Q, int k, int a) {
+    const float4* src = reinterpret_cast<const float4*>(dh + (Q * 2 + a) * RW);
+    const float2* vk = v + (a * 5 + k) * C;
+    float2 z = {0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < C / 2; i++) {
+      const float4 d = src[i];
+      cadd_(z, cmc(vk[2 * i], float2{d.x, d.y}));
+      cadd_(z, cmc(vk[2 * i + 1], float2{d.z, d.w}));
+    }
+    return z;
+  };
+  float2 r0 = w(P, 0, 0), r1 = w(P, 0, 1);
+  for (int j = 0; j < g.M; j++) {
+    const SlotT s = slot_t(g, j, q.x, q.s, true);
+    const size_t Q = q.base + s.y;
+    cadd_(r0, cm(s.t0, w(Q, 1, 0)));
+    cadd_(r0, cm(cm(s.ea, s.t0), w(Q, 2, 1)));
+    cadd_(r1, cm(s.t1, w(Q, 1, 1)));
+    cadd_(r1, cm(cm(s.eb, s.t1), w(Q, 2, 0)));
+  }
+  const float2 up = g.uT[size_t(q.s) * g.V + q.x];
+  const float2 um = cj(g.uT[size_t(q.s == 0 ? g.Lt - 1 : q.s - 1) * g.V + q.x]);
+  const float2 wm = {q.bm * um.x, q.bm * um.y}, wp = {q.bp * up.x, q.bp * up.y};
+  cadd_(r0, cm(wm, w(q.Pm, 3, 0)));
+  cadd_(r1, cm(wm, w(q.Pm, 3, 1)));
+  cadd_(r0, cm(wp, w(q.Pp, 4, 0)));
+  cadd_(r1, cm(wp, w(q.Pp, 4, 1)));
+  const Spin<float> x = in[P];
+  Spin<float> o;
+  o.a = {r0.x + skip * x.a.x, r0.y + skip * x.a.y};
+  o.b = {r1.x + skip * x.b.x, r1.y + skip * x.b.y};
+  out[P] = o;
+}
+
 // out[j] = sum_i buf[i][j] in FP64, one block per column group
 __global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ buf, size_t rows, int cols,
                                                 double* __restrict__ out) {
@@ -545,30 +606,46 @@ Net::Net(Dirac& D, int layers, int seed) : D_(D), L_(layers) {
     nb[2 * i] = float(std::cos(h.slot_alpha_nb[i]));
     nb[2 * i + 1] = float(std::sin(h.slot_alpha_nb[i]));
   }
+  // Gershgorin bound on |D|_2: the 2x2 blocks have norms |c_x|, kappa_xy/2 (r+1) per
+  // spatial hop and kappa_t(x)/2 (r+1) per time hop; row and column sums agree.
+  double bound = 0;
+  for (int x = 0; x < h.V; x++) {
+    double row = std::abs(h.diag[x]) + (h.Lt > 1 ? 2.0 : 1.0) * 0.5 * h.kappa_t[x] * (h.r + 1);
+    for (int j = 0; j < h.nn_count[x]; j++) {
+      const size_t i = size_t(j) * h.V + x;
+      row += std::hypot(h.hopD[4 * i], h.hopD[4 * i + 1]) * (h.r + 1);
+    }
+    bound = std::max(bound, row);
+  }
+  skip_ = bound > 0 ? float(1.0 / bound) : 0.f;
   slot_ = dmalloc<float>(sl.size());
   slot_nb_ = dmalloc<float>(nb.size());
   QED3_HIP(hipMemcpy(slot_, sl.data(), sl.size() * 4, hipMemcpyHostToDevice));
   QED3_HIP(hipMemcpy(slot_nb_, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
-  heads_ = dmalloc<float>(20 * C);
+  std::vector<float> kth(h.V);
+  for (int x = 0; x < h.V; x++) kth[x] = float(0.5 * h.kappa_t[x]);
+  kt_ = dmalloc<float>(h.V);
+  QED3_HIP(hipMemcpy(kt_, kth.data(), kth.size() * 4, hipMemcpyHostToDevice));
+  heads_ = dmalloc<float>(40 * C);
   const int nmid = std::max(L_ - 2, 0);
-  QED3_HIP(hipMalloc(&mixA_, std::max(nmid, 1) * RW * KF * sizeof(bf16)));
-  QED3_HIP(hipMalloc(&mixAT_, std::max(nmid, 1) * RW * KF * sizeof(bf16)));
+  QED3_HIP(hipMalloc(&mixA_, std::max(nmid, 1) * 2 * RW * KF * sizeof(bf16)));
+  QED3_HIP(hipMalloc(&mixAT_, std::max(nmid, 1) * 2 * RW * KF * sizeof(bf16)));
   // initialisation: heads and mixing weights ~ N(0, sigma^2) complex, small
   p_.assign(2 * nparams(), 0.0);
   std::mt19937_64 rng(seed);
   std::normal_distribution<double> nd;
-  for (size_t i = 0; i < 5 * C; i++) {  // v: start near the identity-like lift
+  for (size_t i = 0; i < 10 * C; i++) {  // v
     p_[2 * i] = 0.3 * nd(rng);
     p_[2 * i + 1] = 0.3 * nd(rng);
   }
-  const size_t nw = size_t(nmid) * 5 * C * C;
+  const size_t nw = size_t(nmid) * 10 * C * C;
   for (size_t i = 0; i < nw; i++) {
-    p_[2 * (5 * C + i)] = nd(rng) / std::sqrt(5.0 * C);
-    p_[2 * (5 * C + i) + 1] = nd(rng) / std::sqrt(5.0 * C);
+    p_[2 * (10 * C + i)] = nd(rng) / std::sqrt(5.0 * C);
+    p_[2 * (10 * C + i) + 1] = nd(rng) / std::sqrt(5.0 * C);
   }
-  for (size_t i = 0; i < 5 * C; i++) {
-    p_[2 * (5 * C + nw + i)] = 0.3 * nd(rng) / C;
-    p_[2 * (5 * C + nw + i) + 1] = 0.3 * nd(rng) / C;
+  for (size_t i = 0; i < 10 * C; i++) {  // u
+    p_[2 * (10 * C + nw + i)] = 0.3 * nd(rng) / C;
+    p_[2 * (10 * C + nw + i) + 1] = 0.3 * nd(rng) / C;
   }
   m_.assign(p_.size(), 0.0);
   v_.assign(p_.size(), 0.0);
@@ -576,7 +653,7 @@ Net::Net(Dirac& D, int layers, int seed) : D_(D), L_(layers) {
 }
 
 Net::~Net() {
-  for (void* p : {(void*)slot_, (void*)slot_nb_, (void*)heads_, mixA_, mixAT_, act_, (void*)gfield_,
+  for (void* p : {(void*)slot_, (void*)slot_nb_, (void*)kt_, (void*)heads_, mixA_, mixAT_, act_, (void*)gfield_,
                   (void*)gfield2_, (void*)G_, (void*)out_, (void*)gout_, (void*)partial_, (void*)red_})
     if (p) (void)hipFree(p);
 }
@@ -589,22 +666,24 @@ void Net::set_params(const double* p) {
 }
 
 // Real form of z = W f on interleaved (re, im) rows: R[2c][2d] = Re W, R[2c][2d+1] = -Im W,
-// R[2c+1][2d] = Im W, R[2c+1][2d+1] = Re W, for the five feature blocks side by side.
+// R[2c+1][2d] = Im W, R[2c+1][2d+1] = Re W, for the five feature blocks side by side; one
+// matrix per layer and output spin.
 void Net::upload_params() {
-  std::vector<float> heads(20 * C);
-  const size_t nw = size_t(std::max(L_ - 2, 0)) * 5 * C * C;
-  for (size_t i = 0; i < 10 * C; i++) heads[i] = float(p_[i]);                      // v
-  for (size_t i = 0; i < 10 * C; i++) heads[10 * C + i] = float(p_[2 * (5 * C + nw) + i]);  // u
+  std::vector<float> heads(40 * C);
+  const size_t nw = size_t(std::max(L_ - 2, 0)) * 10 * C * C;
+  for (size_t i = 0; i < 20 * C; i++) heads[i] = float(p_[i]);                        // v
+  for (size_t i = 0; i < 20 * C; i++) heads[20 * C + i] = float(p_[2 * (10 * C + nw) + i]);  // u
   QED3_HIP(hipMemcpy(heads_, heads.data(), heads.size() * 4, hipMemcpyHostToDevice));
   const int nmid = std::max(L_ - 2, 0);
-  std::vector<bf16> A(size_t(std::max(nmid, 1)) * RW * KF), AT(A.size());
+  std::vector<bf16> A(size_t(std::max(nmid, 1)) * 2 * RW * KF), AT(A.size());
   for (int l = 0; l < nmid; l++)
+    for (int sa = 0; sa < 2; sa++)
     for (int k = 0; k < 5; k++)
       for (int c = 0; c < C; c++)
         for (int d = 0; d < C; d++) {
-          const size_t wi = 5 * C + ((size_t(l) * 5 + k) * C + c) * C + d;
+          const size_t wi = 10 * C + (((size_t(l) * 2 + sa) * 5 + k) * C + c) * C + d;
           const float re = float(p_[2 * wi]), im = float(p_[2 * wi + 1]);
-          const size_t base = size_t(l) * RW * KF;
+          const size_t base = (size_t(l) * 2 + sa) * RW * KF;
           const int col = k * RW + 2 * d;
           A[base + (2 * c) * KF + col] = bf16(re);
           A[base + (2 * c) * KF + col + 1] = bf16(-im);
@@ -638,8 +717,8 @@ void Net::ensure(int B, bool training) {
     out_ = dmalloc<Spin<float>>(np);
     gout_ = dmalloc<Spin<float>>(np);
     const size_t nblk = (np + PTS - 1) / PTS;
-    partial_ = reinterpret_cast<double*>(dmalloc<float>(std::max(nblk * RW * KF, np * 10 * C)));
-    red_ = dmalloc<double>(RW * KF + 2 * size_t(std::max(B, capB_)));
+    partial_ = reinterpret_cast<double*>(dmalloc<float>(std::max(nblk * 2 * RW * KF, np * 20 * C)));
+    red_ = dmalloc<double>(2 * RW * KF + 2 * size_t(std::max(B, capB_)));
   }
   capB_ = std::max(B, capB_);
   cap_train_ = cap_train_ || training;
@@ -650,20 +729,48 @@ void Net::apply(Spin<float>* out, const Spin<float>* r, int B) {
   const DiracView<float> dv = D_.view<float>();
   const HopTables& h = D_.tables();
   NetGeo g{dv.nbr, dv.lnk, reinterpret_cast<const float4*>(slot_), reinterpret_cast<const float2*>(slot_nb_),
-           dv.uS, dv.uT, h.V, h.E, h.Lt, h.max_nn, dv.bc, size_t(B) * D_.points()};
+           dv.uS, dv.uT, kt_, h.V, h.E, h.Lt, h.max_nn, dv.bc, size_t(B) * D_.points()};
   hipStream_t st = D_.stream();
   bf16* a0 = static_cast<bf16*>(act_);
   bf16* a1 = a0 + g.np * 2 * RW;
   const unsigned nb = unsigned((g.np + 255) / 256), nbm = unsigned((g.np + PTS - 1) / PTS);
   const float2* v = reinterpret_cast<const float2*>(heads_);
-  const float2* u = v + 5 * C;
+  const float2* u = v + 10 * C;
   hipLaunchKernelGGL(k_net_in, dim3(nb), dim3(256), 0, st, g, r, v, a0);
   for (int l = 0; l < L_ - 2; l++) {
     hipLaunchKernelGGL(k_net_mid, dim3(nbm), dim3(256), 0, st, g, a0,
-                       static_cast<const bf16*>(mixA_) + size_t(l) * RW * KF, a1);
+                       static_cast<const bf16*>(mixA_) + size_t(l) * 2 * RW * KF, a1);
     std::swap(a0, a1);
   }
-  hipLaunchKernelGGL(k_net_out, dim3(nb), dim3(256), 0, st, g, a0, u, out);
+  hipLaunchKernelGGL(k_net_out, dim3(nb), dim3(256), 0, st, g, a0, u, r, skip_, out);
+  QED3_HIP(hipGetLastError());
+}
+
+// out = N^dagger(r): the adjoint layers in reverse order (output head, then each mixing
+// layer as R^T followed by the adjoint feature maps, then the input head).  The net is
+// linear, so no forward activations are needed.
+void Net::apply_adj(Spin<float>* out, const Spin<float>* r, int B) {
+  ensure(B, true);
+  const DiracView<float> dv = D_.view<float>();
+  const HopTables& h = D_.tables();
+  NetGeo g{dv.nbr, dv.lnk, reinterpret_cast<const float4*>(slot_), reinterpret_cast<const float2*>(slot_nb_),
+           dv.uS, dv.uT, kt_, h.V, h.E, h.Lt, h.max_nn, dv.bc, size_t(B) * D_.points()};
+  hipStream_t st = D_.stream();
+  const size_t np = g.np;
+  const unsigned nb = unsigned((np + 255) / 256), nbm = unsigned((np + PTS - 1) / PTS);
+  const float2* v = reinterpret_cast<const float2*>(heads_);
+  const float2* u = v + 10 * C;
+  float* gin = gfield_;
+  float* gnext = gfield2_;
+  hipLaunchKernelGGL(k_net_out_bwd, dim3(nb), dim3(256), 0, st, g, r, static_cast<const bf16*>(nullptr), u,
+                     gin, static_cast<float*>(nullptr));
+  for (int l = L_ - 3; l >= 0; l--) {
+    hipLaunchKernelGGL(k_net_mid_G, dim3(nbm), dim3(256), 0, st, np, gin,
+                       static_cast<const bf16*>(mixAT_) + size_t(l) * 2 * RW * KF, G_);
+    hipLaunchKernelGGL(k_net_adj, dim3(unsigned((4 * np + 255) / 256)), dim3(256), 0, st, g, G_, gnext);
+    std::swap(gin, gnext);
+  }
+  hipLaunchKernelGGL(k_net_in_adj, dim3(nb), dim3(256), 0, st, g, gin, v, r, skip_, out);
   QED3_HIP(hipGetLastError());
 }
 
@@ -672,20 +779,20 @@ double Net::loss_grad(const Spin<float>* r, const Spin<float>* e, int B, std::ve
   const DiracView<float> dv = D_.view<float>();
   const HopTables& h = D_.tables();
   NetGeo g{dv.nbr, dv.lnk, reinterpret_cast<const float4*>(slot_), reinterpret_cast<const float2*>(slot_nb_),
-           dv.uS, dv.uT, h.V, h.E, h.Lt, h.max_nn, dv.bc, size_t(B) * D_.points()};
+           dv.uS, dv.uT, kt_, h.V, h.E, h.Lt, h.max_nn, dv.bc, size_t(B) * D_.points()};
   hipStream_t st = D_.stream();
   const size_t np = g.np, per = D_.points();
   const unsigned nb = unsigned((np + 255) / 256), nbm = unsigned((np + PTS - 1) / PTS);
   const float2* v = reinterpret_cast<const float2*>(heads_);
-  const float2* u = v + 5 * C;
+  const float2* u = v + 10 * C;
   const int nmid = L_ - 2;
   auto act = [&](int j) { return static_cast<bf16*>(act_) + size_t(j) * np * 2 * RW; };
   // forward, keeping h1..h^{L-1}
   hipLaunchKernelGGL(k_net_in, dim3(nb), dim3(256), 0, st, g, r, v, act(0));
   for (int l = 0; l < nmid; l++)
     hipLaunchKernelGGL(k_net_mid, dim3(nbm), dim3(256), 0, st, g, act(l),
-                       static_cast<const bf16*>(mixA_) + size_t(l) * RW * KF, act(l + 1));
-  hipLaunchKernelGGL(k_net_out, dim3(nb), dim3(256), 0, st, g, act(nmid), u, out_);
+                       static_cast<const bf16*>(mixA_) + size_t(l) * 2 * RW * KF, act(l + 1));
+  hipLaunchKernelGGL(k_net_out, dim3(nb), dim3(256), 0, st, g, act(nmid), u, r, skip_, out_);
   // loss
   QED3_HIP(hipMemsetAsync(red_, 0, 2 * B * sizeof(double), st));
   hipLaunchKernelGGL(k_net_loss, dim3(nb), dim3(256), 0, st, out_, e, np, per, red_);
@@ -696,31 +803,33 @@ double Net::loss_grad(const Spin<float>* r, const Spin<float>* e, int B, std::ve
   float* contrib = reinterpret_cast<float*>(partial_);
   double* red = red_ + 2 * B;
   grad.assign(p_.size(), 0.0);
-  const size_t nw = size_t(nmid) * 5 * C * C;
-  std::vector<double> tmp(RW * KF);
+  const size_t nw = size_t(nmid) * 10 * C * C;
+  std::vector<double> tmp(2 * RW * KF);
   hipLaunchKernelGGL(k_net_out_bwd, dim3(nb), dim3(256), 0, st, g, gout_, act(nmid), u, gfield_, contrib);
-  hipLaunchKernelGGL(k_colsum, dim3(10 * C), dim3(256), 0, st, contrib, np, 10 * C, red);
-  QED3_HIP(hipMemcpyAsync(tmp.data(), red, 10 * C * sizeof(double), hipMemcpyDeviceToHost, st));
+  hipLaunchKernelGGL(k_colsum, dim3(20 * C), dim3(256), 0, st, contrib, np, 20 * C, red);
+  QED3_HIP(hipMemcpyAsync(tmp.data(), red, 20 * C * sizeof(double), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
-  for (int i = 0; i < 10 * C; i++) grad[2 * (5 * C + nw) + i] = tmp[i];
+  for (int i = 0; i < 20 * C; i++) grad[2 * (10 * C + nw) + i] = tmp[i];
   // middle layers, last to first
   float* gin = gfield_;
   float* gnext = gfield2_;
   for (int l = nmid - 1; l >= 0; l--) {
     hipLaunchKernelGGL(k_net_mid_dW, dim3(nbm), dim3(256), 0, st, g, act(l), gin, contrib);
-    hipLaunchKernelGGL(k_colsum, dim3(RW * KF), dim3(256), 0, st, contrib, size_t(nbm), RW * KF, red);
-    QED3_HIP(hipMemcpyAsync(tmp.data(), red, RW * KF * sizeof(double), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_colsum, dim3(2 * RW * KF), dim3(256), 0, st, contrib, size_t(nbm), 2 * RW * KF, red);
+    QED3_HIP(hipMemcpyAsync(tmp.data(), red, 2 * RW * KF * sizeof(double), hipMemcpyDeviceToHost, st));
     hipLaunchKernelGGL(k_net_mid_G, dim3(nbm), dim3(256), 0, st, np, gin,
-                       static_cast<const bf16*>(mixAT_) + size_t(l) * RW * KF, G_);
+                       static_cast<const bf16*>(mixAT_) + size_t(l) * 2 * RW * KF, G_);
     hipLaunchKernelGGL(k_net_adj, dim3(unsigned((4 * np + 255) / 256)), dim3(256), 0, st, g, G_, gnext);
     QED3_HIP(hipStreamSynchronize(st));
+    for (int sa = 0; sa < 2; sa++)
     for (int k = 0; k < 5; k++)
       for (int c = 0; c < C; c++)
         for (int d = 0; d < C; d++) {
           const int col = k * RW + 2 * d;
-          const double rr = tmp[(2 * c) * KF + col], ri = tmp[(2 * c) * KF + col + 1];
-          const double ir = tmp[(2 * c + 1) * KF + col], ii = tmp[(2 * c + 1) * KF + col + 1];
-          const size_t wi = 5 * C + ((size_t(l) * 5 + k) * C + c) * C + d;
+          const double* t2 = tmp.data() + size_t(sa) * RW * KF;
+          const double rr = t2[(2 * c) * KF + col], ri = t2[(2 * c) * KF + col + 1];
+          const double ir = t2[(2 * c + 1) * KF + col], ii = t2[(2 * c + 1) * KF + col + 1];
+          const size_t wi = 10 * C + (((size_t(l) * 2 + sa) * 5 + k) * C + c) * C + d;
           grad[2 * wi] = rr + ii;
           grad[2 * wi + 1] = ir - ri;
         }
@@ -728,11 +837,11 @@ double Net::loss_grad(const Spin<float>* r, const Spin<float>* e, int B, std::ve
   }
   // input head
   hipLaunchKernelGGL(k_net_in_bwd, dim3(nb), dim3(256), 0, st, g, r, gin, contrib);
-  hipLaunchKernelGGL(k_colsum, dim3(10 * C), dim3(256), 0, st, contrib, np, 10 * C, red);
-  QED3_HIP(hipMemcpyAsync(tmp.data(), red, 10 * C * sizeof(double), hipMemcpyDeviceToHost, st));
+  hipLaunchKernelGGL(k_colsum, dim3(20 * C), dim3(256), 0, st, contrib, np, 20 * C, red);
+  QED3_HIP(hipMemcpyAsync(tmp.data(), red, 20 * C * sizeof(double), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
   QED3_HIP(hipGetLastError());
-  for (int i = 0; i < 10 * C; i++) grad[i] = tmp[i];
+  for (int i = 0; i < 20 * C; i++) grad[i] = tmp[i];
   double loss = 0;
   for (int b = 0; b < B; b++) loss += pairs[2 * b + 1] / pairs[2 * b] / B;
   return loss;

@@ -5,7 +5,7 @@ import pytest
 
 import oracle as O
 from conftest import rel
-from pairs import geometry, links, oracle_setup, source
+from pairs import geometry, links, oracle_setup, source, wilson_hop_params
 
 
 def test_mt19937_64_known_answer():
@@ -132,3 +132,47 @@ def test_gauge_action_force_fd(compact, lattice):
         assert abs(fd - ft[s, x]) < 1e-6 * max(1, abs(fd))
     if not compact:  # quadratic: S(lambda U) = lambda^2 S(U)
         assert abs(Sg.S(2 * sp, 2 * tp) - 4 * Sg.S(sp, tp)) < 1e-9 * Sg.S(sp, tp)
+
+
+@pytest.mark.parametrize("lattice,bc", [(0, 0), (1, 0), (0, 1)])
+def test_net_features_reproduce_the_wilson_hops(lattice, bc):
+    """The net's five feature maps with spin-dependent head weights give D r - c_x r, checked
+    against the CSR operator and the closed-form diagonal c_x = sum 1/2 r kappa + r kappa_t + M5."""
+    n, Lt, C, r, M5 = 2, 4, 16, 1.0, 0.1
+    L, D, at = oracle_setup(lattice, n, Lt, M5=M5, bc=bc)
+    sp, tp = links(lattice, n, Lt, seed=8)
+    D.update(sp, tp)
+    v = source(lattice, n, Lt, seed=2)
+    out = O.mpnn_forward(D, C, 2, wilson_hop_params(C, r), v[None, :])[0]
+    d = (D.apply(0, v) - out).reshape(Lt, L.V, 2) / v.reshape(Lt, L.V, 2)
+    g = geometry(n)
+    lk = g.links if lattice == 0 else g.dual_links
+    der = L.derived()
+    kap = np.zeros(L.V)
+    np.add.at(kap, lk[:, 0], 0.5 * r * der["kappa"])
+    np.add.at(kap, lk[:, 1], 0.5 * r * der["kappa"])
+    kt = der["dual_area"] / (L.checks()["mean_ell"] * at)
+    c = kap + r * kt + M5
+    assert np.abs(d - c[None, :, None]).max() < 1e-12 * c.max()
+
+
+def test_net_gradient_matches_finite_differences():
+    lattice, n, Lt, C, layers, B = 0, 1, 2, 16, 3, 2
+    L, D, at = oracle_setup(lattice, n, Lt, M5=0.1)
+    sp, tp = links(lattice, n, Lt, seed=4)
+    D.update(sp, tp)
+    npar = 20 * C + (layers - 2) * 10 * C * C
+    rng = np.random.default_rng(3)
+    p = (rng.normal(size=npar) + 1j * rng.normal(size=npar)) * 0.25
+    r = np.stack([source(lattice, n, Lt, seed=s) for s in (1, 2)])
+    e = np.stack([source(lattice, n, Lt, seed=s) for s in (3, 4)])
+    loss, g = O.mpnn_loss_grad(D, C, layers, p, r, e, skip=0.3)
+    h = 1e-6
+    for k in range(4):
+        dlt = rng.normal(size=npar) + 1j * rng.normal(size=npar)
+        lp, _ = O.mpnn_loss_grad(D, C, layers, p + h * dlt, r, e, skip=0.3)
+        lm, _ = O.mpnn_loss_grad(D, C, layers, p - h * dlt, r, e, skip=0.3)
+        fd = (lp - lm) / (2 * h)
+        an = np.real(np.vdot(g, dlt))
+        assert abs(fd - an) < 1e-6 * abs(an), (k, fd, an)
+    assert B == r.shape[0]
