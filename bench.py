@@ -1,10 +1,17 @@
 """Headline benchmark: CG solves/s on D^dag D to a true FP64 relative residual of 1e-10.
 
-Workload (default): simplicial refined icosahedron n=128 (163,842 sites) x Lt=64,
-antiperiodic in time, r=1, M5=m=0.05, von Mises(kappa=1.4) link angles, Gaussian RHS.
-A step is one complete solve (distinct seeded RHS per step).  `value` is the solves/s of
-the fastest method (mixed FP32 CG with FP64 reliable updates); the plain FP64 CG on the
-same GPU is timed beside it and the ratio reported.  Prints one JSON line.
+Learned workload (the value): dual (trivalent) refined icosahedron n=128 (327,680 sites)
+x Lt=64, antiperiodic in time, r=1, M5=m=0.02, von Mises(kappa=1.4) link angles, Gaussian
+RHS.  Three solvers are timed on the same seeded RHS (one complete solve per step): the
+flexible CG with the learned preconditioner M = N N^dagger (FP32 vectors, FP64 reliable
+updates; the net trained first, outside the timed region, on a small lattice of the same
+family, dual n=16 x Lt=16, its weights applied unchanged at full size), the
+unpreconditioned mixed CG (FP32 + FP64 reliable updates) and the plain FP64 CG.  `value`
+is the solves/s of the fastest of them, named in config.solver; every method's figures
+and its ratio to plain FP64 CG are in config.methods.
+
+Second workload: simplicial n=128 (163,842 sites) x Lt=64, m=0.05: mixed CG vs plain FP64
+CG.  Prints one JSON line.
 """
 import argparse
 import json
@@ -32,6 +39,11 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=12)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-learned", action="store_true")
+    p.add_argument("--learned-steps", type=int, default=3)
+    p.add_argument("--train-n", type=int, default=16)
+    p.add_argument("--train-Lt", type=int, default=16)
+    p.add_argument("--train-epochs", type=int, default=40)
     p.add_argument("--kernels-only", action="store_true",
                    help="only launch the D stencils and the calibration copy (for profilers)")
     return p.parse_args()
@@ -127,6 +139,8 @@ def main():
 
     # roofline of the dominant kernel: the FP32 stencil (D) inside the mixed solver
     ms32 = D.time_apply(q.OP_D, q.F32, warmup=5, reps=50)
+    tile32 = tile_info(q, D, q.OP_D, q.F32)
+    D_V, D_E = D.V, D.E
     ms64 = D.time_apply(q.OP_D, q.F64, warmup=5, reps=50)
     b32 = algorithmic_bytes_D(D.V, D.E, a.Lt, 4)
     b64 = algorithmic_bytes_D(D.V, D.E, a.Lt, 8)
@@ -137,34 +151,162 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a, g, at, results.get("fp64", results["mixed"])["iters"], sp, tp)
+    del D, d_x, rhs
+    learned = None if a.no_learned else learned_leg(a, q, g)
 
-    best = results["mixed"]
+    simp = {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
+                        f"m={a.m} tol={a.tol} D^dag D CG", "methods": results,
+            "speedup_mixed_vs_fp64": (round(results["mixed"]["solves_per_s"]
+                                            / results["fp64"]["solves_per_s"], 3)
+                                      if "fp64" in results else None)}
+    solver_names = {"learned": "flexible CG, M = N N^dag (4-layer net), FP32 + FP64 reliable updates",
+                    "mixed": "CG, FP32 + FP64 reliable updates", "fp64": "CG, FP64"}
+    if learned is not None:
+        meth = learned["methods"]
+        winner = max(meth, key=lambda k: meth[k]["solves_per_s"])
+        best = meth[winner]
+        for k in meth:
+            if "fp64" in meth:
+                meth[k]["vs_fp64"] = round(meth[k]["solves_per_s"] / meth["fp64"]["solves_per_s"], 3)
+        steps, ms = a.learned_steps, best["ms_per_solve"]
+        config = {"workload": learned["workload"], "n_refine": a.n, "Lt": a.Lt,
+                  "sites": learned["sites"], "links": learned["links"],
+                  "solver": solver_names[winner],
+                  "methods": learned["methods"], "training": learned["training"],
+                  "speedup_learned_vs_fp64": learned["speedup_vs_fp64"],
+                  "speedup_mixed_vs_fp64": learned["speedup_mixed_vs_fp64"],
+                  "net_apply": learned["net_apply"], "second_workload": simp}
+        dtype = ("f32 inner (bf16 net activations) / f64 reliable updates" if winner == "learned"
+                 else "f32 inner / f64 reliable updates" if winner == "mixed" else "f64")
+    else:
+        best = results["mixed"]
+        steps, ms = a.steps, best["ms_per_solve"]
+        config = {"workload": simp["workload"], "n_refine": a.n, "Lt": a.Lt, "sites": D_V,
+                  "links": D_E, "solver": "FP32 CG + FP64 reliable updates", "methods": results,
+                  "speedup_mixed_vs_fp64": simp["speedup_mixed_vs_fp64"]}
+        dtype = "f32 inner / f64 reliable updates"
     line = {
         "metric": "CG solves/s to 1e-10 true resid, n=128 Lt=64: learned+FP32 vs plain FP64 CG",
         "value": round(best["solves_per_s"], 4),
         "unit": "solves/s",
-        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(best["ms_per_solve"], 3),
+        "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 inner / f64 reliable updates",
+        "dtype": dtype,
         "data": "synthetic: von Mises(kappa=%.2f) links, Gaussian RHS, seeded" % a.beta,
-        "config": {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
-                               f"m={a.m} tol={a.tol} D^dag D CG", "n_refine": a.n, "Lt": a.Lt,
-                   "sites": D.V, "links": D.E, "solver": "FP32 CG + FP64 reliable updates"},
-        "methods": results,
-        "speedup_vs_fp64": (round(best["solves_per_s"] / results["fp64"]["solves_per_s"], 3)
-                            if "fp64" in results else None),
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_stencil<float,6,false,EpiStore> (D, FP32)",
+                     "kernel": ("k_stencil_lds" if tile32["lds"] else "k_stencil")
+                               + "<float,6,false,EpiStore> (D, FP32, simplicial n=128 Lt=64)",
                      "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5),
-                     "tile": tile_info(q, D, q.OP_D, q.F32),
+                     "tile": tile32,
                      "copy_kernel_GBs": round(copy_gbs, 1),
                      "fp64_D": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBS, 4),
                                 "bytes_per_launch": b64, "ms_per_launch": round(ms64, 5)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def timed_solves(q, a, name, run, rhs, warmup, steps):
+    for k in range(warmup):
+        run(rhs[k])
+    q.load().qed3_device_synchronize()
+    t0 = time.time()
+    reps = []
+    for k in range(warmup, warmup + steps):
+        r = run(rhs[k])
+        reps.append(r)
+        print(f"# {name} solve {k}: iters={r.iterations} ru={r.reliable_updates} "
+              f"resid={r.true_resid:.2e} {r.seconds:.3f}s", file=sys.stderr, flush=True)
+    q.load().qed3_device_synchronize()
+    dt = time.time() - t0
+    return dict(solves_per_s=steps / dt, ms_per_solve=1e3 * dt / steps,
+                iters=float(np.mean([r.iterations for r in reps])),
+                reliable_updates=float(np.mean([r.reliable_updates for r in reps])),
+                max_true_resid=float(max(r.true_resid for r in reps)))
+
+
+def learned_leg(a, q, g):
+    """Dual n x Lt at m=0.02: train on dual train_n x train_Lt, then time FCG with the net,
+    mixed CG and FP64 CG on the same seeded RHS."""
+    from qed3_amd import train
+    m, lattice = 0.02, 1
+    t0 = time.time()
+    gt = q.Geometry(a.train_n)
+    Dt = q.WilsonDirac(lattice, a.train_n, a.train_Lt, r=1.0, M5=m, at=0.5 * mean_ell(gt, lattice))
+    data = train.make_pairs(Dt, lattice, a.train_n, a.train_Lt, cfg_seeds=[11, 12, 13],
+                            betas=[1.0, 1.4, 2.0])
+    t_pairs = time.time() - t0
+    net_t = q.Net(Dt, layers=4, seed=5)
+    hist = train.train(net_t, Dt, data, epochs=a.train_epochs, batch=4, lr=3e-3)
+    t_train = time.time() - t0
+    print(f"# trained on dual n={a.train_n} Lt={a.train_Lt}: loss {hist[0]:.3f} -> {hist[-1]:.4f} "
+          f"in {t_train:.1f}s", file=sys.stderr, flush=True)
+    at = 0.5 * mean_ell(g, lattice)
+    D = q.WilsonDirac(lattice, a.n, a.Lt, r=1.0, M5=m, at=at)
+    sp, tp = q.random_gauge(lattice, a.n, a.Lt, seed=4321, kappa=a.beta)
+    D.update(sp, tp)
+    D.autotune()
+    net = q.Net(D, layers=4, seed=5)
+    net.params = net_t.params
+    del net_t, Dt, data
+    nbytes = D.N * 16
+    d_x = q.DeviceArray(nbytes)
+    rhs = [q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=2000 + k))
+           for k in range(a.warmup + a.learned_steps)]
+    methods = {}
+    stats = []
+
+    def run_fcg(b):
+        rep, applies, fb = net.fcg(d_x, b, tol=a.tol)
+        stats.append((applies, fb))
+        return rep
+
+    methods["learned"] = timed_solves(q, a, "learned", run_fcg, rhs, a.warmup, a.learned_steps)
+    methods["learned"]["fallbacks"] = int(sum(fb for _, fb in stats[a.warmup:]))
+    methods["mixed"] = timed_solves(q, a, "mixed", lambda b: D.solve_device(d_x, b, tol=a.tol,
+                                    solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)
+    if not a.no_fp64:
+        methods["fp64"] = timed_solves(q, a, "fp64", lambda b: D.solve_device(d_x, b, tol=a.tol,
+                                       solver=q.CG_F64), rhs, a.warmup, a.learned_steps)
+    # one net application N (and N^dag) at full size, device-resident fields
+    d_y = q.DeviceArray(D.N * 8)
+    d_z = q.DeviceArray(D.N * 8)
+    for fn in (net.apply_device, net.apply_adj_device):
+        fn(d_z, d_y, 1)
+    q.load().qed3_device_synchronize()
+    reps = 10
+    t1 = time.time()
+    for _ in range(reps):
+        net.apply_device(d_z, d_y, 1)
+    q.load().qed3_device_synchronize()
+    ms_net = 1e3 * (time.time() - t1) / reps
+    t1 = time.time()
+    for _ in range(reps):
+        net.apply_adj_device(d_z, d_y, 1)
+    q.load().qed3_device_synchronize()
+    ms_adj = 1e3 * (time.time() - t1) / reps
+    pts = D.V * a.Lt
+    # algorithmic bytes of one N: input spinor read (8 B complex x 2) + 3 layers' BF16
+    # activations written and read (16 channels x 2 spins x 4 B) + output spinor written,
+    # plus the link phases once per layer
+    b_net = pts * (16 + 3 * 2 * 128 + 16) + 4 * (D.E + D.V) * a.Lt * 8
+    fp64 = methods.get("fp64")
+    return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG",
+            "sites": D.V, "links": D.E, "methods": methods,
+            "training": {"lattice": f"dual n={a.train_n} Lt={a.train_Lt}", "pairs": 3 * 40,
+                         "epochs": a.train_epochs, "loss_first": round(hist[0], 4),
+                         "loss_last": round(hist[-1], 5), "seconds": round(t_train, 1),
+                         "pair_generation_s": round(t_pairs, 1)},
+            "speedup_vs_fp64": (round(methods["learned"]["solves_per_s"] / fp64["solves_per_s"], 3)
+                                if fp64 else None),
+            "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
+                                      if fp64 else None),
+            "net_apply": {"ms": round(ms_net, 4), "ms_adjoint": round(ms_adj, 4),
+                          "algorithmic_GB": round(b_net / 1e9, 3),
+                          "achieved_GBs": round(b_net / (ms_net * 1e-3) / 1e9, 1)}}
 
 
 def cpu_baseline(a, g, at, gpu_iters, sp_full, tp_full):

@@ -333,6 +333,12 @@ class Net:
         check(load().qed3_net_apply(self._h, d_out.ptr, d_in.ptr, B))
         return d_out.to_numpy(np.complex64).reshape(r.shape)
 
+    def apply_device(self, d_out, d_in, batch=1):
+        check(load().qed3_net_apply(self._h, d_out.ptr, d_in.ptr, batch))
+
+    def apply_adj_device(self, d_out, d_in, batch=1):
+        check(load().qed3_net_apply_adjoint(self._h, d_out.ptr, d_in.ptr, batch))
+
     def apply_adj(self, r):
         """N^dagger(r) for a host array of shape [B, N] (or [N]) complex."""
         r = np.ascontiguousarray(r, np.complex64)

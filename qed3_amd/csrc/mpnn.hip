@@ -3,6 +3,7 @@
 // weight-gradient GEMMs) and the Adam training loop.  See mpnn.hpp for the network.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <random>
 #include <stdexcept>
 
@@ -127,12 +128,16 @@ __device__ __forceinline__ void feat1(const NetGeo& g, const Spin<float>* __rest
 }
 
 // ---------------------------------------------------------------- forward kernels
+// ADJ = true runs the adjoint network's first stage: adjoint feature maps with the head
+// weights prepared by upload_params (conjugated output-head weights, spins of the gamma
+// term swapped).  The same holds for k_net_mid and k_net_out below.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_net_in(NetGeo g, const Spin<float>* __restrict__ r,
                                                 const float2* __restrict__ v, bf16* __restrict__ h1) {
   const size_t P = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (P >= g.np) return;
   float2 F[5][2];
-  feat1<false>(g, r, P, F);
+  feat1<ADJ>(g, r, P, F);
   for (int a = 0; a < 2; a++) {
     bf16 o[RW];
 #pragma unroll
@@ -159,7 +164,9 @@ __device__ __forceinline__ void load4(const bf16* __restrict__ h, size_t P, int 
   for (int i = 0; i < 4; i++) o[i] = {float(w[2 * i]), float(w[2 * i + 1])};
 }
 
-// Features of a C-channel BF16 field for channels [4 grp, 4 grp + 4) at point P.
+// Features of a C-channel BF16 field for channels [4 grp, 4 grp + 4) at point P (ADJ: the
+// adjoint maps, F_k^dagger in slot k, as in feat1).
+template <bool ADJ>
 __device__ __forceinline__ void feat4(const NetGeo& g, const bf16* __restrict__ h, size_t P, int grp,
                                       float2 F[5][2][4]) {
   const PointIdx q = point_of(g, P);
@@ -168,11 +175,11 @@ __device__ __forceinline__ void feat4(const NetGeo& g, const bf16* __restrict__ 
 #pragma unroll
   for (int i = 0; i < 4; i++) F[1][0][i] = F[1][1][i] = F[2][0][i] = F[2][1][i] = float2{0.f, 0.f};
   for (int j = 0; j < g.M; j++) {
-    const SlotT t = slot_t(g, j, q.x, q.s, false);
+    const SlotT t = slot_t(g, j, q.x, q.s, ADJ);
     float2 h0[4], h1[4];
     load4(h, q.base + t.y, 0, grp, h0);
     load4(h, q.base + t.y, 1, grp, h1);
-    const float2 k0 = cm(t.ea, t.t1), k1 = cm(t.eb, t.t0);
+    const float2 k0 = ADJ ? cm(t.ea, t.t0) : cm(t.ea, t.t1), k1 = ADJ ? cm(t.eb, t.t1) : cm(t.eb, t.t0);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       cadd_(F[1][0][i], cm(t.t0, h0[i]));
@@ -185,13 +192,14 @@ __device__ __forceinline__ void feat4(const NetGeo& g, const bf16* __restrict__ 
   const float2 um = cj(g.uT[size_t(q.s == 0 ? g.Lt - 1 : q.s - 1) * g.V + q.x]);
   const float2 wp = {q.bp * up.x, q.bp * up.y}, wm = {q.bm * um.x, q.bm * um.y};
   float2 t[4];
+  const int i3 = ADJ ? 4 : 3, i4 = ADJ ? 3 : 4;
   for (int a = 0; a < 2; a++) {
     load4(h, q.Pp, a, grp, t);
 #pragma unroll
-    for (int i = 0; i < 4; i++) F[3][a][i] = cm(wp, t[i]);
+    for (int i = 0; i < 4; i++) F[i3][a][i] = cm(wp, t[i]);
     load4(h, q.Pm, a, grp, t);
 #pragma unroll
-    for (int i = 0; i < 4; i++) F[4][a][i] = cm(wm, t[i]);
+    for (int i = 0; i < 4; i++) F[i4][a][i] = cm(wm, t[i]);
   }
 }
 
@@ -200,6 +208,7 @@ __device__ __forceinline__ void feat4(const NetGeo& g, const bf16* __restrict__ 
 // rows 8(i/4) + 4(l/32) + i%4 for accumulator element i.
 // Columns of a block are ordered (spin a, point p): n = a PTS + p, so waves 0-1 hold spin
 // 0 and waves 2-3 spin 1, and each wave multiplies by its spin's weight matrix.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_net_mid(NetGeo g, const bf16* __restrict__ hin,
                                                  const bf16* __restrict__ A /*[2][32][160]*/,
                                                  bf16* __restrict__ hout) {
@@ -211,7 +220,7 @@ __global__ __launch_bounds__(256) void k_net_mid(NetGeo g, const bf16* __restric
     const size_t P = P0 + p;
     float2 F[5][2][4];
     if (P < g.np) {
-      feat4(g, hin, P, grp, F);
+      feat4<ADJ>(g, hin, P, grp, F);
     } else {
       for (int k = 0; k < 5; k++)
         for (int a = 0; a < 2; a++)
@@ -252,6 +261,7 @@ __global__ __launch_bounds__(256) void k_net_mid(NetGeo g, const bf16* __restric
 }
 
 // out = sum_k F_k(hhat_k) + skip r, hhat_k = sum_c u_k[c] h_c (the maps act channel-wise)
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_net_out(NetGeo g, const bf16* __restrict__ h,
                                                  const float2* __restrict__ u,
                                                  const Spin<float>* __restrict__ r, float skip,
@@ -274,22 +284,23 @@ __global__ __launch_bounds__(256) void k_net_out(NetGeo g, const bf16* __restric
   };
   float2 o0 = proj(P, 0, 0, 0), o1 = proj(P, 0, 1, 1);
   for (int j = 0; j < g.M; j++) {
-    const SlotT t = slot_t(g, j, q.x, q.s, false);
+    const SlotT t = slot_t(g, j, q.x, q.s, ADJ);
     const size_t Q = q.base + t.y;
     const float2 a0 = proj(Q, 1, 0, 0), a1 = proj(Q, 1, 1, 1);
     const float2 b0 = proj(Q, 2, 0, 1), b1 = proj(Q, 2, 1, 0);
     cadd_(o0, cm(t.t0, a0));
     cadd_(o1, cm(t.t1, a1));
-    cadd_(o0, cm(cm(t.ea, t.t1), b1));
-    cadd_(o1, cm(cm(t.eb, t.t0), b0));
+    cadd_(o0, cm(cm(t.ea, ADJ ? t.t0 : t.t1), b1));
+    cadd_(o1, cm(cm(t.eb, ADJ ? t.t1 : t.t0), b0));
   }
   const float2 up = g.uT[size_t(q.s) * g.V + q.x];
   const float2 um = cj(g.uT[size_t(q.s == 0 ? g.Lt - 1 : q.s - 1) * g.V + q.x]);
   const float2 wp = {q.bp * up.x, q.bp * up.y}, wm = {q.bm * um.x, q.bm * um.y};
-  cadd_(o0, cm(wp, proj(q.Pp, 3, 0, 0)));
-  cadd_(o1, cm(wp, proj(q.Pp, 3, 1, 1)));
-  cadd_(o0, cm(wm, proj(q.Pm, 4, 0, 0)));
-  cadd_(o1, cm(wm, proj(q.Pm, 4, 1, 1)));
+  const int k3 = ADJ ? 4 : 3, k4 = ADJ ? 3 : 4;  // the adjoint of F3 reads s-1, of F4 s+1
+  cadd_(o0, cm(wp, proj(q.Pp, k3, 0, 0)));
+  cadd_(o1, cm(wp, proj(q.Pp, k3, 1, 1)));
+  cadd_(o0, cm(wm, proj(q.Pm, k4, 0, 0)));
+  cadd_(o1, cm(wm, proj(q.Pm, k4, 1, 1)));
   const Spin<float> rp = r[P];
   Spin<float> s;
   s.a = {o0.x + skip * rp.a.x, o0.y + skip * rp.a.y};
@@ -432,7 +443,7 @@ __global__ __launch_bounds__(256) void k_net_mid_dW(NetGeo g, const bf16* __rest
     const size_t P = P0 + p;
     float2 F[5][2][4];
     const bool ok = P < g.np;
-    if (ok) feat4(g, hin, P, grp, F);
+    if (ok) feat4<false>(g, hin, P, grp, F);
     for (int a = 0; a < 2; a++) {
       const int col = a * PTS + p;
       for (int k = 0; k < 5; k++)
@@ -523,51 +534,6 @@ __global__ __launch_bounds__(256) void k_net_adj(NetGeo g, const float* __restri
   }
 }
 
-// Input head adjoint: out_a = sum_k [F_k^dagger w_k]_a + skip in_a with the single-channel
-// fields w_k(Q)_a = sum_c conj(v[a][k][c]) dh(Q)_c,a, formed on the fly at each gathered point
-// (the adjoint transports as in k_net_adj).
-__global__ __launch_bounds__(256) void k_net_in_adj(NetGeo g, const float* __restrict__ dh,
-                                                    const float2* __restrict__ v,
-                                                    const Spin<float>* __restrict__ in, float skip,
-                                                    Spin<float>* __restrict__ out) {
-  const size_t P = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (P >= g.np) return;
-  const PointIdx q = point_of(g, P);
-  auto w = [&](size_t Q, int k, int a) {
-    const float4* src = reinterpret_cast<const float4*>(dh + (Q * 2 + a) * RW);
-    const float2* vk = v + (a * 5 + k) * C;
-    float2 z = {0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < C / 2; i++) {
-      const float4 d = src[i];
-      cadd_(z, cmc(vk[2 * i], float2{d.x, d.y}));
-      cadd_(z, cmc(vk[2 * i + 1], float2{d.z, d.w}));
-    }
-    return z;
-  };
-  float2 r0 = w(P, 0, 0), r1 = w(P, 0, 1);
-  for (int j = 0; j < g.M; j++) {
-    const SlotT s = slot_t(g, j, q.x, q.s, true);
-    const size_t Q = q.base + s.y;
-    cadd_(r0, cm(s.t0, w(Q, 1, 0)));
-    cadd_(r0, cm(cm(s.ea, s.t0), w(Q, 2, 1)));
-    cadd_(r1, cm(s.t1, w(Q, 1, 1)));
-    cadd_(r1, cm(cm(s.eb, s.t1), w(Q, 2, 0)));
-  }
-  const float2 up = g.uT[size_t(q.s) * g.V + q.x];
-  const float2 um = cj(g.uT[size_t(q.s == 0 ? g.Lt - 1 : q.s - 1) * g.V + q.x]);
-  const float2 wm = {q.bm * um.x, q.bm * um.y}, wp = {q.bp * up.x, q.bp * up.y};
-  cadd_(r0, cm(wm, w(q.Pm, 3, 0)));
-  cadd_(r1, cm(wm, w(q.Pm, 3, 1)));
-  cadd_(r0, cm(wp, w(q.Pp, 4, 0)));
-  cadd_(r1, cm(wp, w(q.Pp, 4, 1)));
-  const Spin<float> x = in[P];
-  Spin<float> o;
-  o.a = {r0.x + skip * x.a.x, r0.y + skip * x.a.y};
-  o.b = {r1.x + skip * x.b.x, r1.y + skip * x.b.y};
-  out[P] = o;
-}
-
 // out[j] = sum_i buf[i][j] in FP64, one block per column group
 __global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ buf, size_t rows, int cols,
                                                 double* __restrict__ out) {
@@ -626,10 +592,11 @@ Net::Net(Dirac& D, int layers, int seed) : D_(D), L_(layers) {
   for (int x = 0; x < h.V; x++) kth[x] = float(0.5 * h.kappa_t[x]);
   kt_ = dmalloc<float>(h.V);
   QED3_HIP(hipMemcpy(kt_, kth.data(), kth.size() * 4, hipMemcpyHostToDevice));
-  heads_ = dmalloc<float>(40 * C);
+  heads_ = dmalloc<float>(80 * C);
   const int nmid = std::max(L_ - 2, 0);
   QED3_HIP(hipMalloc(&mixA_, std::max(nmid, 1) * 2 * RW * KF * sizeof(bf16)));
   QED3_HIP(hipMalloc(&mixAT_, std::max(nmid, 1) * 2 * RW * KF * sizeof(bf16)));
+  QED3_HIP(hipMalloc(&mixAH_, std::max(nmid, 1) * 2 * RW * KF * sizeof(bf16)));
   // initialisation: heads and mixing weights ~ N(0, sigma^2) complex, small
   p_.assign(2 * nparams(), 0.0);
   std::mt19937_64 rng(seed);
@@ -653,7 +620,7 @@ Net::Net(Dirac& D, int layers, int seed) : D_(D), L_(layers) {
 }
 
 Net::~Net() {
-  for (void* p : {(void*)slot_, (void*)slot_nb_, (void*)kt_, (void*)heads_, mixA_, mixAT_, act_, (void*)gfield_,
+  for (void* p : {(void*)slot_, (void*)slot_nb_, (void*)kt_, (void*)heads_, mixA_, mixAT_, mixAH_, act_, (void*)gfield_,
                   (void*)gfield2_, (void*)G_, (void*)out_, (void*)gout_, (void*)partial_, (void*)red_})
     if (p) (void)hipFree(p);
 }
@@ -667,35 +634,68 @@ void Net::set_params(const double* p) {
 
 // Real form of z = W f on interleaved (re, im) rows: R[2c][2d] = Re W, R[2c][2d+1] = -Im W,
 // R[2c+1][2d] = Im W, R[2c+1][2d+1] = Re W, for the five feature blocks side by side; one
-// matrix per layer and output spin.
+// matrix per layer and output spin.  The adjoint network N^dagger uses the same kernels
+// with the adjoint feature maps and, at output spin a, the weights W_a,k^dagger (k != 2) and
+// W_{1-a},2^dagger (the gamma map swaps spins); its heads are the conjugated output and
+// input heads with the same spin swap.
 void Net::upload_params() {
-  std::vector<float> heads(40 * C);
   const size_t nw = size_t(std::max(L_ - 2, 0)) * 10 * C * C;
-  for (size_t i = 0; i < 20 * C; i++) heads[i] = float(p_[i]);                        // v
-  for (size_t i = 0; i < 20 * C; i++) heads[20 * C + i] = float(p_[2 * (10 * C + nw) + i]);  // u
+  auto cpar = [&](size_t i) { return std::complex<double>(p_[2 * i], p_[2 * i + 1]); };
+  auto v_at = [&](int a, int k, int c) { return cpar((size_t(a) * 5 + k) * C + c); };
+  auto u_at = [&](int a, int k, int c) { return cpar(10 * C + nw + (size_t(a) * 5 + k) * C + c); };
+  auto w_at = [&](int l, int a, int k, int c, int d) {
+    return cpar(10 * C + (((size_t(l) * 2 + a) * 5 + k) * C + c) * C + d);
+  };
+  // heads: v, u, then the adjoint network's input head (from u) and output head (from v)
+  std::vector<float> heads(80 * C);
+  for (int a = 0; a < 2; a++)
+    for (int k = 0; k < 5; k++)
+      for (int c = 0; c < C; c++) {
+        const size_t i = (size_t(a) * 5 + k) * C + c;
+        const int sa = k == 2 ? 1 - a : a;
+        const std::complex<double> vv = v_at(a, k, c), uu = u_at(a, k, c);
+        const std::complex<double> vh = std::conj(u_at(sa, k, c)), uh = std::conj(v_at(sa, k, c));
+        heads[2 * i] = float(vv.real());
+        heads[2 * i + 1] = float(vv.imag());
+        heads[20 * C + 2 * i] = float(uu.real());
+        heads[20 * C + 2 * i + 1] = float(uu.imag());
+        heads[40 * C + 2 * i] = float(vh.real());
+        heads[40 * C + 2 * i + 1] = float(vh.imag());
+        heads[60 * C + 2 * i] = float(uh.real());
+        heads[60 * C + 2 * i + 1] = float(uh.imag());
+      }
   QED3_HIP(hipMemcpy(heads_, heads.data(), heads.size() * 4, hipMemcpyHostToDevice));
   const int nmid = std::max(L_ - 2, 0);
-  std::vector<bf16> A(size_t(std::max(nmid, 1)) * 2 * RW * KF), AT(A.size());
+  std::vector<bf16> A(size_t(std::max(nmid, 1)) * 2 * RW * KF), AT(A.size()), AH(A.size());
+  auto put = [](std::vector<bf16>& M, size_t base, int c, int d, int k, std::complex<double> w, bool tr) {
+    const float re = float(w.real()), im = float(w.imag());
+    const int col = k * RW + 2 * d;
+    if (!tr) {
+      M[base + (2 * c) * KF + col] = bf16(re);
+      M[base + (2 * c) * KF + col + 1] = bf16(-im);
+      M[base + (2 * c + 1) * KF + col] = bf16(im);
+      M[base + (2 * c + 1) * KF + col + 1] = bf16(re);
+    } else {
+      M[base + size_t(col) * RW + 2 * c] = bf16(re);
+      M[base + size_t(col + 1) * RW + 2 * c] = bf16(-im);
+      M[base + size_t(col) * RW + 2 * c + 1] = bf16(im);
+      M[base + size_t(col + 1) * RW + 2 * c + 1] = bf16(re);
+    }
+  };
   for (int l = 0; l < nmid; l++)
     for (int sa = 0; sa < 2; sa++)
-    for (int k = 0; k < 5; k++)
-      for (int c = 0; c < C; c++)
-        for (int d = 0; d < C; d++) {
-          const size_t wi = 10 * C + (((size_t(l) * 2 + sa) * 5 + k) * C + c) * C + d;
-          const float re = float(p_[2 * wi]), im = float(p_[2 * wi + 1]);
-          const size_t base = (size_t(l) * 2 + sa) * RW * KF;
-          const int col = k * RW + 2 * d;
-          A[base + (2 * c) * KF + col] = bf16(re);
-          A[base + (2 * c) * KF + col + 1] = bf16(-im);
-          A[base + (2 * c + 1) * KF + col] = bf16(im);
-          A[base + (2 * c + 1) * KF + col + 1] = bf16(re);
-          AT[base + size_t(col) * RW + 2 * c] = bf16(re);
-          AT[base + size_t(col + 1) * RW + 2 * c] = bf16(-im);
-          AT[base + size_t(col) * RW + 2 * c + 1] = bf16(im);
-          AT[base + size_t(col + 1) * RW + 2 * c + 1] = bf16(re);
-        }
+      for (int k = 0; k < 5; k++)
+        for (int c = 0; c < C; c++)
+          for (int d = 0; d < C; d++) {
+            const size_t base = (size_t(l) * 2 + sa) * RW * KF;
+            const std::complex<double> w = w_at(l, sa, k, c, d);
+            put(A, base, c, d, k, w, false);
+            put(AT, base, c, d, k, w, true);
+            put(AH, base, c, d, k, std::conj(w_at(l, k == 2 ? 1 - sa : sa, k, d, c)), false);
+          }
   QED3_HIP(hipMemcpy(mixA_, A.data(), A.size() * sizeof(bf16), hipMemcpyHostToDevice));
   QED3_HIP(hipMemcpy(mixAT_, AT.data(), AT.size() * sizeof(bf16), hipMemcpyHostToDevice));
+  QED3_HIP(hipMemcpy(mixAH_, AH.data(), AH.size() * sizeof(bf16), hipMemcpyHostToDevice));
 }
 
 void Net::ensure(int B, bool training) {
@@ -736,41 +736,37 @@ void Net::apply(Spin<float>* out, const Spin<float>* r, int B) {
   const unsigned nb = unsigned((g.np + 255) / 256), nbm = unsigned((g.np + PTS - 1) / PTS);
   const float2* v = reinterpret_cast<const float2*>(heads_);
   const float2* u = v + 10 * C;
-  hipLaunchKernelGGL(k_net_in, dim3(nb), dim3(256), 0, st, g, r, v, a0);
+  hipLaunchKernelGGL(k_net_in<false>, dim3(nb), dim3(256), 0, st, g, r, v, a0);
   for (int l = 0; l < L_ - 2; l++) {
-    hipLaunchKernelGGL(k_net_mid, dim3(nbm), dim3(256), 0, st, g, a0,
+    hipLaunchKernelGGL(k_net_mid<false>, dim3(nbm), dim3(256), 0, st, g, a0,
                        static_cast<const bf16*>(mixA_) + size_t(l) * 2 * RW * KF, a1);
     std::swap(a0, a1);
   }
-  hipLaunchKernelGGL(k_net_out, dim3(nb), dim3(256), 0, st, g, a0, u, r, skip_, out);
+  hipLaunchKernelGGL(k_net_out<false>, dim3(nb), dim3(256), 0, st, g, a0, u, r, skip_, out);
   QED3_HIP(hipGetLastError());
 }
 
-// out = N^dagger(r): the adjoint layers in reverse order (output head, then each mixing
-// layer as R^T followed by the adjoint feature maps, then the input head).  The net is
-// linear, so no forward activations are needed.
+// out = N^dagger(r): the forward kernels in adjoint mode, layers in reverse order (the
+// net is linear, so no forward activations are needed); costs the same as N.
 void Net::apply_adj(Spin<float>* out, const Spin<float>* r, int B) {
-  ensure(B, true);
+  ensure(B, false);
   const DiracView<float> dv = D_.view<float>();
   const HopTables& h = D_.tables();
   NetGeo g{dv.nbr, dv.lnk, reinterpret_cast<const float4*>(slot_), reinterpret_cast<const float2*>(slot_nb_),
            dv.uS, dv.uT, kt_, h.V, h.E, h.Lt, h.max_nn, dv.bc, size_t(B) * D_.points()};
   hipStream_t st = D_.stream();
-  const size_t np = g.np;
-  const unsigned nb = unsigned((np + 255) / 256), nbm = unsigned((np + PTS - 1) / PTS);
-  const float2* v = reinterpret_cast<const float2*>(heads_);
-  const float2* u = v + 10 * C;
-  float* gin = gfield_;
-  float* gnext = gfield2_;
-  hipLaunchKernelGGL(k_net_out_bwd, dim3(nb), dim3(256), 0, st, g, r, static_cast<const bf16*>(nullptr), u,
-                     gin, static_cast<float*>(nullptr));
+  bf16* a0 = static_cast<bf16*>(act_);
+  bf16* a1 = a0 + g.np * 2 * RW;
+  const unsigned nb = unsigned((g.np + 255) / 256), nbm = unsigned((g.np + PTS - 1) / PTS);
+  const float2* vh = reinterpret_cast<const float2*>(heads_) + 20 * C;
+  const float2* uh = vh + 10 * C;
+  hipLaunchKernelGGL(k_net_in<true>, dim3(nb), dim3(256), 0, st, g, r, vh, a0);
   for (int l = L_ - 3; l >= 0; l--) {
-    hipLaunchKernelGGL(k_net_mid_G, dim3(nbm), dim3(256), 0, st, np, gin,
-                       static_cast<const bf16*>(mixAT_) + size_t(l) * 2 * RW * KF, G_);
-    hipLaunchKernelGGL(k_net_adj, dim3(unsigned((4 * np + 255) / 256)), dim3(256), 0, st, g, G_, gnext);
-    std::swap(gin, gnext);
+    hipLaunchKernelGGL(k_net_mid<true>, dim3(nbm), dim3(256), 0, st, g, a0,
+                       static_cast<const bf16*>(mixAH_) + size_t(l) * 2 * RW * KF, a1);
+    std::swap(a0, a1);
   }
-  hipLaunchKernelGGL(k_net_in_adj, dim3(nb), dim3(256), 0, st, g, gin, v, r, skip_, out);
+  hipLaunchKernelGGL(k_net_out<true>, dim3(nb), dim3(256), 0, st, g, a0, uh, r, skip_, out);
   QED3_HIP(hipGetLastError());
 }
 
@@ -788,11 +784,11 @@ double Net::loss_grad(const Spin<float>* r, const Spin<float>* e, int B, std::ve
   const int nmid = L_ - 2;
   auto act = [&](int j) { return static_cast<bf16*>(act_) + size_t(j) * np * 2 * RW; };
   // forward, keeping h1..h^{L-1}
-  hipLaunchKernelGGL(k_net_in, dim3(nb), dim3(256), 0, st, g, r, v, act(0));
+  hipLaunchKernelGGL(k_net_in<false>, dim3(nb), dim3(256), 0, st, g, r, v, act(0));
   for (int l = 0; l < nmid; l++)
-    hipLaunchKernelGGL(k_net_mid, dim3(nbm), dim3(256), 0, st, g, act(l),
+    hipLaunchKernelGGL(k_net_mid<false>, dim3(nbm), dim3(256), 0, st, g, act(l),
                        static_cast<const bf16*>(mixA_) + size_t(l) * 2 * RW * KF, act(l + 1));
-  hipLaunchKernelGGL(k_net_out, dim3(nb), dim3(256), 0, st, g, act(nmid), u, r, skip_, out_);
+  hipLaunchKernelGGL(k_net_out<false>, dim3(nb), dim3(256), 0, st, g, act(nmid), u, r, skip_, out_);
   // loss
   QED3_HIP(hipMemsetAsync(red_, 0, 2 * B * sizeof(double), st));
   hipLaunchKernelGGL(k_net_loss, dim3(nb), dim3(256), 0, st, out_, e, np, per, red_);

@@ -47,7 +47,7 @@ class Net {
 
   // out = N(r) for B fields of D.points() FP32 spinors each (device)
   void apply(Spin<float>* out, const Spin<float>* r, int B);
-  // out = N^dagger(r) (uses the training buffers; same layout)
+  // out = N^dagger(r) (same layout and cost as apply)
   void apply_adj(Spin<float>* out, const Spin<float>* r, int B);
   // loss = (1/B) sum_b |N(r_b) - e_b|^2/|e_b|^2 (N including the skip term) and its gradient (d/dRe + i d/dIm)
   double loss_grad(const Spin<float>* r, const Spin<float>* e, int B, std::vector<double>& grad);
@@ -66,9 +66,10 @@ class Net {
   float* slot_ = nullptr;      // [M][V] float4 (C, S, cos a_xy, sin a_xy)
   float* slot_nb_ = nullptr;   // [M][V] float2 (cos a_yx, sin a_yx)
   float* kt_ = nullptr;        // [V] kappa_t(x)/2
-  float* heads_ = nullptr;     // v[2][5][C], u[2][5][C] interleaved complex FP32
+  float* heads_ = nullptr;     // v, u and the adjoint network's two heads, [2][5][C] complex FP32 each
   void* mixA_ = nullptr;       // per middle layer and output spin: R (32 x 160) bf16
   void* mixAT_ = nullptr;      // per middle layer and output spin: R^T (160 x 32) bf16
+  void* mixAH_ = nullptr;      // adjoint network: per layer and output spin (32 x 160) bf16
   void* act_ = nullptr;        // activations h1..h^{L-1}: [L-1][B*points][2][32] bf16
   float* gfield_ = nullptr;    // gradient field [B*points][2][32] FP32 (two buffers)
   float* gfield2_ = nullptr;

@@ -5,6 +5,7 @@ import pytest
 
 import oracle as O
 import qed3_amd as q
+from qed3_amd import train
 from conftest import rel
 from pairs import geometry, links, oracle_setup, source, wilson_hop_params
 
@@ -134,42 +135,6 @@ def test_net_is_gauge_covariant(gpu):
     assert rel(out1, G * out0) < 3e-2
 
 
-def make_pairs(Dq, lattice, n, Lt, cfg_seeds, rhs_seeds, ks, betas, nrandom=8):
-    """(D e, e) pairs per gauge configuration, for a net that approximates D^{-1}: Gaussian
-    e (every mode weighs the same) and the errors of the solver's own CG iterates (low
-    modes dominate), each pair scaled to |D e| = 1."""
-    data = []
-    for cs, beta in zip(cfg_seeds, betas):
-        sp, tp = links(lattice, n, Lt, seed=cs, kappa=beta)
-        Dq.update(sp, tp)
-        rs, es = [], []
-        for j in range(nrandom):
-            e = source(lattice, n, Lt, seed=1000 * cs + j)
-            r = Dq.mult(e)
-            s = 1.0 / np.linalg.norm(r)
-            rs.append(r * s)
-            es.append(e * s)
-        for rsd in rhs_seeds:
-            b = source(lattice, n, Lt, seed=rsd)
-            xs, _ = Dq.solve(b, tol=1e-11, solver=q.CG_F64)
-            for k in ks:
-                if k == 0:
-                    xk = np.zeros_like(b)  # x0 = 0: r = b, e = x*
-                else:
-                    d_b = q.DeviceArray.from_numpy(b)
-                    d_x = q.DeviceArray(b.nbytes)
-                    Dq.solve_device(d_x, d_b, tol=1e-14, maxiter=k, solver=q.CG_F64,
-                                    raise_on_fail=False)
-                    xk = d_x.to_numpy(np.complex128)
-                ek = xs - xk
-                rk = Dq.mult(ek)
-                s = 1.0 / np.linalg.norm(rk)
-                rs.append(rk * s)
-                es.append(ek * s)
-        data.append((sp, tp, np.array(rs), np.array(es)))
-    return data
-
-
 @pytest.mark.gpu
 def test_training_and_fcg_gain(gpu):
     """Train the net as an approximate inverse of D on (D e, e) pairs (three beta), then
@@ -178,18 +143,10 @@ def test_training_and_fcg_gain(gpu):
     lattice, n, Lt, M5 = 1, 4, 8, 0.02
     L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
     Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=M5, at=at)
-    data = make_pairs(Dq, lattice, n, Lt, cfg_seeds=[11, 12, 13], rhs_seeds=[1, 2, 3, 4],
-                      ks=[0, 1, 2, 4, 8, 16, 32, 64], betas=[1.0, 1.4, 2.0])
+    data = train.make_pairs(Dq, lattice, n, Lt, cfg_seeds=[11, 12, 13], betas=[1.0, 1.4, 2.0])
     net = q.Net(Dq, layers=4, seed=5)
-    first, last = None, None
-    for ep in range(40):
-        for sp, tp, rs, es in data:
-            Dq.update(sp, tp)
-            d_r = q.DeviceArray.from_numpy(rs.astype(np.complex64))
-            d_e = q.DeviceArray.from_numpy(es.astype(np.complex64))
-            h = net.train_device(d_r, d_e, npairs=len(rs), batch=4, epochs=1, lr=3e-3)
-            first = h[0] if first is None else first
-            last = h[-1]
+    hist = train.train(net, Dq, data, epochs=40, batch=4, lr=3e-3)
+    first, last = hist[0], hist[-1]
     print(f"training loss {first:.4f} -> {last:.4f}", flush=True)
     assert last < 0.7 * first
     # unseen configuration and RHS
