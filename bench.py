@@ -22,6 +22,10 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# Bytes moved per FP32 D launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
+# 1 GiB copy kernel) from two rocprofv3 --pmc runs of `bench.py --kernels-only --lattice 1
+# --m 0.02` at the autotuned tile; summary in profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.
+MEASURED_TRAFFIC = {(1, 128, 64): 1.424e9}
 
 
 def parse():
@@ -152,7 +156,7 @@ def main():
     if not a.no_cpu:
         cpu = cpu_baseline(a, g, at, results.get("fp64", results["mixed"])["iters"], sp, tp)
     del D, d_x, rhs
-    learned = None if a.no_learned else learned_leg(a, q, g)
+    learned = learned_leg(a, q, g, with_net=not a.no_learned)
 
     simp = {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
                         f"m={a.m} tol={a.tol} D^dag D CG", "methods": results,
@@ -161,7 +165,7 @@ def main():
                                       if "fp64" in results else None)}
     solver_names = {"learned": "flexible CG, M = N N^dag (4-layer net), FP32 + FP64 reliable updates",
                     "mixed": "CG, FP32 + FP64 reliable updates", "fp64": "CG, FP64"}
-    if learned is not None:
+    if True:
         meth = learned["methods"]
         winner = max(meth, key=lambda k: meth[k]["solves_per_s"])
         best = meth[winner]
@@ -176,15 +180,9 @@ def main():
                   "speedup_learned_vs_fp64": learned["speedup_vs_fp64"],
                   "speedup_mixed_vs_fp64": learned["speedup_mixed_vs_fp64"],
                   "net_apply": learned["net_apply"], "second_workload": simp}
+        roof = learned["roofline"]
         dtype = ("f32 inner (bf16 net activations) / f64 reliable updates" if winner == "learned"
                  else "f32 inner / f64 reliable updates" if winner == "mixed" else "f64")
-    else:
-        best = results["mixed"]
-        steps, ms = a.steps, best["ms_per_solve"]
-        config = {"workload": simp["workload"], "n_refine": a.n, "Lt": a.Lt, "sites": D_V,
-                  "links": D_E, "solver": "FP32 CG + FP64 reliable updates", "methods": results,
-                  "speedup_mixed_vs_fp64": simp["speedup_mixed_vs_fp64"]}
-        dtype = "f32 inner / f64 reliable updates"
     line = {
         "metric": "CG solves/s to 1e-10 true resid, n=128 Lt=64: learned+FP32 vs plain FP64 CG",
         "value": round(best["solves_per_s"], 4),
@@ -195,15 +193,17 @@ def main():
         "dtype": dtype,
         "data": "synthetic: von Mises(kappa=%.2f) links, Gaussian RHS, seeded" % a.beta,
         "config": config,
-        "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": ("k_stencil_lds" if tile32["lds"] else "k_stencil")
-                               + "<float,6,false,EpiStore> (D, FP32, simplicial n=128 Lt=64)",
-                     "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5),
-                     "tile": tile32,
-                     "copy_kernel_GBs": round(copy_gbs, 1),
-                     "fp64_D": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-                                "bytes_per_launch": b64, "ms_per_launch": round(ms64, 5)}},
+        "roofline": dict(roof, copy_kernel_GBs=round(copy_gbs, 1),
+                         simplicial={"kernel": ("k_stencil_lds" if tile32["lds"] else "k_stencil")
+                                     + "<float,6,false,EpiStore> (D, FP32, simplicial n=128 Lt=64)",
+                                     "achieved": round(gbs32, 1),
+                                     "frac": round(gbs32 / HBM_PEAK_GBS, 4),
+                                     "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5),
+                                     "tile": tile32,
+                                     "fp64_D": {"achieved": round(gbs64, 1),
+                                                "frac": round(gbs64 / HBM_PEAK_GBS, 4),
+                                                "bytes_per_launch": b64,
+                                                "ms_per_launch": round(ms64, 5)}}),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -228,11 +228,14 @@ def timed_solves(q, a, name, run, rhs, warmup, steps):
                 max_true_resid=float(max(r.true_resid for r in reps)))
 
 
-def learned_leg(a, q, g):
+def learned_leg(a, q, g, with_net=True):
     """Dual n x Lt at m=0.02: train on dual train_n x train_Lt, then time FCG with the net,
-    mixed CG and FP64 CG on the same seeded RHS."""
+    mixed CG and FP64 CG on the same seeded RHS (with_net=False: the two CGs only), and the
+    roofline of this workload's FP32 stencil."""
     from qed3_amd import train
     m, lattice = 0.02, 1
+    if not with_net:
+        return plain_leg(a, q, g, m, lattice)
     t0 = time.time()
     gt = q.Geometry(a.train_n)
     Dt = q.WilsonDirac(lattice, a.train_n, a.train_Lt, r=1.0, M5=m, at=0.5 * mean_ell(gt, lattice))
@@ -245,10 +248,7 @@ def learned_leg(a, q, g):
     print(f"# trained on dual n={a.train_n} Lt={a.train_Lt}: loss {hist[0]:.3f} -> {hist[-1]:.4f} "
           f"in {t_train:.1f}s", file=sys.stderr, flush=True)
     at = 0.5 * mean_ell(g, lattice)
-    D = q.WilsonDirac(lattice, a.n, a.Lt, r=1.0, M5=m, at=at)
-    sp, tp = q.random_gauge(lattice, a.n, a.Lt, seed=4321, kappa=a.beta)
-    D.update(sp, tp)
-    D.autotune()
+    D = dual_context(a, q, g, m, lattice)
     net = q.Net(D, layers=4, seed=5)
     net.params = net_t.params
     del net_t, Dt, data
@@ -295,6 +295,7 @@ def learned_leg(a, q, g):
     b_net = pts * (16 + 3 * 2 * 128 + 16) + 4 * (D.E + D.V) * a.Lt * 8
     fp64 = methods.get("fp64")
     return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG",
+            "roofline": stencil_roofline(a, q, D),
             "sites": D.V, "links": D.E, "methods": methods,
             "training": {"lattice": f"dual n={a.train_n} Lt={a.train_Lt}", "pairs": 3 * 40,
                          "epochs": a.train_epochs, "loss_first": round(hist[0], 4),
@@ -307,6 +308,48 @@ def learned_leg(a, q, g):
             "net_apply": {"ms": round(ms_net, 4), "ms_adjoint": round(ms_adj, 4),
                           "algorithmic_GB": round(b_net / 1e9, 3),
                           "achieved_GBs": round(b_net / (ms_net * 1e-3) / 1e9, 1)}}
+
+
+def dual_context(a, q, g, m, lattice):
+    at = 0.5 * mean_ell(g, lattice)
+    D = q.WilsonDirac(lattice, a.n, a.Lt, r=1.0, M5=m, at=at)
+    sp, tp = q.random_gauge(lattice, a.n, a.Lt, seed=4321, kappa=a.beta)
+    D.update(sp, tp)
+    D.autotune()
+    return D
+
+
+def stencil_roofline(a, q, D):
+    """The dominant kernel of the CG: the FP32 stencil (one D apply), timed with HIP events
+    on the context's stream, against B_D = 2 B_vec + B_link."""
+    ms32 = D.time_apply(q.OP_D, q.F32, warmup=5, reps=50)
+    b32 = algorithmic_bytes_D(D.V, D.E, a.Lt, 4)
+    gbs = b32 / (ms32 * 1e-3) / 1e9
+    tile = tile_info(q, D, q.OP_D, q.F32)
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": MEASURED_TRAFFIC.get((D.lattice, a.n, a.Lt)),
+            "kernel": ("k_stencil_lds" if tile["lds"] else "k_stencil")
+            + f"<float,{6 if D.lattice == 0 else 3},false,EpiStore> (D, FP32)",
+            "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5), "tile": tile}
+
+
+def plain_leg(a, q, g, m, lattice):
+    D = dual_context(a, q, g, m, lattice)
+    d_x = q.DeviceArray(D.N * 16)
+    rhs = [q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=2000 + k))
+           for k in range(a.warmup + a.learned_steps)]
+    methods = {"mixed": timed_solves(q, a, "mixed", lambda b: D.solve_device(
+        d_x, b, tol=a.tol, solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)}
+    if not a.no_fp64:
+        methods["fp64"] = timed_solves(q, a, "fp64", lambda b: D.solve_device(
+            d_x, b, tol=a.tol, solver=q.CG_F64), rhs, a.warmup, a.learned_steps)
+    fp64 = methods.get("fp64")
+    return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG",
+            "roofline": stencil_roofline(a, q, D), "sites": D.V, "links": D.E,
+            "methods": methods, "training": None, "speedup_vs_fp64": None,
+            "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
+                                      if fp64 else None), "net_apply": None}
 
 
 def cpu_baseline(a, g, at, gpu_iters, sp_full, tp_full):

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "batch.hpp"
 #include "cg_epi.hpp"
@@ -254,7 +255,7 @@ std::vector<CGResult> batch_cg(const std::vector<Dirac*>& Ds, const std::vector<
   const int chunk = o.chunk > 0 ? o.chunk : 8;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  if (o.use_graph) {
+  if (o.use_graph && std::getenv("QED3_NO_GRAPH") == nullptr) {
     QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < chunk; i++) one_iter();
     QED3_HIP(hipStreamEndCapture(st, &graph));

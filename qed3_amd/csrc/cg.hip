@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "cg.hpp"
@@ -168,7 +169,7 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  if (o.use_graph) {
+  if (o.use_graph && std::getenv("QED3_NO_GRAPH") == nullptr) {
     QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < chunk; i++) one_iter();
     QED3_HIP(hipStreamEndCapture(st, &graph));

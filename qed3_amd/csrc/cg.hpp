@@ -28,7 +28,8 @@ struct CGOptions {
   bool mixed = false;
   double delta = 0.1;
   int chunk = 0;  // iterations per graph replay (0 = auto)
-  bool use_graph = true;
+  bool use_graph = true;  // replay chunks of iterations as a HIP graph (QED3_NO_GRAPH=1 in the
+                          // environment launches the same kernels directly, e.g. for profilers)
   int fcg_m = 4;  // directions kept by flexible CG
 };
 
