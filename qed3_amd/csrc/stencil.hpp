@@ -15,8 +15,9 @@
 // stencil with H_xy -> H_yx^dagger and the two temporal projectors swapped.
 //
 // The epilogue functor receives each output point and may fuse BLAS-1 work and a
-// FP64 partial sum; kernels with a reducing epilogue finish with a single-pass grid
-// reduction whose last block runs the epilogue's finaliser.
+// FP64 partial sum; kernels with a reducing epilogue store one partial per block, and
+// launch_stencil follows them with a one-block finaliser kernel that sums the partials
+// in block order and runs the epilogue's finaliser (no device-scope fences).
 #pragma once
 #include "device_common.hpp"
 #include "dirac.hpp"

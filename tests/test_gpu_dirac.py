@@ -107,3 +107,22 @@ def test_imported_dat_geometry_gives_same_operator(gpu, tmp_path):
     B.update(sp, tp)
     v = source(lattice, n, Lt, seed=1)
     assert rel(B.mult(v), A.mult(v)) < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_f32_phases_over_the_whole_angle_range(gpu, lattice):
+    """Uniform angles, shifted by multiples of 2 pi up to +-40 pi, give the FP64 operator
+    to FP32 accuracy in the FP32 kernels (phases formed in FP64, then rounded)."""
+    n, Lt = 4, 4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.1)
+    sp, tp = links(lattice, n, Lt, seed=17, kappa=0.0)
+    rng = np.random.default_rng(5)
+    sp = sp + 2 * np.pi * rng.integers(-20, 21, sp.shape)
+    tp = tp + 2 * np.pi * rng.integers(-20, 21, tp.shape)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.1, at=at)
+    Dq.update(sp, tp)
+    v = source(lattice, n, Lt, seed=4)
+    for op in (q.OP_D, q.OP_DDAG, q.OP_DDAGD):
+        assert rel(Dq.apply(op, v.astype(np.complex64), q.F32), Do.apply(op, v)) < 2e-6
