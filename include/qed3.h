@@ -245,6 +245,31 @@ void qed3_hmc_destroy(qed3_hmc* h);
 int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_reject,
                         qed3_hmc_report* rep);
 
+/* ---- checkpoint and resume (hmc.cu:423-447 scans ckpoint_lat.k / ckpoint_rng.k) ------
+ * Gauge file: raw native-endian FP64, spatial[Lt][E] then temporal[Lt][V], nothing else
+ * (GaugeExt::ckpoint/read, gauge_ext.h:225-274).  qed3_gauge_read fails unless the file
+ * holds exactly Lt*(E+V) doubles. */
+int qed3_gauge_write(const char* path, int Lt, int E, int V, const double* spatial,
+                     const double* temporal);
+int qed3_gauge_read(const char* path, int Lt, int E, int V, double* spatial, double* temporal);
+
+/* RNG streams of ParallelRngExt (rng.h:91-168) on the host, no device needed: the master
+ * mt19937_64 seeded with `seed` reseeds one stream per (t, link) and then per (t, site).
+ * The checkpoint text holds four lines per stream (engine, normal, uniform, {1,2}
+ * distributions), master first, then links [t][l], then sites [t][x].  Loading restores
+ * the engines and resets the distributions, which is what the reference reader ends up
+ * doing (rng.h:36-52). */
+typedef struct qed3_rng qed3_rng;
+int qed3_rng_create(int Lt, int E, int V, int seed, qed3_rng** out);
+void qed3_rng_destroy(qed3_rng* r);
+int qed3_rng_save(const qed3_rng* r, const char* path);
+int qed3_rng_load(qed3_rng* r, const char* path);
+/* next N(0,1) of stream `which` (0 master, 1 link (s, i), 2 site (s, i)) */
+int qed3_rng_gaussian(qed3_rng* r, int which, int s, int i, double* out);
+/* the HMC handle's own streams */
+int qed3_hmc_rng_save(const qed3_hmc* h, const char* path);
+int qed3_hmc_rng_load(qed3_hmc* h, const char* path);
+
 /* Pick the stencil launch shape (threads per block, time slices per thread) for this
  * lattice by timing every candidate on the device (project-written autotuner). */
 int qed3_autotune(qed3_ctx* ctx);

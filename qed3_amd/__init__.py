@@ -405,6 +405,12 @@ class HMC:
         except Exception:
             pass
 
+    def rng_save(self, path):
+        check(load().qed3_hmc_rng_save(self._h, str(path).encode()))
+
+    def rng_load(self, path):
+        check(load().qed3_hmc_rng_load(self._h, str(path).encode()))
+
     def trajectory(self, spatial, temporal, no_reject=False):
         """Runs one trajectory; spatial/temporal (float64 arrays) are updated in place."""
         assert spatial.dtype == np.float64 and spatial.flags.c_contiguous
@@ -413,3 +419,68 @@ class HMC:
         check(load().qed3_hmc_trajectory(self._h, _dp(spatial), _dp(temporal), int(no_reject),
                                          C.byref(rep)))
         return {k: getattr(rep, k) for k, _ in rep._fields_}
+
+
+# ---- checkpoint and resume (gauge_ext.h:225-274, rng.h:131-168, hmc.cu:423-447) ----------
+def gauge_write(path, spatial, temporal):
+    """Raw FP64 gauge file: spatial[Lt][E] then temporal[Lt][V]."""
+    sp = np.ascontiguousarray(spatial, np.float64)
+    tp = np.ascontiguousarray(temporal, np.float64)
+    Lt, E = sp.shape
+    check(load().qed3_gauge_write(str(path).encode(), Lt, E, tp.shape[1], _dp(sp), _dp(tp)))
+
+
+def gauge_read(path, Lt, E, V):
+    sp = np.zeros((Lt, E))
+    tp = np.zeros((Lt, V))
+    check(load().qed3_gauge_read(str(path).encode(), Lt, E, V, _dp(sp), _dp(tp)))
+    return sp, tp
+
+
+class Rng:
+    """ParallelRngExt streams on the host (master, Lt*E link streams, Lt*V site streams)."""
+
+    MASTER, LINK, SITE = 0, 1, 2
+
+    def __init__(self, Lt, E, V, seed):
+        self._h = C.c_void_p()
+        check(load().qed3_rng_create(Lt, E, V, seed, C.byref(self._h)))
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().qed3_rng_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def gaussian(self, which=0, s=0, i=0):
+        out = C.c_double()
+        check(load().qed3_rng_gaussian(self._h, which, s, i, C.byref(out)))
+        return out.value
+
+    def save(self, path):
+        check(load().qed3_rng_save(self._h, str(path).encode()))
+
+    def load(self, path):
+        check(load().qed3_rng_load(self._h, str(path).encode()))
+
+
+def checkpoint_paths(directory, k):
+    """The reference's checkpoint names for trajectory k (hmc.cu:431-432)."""
+    import os
+    return (os.path.join(directory, f"ckpoint_lat.{k}"), os.path.join(directory, f"ckpoint_rng.{k}"))
+
+
+def resume_point(directory, k_ckpoint=10, kmax=100000):
+    """Largest k = k_ckpoint, 2 k_ckpoint, ... with both checkpoint files present and every
+    earlier multiple present too (the scan of hmc.cu:427-439 stops at the first gap); 0 when
+    there is none, i.e. start from scratch."""
+    import os
+    k = k_ckpoint
+    while k <= kmax:
+        lat, rng = checkpoint_paths(directory, k)
+        if not (os.path.exists(lat) and os.path.exists(rng)):
+            break
+        k += k_ckpoint
+    return k - k_ckpoint

@@ -92,6 +92,15 @@ SIGNATURES = {
     "qed3_hmc_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "qed3_hmc_destroy": (None, [C.c_void_p]),
     "qed3_hmc_trajectory": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_int, C.c_void_p]),
+    "qed3_hmc_rng_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "qed3_hmc_rng_load": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "qed3_gauge_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
+    "qed3_gauge_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
+    "qed3_rng_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "qed3_rng_destroy": (None, [C.c_void_p]),
+    "qed3_rng_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "qed3_rng_load": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "qed3_rng_gaussian": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_double_p]),
     "qed3_autotune": (C.c_int, [C.c_void_p]),
     "qed3_time_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]),
     "qed3_time_copy": (C.c_int, [C.c_size_t, C.c_int, c_double_p]),

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -30,6 +31,10 @@ struct qed3_ctx {
 struct qed3_hmc {
   qed3_ctx* ctx;
   std::unique_ptr<qed3::HMC> hmc;
+};
+
+struct qed3_rng {
+  std::unique_ptr<qed3::ParallelRng> rng;
 };
 
 struct qed3_net {
@@ -620,6 +625,92 @@ int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_r
       rep->max_resid = r.max_resid;
       rep->seconds = r.seconds;
     }
+    return QED3_OK;
+  });
+}
+
+int qed3_gauge_write(const char* path, int Lt, int E, int V, const double* spatial,
+                     const double* temporal) {
+  if (!path || !spatial || !temporal || Lt < 1 || E < 1 || V < 1) return fail(QED3_ERR_ARG, "bad argument");
+  std::FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(QED3_ERR_ARG, std::string("cannot write ") + path);
+  const size_t nS = size_t(Lt) * E, nT = size_t(Lt) * V;
+  const bool ok = std::fwrite(spatial, sizeof(double), nS, f) == nS &&
+                  std::fwrite(temporal, sizeof(double), nT, f) == nT;
+  const bool closed = std::fclose(f) == 0;
+  if (!ok || !closed) return fail(QED3_ERR_ARG, std::string("write failed: ") + path);
+  return QED3_OK;
+}
+
+int qed3_gauge_read(const char* path, int Lt, int E, int V, double* spatial, double* temporal) {
+  if (!path || !spatial || !temporal || Lt < 1 || E < 1 || V < 1) return fail(QED3_ERR_ARG, "bad argument");
+  std::FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(QED3_ERR_ARG, std::string("cannot read ") + path);
+  const size_t nS = size_t(Lt) * E, nT = size_t(Lt) * V;
+  bool ok = std::fread(spatial, sizeof(double), nS, f) == nS &&
+            std::fread(temporal, sizeof(double), nT, f) == nT;
+  char extra;
+  ok = ok && std::fread(&extra, 1, 1, f) == 0;  // exactly Lt*(E+V) doubles
+  std::fclose(f);
+  if (!ok) return fail(QED3_ERR_ARG, std::string("gauge file size does not match Lt*(E+V): ") + path);
+  return QED3_OK;
+}
+
+int qed3_rng_create(int Lt, int E, int V, int seed, qed3_rng** out) {
+  if (!out || Lt < 1 || E < 1 || V < 1) return fail(QED3_ERR_ARG, "bad argument");
+  return guarded([&]() -> int {
+    auto r = std::make_unique<qed3_rng>();
+    r->rng = std::make_unique<qed3::ParallelRng>(Lt, E, V, seed);
+    *out = r.release();
+    return QED3_OK;
+  });
+}
+
+void qed3_rng_destroy(qed3_rng* r) { delete r; }
+
+int qed3_rng_save(const qed3_rng* r, const char* path) {
+  if (!r || !path) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    r->rng->save(path);
+    return QED3_OK;
+  });
+}
+
+int qed3_rng_load(qed3_rng* r, const char* path) {
+  if (!r || !path) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    r->rng->load(path);
+    return QED3_OK;
+  });
+}
+
+int qed3_rng_gaussian(qed3_rng* r, int which, int s, int i, double* out) {
+  if (!r || !out) return fail(QED3_ERR_ARG, "null argument");
+  const qed3::ParallelRng& p = *r->rng;
+  if (which == 0) {
+    *out = r->rng->gaussian_master();
+  } else if (which == 1 && s >= 0 && s < p.Lt() && i >= 0 && i < p.E()) {
+    *out = r->rng->gaussian_link(s, i);
+  } else if (which == 2 && s >= 0 && s < p.Lt() && i >= 0 && i < p.V()) {
+    *out = r->rng->gaussian_site(s, i);
+  } else {
+    return fail(QED3_ERR_ARG, "stream index out of range");
+  }
+  return QED3_OK;
+}
+
+int qed3_hmc_rng_save(const qed3_hmc* h, const char* path) {
+  if (!h || !path) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    h->hmc->rng().save(path);
+    return QED3_OK;
+  });
+}
+
+int qed3_hmc_rng_load(qed3_hmc* h, const char* path) {
+  if (!h || !path) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    h->hmc->rng().load(path);
     return QED3_OK;
   });
 }

@@ -409,7 +409,7 @@ void Dirac::tune_one(bool dag) {
   for (int lds : {0, 1})
   for (int remap : {1, 0})
   for (int block : {64, 128, 256}) {
-    for (int ts : {1, 2, 4, 8, 16}) {
+    for (int ts : {1, 2, 4, 8, 16, 32, 64}) {
       if (ts > h_.Lt) continue;
       tile_[f][dag] = StencilTile{block, ts, remap, lds};
       for (int i = 0; i < 2; i++) apply<T>(dag, b, a, false);

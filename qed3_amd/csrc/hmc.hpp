@@ -59,6 +59,8 @@ class HMC {
   ~HMC();
   // one trajectory; u/t (host, gauge layout) are replaced by the new field on accept
   HMCReport trajectory(double* u, double* t, bool no_reject = false);
+  ParallelRng& rng() { return *rng_; }
+  const ParallelRng& rng() const { return *rng_; }
 
  private:
   void set_gauge(const std::vector<double>& u, const std::vector<double>& t);

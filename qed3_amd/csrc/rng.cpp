@@ -2,6 +2,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
 #include <thread>
 
 namespace qed3 {
@@ -91,6 +94,42 @@ void draw_source(int Lt, int E, int V, int seed, double* xi) {
       xi[4 * k + 2 * a + 1] = im * inv;
     }
   });
+}
+
+void RngStream::write_text(std::ostream& os) const {
+  os << mt << '\n' << gauss << '\n' << unif << '\n' << z2 << '\n';
+}
+
+void RngStream::read_text(std::istream& is) {
+  std::string line;
+  if (!std::getline(is, line)) throw std::runtime_error("rng checkpoint: missing engine line");
+  std::istringstream ss(line);
+  std::mt19937_64 m;
+  if (!(ss >> m)) throw std::runtime_error("rng checkpoint: bad engine line");
+  for (int k = 0; k < 3; k++)
+    if (!std::getline(is, line)) throw std::runtime_error("rng checkpoint: missing distribution line");
+  mt = m;
+  gauss.reset();
+  gauss = std::normal_distribution<double>();
+  unif = std::uniform_real_distribution<double>();
+  z2 = std::uniform_int_distribution<int>(1, 2);
+}
+
+void ParallelRng::save(const std::string& path) const {
+  std::ofstream os(path);
+  if (!os) throw std::runtime_error("cannot write " + path);
+  master_.write_text(os);
+  for (const auto& r : links_) r.write_text(os);
+  for (const auto& r : sites_) r.write_text(os);
+  if (!os) throw std::runtime_error("write failed: " + path);
+}
+
+void ParallelRng::load(const std::string& path) {
+  std::ifstream is(path);
+  if (!is) throw std::runtime_error("cannot read " + path);
+  master_.read_text(is);
+  for (auto& r : links_) r.read_text(is);
+  for (auto& r : sites_) r.read_text(is);
 }
 
 }  // namespace qed3

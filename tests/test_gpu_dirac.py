@@ -43,7 +43,7 @@ def test_dirac_f32_matches_oracle(gpu, lattice):
 
 TILES = [dict(block=b, ts=t, remap=r, lds=l) for (b, t, r, l) in
          [(64, 1, 0, 0), (128, 3, 1, 0), (256, 8, 1, 0), (64, 1, 0, 1), (128, 2, 1, 1),
-          (256, 5, 0, 1)]]
+          (256, 5, 0, 1), (256, 32, 1, 1), (64, 64, 0, 0)]]
 
 
 @pytest.mark.gpu
