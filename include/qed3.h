@@ -214,10 +214,17 @@ int qed3_pf_force(qed3_ctx* ctx, const void* d_eta, double* d_fs, double* d_ft);
 int qed3_gauge_action(qed3_ctx* ctx, double gsq, int compact, const double* spatial,
                       const double* temporal, double* S, double* fs, double* ft);
 
+/* Plaquette-angle correlator of gauge.cu:313-343 on the host (no device needed):
+ * corr[t] = sum_{s,f} [P_f(s) P_f(s+t) + P_f(s) P_f(s-t)] / vol_f^2 / (2 Lt F), t = 0..Lt-1,
+ * times mod Lt, P_f(s) the oriented sum of the spatial angles around face f at slice s
+ * (faces of the params' lattice: triangles, or the dual cells), vol_f its area. */
+int qed3_plaquette_correlator(const qed3_geom* g, const qed3_params* p, const double* spatial,
+                              double* corr);
+
 typedef struct qed3_hmc_params {
   double gsq;       /* gauge coupling g^2 = 1/beta */
   int compact;      /* gauge action form (see qed3_gauge_action) */
-  int npf;          /* pseudofermions (Nf = 2 npf) */
+  int npf;          /* pseudofermions (Nf = 2 npf); 0: pure-gauge HMC (hmc.h:212-296) */
   double tau;       /* trajectory length */
   int nsteps;       /* outer (fermion) leapfrog steps */
   int nsteps_inner; /* gauge steps per outer step (integrator.h:58-73) */

@@ -94,6 +94,7 @@ SIGNATURES = {
     "qed3_hmc_trajectory": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_int, C.c_void_p]),
     "qed3_hmc_rng_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "qed3_hmc_rng_load": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "qed3_plaquette_correlator": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
     "qed3_gauge_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "qed3_gauge_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "qed3_rng_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),

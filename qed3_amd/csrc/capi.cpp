@@ -585,6 +585,18 @@ int qed3_gauge_action(qed3_ctx* ctx, double gsq, int compact, const double* spat
   });
 }
 
+int qed3_plaquette_correlator(const qed3_geom* g, const qed3_params* p, const double* spatial,
+                              double* corr) {
+  if (!g || !p || !spatial || !corr) return fail(QED3_ERR_ARG, "null argument");
+  if ((p->lattice != 0 && p->lattice != 1) || p->Lt < 1) return fail(QED3_ERR_ARG, "bad params");
+  return guarded([&]() -> int {
+    const qed3::HopTables h = qed3::make_hop_tables(g->g, p->lattice, p->Lt, p->r, p->M5, p->at);
+    const qed3::GaugeAction A(h, 1.0, p->at, false);
+    A.plaquette_correlator(spatial, corr);
+    return QED3_OK;
+  });
+}
+
 int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
   if (!ctx || !p || !out) return fail(QED3_ERR_ARG, "null argument");
   if (p->solver != QED3_SOLVER_CG_F64 && p->solver != QED3_SOLVER_CG_MIXED)

@@ -33,6 +33,28 @@ GaugeAction::GaugeAction(const HopTables& h, double gsq, double at, bool compact
   }
 }
 
+// gauge.cu:319-335: C(t) = < P_f(s) P_f(s+t) + P_f(s) P_f(s-t) > / vol_f^2 averaged over
+// slices s and faces f (2 Lt F terms), times taken mod Lt.
+void GaugeAction::plaquette_correlator(const double* u, double* corr) const {
+  const int F = int(h_.face_rings.size());
+  std::vector<double> P(size_t(Lt_) * F);
+  for (int s = 0; s < Lt_; s++)
+    for (int f = 0; f < F; f++) P[size_t(s) * F + f] = face_angle(u, s, f);
+  for (int t = 0; t < Lt_; t++) {
+    double acc = 0;
+    for (int s = 0; s < Lt_; s++) {
+      const double* a = &P[size_t(s) * F];
+      const double* b = &P[size_t((s + t) % Lt_) * F];
+      const double* c = &P[size_t(((s - t) % Lt_ + Lt_) % Lt_) * F];
+      for (int f = 0; f < F; f++) {
+        const double w = 1.0 / (h_.face_area[f] * h_.face_area[f]);
+        acc += a[f] * b[f] * w + a[f] * c[f] * w;
+      }
+    }
+    corr[t] = acc / (2.0 * Lt_ * F);
+  }
+}
+
 double GaugeAction::face_angle(const double* u, int s, int f) const {
   const double* us = u + size_t((s + Lt_) % Lt_) * h_.E;
   double sum = 0;
@@ -102,7 +124,9 @@ void GaugeAction::force(const double* u, const double* t, double* fs, double* ft
 
 HMC::HMC(Dirac& D, const HMCParams& p)
     : D_(D), p_(p), Sg_(D.tables(), p.gsq, D.tables().at, p.compact) {
-  if (p.npf < 1 || p.nsteps < 1 || p.nsteps_inner < 1) throw std::invalid_argument("bad HMC parameters");
+  // npf = 0 is the pure-gauge HMC (HMCPureGauge, hmc.h:212-296): with no fermion kicks the
+  // two-level leapfrog is one leapfrog of nsteps * nsteps_inner gauge steps
+  if (p.npf < 0 || p.nsteps < 1 || p.nsteps_inner < 1) throw std::invalid_argument("bad HMC parameters");
   const HopTables& h = D.tables();
   rng_ = std::make_unique<ParallelRng>(h.Lt, h.E, h.V, p.seed);
   const size_t n = D.points();

@@ -21,6 +21,8 @@ class GaugeAction {
   GaugeAction(const HopTables& h, double gsq, double at, bool compact);
   double S(const double* u, const double* t) const;
   void force(const double* u, const double* t, double* fs, double* ft) const;
+  // plaquette-angle correlator over time separations t = 0..Lt-1 (corr[Lt])
+  void plaquette_correlator(const double* u, double* corr) const;
 
  private:
   double face_angle(const double* u, int s, int f) const;

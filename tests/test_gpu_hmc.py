@@ -23,7 +23,7 @@ def test_gauge_action_matches_oracle(gpu, lattice, compact):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lattice,compact,npf", [(0, True, 1), (1, False, 2)])
+@pytest.mark.parametrize("lattice,compact,npf", [(0, True, 1), (1, False, 2), (1, True, 0)])
 def test_hmc_trajectory_matches_oracle(gpu, lattice, compact, npf):
     n, Lt, M5, gsq = 1, 4, 0.1, 1 / 1.4
     print(f"case lattice={lattice} compact={compact} npf={npf}", flush=True)
