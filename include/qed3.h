@@ -252,6 +252,29 @@ void qed3_hmc_destroy(qed3_hmc* h);
 int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_reject,
                         qed3_hmc_report* rep);
 
+/* ---- multi-shift CG and the overlap operator (overlap.h:7-465) --------------------
+ * Zolotarev coefficients of the overlap's sign approximation (Zolotarev, overlap.h:7-135):
+ * size = n/2 + 1 entries in A, c, cp (index 0 unused), consts = {M, lambda_inv, E, C,
+ * Delta}; eps(x) = C x (1 + sum_m A_m / (x^2 - k^2/cp_m)), max |eps - 1| = Delta on
+ * [k, 1].  Host only. */
+int qed3_zolotarev(int n, double k, int* size, double* A, double* c, double* cp, double* consts);
+/* (D^dag D + shifts[i]) x_i = b for nshift <= 32 shifts >= 0 at once (one Krylov space,
+ * FP64); reps[i] holds each shift's iterations and FP64 true residual.  Replaces the
+ * size-1 independent MatPoly::solve calls of Overlap::mult/adj (overlap.h:309-361). */
+int qed3_multishift_cg(qed3_ctx* ctx, int nshift, const double* shifts, void* const* d_x,
+                       const void* d_b, const qed3_cg_opts* opts, qed3_cg_report* reps);
+/* Extreme eigenvalues of D^dag D (power and inverse iteration to relative change tol);
+ * Overlap::compute_lambda_max (overlap.h:217-306) derives lambda_min/max from them. */
+int qed3_spectral_bounds(qed3_ctx* ctx, double tol, int maxiter, double* mu_min, double* mu_max);
+/* Overlap operator D_ov = 1 + X eps(X^dag X), X = D / lambda_max, k = lambda_min/lambda_max,
+ * Zolotarev degree n (<= 65), shifted solves to tol (Overlap, overlap.h:137-361);
+ * op = QED3_OP_D applies D_ov, QED3_OP_DDAG its adjoint.  FP64 device fields. */
+typedef struct qed3_overlap qed3_overlap;
+int qed3_overlap_create(qed3_ctx* ctx, int n, double lambda_min, double lambda_max, double tol,
+                        qed3_overlap** out);
+void qed3_overlap_destroy(qed3_overlap* ov);
+int qed3_overlap_apply(qed3_overlap* ov, int op, void* d_out, const void* d_in);
+
 /* ---- checkpoint and resume (hmc.cu:423-447 scans ckpoint_lat.k / ckpoint_rng.k) ------
  * Gauge file: raw native-endian FP64, spatial[Lt][E] then temporal[Lt][V], nothing else
  * (GaugeExt::ckpoint/read, gauge_ext.h:225-274).  qed3_gauge_read fails unless the file

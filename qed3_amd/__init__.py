@@ -484,3 +484,69 @@ def resume_point(directory, k_ckpoint=10, kmax=100000):
             break
         k += k_ckpoint
     return k - k_ckpoint
+
+
+# ---- multi-shift CG and the overlap operator (overlap.h:7-465) -----------------------------
+def zolotarev(n, k):
+    """Coefficients of eps(x) = C x (1 + sum A_m/(x^2 - k^2/cp_m)): dict with A, c, cp
+    (index 0 unused), M, lambda_inv, E, C, Delta and the shifts sigma_m = -k^2/cp_m."""
+    size = n // 2 + 1
+    A, c, cp, consts = np.zeros(size), np.zeros(size), np.zeros(size), np.zeros(5)
+    sz = C.c_int()
+    check(load().qed3_zolotarev(n, k, C.byref(sz), _dp(A), _dp(c), _dp(cp), _dp(consts)))
+    out = dict(A=A, c=c, cp=cp, M=consts[0], lambda_inv=consts[1], E=consts[2], C=consts[3],
+               Delta=consts[4], k=k, n=n)
+    out["shifts"] = -k * k / cp[1:]
+    return out
+
+
+def multishift_solve(D, b, shifts, tol=1e-10, maxiter=100000):
+    """x_i with (D^dag D + shifts[i]) x_i = b (host arrays); returns (list of x_i, reports)."""
+    b = np.ascontiguousarray(b, np.complex128)
+    d_b = DeviceArray.from_numpy(b)
+    d_x = [DeviceArray(b.nbytes) for _ in shifts]
+    ptrs = (C.c_void_p * len(shifts))(*[d.ptr for d in d_x])
+    sh = np.ascontiguousarray(shifts, np.float64)
+    reps = (CGReport * len(shifts))()
+    opts = CGOpts(tol, maxiter, CG_F64, 0.1, 0)
+    check(load().qed3_multishift_cg(D._h, len(shifts), _dp(sh), ptrs, d_b.ptr, C.byref(opts), reps))
+    return [d.to_numpy(np.complex128) for d in d_x], list(reps)
+
+
+def spectral_bounds(D, tol=1e-8, maxiter=2000):
+    """(mu_min, mu_max) of D^dag D."""
+    lo, hi = C.c_double(), C.c_double()
+    check(load().qed3_spectral_bounds(D._h, tol, maxiter, C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
+
+
+class Overlap:
+    """D_ov = 1 + X eps(X^dag X), X = D/lambda_max, Zolotarev degree n (Overlap of
+    overlap.h:137-361).  Without explicit bounds they come from the spectrum of D^dag D
+    with the reference's margins: lambda_min = sqrt((1 - 100 tol) mu_min),
+    lambda_max = sqrt((1 + 100 tol) mu_max) (overlap.h:298-301)."""
+
+    def __init__(self, D, n=21, lambda_min=None, lambda_max=None, tol=1e-10, bound_tol=1e-8):
+        self.D = D
+        if lambda_min is None or lambda_max is None:
+            mu_min, mu_max = spectral_bounds(D, bound_tol)
+            lambda_min = np.sqrt((1 - 100 * bound_tol) * mu_min)
+            lambda_max = np.sqrt((1 + 100 * bound_tol) * mu_max)
+        self.lambda_min, self.lambda_max, self.n = lambda_min, lambda_max, n
+        self._h = C.c_void_p()
+        check(load().qed3_overlap_create(D._h, n, lambda_min, lambda_max, tol, C.byref(self._h)))
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().qed3_overlap_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def apply(self, v, dag=False):
+        v = np.ascontiguousarray(v, np.complex128)
+        d_in = DeviceArray.from_numpy(v)
+        d_out = DeviceArray(v.nbytes)
+        check(load().qed3_overlap_apply(self._h, OP_DDAG if dag else OP_D, d_out.ptr, d_in.ptr))
+        return d_out.to_numpy(np.complex128)

@@ -95,6 +95,15 @@ SIGNATURES = {
     "qed3_hmc_rng_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "qed3_hmc_rng_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "qed3_plaquette_correlator": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
+    "qed3_zolotarev": (C.c_int, [C.c_int, C.c_double, c_int_p, c_double_p, c_double_p, c_double_p,
+                                 c_double_p]),
+    "qed3_multishift_cg": (C.c_int, [C.c_void_p, C.c_int, c_double_p, C.POINTER(C.c_void_p),
+                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qed3_spectral_bounds": (C.c_int, [C.c_void_p, C.c_double, C.c_int, c_double_p, c_double_p]),
+    "qed3_overlap_create": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.POINTER(C.c_void_p)]),
+    "qed3_overlap_destroy": (None, [C.c_void_p]),
+    "qed3_overlap_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "qed3_gauge_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "qed3_gauge_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "qed3_rng_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),

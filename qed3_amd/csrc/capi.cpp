@@ -16,6 +16,7 @@
 #include "fcg.hpp"
 #include "hmc.hpp"
 #include "mpnn.hpp"
+#include "msolve.hpp"
 #include "rng.hpp"
 
 struct qed3_geom {
@@ -31,6 +32,11 @@ struct qed3_ctx {
 struct qed3_hmc {
   qed3_ctx* ctx;
   std::unique_ptr<qed3::HMC> hmc;
+};
+
+struct qed3_overlap {
+  qed3_ctx* ctx;
+  std::unique_ptr<qed3::Overlap> ov;
 };
 
 struct qed3_rng {
@@ -637,6 +643,90 @@ int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_r
       rep->max_resid = r.max_resid;
       rep->seconds = r.seconds;
     }
+    return QED3_OK;
+  });
+}
+
+int qed3_zolotarev(int n, double k, int* size, double* A, double* c, double* cp, double* consts) {
+  if (!size || !A || !c || !cp || !consts) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    const qed3::Zolotarev z(n, k);
+    *size = z.size;
+    for (int m = 0; m < z.size; m++) {
+      A[m] = z.A[m];
+      c[m] = z.c[m];
+      cp[m] = z.cp[m];
+    }
+    consts[0] = z.M;
+    consts[1] = z.lambda_inv;
+    consts[2] = z.E;
+    consts[3] = z.C;
+    consts[4] = z.delta();
+    return QED3_OK;
+  });
+}
+
+int qed3_multishift_cg(qed3_ctx* ctx, int nshift, const double* shifts, void* const* d_x,
+                       const void* d_b, const qed3_cg_opts* opts, qed3_cg_report* reps) {
+  if (!ctx || !shifts || !d_x || !d_b || !opts || nshift < 1) return fail(QED3_ERR_ARG, "bad argument");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    qed3::CGOptions o;
+    o.tol = opts->tol;
+    o.maxiter = opts->maxiter;
+    std::vector<double> sh(shifts, shifts + nshift);
+    std::vector<qed3::Spin<double>*> xs(nshift);
+    for (int i = 0; i < nshift; i++) {
+      if (!d_x[i] || d_x[i] == d_b) throw std::invalid_argument("solution vectors must be distinct from b");
+      xs[i] = static_cast<qed3::Spin<double>*>(d_x[i]);
+    }
+    const auto r = qed3::multishift_cg(*ctx->D, sh, xs, static_cast<const qed3::Spin<double>*>(d_b), o);
+    int worst = QED3_OK;
+    for (int i = 0; i < nshift; i++) {
+      if (reps) {
+        reps[i].status = r[i].status;
+        reps[i].iterations = r[i].iterations;
+        reps[i].reliable_updates = 0;
+        reps[i].true_resid = r[i].true_resid;
+        reps[i].seconds = r[i].seconds;
+      }
+      if (r[i].status != 0) worst = QED3_ERR_NOCONV;
+    }
+    if (worst != QED3_OK) return fail(worst, "a shifted system did not reach the tolerance");
+    return QED3_OK;
+  });
+}
+
+int qed3_spectral_bounds(qed3_ctx* ctx, double tol, int maxiter, double* mu_min, double* mu_max) {
+  if (!ctx || !mu_min || !mu_max || !(tol > 0) || maxiter < 1) return fail(QED3_ERR_ARG, "bad argument");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    qed3::spectral_bounds(*ctx->D, tol, maxiter, mu_min, mu_max);
+    return QED3_OK;
+  });
+}
+
+int qed3_overlap_create(qed3_ctx* ctx, int n, double lambda_min, double lambda_max, double tol,
+                        qed3_overlap** out) {
+  if (!ctx || !out || n < 1 || !(tol > 0)) return fail(QED3_ERR_ARG, "bad argument");
+  return guarded([&]() -> int {
+    auto o = std::make_unique<qed3_overlap>();
+    o->ctx = ctx;
+    o->ov = std::make_unique<qed3::Overlap>(*ctx->D, n, lambda_min, lambda_max, tol);
+    *out = o.release();
+    return QED3_OK;
+  });
+}
+
+void qed3_overlap_destroy(qed3_overlap* ov) { delete ov; }
+
+int qed3_overlap_apply(qed3_overlap* ov, int op, void* d_out, const void* d_in) {
+  if (!ov || !d_out || !d_in || d_out == d_in) return fail(QED3_ERR_ARG, "bad argument");
+  if (op != QED3_OP_D && op != QED3_OP_DDAG) return fail(QED3_ERR_ARG, "op must be QED3_OP_D or QED3_OP_DDAG");
+  if (!ov->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    ov->ov->apply(op == QED3_OP_DDAG, static_cast<qed3::Spin<double>*>(d_out),
+                  static_cast<const qed3::Spin<double>*>(d_in));
     return QED3_OK;
   });
 }

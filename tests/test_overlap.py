@@ -1,0 +1,118 @@
+"""Zolotarev sign approximation, multi-shift CG and the overlap operator (overlap.h)."""
+import numpy as np
+import pytest
+
+import qed3_amd as q
+from conftest import rel
+from pairs import links, oracle_setup, source
+
+
+@pytest.mark.parametrize("n,k", [(21, 0.01), (13, 0.05), (8, 0.2)])
+def test_zolotarev_is_the_optimal_sign_approximation(n, k):
+    """max_{k<=x<=1} |eps(x) - 1| = Delta, reached with alternating sign (equioscillation
+    of the optimal rational approximation); eps is odd; partial fractions = product form."""
+    z = q.zolotarev(n, k)
+    x = np.concatenate([np.geomspace(k, 1, 200001), [k, 1.0]])
+    A, cp, c = z["A"][1:], z["cp"][1:], z["c"][1:]
+
+    def eps(t):
+        return z["C"] * t * (1 + np.sum(A[None, :] / (t[:, None] ** 2 - k * k / cp[None, :]), axis=1))
+
+    def eps_prod(t):
+        r = 2 / (1 + z["lambda_inv"]) * t / (k * z["M"])
+        return r * np.prod((k * k - c[None, :] * t[:, None] ** 2)
+                           / (k * k - cp[None, :] * t[:, None] ** 2), axis=1)
+
+    e = eps(x) - 1
+    D = z["Delta"]
+    assert 0 < D < 0.1
+    assert np.abs(e).max() <= D * (1 + 1e-6)
+    assert np.abs(e).max() >= D * (1 - 1e-3)
+    # alternation: the error reaches +Delta and -Delta
+    assert e.max() >= D * (1 - 1e-3) and e.min() <= -D * (1 - 1e-3)
+    assert np.allclose(eps(-x), -eps(x), rtol=1e-14)
+    assert np.allclose(eps_prod(x), eps(x), rtol=1e-10)
+    assert np.all(z["shifts"] > 0)
+
+
+def test_zolotarev_error_falls_with_degree():
+    d = [q.zolotarev(n, 0.05)["Delta"] for n in (6, 10, 14, 18)]
+    assert all(a > 10 * b for a, b in zip(d, d[1:]))
+
+
+def test_zolotarev_rejects_bad_modulus():
+    with pytest.raises(q.Qed3Error):
+        q.zolotarev(21, 1.5)
+
+
+def _dense(Do, N, op):
+    return np.stack([Do.apply(op, np.eye(N, dtype=complex)[i]) for i in range(N)], axis=1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_multishift_matches_dense_solves(gpu, lattice):
+    n, Lt = 2, 4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.05)
+    sp, tp = links(lattice, n, Lt, seed=6)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.05, at=at)
+    Dq.update(sp, tp)
+    N = 2 * Lt * Dq.V
+    A = _dense(Do, N, 2)
+    b = source(lattice, n, Lt, seed=2)
+    shifts = [0.0, 0.03, 0.4, 5.0]
+    xs, reps = q.multishift_solve(Dq, b, shifts, tol=1e-11)
+    for s, x, r in zip(shifts, xs, reps):
+        ref = np.linalg.solve(A + s * np.eye(N), b)
+        assert rel(x, ref) < 1e-9, (s, rel(x, ref))
+        assert r.status == 0 and r.true_resid <= 1e-11
+    # larger shifts converge in fewer iterations
+    its = [r.iterations for r in reps]
+    assert its == sorted(its, reverse=True)
+
+
+@pytest.mark.gpu
+def test_spectral_bounds_match_dense_eigenvalues(gpu):
+    lattice, n, Lt = 1, 2, 4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.1)
+    sp, tp = links(lattice, n, Lt, seed=9)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.1, at=at)
+    Dq.update(sp, tp)
+    ev = np.linalg.eigvalsh(_dense(Do, 2 * Lt * Dq.V, 2))
+    lo, hi = q.spectral_bounds(Dq, tol=1e-10, maxiter=20000)
+    assert abs(lo - ev[0]) < 1e-6 * ev[0] and abs(hi - ev[-1]) < 1e-6 * ev[-1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_overlap_matches_dense(gpu, lattice):
+    """D_ov = the Zolotarev rational function of the dense operator (1e-9), close to the
+    exact 1 + X (X^dag X)^{-1/2} (within Delta), its adjoint, and |(D_ov - 1) v| = |v|."""
+    n, Lt, nz = 2, 4, 21
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=-0.5)
+    sp, tp = links(lattice, n, Lt, seed=4)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=-0.5, at=at)
+    Dq.update(sp, tp)
+    N = 2 * Lt * Dq.V
+    Dd = _dense(Do, N, 0)
+    ev = np.linalg.eigvalsh(Dd.conj().T @ Dd)
+    lmin, lmax = np.sqrt(ev[0]) * 0.999, np.sqrt(ev[-1]) * 1.001
+    ov = q.Overlap(Dq, n=nz, lambda_min=lmin, lambda_max=lmax, tol=1e-12)
+    z = q.zolotarev(nz, lmin / lmax)
+    H = Dd.conj().T @ Dd / lmax ** 2
+    R = np.eye(N, dtype=complex)
+    for m in range(1, z["A"].size):
+        R += z["A"][m] * np.linalg.inv(H + z["shifts"][m - 1] * np.eye(N))
+    Dzol = np.eye(N) + z["C"] / lmax * Dd @ R
+    w, U = np.linalg.eigh(H)
+    Dex = np.eye(N) + (Dd / lmax) @ (U @ np.diag(w ** -0.5) @ U.conj().T)
+    for seed in (1, 2):
+        v = source(lattice, n, Lt, seed=seed)
+        got = ov.apply(v)
+        assert rel(got, Dzol @ v) < 1e-9
+        assert rel(got, Dex @ v) < 4 * z["Delta"] + 1e-9
+        assert rel(ov.apply(v, dag=True), Dzol.conj().T @ v) < 1e-9
+        assert abs(np.linalg.norm(got - v) / np.linalg.norm(v) - 1) < 4 * z["Delta"] + 1e-9
