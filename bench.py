@@ -44,7 +44,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-learned", action="store_true")
-    p.add_argument("--learned-steps", type=int, default=3)
+    p.add_argument("--no-hmc", action="store_true")
+    p.add_argument("--hmc-traj", type=int, default=2)
+    p.add_argument("--learned-steps", type=int, default=2)
     p.add_argument("--train-n", type=int, default=16)
     p.add_argument("--train-Lt", type=int, default=16)
     p.add_argument("--train-epochs", type=int, default=40)
@@ -157,6 +159,7 @@ def main():
         cpu = cpu_baseline(a, g, at, results.get("fp64", results["mixed"])["iters"], sp, tp)
     del D, d_x, rhs
     learned = learned_leg(a, q, g, with_net=not a.no_learned)
+    hmc = None if a.no_hmc else hmc_leg(a, q)
 
     simp = {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
                         f"m={a.m} tol={a.tol} D^dag D CG", "methods": results,
@@ -179,7 +182,8 @@ def main():
                   "methods": learned["methods"], "training": learned["training"],
                   "speedup_learned_vs_fp64": learned["speedup_vs_fp64"],
                   "speedup_mixed_vs_fp64": learned["speedup_mixed_vs_fp64"],
-                  "net_apply": learned["net_apply"], "second_workload": simp}
+                  "net_apply": learned["net_apply"], "solution_agreement": learned["agreement"],
+                  "second_workload": simp, "hmc": hmc}
         roof = learned["roofline"]
         dtype = ("f32 inner (bf16 net activations) / f64 reliable updates" if winner == "learned"
                  else "f32 inner / f64 reliable updates" if winner == "mixed" else "f64")
@@ -265,12 +269,20 @@ def learned_leg(a, q, g, with_net=True):
         return rep
 
     methods["learned"] = timed_solves(q, a, "learned", run_fcg, rhs, a.warmup, a.learned_steps)
+    x_learned = d_x.to_numpy(np.complex128)
     methods["learned"]["fallbacks"] = int(sum(fb for _, fb in stats[a.warmup:]))
     methods["mixed"] = timed_solves(q, a, "mixed", lambda b: D.solve_device(d_x, b, tol=a.tol,
                                     solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)
+    x_mixed = d_x.to_numpy(np.complex128)
+    agreement = None
     if not a.no_fp64:
         methods["fp64"] = timed_solves(q, a, "fp64", lambda b: D.solve_device(d_x, b, tol=a.tol,
                                        solver=q.CG_F64), rhs, a.warmup, a.learned_steps)
+        x64 = d_x.to_numpy(np.complex128)
+        nrm = np.linalg.norm(x64)
+        agreement = {"rhs_seed": 2000 + a.warmup + a.learned_steps - 1,
+                     "mixed_vs_fp64": float(np.linalg.norm(x_mixed - x64) / nrm),
+                     "learned_vs_fp64": float(np.linalg.norm(x_learned - x64) / nrm)}
     # one net application N (and N^dag) at full size, device-resident fields
     d_y = q.DeviceArray(D.N * 8)
     d_z = q.DeviceArray(D.N * 8)
@@ -295,6 +307,7 @@ def learned_leg(a, q, g, with_net=True):
     b_net = pts * (16 + 3 * 2 * 128 + 16) + 4 * (D.E + D.V) * a.Lt * 8
     fp64 = methods.get("fp64")
     return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG",
+            "agreement": agreement,
             "roofline": stencil_roofline(a, q, D),
             "sites": D.V, "links": D.E, "methods": methods,
             "training": {"lattice": f"dual n={a.train_n} Lt={a.train_Lt}", "pairs": 3 * 40,
@@ -341,15 +354,86 @@ def plain_leg(a, q, g, m, lattice):
            for k in range(a.warmup + a.learned_steps)]
     methods = {"mixed": timed_solves(q, a, "mixed", lambda b: D.solve_device(
         d_x, b, tol=a.tol, solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)}
+    x_mixed = d_x.to_numpy(np.complex128)
+    agreement = None
     if not a.no_fp64:
         methods["fp64"] = timed_solves(q, a, "fp64", lambda b: D.solve_device(
             d_x, b, tol=a.tol, solver=q.CG_F64), rhs, a.warmup, a.learned_steps)
+        x64 = d_x.to_numpy(np.complex128)
+        agreement = {"rhs_seed": 2000 + a.warmup + a.learned_steps - 1,
+                     "mixed_vs_fp64": float(np.linalg.norm(x_mixed - x64) / np.linalg.norm(x64))}
     fp64 = methods.get("fp64")
-    return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG",
+    return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG", "agreement": agreement,
             "roofline": stencil_roofline(a, q, D), "sites": D.V, "links": D.E,
             "methods": methods, "training": None, "speedup_vs_fp64": None,
             "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
                                       if fp64 else None), "net_apply": None}
+
+
+def hmc_leg(a, q):
+    """Full HMC (simplicial n=64, Lt=32, m=0.1, compact U(1) at beta=1.4, tau=1, 20 outer
+    steps, Nf=2 via one pseudofermion, mixed CG): seconds per trajectory, dH, accept and CG
+    work; the pseudofermion force at the same size against the FP64 CPU path (the oracle's
+    local route) on the same eta, and the force launch against its algorithmic bytes."""
+    import oracle as O
+    n, Lt, m, lattice = 64, 32, 0.1, 0
+    g = q.Geometry(n)
+    at = 0.5 * mean_ell(g, lattice)
+    D = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=m, at=at)
+    sp, tp = q.random_gauge(lattice, n, Lt, seed=55, kappa=1.4)
+    sp, tp = np.ascontiguousarray(sp), np.ascontiguousarray(tp)
+    D.update(sp, tp)
+    D.autotune()
+    H = q.HMC(D, 1 / 1.4, compact=True, npf=1, tau=1.0, nsteps=20, nsteps_inner=5, tol=1e-10,
+              solver=q.CG_MIXED, seed=7)
+    H.trajectory(sp, tp)  # warmup
+    trajs = []
+    for k in range(a.hmc_traj):
+        t0 = time.time()
+        r = H.trajectory(sp, tp)
+        r["wall_s"] = time.time() - t0
+        trajs.append(r)
+        print(f"# hmc traj {k}: dH={r['dH']:.3e} accept={r['accept']} cg={r['cg_solves']}/"
+              f"{r['cg_iterations']} {r['wall_s']:.2f}s", file=sys.stderr, flush=True)
+    # force at this size: GPU vs the FP64 CPU path on the same eta
+    D.update(sp, tp)
+    phi = q.random_source(lattice, n, Lt, seed=3)
+    eta, _ = D.solve(phi, tol=1e-12, solver=q.CG_F64)
+    d_eta = q.DeviceArray.from_numpy(eta)
+    d_fs, d_ft = q.DeviceArray(8 * Lt * D.E), q.DeviceArray(8 * Lt * D.V)
+    for _ in range(2):
+        q.check(q.load().qed3_pf_force(D._h, d_eta.ptr, d_fs.ptr, d_ft.ptr))
+    q.load().qed3_device_synchronize()
+    t0 = time.time()
+    reps = 20
+    for _ in range(reps):
+        q.check(q.load().qed3_pf_force(D._h, d_eta.ptr, d_fs.ptr, d_ft.ptr))
+    q.load().qed3_device_synchronize()
+    ms_force = 1e3 * (time.time() - t0) / reps
+    fs = d_fs.to_numpy(np.float64).reshape(Lt, D.E)
+    ft = d_ft.to_numpy(np.float64).reshape(Lt, D.V)
+    t0 = time.time()
+    L = O.Lattice(g, lattice)
+    Do = O.Dirac(L, Lt, r=1.0, M5=m, at=at)
+    Do.update(sp, tp)
+    t_setup = time.time() - t0
+    t0 = time.time()
+    fs_o, ft_o = Do.force(eta)
+    t_cpu = time.time() - t0
+    scale = max(np.abs(fs_o).max(), np.abs(ft_o).max())
+    err = max(np.abs(fs - fs_o).max(), np.abs(ft - ft_o).max()) / scale
+    vec = 2 * D.V * Lt * 16
+    link = (D.E + D.V) * Lt * 16
+    # one FP64 D apply (D eta), then the link kernel: eta and D eta in, phases, one double out
+    b_force = (2 * vec + link) + (2 * vec + link + (D.E + D.V) * Lt * 8)
+    return {"workload": f"simplicial n={n} Lt={Lt} m={m} compact beta=1.4 tau=1 nsteps=20x5 npf=1",
+            "seconds_per_trajectory": round(float(np.mean([t["wall_s"] for t in trajs])), 3),
+            "trajectories": [{k: t[k] for k in ("dH", "accept", "cg_solves", "cg_iterations",
+                                                "max_resid")} for t in trajs],
+            "force": {"ms_gpu": round(ms_force, 4), "algorithmic_GB": round(b_force / 1e9, 4),
+                      "achieved_GBs": round(b_force / (ms_force * 1e-3) / 1e9, 1),
+                      "max_rel_diff_vs_cpu_fp64": float(err), "cpu_fp64_s": round(t_cpu, 3),
+                      "cpu_setup_s": round(t_setup, 1)}}
 
 
 def cpu_baseline(a, g, at, gpu_iters, sp_full, tp_full):

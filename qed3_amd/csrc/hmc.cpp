@@ -4,6 +4,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 
@@ -72,23 +74,21 @@ double GaugeAction::link_angle(const double* u, const double* t, int s, int l) c
 double GaugeAction::S(const double* u, const double* t) const {
   const int F = h_.F, E = h_.E;
   std::vector<long double> part(size_t(Lt_) * 2, 0.0L);
-  parallel_ranges(size_t(Lt_) * 2, [&](size_t a, size_t b) {
-    for (size_t k = a; k < b; k++) {
-      const int s = int(k / 2);
-      long double acc = 0;
-      if (k % 2 == 0) {
-        for (int f = 0; f < F; f++) {
-          const double P = face_angle(u, s, f);
-          acc += compact_ ? beta_s_[f] * (1.0 - std::cos(P)) : 0.5 * beta_s_[f] * P * P;
-        }
-      } else {
-        for (int l = 0; l < E; l++) {
-          const double P = link_angle(u, t, s, l);
-          acc += compact_ ? beta_t_[l] * (1.0 - std::cos(P)) : 0.5 * beta_t_[l] * P * P;
-        }
+  parallel_tasks(size_t(Lt_) * 2, [&](size_t k) {
+    const int s = int(k / 2);
+    long double acc = 0;
+    if (k % 2 == 0) {
+      for (int f = 0; f < F; f++) {
+        const double P = face_angle(u, s, f);
+        acc += compact_ ? beta_s_[f] * (1.0 - std::cos(P)) : 0.5 * beta_s_[f] * P * P;
       }
-      part[k] = acc;
+    } else {
+      for (int l = 0; l < E; l++) {
+        const double P = link_angle(u, t, s, l);
+        acc += compact_ ? beta_t_[l] * (1.0 - std::cos(P)) : 0.5 * beta_t_[l] * P * P;
+      }
     }
+    part[k] = acc;
   });
   long double tot = 0;
   for (auto v : part) tot += v;
@@ -100,24 +100,22 @@ void GaugeAction::force(const double* u, const double* t, double* fs, double* ft
   auto g = [&](double b, double P) { return compact_ ? b * std::sin(P) : b * P; };
   // one slice per task: every write lands in slice s (spatial faces), or in slice s of
   // fs/ft from the temporal plaquettes at s and s-1, so slices are independent.
-  parallel_ranges(size_t(Lt_), [&](size_t a, size_t b) {
-    for (size_t ss = a; ss < b; ss++) {
-      const int s = int(ss);
-      double* fss = fs + size_t(s) * E;
-      double* fts = ft + size_t(s) * V;
-      for (int l = 0; l < E; l++) fss[l] = 0;
-      for (int x = 0; x < V; x++) fts[x] = 0;
-      for (int f = 0; f < F; f++) {
-        const double grad = g(beta_s_[f], face_angle(u, s, f));
-        for (int k = fe_off_[f]; k < fe_off_[f + 1]; k++) fss[fe_link_[k]] += grad * fe_sign_[k];
-      }
-      for (int l = 0; l < E; l++) {
-        const double grad = g(beta_t_[l], link_angle(u, t, s, l));
-        const double grad2 = g(beta_t_[l], link_angle(u, t, s - 1, l));
-        fss[l] += grad - grad2;
-        fts[h_.links[l][1]] += grad;
-        fts[h_.links[l][0]] -= grad;
-      }
+  parallel_tasks(size_t(Lt_), [&](size_t ss) {
+    const int s = int(ss);
+    double* fss = fs + size_t(s) * E;
+    double* fts = ft + size_t(s) * V;
+    for (int l = 0; l < E; l++) fss[l] = 0;
+    for (int x = 0; x < V; x++) fts[x] = 0;
+    for (int f = 0; f < F; f++) {
+      const double grad = g(beta_s_[f], face_angle(u, s, f));
+      for (int k = fe_off_[f]; k < fe_off_[f + 1]; k++) fss[fe_link_[k]] += grad * fe_sign_[k];
+    }
+    for (int l = 0; l < E; l++) {
+      const double grad = g(beta_t_[l], link_angle(u, t, s, l));
+      const double grad2 = g(beta_t_[l], link_angle(u, t, s - 1, l));
+      fss[l] += grad - grad2;
+      fts[h_.links[l][1]] += grad;
+      fts[h_.links[l][0]] -= grad;
     }
   });
 }
@@ -170,15 +168,17 @@ void HMC::pf_gen(int k) {
   const size_t n = D_.points();
   std::vector<double> xi(4 * n);
   const double inv = 1.0 / std::sqrt(2.0);
-  for (int s = 0; s < h.Lt; s++)
-    for (int x = 0; x < h.V; x++)
+  parallel_ranges(n, [&](size_t p0, size_t p1) {  // one site stream per point, drawn in order
+    for (size_t P = p0; P < p1; P++) {
+      const int s = int(P / h.V), x = int(P % h.V);
       for (int a = 0; a < 2; a++) {
         const double re = rng_->gaussian_site(s, x);
         const double im = rng_->gaussian_site(s, x);
-        const size_t o = 4 * (size_t(s) * h.V + x) + 2 * a;
-        xi[o] = re * inv;
-        xi[o + 1] = im * inv;
+        xi[4 * P + 2 * a] = re * inv;
+        xi[4 * P + 2 * a + 1] = im * inv;
       }
+    }
+  });
   QED3_HIP(hipMemcpy(xi_, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice));
   D_.apply<double>(true, phi_[k], xi_, true);
   update_eta(k);
@@ -211,47 +211,89 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
   const HopTables& h = D_.tables();
   const size_t nS = size_t(h.Lt) * h.E, nT = size_t(h.Lt) * h.V;
   rep_ = HMCReport();
+  // per-phase host wall time, printed with QED3_HMC_TRACE set in the environment
+  const bool trace = std::getenv("QED3_HMC_TRACE") != nullptr;
+  double t_rng = 0, t_gauge = 0, t_fermion = 0, t_upload = 0, t_h = 0;
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto since = [&](std::chrono::steady_clock::time_point a) {
+    return std::chrono::duration<double>(now() - a).count();
+  };
+  auto ta = now();
   u_.assign(u, u + nS);
   t_.assign(t, t + nT);
   pu_.resize(nS);
   pt_.resize(nT);
-  // pi.gaussian(rng) (gauge_ext.h:146-149)
-  for (int s = 0; s < h.Lt; s++)
-    for (int l = 0; l < h.E; l++) pu_[size_t(s) * h.E + l] = rng_->gaussian_link(s, l);
-  for (int s = 0; s < h.Lt; s++)
-    for (int x = 0; x < h.V; x++) pt_[size_t(s) * h.V + x] = rng_->gaussian_site(s, x);
+  // pi.gaussian(rng) (gauge_ext.h:146-149); every entry has a stream of its own, so the
+  // draws run in parallel without changing any stream's sequence
+  parallel_ranges(nS, [&](size_t a, size_t b) {
+    for (size_t i = a; i < b; i++) pu_[i] = rng_->gaussian_link(int(i / h.E), int(i % h.E));
+  });
+  parallel_ranges(nT, [&](size_t a, size_t b) {
+    for (size_t i = a; i < b; i++) pt_[i] = rng_->gaussian_site(int(i / h.V), int(i % h.V));
+  });
+  t_rng += since(ta);
+  ta = now();
   set_gauge(u_, t_);
+  t_upload += since(ta);
+  ta = now();
   for (int k = 0; k < p_.npf; k++) pf_gen(k);
+  t_fermion += since(ta);
+  ta = now();
   const double h0 = H();
+  t_h += since(ta);
 
   // ExplicitLeapfrogML::integrate, multi-pseudofermion form (integrator.h:132-168)
   const double tau = p_.tau / p_.nsteps, tau_in = tau / p_.nsteps_inner;
   std::vector<double> gs(nS), gt(nT), fs(nS), ft(nT);
   auto kick = [&](const std::vector<double>& a, const std::vector<double>& b, double c) {
-    for (size_t i = 0; i < nS; i++) pu_[i] += c * a[i];
-    for (size_t i = 0; i < nT; i++) pt_[i] += c * b[i];
+    parallel_ranges(nS, [&](size_t i0, size_t i1) {
+      for (size_t i = i0; i < i1; i++) pu_[i] += c * a[i];
+    });
+    parallel_ranges(nT, [&](size_t i0, size_t i1) {
+      for (size_t i = i0; i < i1; i++) pt_[i] += c * b[i];
+    });
   };
+  auto drift = [&](double c) {
+    parallel_ranges(nS, [&](size_t i0, size_t i1) {
+      for (size_t i = i0; i < i1; i++) u_[i] += c * pu_[i];
+    });
+    parallel_ranges(nT, [&](size_t i0, size_t i1) {
+      for (size_t i = i0; i < i1; i++) t_[i] += c * pt_[i];
+    });
+  };
+  ta = now();
   for (int k = 0; k < p_.npf; k++) {
     fermion_force(k, fs, ft);
     kick(fs, ft, -0.5 * tau);
   }
+  t_fermion += since(ta);
   for (int n = 0; n < p_.nsteps; n++) {
+    ta = now();
     Sg_.force(u_.data(), t_.data(), gs.data(), gt.data());
     kick(gs, gt, -0.5 * tau_in);
     for (int ni = 0; ni < p_.nsteps_inner; ni++) {
-      for (size_t i = 0; i < nS; i++) u_[i] += tau_in * pu_[i];
-      for (size_t i = 0; i < nT; i++) t_[i] += tau_in * pt_[i];
+      drift(tau_in);
       Sg_.force(u_.data(), t_.data(), gs.data(), gt.data());
       kick(gs, gt, (ni != p_.nsteps_inner - 1) ? -1.0 * tau_in : -0.5 * tau_in);
     }
+    t_gauge += since(ta);
+    ta = now();
     set_gauge(u_, t_);
+    t_upload += since(ta);
+    ta = now();
     for (int k = 0; k < p_.npf; k++) {
       update_eta(k);
       fermion_force(k, fs, ft);
       kick(fs, ft, (n != p_.nsteps - 1) ? -1.0 * tau : -0.5 * tau);
     }
+    t_fermion += since(ta);
   }
+  ta = now();
   const double h1 = H();
+  t_h += since(ta);
+  if (trace)
+    std::fprintf(stderr, "hmc trajectory: rng %.3fs gauge %.3fs fermion %.3fs upload %.3fs H %.3fs\n",
+                 t_rng, t_gauge, t_fermion, t_upload, t_h);
   rep_.h0 = h0;
   rep_.h1 = h1;
   rep_.dH = h1 - h0;

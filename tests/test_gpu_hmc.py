@@ -63,3 +63,23 @@ def test_hmc_dH_scales_as_tau_squared(gpu):
         dh.append(abs(rep["dH"]))
     ratio = dh[0] / dh[1]
     assert 2.5 < ratio < 6.0, (dh, ratio)
+
+
+@pytest.mark.gpu
+def test_hmc_parallel_host_path_matches_oracle(gpu):
+    """At n=8, Lt=16 (10,272 points, 30,720 links) the momentum and noise draws and the
+    leapfrog updates run on host threads; the trajectory still equals the oracle's."""
+    lattice, n, Lt, M5, gsq = 0, 8, 16, 0.1, 1 / 1.4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    sp, tp = links(lattice, n, Lt, seed=8, kappa=1.4)
+    Sg = O.GaugeAction(L, Lt, gsq=gsq, at=at, compact=True)
+    ref = O.hmc_trajectory(Do, Sg, seed=21, npf=1, tau=0.2, nsteps=3, nsteps_inner=2,
+                           tol=1e-13, sp=sp, tp=tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, M5=M5, at=at)
+    H = q.HMC(Dq, gsq, compact=True, npf=1, tau=0.2, nsteps=3, nsteps_inner=2, tol=1e-12,
+              solver=q.CG_F64, seed=21)
+    u, t = sp.copy(), tp.copy()
+    rep = H.trajectory(u, t, no_reject=True)
+    assert abs(rep["h0"] - ref["h0"]) < 1e-9 * abs(ref["h0"])
+    assert abs(rep["dH"] - ref["dH"]) < 1e-9 * max(1.0, abs(ref["h0"]))
+    assert rel(u, ref["u"]) < 1e-10 and rel(t, ref["t"]) < 1e-10
