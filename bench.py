@@ -45,7 +45,10 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-learned", action="store_true")
     p.add_argument("--no-hmc", action="store_true")
-    p.add_argument("--hmc-traj", type=int, default=2)
+    p.add_argument("--hmc-traj", type=int, default=3)
+    p.add_argument("--hmc-therm", type=int, default=6)
+    p.add_argument("--hmc-inner", type=int, default=50)
+    p.add_argument("--hmc-nsteps", type=int, default=20)
     p.add_argument("--learned-steps", type=int, default=2)
     p.add_argument("--train-n", type=int, default=16)
     p.add_argument("--train-Lt", type=int, default=16)
@@ -372,9 +375,13 @@ def plain_leg(a, q, g, m, lattice):
 
 def hmc_leg(a, q):
     """Full HMC (simplicial n=64, Lt=32, m=0.1, compact U(1) at beta=1.4, tau=1, 20 outer
-    steps, Nf=2 via one pseudofermion, mixed CG): seconds per trajectory, dH, accept and CG
-    work; the pseudofermion force at the same size against the FP64 CPU path (the oracle's
-    local route) on the same eta, and the force launch against its algorithmic bytes."""
+    (fermion) steps of hmc_inner gauge steps each, Nf=2 via one pseudofermion, mixed CG):
+    hmc_therm forced-accept trajectories from the von Mises start, then hmc_traj timed
+    trajectories with the accept step (seconds per trajectory, dH, acceptance, CG work);
+    the pseudofermion force at the same size against the FP64 CPU path (the oracle's local
+    route) on the same eta, and the force launch against its algorithmic bytes.  The gauge
+    coupling per face, beta_s = a_t/(vol_f g^2), is ~220 at n=64 with a_t = <l>/2, which
+    sets the small gauge step."""
     import oracle as O
     n, Lt, m, lattice = 64, 32, 0.1, 0
     g = q.Geometry(n)
@@ -384,9 +391,11 @@ def hmc_leg(a, q):
     sp, tp = np.ascontiguousarray(sp), np.ascontiguousarray(tp)
     D.update(sp, tp)
     D.autotune()
-    H = q.HMC(D, 1 / 1.4, compact=True, npf=1, tau=1.0, nsteps=20, nsteps_inner=5, tol=1e-10,
-              solver=q.CG_MIXED, seed=7)
-    H.trajectory(sp, tp)  # warmup
+    H = q.HMC(D, 1 / 1.4, compact=True, npf=1, tau=1.0, nsteps=a.hmc_nsteps, nsteps_inner=a.hmc_inner,
+              tol=1e-10, solver=q.CG_MIXED, seed=7)
+    for k in range(a.hmc_therm):
+        r = H.trajectory(sp, tp, no_reject=True)
+        print(f"# hmc thermalisation {k}: dH={r['dH']:.3e}", file=sys.stderr, flush=True)
     trajs = []
     for k in range(a.hmc_traj):
         t0 = time.time()
@@ -426,7 +435,10 @@ def hmc_leg(a, q):
     link = (D.E + D.V) * Lt * 16
     # one FP64 D apply (D eta), then the link kernel: eta and D eta in, phases, one double out
     b_force = (2 * vec + link) + (2 * vec + link + (D.E + D.V) * Lt * 8)
-    return {"workload": f"simplicial n={n} Lt={Lt} m={m} compact beta=1.4 tau=1 nsteps=20x5 npf=1",
+    return {"workload": f"simplicial n={n} Lt={Lt} m={m} compact beta=1.4 tau=1 "
+                        f"nsteps={a.hmc_nsteps}x{a.hmc_inner} npf=1, after {a.hmc_therm} forced-accept "
+                        "trajectories",
+            "acceptance": float(np.mean([t["accept"] for t in trajs])),
             "seconds_per_trajectory": round(float(np.mean([t["wall_s"] for t in trajs])), 3),
             "trajectories": [{k: t[k] for k in ("dH", "accept", "cg_solves", "cg_iterations",
                                                 "max_resid")} for t in trajs],

@@ -137,6 +137,11 @@ HMC::HMC(Dirac& D, const HMCParams& p)
   QED3_HIP(hipMalloc(&xi_, n * sizeof(Spin<double>)));
   QED3_HIP(hipMalloc(&dfs_, size_t(h.Lt) * h.E * sizeof(double)));
   QED3_HIP(hipMalloc(&dft_, size_t(h.Lt) * h.V * sizeof(double)));
+  dg_ = std::make_unique<DeviceGauge>(h, Sg_.beta_s(), Sg_.beta_t(), Sg_.face_edge_offsets(),
+                                      Sg_.face_edge_links(), Sg_.face_edge_signs(), Sg_.compact());
+  const size_t nS = size_t(h.Lt) * h.E, nT = size_t(h.Lt) * h.V;
+  for (double** p : {&du_, &dpu_, &dgs_}) QED3_HIP(hipMalloc(p, nS * sizeof(double)));
+  for (double** p : {&dt_, &dpt_, &dgt_}) QED3_HIP(hipMalloc(p, nT * sizeof(double)));
 }
 
 HMC::~HMC() {
@@ -145,10 +150,7 @@ HMC::~HMC() {
   (void)hipFree(xi_);
   (void)hipFree(dfs_);
   (void)hipFree(dft_);
-}
-
-void HMC::set_gauge(const std::vector<double>& u, const std::vector<double>& t) {
-  D_.set_gauge(u.data(), t.data(), false);
+  for (double* p : {du_, dt_, dpu_, dpt_, dgs_, dgt_}) (void)hipFree(p);
 }
 
 void HMC::update_eta(int k) {
@@ -190,18 +192,12 @@ double HMC::Sf(int k) {  // Re <phi, eta> (pseudofermion.h:53-58)
   return re;
 }
 
-void HMC::fermion_force(int k, std::vector<double>& fs, std::vector<double>& ft) {
-  D_.force(eta_[k], dfs_, dft_);
-  QED3_HIP(hipMemcpy(fs.data(), dfs_, fs.size() * sizeof(double), hipMemcpyDeviceToHost));
-  QED3_HIP(hipMemcpy(ft.data(), dft_, ft.size() * sizeof(double), hipMemcpyDeviceToHost));
-}
-
-double HMC::H() {  // hmc.h:159-166
-  long double kin = 0;
-  for (double v : pu_) kin += (long double)v * v;
-  for (double v : pt_) kin += (long double)v * v;
-  long double res = 0.5L * kin;
-  res += Sg_.S(u_.data(), t_.data());
+double HMC::H() {  // hmc.h:159-166: 1/2 |pi|^2 + S_g + sum_k S_f (device state, FP64)
+  const HopTables& h = D_.tables();
+  hipStream_t st = D_.stream();
+  const size_t nS = size_t(h.Lt) * h.E, nT = size_t(h.Lt) * h.V;
+  long double res = 0.5L * ((long double)dg_->norm2(dpu_, nS, st) + dg_->norm2(dpt_, nT, st));
+  res += dg_->action(du_, dt_, st);
   for (int k = 0; k < p_.npf; k++) res += Sf(k);
   return double(res);
 }
@@ -210,17 +206,17 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
   const auto t0 = std::chrono::steady_clock::now();
   const HopTables& h = D_.tables();
   const size_t nS = size_t(h.Lt) * h.E, nT = size_t(h.Lt) * h.V;
+  hipStream_t st = D_.stream();
   rep_ = HMCReport();
   // per-phase host wall time, printed with QED3_HMC_TRACE set in the environment
   const bool trace = std::getenv("QED3_HMC_TRACE") != nullptr;
-  double t_rng = 0, t_gauge = 0, t_fermion = 0, t_upload = 0, t_h = 0;
+  double t_rng = 0, t_gauge = 0, t_fermion = 0, t_h = 0;
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto since = [&](std::chrono::steady_clock::time_point a) {
+    QED3_HIP(hipStreamSynchronize(st));
     return std::chrono::duration<double>(now() - a).count();
   };
   auto ta = now();
-  u_.assign(u, u + nS);
-  t_.assign(t, t + nT);
   pu_.resize(nS);
   pt_.resize(nT);
   // pi.gaussian(rng) (gauge_ext.h:146-149); every entry has a stream of its own, so the
@@ -231,10 +227,13 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
   parallel_ranges(nT, [&](size_t a, size_t b) {
     for (size_t i = a; i < b; i++) pt_[i] = rng_->gaussian_site(int(i / h.V), int(i % h.V));
   });
+  // the integrator's state lives on the device from here to the accept step
+  QED3_HIP(hipMemcpyAsync(dpu_, pu_.data(), nS * sizeof(double), hipMemcpyHostToDevice, st));
+  QED3_HIP(hipMemcpyAsync(dpt_, pt_.data(), nT * sizeof(double), hipMemcpyHostToDevice, st));
+  QED3_HIP(hipMemcpyAsync(du_, u, nS * sizeof(double), hipMemcpyHostToDevice, st));
+  QED3_HIP(hipMemcpyAsync(dt_, t, nT * sizeof(double), hipMemcpyHostToDevice, st));
+  D_.set_gauge(du_, dt_, true);
   t_rng += since(ta);
-  ta = now();
-  set_gauge(u_, t_);
-  t_upload += since(ta);
   ta = now();
   for (int k = 0; k < p_.npf; k++) pf_gen(k);
   t_fermion += since(ta);
@@ -244,47 +243,34 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
 
   // ExplicitLeapfrogML::integrate, multi-pseudofermion form (integrator.h:132-168)
   const double tau = p_.tau / p_.nsteps, tau_in = tau / p_.nsteps_inner;
-  std::vector<double> gs(nS), gt(nT), fs(nS), ft(nT);
-  auto kick = [&](const std::vector<double>& a, const std::vector<double>& b, double c) {
-    parallel_ranges(nS, [&](size_t i0, size_t i1) {
-      for (size_t i = i0; i < i1; i++) pu_[i] += c * a[i];
-    });
-    parallel_ranges(nT, [&](size_t i0, size_t i1) {
-      for (size_t i = i0; i < i1; i++) pt_[i] += c * b[i];
-    });
+  auto kick = [&](const double* fs, const double* ft, double c) {
+    daxpy(dpu_, fs, c, nS, st);
+    daxpy(dpt_, ft, c, nT, st);
   };
-  auto drift = [&](double c) {
-    parallel_ranges(nS, [&](size_t i0, size_t i1) {
-      for (size_t i = i0; i < i1; i++) u_[i] += c * pu_[i];
-    });
-    parallel_ranges(nT, [&](size_t i0, size_t i1) {
-      for (size_t i = i0; i < i1; i++) t_[i] += c * pt_[i];
-    });
-  };
+  auto gauge_force = [&] { dg_->force(du_, dt_, dgs_, dgt_, st); };
   ta = now();
   for (int k = 0; k < p_.npf; k++) {
-    fermion_force(k, fs, ft);
-    kick(fs, ft, -0.5 * tau);
+    D_.force(eta_[k], dfs_, dft_);
+    kick(dfs_, dft_, -0.5 * tau);
   }
   t_fermion += since(ta);
   for (int n = 0; n < p_.nsteps; n++) {
     ta = now();
-    Sg_.force(u_.data(), t_.data(), gs.data(), gt.data());
-    kick(gs, gt, -0.5 * tau_in);
+    gauge_force();
+    kick(dgs_, dgt_, -0.5 * tau_in);
     for (int ni = 0; ni < p_.nsteps_inner; ni++) {
-      drift(tau_in);
-      Sg_.force(u_.data(), t_.data(), gs.data(), gt.data());
-      kick(gs, gt, (ni != p_.nsteps_inner - 1) ? -1.0 * tau_in : -0.5 * tau_in);
+      daxpy(du_, dpu_, tau_in, nS, st);
+      daxpy(dt_, dpt_, tau_in, nT, st);
+      gauge_force();
+      kick(dgs_, dgt_, (ni != p_.nsteps_inner - 1) ? -1.0 * tau_in : -0.5 * tau_in);
     }
+    D_.set_gauge(du_, dt_, true);
     t_gauge += since(ta);
-    ta = now();
-    set_gauge(u_, t_);
-    t_upload += since(ta);
     ta = now();
     for (int k = 0; k < p_.npf; k++) {
       update_eta(k);
-      fermion_force(k, fs, ft);
-      kick(fs, ft, (n != p_.nsteps - 1) ? -1.0 * tau : -0.5 * tau);
+      D_.force(eta_[k], dfs_, dft_);
+      kick(dfs_, dft_, (n != p_.nsteps - 1) ? -1.0 * tau : -0.5 * tau);
     }
     t_fermion += since(ta);
   }
@@ -292,8 +278,8 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
   const double h1 = H();
   t_h += since(ta);
   if (trace)
-    std::fprintf(stderr, "hmc trajectory: rng %.3fs gauge %.3fs fermion %.3fs upload %.3fs H %.3fs\n",
-                 t_rng, t_gauge, t_fermion, t_upload, t_h);
+    std::fprintf(stderr, "hmc trajectory: rng+upload %.3fs gauge %.3fs fermion %.3fs H %.3fs\n",
+                 t_rng, t_gauge, t_fermion, t_h);
   rep_.h0 = h0;
   rep_.h1 = h1;
   rep_.dH = h1 - h0;
@@ -301,8 +287,9 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
   const double a = rng_->uniform();
   rep_.accept = (a < r || no_reject) ? 1 : 0;
   if (rep_.accept) {
-    std::copy(u_.begin(), u_.end(), u);
-    std::copy(t_.begin(), t_.end(), t);
+    QED3_HIP(hipMemcpyAsync(u, du_, nS * sizeof(double), hipMemcpyDeviceToHost, st));
+    QED3_HIP(hipMemcpyAsync(t, dt_, nT * sizeof(double), hipMemcpyDeviceToHost, st));
+    QED3_HIP(hipStreamSynchronize(st));
   } else {
     D_.set_gauge(u, t, false);
   }

@@ -11,6 +11,7 @@
 
 #include "cg.hpp"
 #include "dirac.hpp"
+#include "gauge_dev.hpp"
 #include "rng.hpp"
 
 namespace qed3 {
@@ -23,6 +24,12 @@ class GaugeAction {
   void force(const double* u, const double* t, double* fs, double* ft) const;
   // plaquette-angle correlator over time separations t = 0..Lt-1 (corr[Lt])
   void plaquette_correlator(const double* u, double* corr) const;
+  const std::vector<double>& beta_s() const { return beta_s_; }
+  const std::vector<double>& beta_t() const { return beta_t_; }
+  const std::vector<int>& face_edge_offsets() const { return fe_off_; }
+  const std::vector<int>& face_edge_links() const { return fe_link_; }
+  const std::vector<int>& face_edge_signs() const { return fe_sign_; }
+  bool compact() const { return compact_; }
 
  private:
   double face_angle(const double* u, int s, int f) const;
@@ -69,7 +76,6 @@ class HMC {
   void pf_gen(int k);
   void update_eta(int k);
   double Sf(int k);
-  void fermion_force(int k, std::vector<double>& fs, std::vector<double>& ft);
   double H();
 
   Dirac& D_;
@@ -80,7 +86,11 @@ class HMC {
   Spin<double>* xi_ = nullptr;
   double* dfs_ = nullptr;
   double* dft_ = nullptr;
-  std::vector<double> u_, t_, pu_, pt_;
+  std::vector<double> pu_, pt_;  // host momenta (drawn from the host streams)
+  // device state of the integrator: angles, momenta, gauge force
+  std::unique_ptr<DeviceGauge> dg_;
+  double *du_ = nullptr, *dt_ = nullptr, *dpu_ = nullptr, *dpt_ = nullptr;
+  double *dgs_ = nullptr, *dgt_ = nullptr;
   HMCReport rep_;
 };
 
