@@ -49,6 +49,9 @@ def parse():
     p.add_argument("--hmc-therm", type=int, default=6)
     p.add_argument("--hmc-inner", type=int, default=50)
     p.add_argument("--hmc-nsteps", type=int, default=20)
+    p.add_argument("--batched", action="store_true",
+                   help="also time the 64-system batched CG (uneven sizes) against one-by-one solves")
+    p.add_argument("--batch-nsys", type=int, default=64)
     p.add_argument("--learned-steps", type=int, default=2)
     p.add_argument("--train-n", type=int, default=16)
     p.add_argument("--train-Lt", type=int, default=16)
@@ -163,6 +166,7 @@ def main():
     del D, d_x, rhs
     learned = learned_leg(a, q, g, with_net=not a.no_learned)
     hmc = None if a.no_hmc else hmc_leg(a, q)
+    batched = batched_leg(a, q) if a.batched else None
 
     simp = {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
                         f"m={a.m} tol={a.tol} D^dag D CG", "methods": results,
@@ -186,7 +190,7 @@ def main():
                   "speedup_learned_vs_fp64": learned["speedup_vs_fp64"],
                   "speedup_mixed_vs_fp64": learned["speedup_mixed_vs_fp64"],
                   "net_apply": learned["net_apply"], "solution_agreement": learned["agreement"],
-                  "second_workload": simp, "hmc": hmc}
+                  "second_workload": simp, "hmc": hmc, "batched": batched}
         roof = learned["roofline"]
         dtype = ("f32 inner (bf16 net activations) / f64 reliable updates" if winner == "learned"
                  else "f32 inner / f64 reliable updates" if winner == "mixed" else "f64")
@@ -446,6 +450,54 @@ def hmc_leg(a, q):
                       "achieved_GBs": round(b_force / (ms_force * 1e-3) / 1e9, 1),
                       "max_rel_diff_vs_cpu_fp64": float(err), "cpu_fp64_s": round(t_cpu, 3),
                       "cpu_setup_s": round(t_setup, 1)}}
+
+
+def batched_leg(a, q):
+    """Batched mixed CG over nsys systems of uneven shapes: refinement n in {32,64,96,128},
+    Lt in {16,32,64}, both lattices, m ~ U[0.02, 0.2] (seeded), von Mises(1.4) links, one
+    Gaussian RHS each; one batched call against the same systems solved one by one."""
+    rng = np.random.default_rng(31)
+    geos = {}
+    ops, xs, bs, desc = [], [], [], []
+    t0 = time.time()
+    for k in range(a.batch_nsys):
+        n = int(rng.choice([32, 64, 96, 128]))
+        Lt = int(rng.choice([16, 32, 64]))
+        lattice = int(rng.integers(0, 2))
+        m = float(rng.uniform(0.02, 0.2))
+        if n not in geos:
+            geos[n] = q.Geometry(n)
+        D = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=m, at=0.5 * mean_ell(geos[n], lattice))
+        sp, tp = q.random_gauge(lattice, n, Lt, seed=500 + k, kappa=a.beta)
+        D.update(sp, tp)
+        ops.append(D)
+        b = q.random_source(lattice, n, Lt, seed=900 + k)
+        bs.append(q.DeviceArray.from_numpy(b))
+        xs.append(q.DeviceArray(b.nbytes))
+        desc.append((lattice, n, Lt, round(m, 4)))
+    t_setup = time.time() - t0
+    pts = sum(D.V * D.Lt for D in ops)
+    print(f"# batched: {a.batch_nsys} systems, {pts} points, setup {t_setup:.1f}s",
+          file=sys.stderr, flush=True)
+    q.batch_solve(ops[:2], xs[:2], bs[:2], tol=a.tol)  # warm the kernels
+    q.load().qed3_device_synchronize()
+    t0 = time.time()
+    reps = q.batch_solve(ops, xs, bs, tol=a.tol)
+    q.load().qed3_device_synchronize()
+    t_batch = time.time() - t0
+    t0 = time.time()
+    seq = [D.solve_device(x, b, tol=a.tol, solver=q.CG_MIXED) for D, x, b in zip(ops, xs, bs)]
+    q.load().qed3_device_synchronize()
+    t_seq = time.time() - t0
+    return {"systems": a.batch_nsys, "points": int(pts), "setup_s": round(t_setup, 1),
+            "batched_s": round(t_batch, 3), "one_by_one_s": round(t_seq, 3),
+            "solves_per_s_batched": round(a.batch_nsys / t_batch, 2),
+            "solves_per_s_one_by_one": round(a.batch_nsys / t_seq, 2),
+            "iterations_max": int(max(r.iterations for r in reps)),
+            "iterations_mean": float(np.mean([r.iterations for r in reps])),
+            "max_true_resid": float(max(r.true_resid for r in reps)),
+            "one_by_one_iterations_mean": float(np.mean([r.iterations for r in seq])),
+            "shapes": desc[:8]}
 
 
 def cpu_baseline(a, g, at, gpu_iters, sp_full, tp_full):
