@@ -216,7 +216,7 @@ class WilsonDirac:
     def DHD(self, v, prec=F64):
         return self.apply(OP_DDAGD, v, prec)
 
-    def solve_device(self, d_x, d_b, tol=1e-10, maxiter=100000, solver=CG_F64, delta=0.1,
+    def solve_device(self, d_x, d_b, tol=1e-10, maxiter=100000, solver=CG_F64, delta=0.01,
                      check_every=0, raise_on_fail=True):
         opts = CGOpts(tol, maxiter, solver, delta, check_every)
         rep = CGReport()
@@ -277,7 +277,7 @@ class WilsonDirac:
         return ms.value
 
 
-def batch_solve(ops, xs, bs, tol=1e-10, maxiter=100000, delta=0.1, check_every=0,
+def batch_solve(ops, xs, bs, tol=1e-10, maxiter=100000, delta=0.01, check_every=0,
                 raise_on_fail=True):
     """Batched mixed CG over WilsonDirac systems of any shapes; xs/bs DeviceArrays."""
     n = len(ops)
@@ -375,7 +375,7 @@ class Net:
                                     _dp(hist)))
         return hist
 
-    def fcg(self, d_x, d_b, tol=1e-10, maxiter=20000, delta=0.1, raise_on_fail=True):
+    def fcg(self, d_x, d_b, tol=1e-10, maxiter=20000, delta=0.01, raise_on_fail=True):
         opts = CGOpts(tol, maxiter, CG_MIXED, delta, 0)
         rep = CGReport()
         pa, fb = C.c_int(), C.c_int()

@@ -79,23 +79,6 @@ __global__ __launch_bounds__(256) void k_flush(Spin<double>* x64, Spin<T>* xin, 
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_pupdate(Spin<T>* p, const Spin<T>* __restrict__ r,
-                                                 size_t n, const CGScalars* sc) {
-  if (sc->done != 0) return;
-  const T beta = T(sc->beta);
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    const Spin<T> rr = r[i];
-    Spin<T> pp = p[i];
-    pp.a.x = rr.a.x + beta * pp.a.x;
-    pp.a.y = rr.a.y + beta * pp.a.y;
-    pp.b.x = rr.b.x + beta * pp.b.x;
-    pp.b.y = rr.b.y + beta * pp.b.y;
-    p[i] = pp;
-  }
-}
-
 __global__ __launch_bounds__(256) void k_resid_norms(const Spin<double>* __restrict__ b,
                                                      const Spin<double>* __restrict__ ax, size_t n,
                                                      GridReduce gr, double2* out) {
@@ -114,6 +97,7 @@ template <typename T>
 struct Work {
   Spin<T>* r = nullptr;
   Spin<T>* p = nullptr;
+  Spin<T>* p2 = nullptr;  // the other buffer of the double-buffered search direction
   Spin<T>* w = nullptr;
   Spin<T>* xin = nullptr;
   Spin<double>* t = nullptr;
@@ -130,6 +114,7 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   Work<T> W;
   QED3_HIP(hipMallocAsync((void**)&W.r, n * sizeof(Spin<T>), st));
   QED3_HIP(hipMallocAsync((void**)&W.p, n * sizeof(Spin<T>), st));
+  QED3_HIP(hipMallocAsync((void**)&W.p2, n * sizeof(Spin<T>), st));
   QED3_HIP(hipMallocAsync((void**)&W.w, n * sizeof(Spin<T>), st));
   if (kMixed) {
     QED3_HIP(hipMallocAsync((void**)&W.xin, n * sizeof(Spin<T>), st));
@@ -156,22 +141,33 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   const int nn = D.tables().max_nn;
   const HaloView hvf = D.halo_for<T>(false), hvt = D.halo_for<T>(true);
   const HaloView hv64f = D.halo_for<double>(false), hv64t = D.halo_for<double>(true);
-  auto one_iter = [&]() {
-    launch_stencil<T, false>(vin, nn, W.p, EpiK1<T>{W.w, sc}, D.tile<T>(false), gr, st, hvf);
-    launch_stencil<T, true>(vin, nn, W.w, EpiK2<T>{W.xin, W.r, W.p, sc, T(0)}, D.tile<T>(true), gr, st, hvt);
+  // One iteration.  K1 forms p = r + beta p_old on the fly (beta = 0 and p_old = r at the
+  // start; after a reliable update r and beta are the true residual's), stores that p in
+  // the other buffer and applies D to it; K2 reads the new p.  p_old is read at halo points
+  // of other blocks while the new p is written, hence two buffers alternating by parity.
+  Spin<T>* P[2] = {W.p, W.p2};
+  auto one_iter = [&](int par) {
+    Spin<T>* pold = P[par];
+    Spin<T>* pnew = P[par ^ 1];
+    EpiK1P<T> k1;
+    k1.w = W.w;
+    k1.sc = sc;
+    k1.pnew = pnew;
+    launch_stencil_src<T, false>(vin, nn, SrcPUpdate<T>{W.r, pold, sc}, k1, D.tile<T>(false), gr, st, hvf);
+    launch_stencil<T, true>(vin, nn, W.w, EpiK2<T>{W.xin, W.r, pnew, sc, T(0)}, D.tile<T>(true), gr, st, hvt);
     if (kMixed)
       hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 0);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{W.t, sc}, D.tile<double>(false), gr, st, hv64f);
     launch_stencil<double, true>(v64, nn, W.t, EpiResid<T>{b, W.r, sc}, D.tile<double>(true), gr, st, hv64t);
-    hipLaunchKernelGGL(k_pupdate<T>, dim3(nbk), dim3(256), 0, st, W.p, W.r, n, sc);
   };
 
   int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
+  chunk += chunk & 1;  // even, so every replay of a captured chunk starts at parity 0
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   if (o.use_graph && std::getenv("QED3_NO_GRAPH") == nullptr) {
     QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; i++) one_iter();
+    for (int i = 0; i < chunk; i++) one_iter(i & 1);
     QED3_HIP(hipStreamEndCapture(st, &graph));
     QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
@@ -180,7 +176,7 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
     if (exec)
       QED3_HIP(hipGraphLaunch(exec, st));
     else
-      for (int i = 0; i < chunk; i++) one_iter();
+      for (int i = 0; i < chunk; i++) one_iter(i & 1);
     QED3_HIP(hipGetLastError());
     QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
     QED3_HIP(hipStreamSynchronize(st));
@@ -197,6 +193,7 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   if (graph) (void)hipGraphDestroy(graph);
   (void)hipFreeAsync(W.r, st);
   (void)hipFreeAsync(W.p, st);
+  (void)hipFreeAsync(W.p2, st);
   (void)hipFreeAsync(W.w, st);
   if (kMixed) {
     (void)hipFreeAsync(W.xin, st);

@@ -26,7 +26,7 @@ struct CGOptions {
   double tol = 1e-10;
   int maxiter = 100000;
   bool mixed = false;
-  double delta = 0.1;
+  double delta = 0.01;  // reliable update when |r| falls below delta * max |r| since the last one
   int chunk = 0;  // iterations per graph replay (0 = auto)
   bool use_graph = true;  // replay chunks of iterations as a HIP graph (QED3_NO_GRAPH=1 in the
                           // environment launches the same kernels directly, e.g. for profilers)

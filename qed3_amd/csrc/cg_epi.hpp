@@ -31,6 +31,14 @@ struct EpiK1 {
   }
 };
 
+// K1 reading p = r + beta p_old (SrcPUpdate) and storing that p at every point it owns for
+// K2 and the next iteration (the search-direction update fused into the matvec)
+template <typename T>
+struct EpiK1P : EpiK1<T> {
+  Spin<T>* pnew;
+  __device__ void center(size_t i, const Spin<T>& v) const { st(pnew + i, v); }
+};
+
 // K2: q = D^dag w; x += alpha p; r -= alpha q; |r|^2 and the reliable-update decision
 template <typename T>
 struct EpiK2 {

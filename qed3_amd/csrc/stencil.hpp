@@ -47,11 +47,54 @@ __device__ __forceinline__ void tile_of(const TileMap& m, int b, int& xr, int& s
   }
 }
 
+// Input sources of the stencil.  bind() runs once per thread at kernel start and returns
+// the per-point reader.  SrcPlain reads a field; SrcPUpdate forms the CG search direction
+// p = r + beta p_old on the fly (beta from the solver scalars), so the p update needs no
+// pass of its own.
+template <typename T>
+struct SrcPlain {
+  const Spin<T>* __restrict__ p;
+  __device__ SrcPlain bind() const { return *this; }
+  __device__ Spin<T> operator()(size_t i) const { return ld(p + i); }
+};
+
+template <typename T>
+struct SrcPUpdate {
+  const Spin<T>* r;
+  const Spin<T>* p;
+  const CGScalars* sc;
+  struct Bound {
+    const Spin<T>* __restrict__ r;
+    const Spin<T>* __restrict__ p;
+    T beta;
+    __device__ Spin<T> operator()(size_t i) const {
+      Spin<T> a = ld(r + i);
+      const Spin<T> c = ld(p + i);
+      a.a.x += beta * c.a.x;
+      a.a.y += beta * c.a.y;
+      a.b.x += beta * c.b.x;
+      a.b.y += beta * c.b.y;
+      return a;
+    }
+  };
+  __device__ Bound bind() const { return Bound{r, p, T(sc->beta)}; }
+};
+
+// Epilogues may take the stencil's input value at each output point (its centre) through
+// center(i, v); the others ignore it.
+template <class Epi, typename T>
+__device__ __forceinline__ auto call_center(const Epi& e, size_t i, const Spin<T>& v, int)
+    -> decltype(e.center(i, v), void()) {
+  e.center(i, v);
+}
+template <class Epi, typename T>
+__device__ __forceinline__ void call_center(const Epi&, size_t, const Spin<T>&, long) {}
+
 // The stencil for site x over the time slices [s0, s0+ts), handing every output point
 // to the epilogue (shared by the single-system and the batched kernels).
-template <typename T, int NS, bool DAG, class Epi>
-__device__ __forceinline__ void stencil_site(const DiracView<T>& d, const Spin<T>* __restrict__ in,
-                                             const Epi& epi, int x, int s0, int ts, double2& acc) {
+template <typename T, int NS, bool DAG, class Epi, class In>
+__device__ __forceinline__ void stencil_site_src(const DiracView<T>& d, const In& in,
+                                                 const Epi& epi, int x, int s0, int ts, double2& acc) {
   using C = cplx_t<T>;
   const int V = d.V, Lt = d.Lt;
   if (x < V && s0 < Lt) {
@@ -72,22 +115,22 @@ __device__ __forceinline__ void stencil_site(const DiracView<T>& d, const Spin<T
     const T fw0 = DAG ? m0 : m1, fw1 = DAG ? m1 : m0;
     const int s1 = min(s0 + ts, Lt);
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
-    Spin<T> prev = ld(in + size_t(sm) * V + x);
-    Spin<T> cur = ld(in + size_t(s0) * V + x);
+    Spin<T> prev = in(size_t(sm) * V + x);
+    Spin<T> cur = in(size_t(s0) * V + x);
     C uprev = d.uT[size_t(sm) * V + x];
     for (int s = s0; s < s1; s++) {
       const int sp = (s == Lt - 1) ? 0 : s + 1;
-      const Spin<T> next = ld(in + size_t(sp) * V + x);
+      const Spin<T> next = in(size_t(sp) * V + x);
       const C ucur = d.uT[size_t(s) * V + x];
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
       const C* uS = d.uS + size_t(s) * d.E;
-      const Spin<T>* ins = in + size_t(s) * V;
+      const size_t sV = size_t(s) * V;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
         const int l = lk[j];
         C u = uS[l >= 0 ? l : ~l];
         if (l < 0) u.y = -u.y;
-        const Spin<T> py = ld(ins + nb[j]);
+        const Spin<T> py = in(sV + nb[j]);
         const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
         const T Cc = hp[j].x, Ss = hp[j].y, A = hp[j].z, B = hp[j].w;
         // H = [[-r(C - iS), a - ib], [a + ib, -r(C + iS)]]
@@ -117,6 +160,7 @@ __device__ __forceinline__ void stencil_site(const DiracView<T>& d, const Spin<T
       o.a = o0;
       o.b = o1;
       epi(size_t(s) * V + x, o, acc);
+      call_center(epi, size_t(s) * V + x, cur, 0);
       prev = cur;
       cur = next;
       uprev = ucur;
@@ -125,15 +169,21 @@ __device__ __forceinline__ void stencil_site(const DiracView<T>& d, const Spin<T
 }
 
 template <typename T, int NS, bool DAG, class Epi>
-__global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* __restrict__ in,
-                                                  Epi epi, int ts, TileMap tm, GridReduce gr) {
+__device__ __forceinline__ void stencil_site(const DiracView<T>& d, const Spin<T>* __restrict__ in,
+                                             const Epi& epi, int x, int s0, int ts, double2& acc) {
+  stencil_site_src<T, NS, DAG>(d, SrcPlain<T>{in}, epi, x, s0, ts, acc);
+}
+
+template <typename T, int NS, bool DAG, class Epi, class Src>
+__global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, Src in, Epi epi, int ts, TileMap tm,
+                                                  GridReduce gr) {
   if (epi.skip()) return;  // uniform: solver already finished
   int xr, st;
   tile_of(tm, blockIdx.x, xr, st);
   const int x = (xr < tm.nxr) ? xr * blockDim.x + threadIdx.x : d.V;
   double2 acc = {0.0, 0.0};
   epi.begin();
-  stencil_site<T, NS, DAG>(d, in, epi, x, st * ts, ts, acc);
+  stencil_site_src<T, NS, DAG>(d, in.bind(), epi, x, st * ts, ts, acc);
   if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
 }
 
@@ -142,11 +192,11 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, const Spin<T>* 
 // (the outside neighbours of the range) in LDS with one coalesced load of the range
 // and one gather of the halo, then reads all spatial neighbours from LDS through a
 // 16-bit local index table.  Every spinor is fetched once per block and slice.
-template <typename T, int NS, bool DAG, class Epi>
-__global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, const Spin<T>* __restrict__ in,
-                                                      Epi epi, int ts, TileMap tm, HaloView hv,
-                                                      GridReduce gr) {
+template <typename T, int NS, bool DAG, class Epi, class Src>
+__global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, Src src, Epi epi, int ts, TileMap tm,
+                                                      HaloView hv, GridReduce gr) {
   if (epi.skip()) return;
+  const auto in = src.bind();
   extern __shared__ unsigned char smem_raw[];
   Spin<T>* tileS = reinterpret_cast<Spin<T>*>(smem_raw);
   using C = cplx_t<T>;
@@ -184,19 +234,19 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, const Spin<
     fw0 = DAG ? m0 : m1;
     fw1 = DAG ? m1 : m0;
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
-    prev = ld(in + size_t(sm) * V + x);
-    cur = ld(in + size_t(s0) * V + x);
+    prev = in(size_t(sm) * V + x);
+    cur = in(size_t(s0) * V + x);
     uprev = d.uT[size_t(sm) * V + x];
   }
   for (int s = s0; s < s1; s++) {
-    const Spin<T>* ins = in + size_t(s) * V;
+    const size_t sV = size_t(s) * V;
     __syncthreads();  // previous slice's LDS reads are done
     if (act) tileS[threadIdx.x] = cur;
-    for (int k = threadIdx.x; k < nh; k += R) tileS[R + k] = ld(ins + hv.halo[h0 + k]);
+    for (int k = threadIdx.x; k < nh; k += R) tileS[R + k] = in(sV + hv.halo[h0 + k]);
     __syncthreads();
     if (act) {
       const int sp = (s == Lt - 1) ? 0 : s + 1;
-      const Spin<T> next = ld(in + size_t(sp) * V + x);
+      const Spin<T> next = in(size_t(sp) * V + x);
       const C ucur = d.uT[size_t(s) * V + x];
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
       const C* uS = d.uS + size_t(s) * d.E;
@@ -234,6 +284,7 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, const Spin<
       o.a = o0;
       o.b = o1;
       epi(size_t(s) * V + x, o, acc);
+      call_center(epi, size_t(s) * V + x, cur, 0);
       prev = cur;
       cur = next;
       uprev = ucur;
@@ -259,9 +310,9 @@ __device__ __forceinline__ double norm2(const Spin<T>& v) {
          double(v.b.y) * v.b.y;
 }
 
-template <typename T, bool DAG, class Epi>
-inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in, const Epi& epi,
-                           StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv) {
+template <typename T, bool DAG, class Epi, class Src>
+inline void launch_stencil_src(const DiracView<T>& v, int max_nn, const Src& in, const Epi& epi,
+                               StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv) {
   TileMap tm;
   tm.nxr = (v.V + t.block - 1) / t.block;
   tm.nxr_pad = (tm.nxr + 7) / 8 * 8;
@@ -275,18 +326,24 @@ inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in,
     if (!hv.nbl) throw std::logic_error("LDS stencil launched without halo tables");
     const size_t smem = size_t(t.block + hv.hmax) * sizeof(Spin<T>);
     if (max_nn == 6)
-      hipLaunchKernelGGL((k_stencil_lds<T, 6, DAG, Epi>), grid, dim3(t.block), smem, st, v, in, epi,
+      hipLaunchKernelGGL((k_stencil_lds<T, 6, DAG, Epi, Src>), grid, dim3(t.block), smem, st, v, in, epi,
                          t.ts, tm, hv, gr);
     else
-      hipLaunchKernelGGL((k_stencil_lds<T, 3, DAG, Epi>), grid, dim3(t.block), smem, st, v, in, epi,
+      hipLaunchKernelGGL((k_stencil_lds<T, 3, DAG, Epi, Src>), grid, dim3(t.block), smem, st, v, in, epi,
                          t.ts, tm, hv, gr);
   } else if (max_nn == 6) {
-    hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
+    hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi, Src>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
   } else {
-    hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
+    hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi, Src>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
   }
   if constexpr (Epi::kReduce)
     hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, unsigned(nblk), epi);
+}
+
+template <typename T, bool DAG, class Epi>
+inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in, const Epi& epi,
+                           StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv) {
+  launch_stencil_src<T, DAG>(v, max_nn, SrcPlain<T>{in}, epi, t, gr, st, hv);
 }
 
 }  // namespace qed3
