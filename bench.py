@@ -24,8 +24,10 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 # Bytes moved per FP32 D launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
 # 1 GiB copy kernel) from two rocprofv3 --pmc runs of `bench.py --kernels-only --lattice 1
-# --m 0.02` at the autotuned tile; summary in profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.
-MEASURED_TRAFFIC = {(1, 128, 64): 1.424e9}
+# --m 0.02 --tile <block,ts,remap,lds>` per launch shape the autotuner picks; summary in
+# profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (lattice, n, Lt, tile).
+MEASURED_TRAFFIC = {(1, 128, 64, 128, 64, 1, 1): 1.1690e9, (1, 128, 64, 256, 64, 1, 1): 1.1688e9,
+                    (1, 128, 64, 256, 8, 1, 1): 1.4255e9}
 
 
 def parse():
@@ -56,6 +58,8 @@ def parse():
     p.add_argument("--train-n", type=int, default=16)
     p.add_argument("--train-Lt", type=int, default=16)
     p.add_argument("--train-epochs", type=int, default=40)
+    p.add_argument("--tile", type=str, default=None,
+                   help="block,ts,remap,lds for the FP32 D stencil instead of the autotuned one")
     p.add_argument("--kernels-only", action="store_true",
                    help="only launch the D stencils and the calibration copy (for profilers)")
     return p.parse_args()
@@ -109,6 +113,9 @@ def main():
           file=sys.stderr, flush=True)
 
     if a.kernels_only:
+        if a.tile:
+            b_, t_, r_, l_ = (int(v) for v in a.tile.split(","))
+            D.set_tile(b_, t_, r_, l_, ops=(q.OP_D,), precs=(q.F32,))
         for prec, nm in ((q.F32, "f32"), (q.F64, "f64")):
             for op in (q.OP_D, q.OP_DDAG):
                 ms = D.time_apply(op, prec, warmup=2, reps=10)
@@ -348,7 +355,8 @@ def stencil_roofline(a, q, D):
     tile = tile_info(q, D, q.OP_D, q.F32)
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "traffic": MEASURED_TRAFFIC.get((D.lattice, a.n, a.Lt)),
+            "traffic": MEASURED_TRAFFIC.get((D.lattice, a.n, a.Lt, tile["block"], tile["ts"],
+                                             tile["xcd_remap"], tile["lds"])),
             "kernel": ("k_stencil_lds" if tile["lds"] else "k_stencil")
             + f"<float,{6 if D.lattice == 0 else 3},false,EpiStore> (D, FP32)",
             "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5), "tile": tile}
