@@ -44,6 +44,34 @@ def test_argument_errors_without_device():
     assert lib.qed3_memcpy(None, None, 0, 9) == 1
 
 
+def test_argument_errors_of_the_host_entry_points():
+    """Entry points that need no device reject bad arguments with QED3_ERR_ARG (1)."""
+    lib = _lib.load()
+    r = C.c_void_p()
+    assert lib.qed3_rng_create(0, 5, 3, 1, C.byref(r)) == 1
+    assert lib.qed3_rng_create(2, 5, 3, 1, C.byref(r)) == 0
+    out = C.c_double()
+    assert lib.qed3_rng_gaussian(r, 1, 2, 0, C.byref(out)) == 1   # slice out of range
+    assert lib.qed3_rng_gaussian(r, 2, 0, 3, C.byref(out)) == 1   # site out of range
+    assert lib.qed3_rng_gaussian(r, 7, 0, 0, C.byref(out)) == 1   # no such stream kind
+    assert lib.qed3_rng_gaussian(r, 1, 1, 4, C.byref(out)) == 0
+    assert lib.qed3_rng_load(r, b"/nonexistent/rng") == 1
+    lib.qed3_rng_destroy(r)
+    assert lib.qed3_gauge_write(b"/tmp/x", 0, 1, 1, None, None) == 1
+    assert lib.qed3_gauge_read(b"/nonexistent/lat", 1, 1, 1, None, None) == 1
+    size = C.c_int()
+    buf = (C.c_double * 16)()
+    assert lib.qed3_zolotarev(21, 0.0, C.byref(size), buf, buf, buf, buf) == 1
+    assert lib.qed3_zolotarev(0, 0.1, C.byref(size), buf, buf, buf, buf) == 1
+    g = C.c_void_p()
+    assert lib.qed3_geom_create(1, C.byref(g)) == 0
+    p = _lib.Params(5, 1, 2, 0, 1.0, 0.1, 0.2)
+    assert lib.qed3_plaquette_correlator(g, C.byref(p), buf, buf) == 1
+    lib.qed3_geom_destroy(g)
+    assert lib.qed3_overlap_apply(None, 0, None, None) == 1
+    assert lib.qed3_multishift_cg(None, 1, None, None, None, None, None) == 1
+
+
 def test_seeded_inputs_are_reproducible():
     sp1, tp1 = qed3_amd.random_gauge(0, 2, 3, seed=5, kappa=1.4)
     sp2, tp2 = qed3_amd.random_gauge(0, 2, 3, seed=5, kappa=1.4)
