@@ -41,8 +41,8 @@ def parse():
     p.add_argument("--m", type=float, default=0.05)
     p.add_argument("--beta", type=float, default=1.4)
     p.add_argument("--tol", type=float, default=1e-10)
-    p.add_argument("--cpu-lt", type=int, default=4)
-    p.add_argument("--cpu-iters", type=int, default=12)
+    p.add_argument("--cpu-lt", type=int, default=8)
+    p.add_argument("--cpu-iters", type=int, default=600)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-learned", action="store_true")
@@ -265,6 +265,18 @@ def learned_leg(a, q, g, with_net=True):
     t_train = time.time() - t0
     print(f"# trained on dual n={a.train_n} Lt={a.train_Lt}: loss {hist[0]:.3f} -> {hist[-1]:.4f} "
           f"in {t_train:.1f}s", file=sys.stderr, flush=True)
+    # on the training lattice itself: an unseen configuration and RHS
+    sp_s, tp_s = q.random_gauge(lattice, a.train_n, a.train_Lt, seed=99, kappa=a.beta)
+    Dt.update(sp_s, tp_s)
+    b_s = q.DeviceArray.from_numpy(q.random_source(lattice, a.train_n, a.train_Lt, seed=77))
+    x_s = q.DeviceArray(Dt.N * 16)
+    small_cg = Dt.solve_device(x_s, b_s, tol=a.tol, solver=q.CG_F64)
+    small_fcg, _, small_fb = net_t.fcg(x_s, b_s, tol=a.tol)
+    small = {"lattice": f"dual n={a.train_n} Lt={a.train_Lt} (the training lattice, unseen links and RHS)",
+             "plain_cg_iterations": small_cg.iterations, "learned_fcg_iterations": small_fcg.iterations,
+             "fallback": small_fb}
+    print(f"# on the training lattice: plain CG {small_cg.iterations} its, learned FCG "
+          f"{small_fcg.iterations} its", file=sys.stderr, flush=True)
     at = 0.5 * mean_ell(g, lattice)
     D = dual_context(a, q, g, m, lattice)
     net = q.Net(D, layers=4, seed=5)
@@ -326,6 +338,7 @@ def learned_leg(a, q, g, with_net=True):
             "sites": D.V, "links": D.E, "methods": methods,
             "training": {"lattice": f"dual n={a.train_n} Lt={a.train_Lt}", "pairs": 3 * 40,
                          "epochs": a.train_epochs, "loss_first": round(hist[0], 4),
+                         "on_training_lattice": small,
                          "loss_last": round(hist[-1], 5), "seconds": round(t_train, 1),
                          "pair_generation_s": round(t_pairs, 1)},
             "speedup_vs_fp64": (round(methods["learned"]["solves_per_s"] / fp64["solves_per_s"], 3)
