@@ -224,7 +224,7 @@ CGResult fcg_solve(Dirac& D, Net& net, Spin<double>* x, const Spin<double>* b, c
     }
   }
   if (!done) {
-    // fold the FP32 correction in and let plain mixed CG finish (or judge) the solve
+    // fold the FP32 correction in and let plain mixed CG finish (or decide) the solve
     hipLaunchKernelGGL(k_acc64, dim3(nbk), dim3(256), 0, st, x, B.x, n);
     rr = true_resid();
     if (rr > tol2 && rr > 0) {
