@@ -34,7 +34,9 @@ enum { QED3_LATTICE_SIMPLICIAL = 0, QED3_LATTICE_DUAL = 1 };
 enum { QED3_BC_ANTIPERIODIC = 0, QED3_BC_PERIODIC = 1 };
 enum { QED3_OP_D = 0, QED3_OP_DDAG = 1, QED3_OP_DDAGD = 2, QED3_OP_DDDAG = 3 };
 enum { QED3_PREC_F64 = 0, QED3_PREC_F32 = 1 };
-enum { QED3_SOLVER_CG_F64 = 0, QED3_SOLVER_CG_MIXED = 1 };
+/* FP64 CG; FP32 inner CG with FP64 reliable updates; the same with the inner r, p and w
+ * stored in IEEE binary16 (scaled to unit rms at every reliable update) */
+enum { QED3_SOLVER_CG_F64 = 0, QED3_SOLVER_CG_MIXED = 1, QED3_SOLVER_CG_HALF = 2 };
 enum { QED3_HOST = 0, QED3_DEVICE = 1 };
 
 const char* qed3_last_error(void);

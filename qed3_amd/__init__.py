@@ -18,7 +18,7 @@ SIMPLICIAL, DUAL = 0, 1
 ANTIPERIODIC, PERIODIC = 0, 1
 OP_D, OP_DDAG, OP_DDAGD, OP_DDDAG = 0, 1, 2, 3
 F64, F32 = 0, 1
-CG_F64, CG_MIXED = 0, 1
+CG_F64, CG_MIXED, CG_HALF = 0, 1, 2
 
 
 def _dp(a):

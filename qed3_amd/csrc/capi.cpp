@@ -374,14 +374,16 @@ int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
   if (!ctx || !d_x || !d_b || !opts) return fail(QED3_ERR_ARG, "null argument");
   if (d_x == d_b) return fail(QED3_ERR_ARG, "x and b must differ");
   if (!(opts->tol > 0) || opts->maxiter < 1) return fail(QED3_ERR_ARG, "bad tolerance/maxiter");
-  if (opts->solver != QED3_SOLVER_CG_F64 && opts->solver != QED3_SOLVER_CG_MIXED)
+  if (opts->solver != QED3_SOLVER_CG_F64 && opts->solver != QED3_SOLVER_CG_MIXED &&
+      opts->solver != QED3_SOLVER_CG_HALF)
     return fail(QED3_ERR_ARG, "bad solver");
   if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
   return guarded([&]() -> int {
     qed3::CGOptions o;
     o.tol = opts->tol;
     o.maxiter = opts->maxiter;
-    o.mixed = opts->solver == QED3_SOLVER_CG_MIXED;
+    o.mixed = opts->solver != QED3_SOLVER_CG_F64;
+    o.half = opts->solver == QED3_SOLVER_CG_HALF;
     o.delta = opts->delta > 0 ? opts->delta : 0.1;
     o.chunk = opts->check_every;
     const qed3::CGResult r = qed3::cg_solve(*ctx->D, static_cast<qed3::Spin<double>*>(d_x),
@@ -605,7 +607,8 @@ int qed3_plaquette_correlator(const qed3_geom* g, const qed3_params* p, const do
 
 int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
   if (!ctx || !p || !out) return fail(QED3_ERR_ARG, "null argument");
-  if (p->solver != QED3_SOLVER_CG_F64 && p->solver != QED3_SOLVER_CG_MIXED)
+  if (p->solver != QED3_SOLVER_CG_F64 && p->solver != QED3_SOLVER_CG_MIXED &&
+      p->solver != QED3_SOLVER_CG_HALF)
     return fail(QED3_ERR_ARG, "bad solver");
   return guarded([&]() -> int {
     qed3::HMCParams hp;
@@ -616,7 +619,8 @@ int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
     hp.nsteps = p->nsteps;
     hp.nsteps_inner = p->nsteps_inner;
     hp.tol = p->tol;
-    hp.mixed = p->solver == QED3_SOLVER_CG_MIXED;
+    hp.mixed = p->solver != QED3_SOLVER_CG_F64;
+    hp.half = p->solver == QED3_SOLVER_CG_HALF;
     hp.seed = p->seed;
     auto h = std::make_unique<qed3_hmc>();
     h->ctx = ctx;

@@ -46,6 +46,8 @@ __global__ __launch_bounds__(256) void k_init(const Spin<double>* __restrict__ b
     sc->bnorm2 = t.x;
     sc->maxr2 = t.x;
     sc->true_r2 = t.x;
+    sc->scale = 1.0;
+    sc->rescale = 0;
     sc->gam = 0;
     sc->alpha = 0;
     sc->beta = 0;
@@ -63,19 +65,95 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_flush(Spin<double>* x64, Spin<T>* xin, size_t n,
                                                const CGScalars* sc, int always) {
   if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
+  const double s = sc->scale;  // 1 unless the inner fields are stored scaled
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
     const Spin<T> a = xin[i];
     Spin<double> xx = x64[i];
-    xx.a.x += a.a.x;
-    xx.a.y += a.a.y;
-    xx.b.x += a.b.x;
-    xx.b.y += a.b.y;
+    xx.a.x += s * a.a.x;
+    xx.a.y += s * a.a.y;
+    xx.b.x += s * a.b.x;
+    xx.b.y += s * a.b.y;
     x64[i] = xx;
     Spin<T> z;
     z.a = {T(0), T(0)};
     z.b = {T(0), T(0)};
     xin[i] = z;
+  }
+}
+
+// Half-storage start: x = 0, p_old = 0, |b|^2; the scale makes the stored residual
+// unit-rms per component (k_store_b0 then stores r = b / scale).
+__global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__ b, Spin<double>* x64,
+                                                Spin<float>* xin, SpinH* p, size_t n, CGScalars* sc,
+                                                GridReduce gr, double tol2, double delta2, int maxiter) {
+  double2 acc = {0.0, 0.0};
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<double> bb = b[i];
+    Spin<float> zf;
+    zf.a = {0.f, 0.f};
+    zf.b = {0.f, 0.f};
+    xin[i] = zf;
+    p[i] = to_half(zf);
+    Spin<double> z;
+    z.a = {0.0, 0.0};
+    z.b = {0.0, 0.0};
+    x64[i] = z;
+    acc.x += norm2(bb);
+  }
+  const double ncomp = 4.0 * double(n);
+  grid_reduce(acc, gr, [=](double2 t) {
+    const double s = t.x > 0.0 ? sqrt(t.x / ncomp) : 1.0;
+    sc->scale = s;
+    sc->mu = t.x / (s * s);
+    sc->bnorm2 = t.x;
+    sc->maxr2 = sc->mu;
+    sc->true_r2 = t.x;
+    sc->rfac = 1.0;
+    sc->rescale = 0;
+    sc->gam = 0;
+    sc->alpha = 0;
+    sc->beta = 0;
+    sc->tol2 = tol2;
+    sc->delta2 = delta2;
+    sc->iter = 0;
+    sc->n_ru = 0;
+    sc->maxiter = maxiter;
+    sc->need_ru = 0;
+    sc->done = (t.x == 0.0) ? 1 : (t.x < 1e300 ? 0 : 2);
+  });
+}
+
+// r = b / scale in half (FP64 arithmetic, so any finite b is representable)
+__global__ __launch_bounds__(256) void k_store_b0(const Spin<double>* __restrict__ b, SpinH* r, size_t n,
+                                                  const CGScalars* sc) {
+  if (sc->done != 0) return;
+  const double inv = 1.0 / sc->scale;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<double> bb = b[i];
+    Spin<float> v;
+    v.a = {float(bb.a.x * inv), float(bb.a.y * inv)};
+    v.b = {float(bb.b.x * inv), float(bb.b.y * inv)};
+    r[i] = to_half(v);
+  }
+}
+
+// r = rfac * rt in half after a reliable update (rt holds the true residual at the
+// outgoing scale, rfac = old scale / new scale)
+__global__ __launch_bounds__(256) void k_rescale(const Spin<float>* __restrict__ rt, SpinH* r, size_t n,
+                                                 const CGScalars* sc) {
+  if (sc->done != 0 || sc->rescale == 0) return;
+  const float inv = float(sc->rfac);
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    Spin<float> v = rt[i];
+    v.a.x *= inv;
+    v.a.y *= inv;
+    v.b.x *= inv;
+    v.b.y *= inv;
+    r[i] = to_half(v);
   }
 }
 
@@ -104,6 +182,37 @@ struct Work {
 };
 
 size_t stream_blocks(size_t n) { return std::min<size_t>((n + 255) / 256, 8192); }
+
+// Runs one_iter(parity) in chunks of iterations, captured once as a HIP graph unless
+// disabled, with one host poll of the device scalars per chunk, until the device marks
+// the solve done.  On return hsc holds the final scalars.
+template <class Iter>
+void drive_chunks(const Iter& one_iter, const CGOptions& o, size_t n, hipStream_t st,
+                  const CGScalars* sc, CGScalars* hsc) {
+  int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
+  chunk += chunk & 1;  // even, so every replay of a captured chunk starts at parity 0
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (o.use_graph && std::getenv("QED3_NO_GRAPH") == nullptr) {
+    QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk; i++) one_iter(i & 1);
+    QED3_HIP(hipStreamEndCapture(st, &graph));
+    QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  const int max_chunks = o.maxiter / chunk + 4;
+  for (int c = 0; c < max_chunks; c++) {
+    if (exec)
+      QED3_HIP(hipGraphLaunch(exec, st));
+    else
+      for (int i = 0; i < chunk; i++) one_iter(i & 1);
+    QED3_HIP(hipGetLastError());
+    QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+    QED3_HIP(hipStreamSynchronize(st));
+    if (hsc->done != 0) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+}
 
 template <typename T>
 CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
@@ -161,27 +270,7 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
     launch_stencil<double, true>(v64, nn, W.t, EpiResid<T>{b, W.r, sc}, D.tile<double>(true), gr, st, hv64t);
   };
 
-  int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
-  chunk += chunk & 1;  // even, so every replay of a captured chunk starts at parity 0
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (o.use_graph && std::getenv("QED3_NO_GRAPH") == nullptr) {
-    QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; i++) one_iter(i & 1);
-    QED3_HIP(hipStreamEndCapture(st, &graph));
-    QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
-  const int max_chunks = o.maxiter / chunk + 4;
-  for (int c = 0; c < max_chunks; c++) {
-    if (exec)
-      QED3_HIP(hipGraphLaunch(exec, st));
-    else
-      for (int i = 0; i < chunk; i++) one_iter(i & 1);
-    QED3_HIP(hipGetLastError());
-    QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
-    QED3_HIP(hipStreamSynchronize(st));
-    if (hsc->done != 0) break;
-  }
+  drive_chunks(one_iter, o, n, st, sc, hsc);
   if (kMixed) hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 1);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
@@ -189,8 +278,6 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   res.iterations = hsc->iter;
   res.reliable_updates = hsc->n_ru;
   res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
   (void)hipFreeAsync(W.r, st);
   (void)hipFreeAsync(W.p, st);
   (void)hipFreeAsync(W.p2, st);
@@ -200,6 +287,73 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
     (void)hipFreeAsync(W.t, st);
   }
   (void)hipFreeAsync(sc, st);
+  QED3_HIP(hipStreamSynchronize(st));
+  (void)hipHostFree(hsc);
+  res.true_resid = true_residual(D, x, b);
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
+  return res;
+}
+
+// FP32 arithmetic on binary16-stored r, p, w (scaled to unit rms at the start and at
+// every reliable update), FP32 x_inner, FP64 reliable updates.  Same kernel sequence as
+// the mixed solver plus the rescale pass, which only runs after an update.
+CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
+  const size_t n = D.points();
+  hipStream_t st = D.stream();
+  auto t0 = std::chrono::steady_clock::now();
+  SpinH *r = nullptr, *p = nullptr, *p2 = nullptr, *w = nullptr;
+  Spin<float>*xin = nullptr, *rt = nullptr;
+  Spin<double>* t = nullptr;
+  QED3_HIP(hipMallocAsync((void**)&r, n * sizeof(SpinH), st));
+  QED3_HIP(hipMallocAsync((void**)&p, n * sizeof(SpinH), st));
+  QED3_HIP(hipMallocAsync((void**)&p2, n * sizeof(SpinH), st));
+  QED3_HIP(hipMallocAsync((void**)&w, n * sizeof(SpinH), st));
+  QED3_HIP(hipMallocAsync((void**)&xin, n * sizeof(Spin<float>), st));
+  QED3_HIP(hipMallocAsync((void**)&rt, n * sizeof(Spin<float>), st));
+  QED3_HIP(hipMallocAsync((void**)&t, n * sizeof(Spin<double>), st));
+  CGScalars* sc = nullptr;
+  QED3_HIP(hipMallocAsync((void**)&sc, sizeof(CGScalars), st));
+  CGScalars* hsc = nullptr;
+  QED3_HIP(hipHostMalloc((void**)&hsc, sizeof(CGScalars), hipHostMallocDefault));
+
+  const GridReduce gr = D.reducer();
+  const int nbk = int(stream_blocks(n));
+  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, p, n, sc, gr,
+                     o.tol * o.tol, o.delta * o.delta, o.maxiter);
+  hipLaunchKernelGGL(k_store_b0, dim3(nbk), dim3(256), 0, st, b, r, n, sc);
+  QED3_HIP(hipGetLastError());
+
+  const DiracView<float> vin = D.view<float>();
+  const DiracView<double> v64 = D.view<double>();
+  const int nn = D.tables().max_nn;
+  const HaloView hvf = D.halo_for<float>(false), hvt = D.halo_for<float>(true);
+  const HaloView hv64f = D.halo_for<double>(false), hv64t = D.halo_for<double>(true);
+  const double inv_ncomp = 1.0 / (4.0 * double(n));
+  SpinH* P[2] = {p, p2};
+  auto one_iter = [&](int par) {
+    SpinH* pold = P[par];
+    SpinH* pnew = P[par ^ 1];
+    launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{r, pold, sc}, EpiK1PH{w, pnew, sc},
+                                     D.tile<float>(false), gr, st, hvf);
+    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, EpiK2H{xin, r, pnew, sc, 0.f},
+                                    D.tile<float>(true), gr, st, hvt);
+    hipLaunchKernelGGL(k_flush<float>, dim3(nbk), dim3(256), 0, st, x, xin, n, sc, 0);
+    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
+    launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr,
+                                 st, hv64t);
+    hipLaunchKernelGGL(k_rescale, dim3(nbk), dim3(256), 0, st, rt, r, n, sc);
+  };
+  drive_chunks(one_iter, o, n, st, sc, hsc);
+  hipLaunchKernelGGL(k_flush<float>, dim3(nbk), dim3(256), 0, st, x, xin, n, sc, 1);
+  QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  QED3_HIP(hipStreamSynchronize(st));
+  CGResult res;
+  res.iterations = hsc->iter;
+  res.reliable_updates = hsc->n_ru;
+  res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
+  for (void* q : {(void*)r, (void*)p, (void*)p2, (void*)w, (void*)xin, (void*)rt, (void*)t, (void*)sc})
+    (void)hipFreeAsync(q, st);
   QED3_HIP(hipStreamSynchronize(st));
   (void)hipHostFree(hsc);
   res.true_resid = true_residual(D, x, b);
@@ -233,6 +387,7 @@ double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b) {
 }
 
 CGResult cg_solve(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
+  if (o.half) return run_cg_half(D, x, b, o);
   if (o.mixed) return run_cg<float>(D, x, b, o);
   return run_cg<double>(D, x, b, o);
 }

@@ -1,13 +1,13 @@
 // Device-driven CG on A = D^dagger D (the role of MatPoly::solve,
 // src/both_3d/matpoly.h:204-267).
 //
-// Each iteration is six kernels whose control flow lives on the device:
-//   K1  w = D p,           gam = |w|^2           (stencil + grid reduction)
+// Each iteration is five kernels (plus the one-block reduction finalisers) whose control
+// flow lives on the device:
+//   K1  p = r + beta p_old (formed while gathering), w = D p, gam = |w|^2
 //   K2  q = D^dag w,       alpha = mu/gam, x += alpha p, r -= alpha q, |r|^2
 //   U1  x64 += x_inner                           (only when a reliable update is due)
 //   U2  t = D x64                                (FP64, only when due)
 //   U3  r64 = b - D^dag t, r_inner = r64, |r64|^2 (FP64, only when due)
-//   K3  p = r + beta p
 // Kernels read the scalars (alpha, beta, flags) from device memory and exit at once
 // when the solve is finished or no update is due, so a chunk of iterations is
 // captured once in a hipGraph and replayed with one host poll per chunk.
@@ -16,7 +16,10 @@
 // Vorst 1996; Clark et al., CPC 181 (2010) 1517): the FP64 residual is recomputed
 // whenever |r| has dropped by `delta` since the last update, and convergence is only
 // declared on the recomputed FP64 residual.  Inner precision double is plain FP64 CG
-// whose stop test is likewise confirmed on the true residual.
+// whose stop test is likewise confirmed on the true residual.  With `half`, r, p and w
+// are stored in IEEE binary16 (FP32 arithmetic, FP32 x_inner), scaled to unit rms per
+// component at the start and after every reliable update (one extra pass that only runs
+// then): half the field bytes of the FP32 iteration.
 #pragma once
 #include "dirac.hpp"
 
@@ -26,6 +29,7 @@ struct CGOptions {
   double tol = 1e-10;
   int maxiter = 100000;
   bool mixed = false;
+  bool half = false;    // binary16-stored inner fields (implies the mixed scheme)
   double delta = 0.01;  // reliable update when |r| falls below delta * max |r| since the last one
   int chunk = 0;  // iterations per graph replay (0 = auto)
   bool use_graph = true;  // replay chunks of iterations as a HIP graph (QED3_NO_GRAPH=1 in the

@@ -9,6 +9,15 @@ namespace cgepi {
 
 __device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < 1e300; }
 
+__device__ __forceinline__ void k1_finish(CGScalars* sc, double gam) {
+  sc->gam = gam;
+  if (!finite_pos(gam)) {
+    sc->done = 2;
+    return;
+  }
+  sc->alpha = sc->mu / gam;
+}
+
 // K1: w = D p, gam = |w|^2, alpha = mu / gam
 template <typename T>
 struct EpiK1 {
@@ -21,14 +30,7 @@ struct EpiK1 {
     st(w + i, v);
     acc.x += norm2(v);
   }
-  __device__ void operator()(double2 t) const {
-    sc->gam = t.x;
-    if (!finite_pos(t.x)) {
-      sc->done = 2;
-      return;
-    }
-    sc->alpha = sc->mu / t.x;
-  }
+  __device__ void operator()(double2 t) const { k1_finish(sc, t.x); }
 };
 
 // K1 reading p = r + beta p_old (SrcPUpdate) and storing that p at every point it owns for
@@ -130,6 +132,124 @@ struct EpiResid {
     sc->beta = m / sc->mu;
     sc->mu = m;
     sc->maxr2 = m;
+    if (sc->iter >= sc->maxiter) sc->done = 3;
+  }
+};
+
+// ---- half-storage CG (r, p, w in binary16 scaled by sc->scale; x_inner in FP32) -----
+// Norms and decisions are taken on the rounded values that are stored, so the scalars
+// describe exactly the fields the next kernel reads.
+
+// K1: w = D p stored in half, gam = |w|^2; p (already rounded by SrcPUpdateH) stored at
+// every point the block owns; clears the rescale flag of the previous reliable update
+struct EpiK1PH {
+  static constexpr bool kReduce = true;
+  SpinH* w;
+  SpinH* pnew;
+  CGScalars* sc;
+  __device__ bool skip() const { return sc->done != 0; }
+  __device__ void begin() {}
+  __device__ void operator()(size_t i, const Spin<float>& v, double2& acc) const {
+    const SpinH h = to_half(v);
+    w[i] = h;
+    acc.x += norm2(from_half(h));
+  }
+  __device__ void center(size_t i, const Spin<float>& v) const { pnew[i] = to_half(v); }
+  __device__ void operator()(double2 t) const {
+    sc->rescale = 0;
+    k1_finish(sc, t.x);
+  }
+};
+
+// K2: q = D^dag w; x += alpha p (FP32); r -= alpha q (stored in half); |r|^2 in stored
+// units, compared with the tolerance in true units
+struct EpiK2H {
+  static constexpr bool kReduce = true;
+  Spin<float>* x;
+  SpinH* r;
+  const SpinH* p;
+  CGScalars* sc;
+  float alpha;
+  __device__ bool skip() const { return sc->done != 0; }
+  __device__ void begin() { alpha = float(sc->alpha); }
+  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const {
+    const Spin<float> pp = ldh(p + i);
+    Spin<float> xx = ld(x + i), rr = ldh(r + i);
+    xx.a.x += alpha * pp.a.x;
+    xx.a.y += alpha * pp.a.y;
+    xx.b.x += alpha * pp.b.x;
+    xx.b.y += alpha * pp.b.y;
+    rr.a.x -= alpha * q.a.x;
+    rr.a.y -= alpha * q.a.y;
+    rr.b.x -= alpha * q.b.x;
+    rr.b.y -= alpha * q.b.y;
+    st(x + i, xx);
+    const SpinH rh = to_half(rr);
+    r[i] = rh;
+    acc.x += norm2(from_half(rh));
+  }
+  __device__ void operator()(double2 t) const {
+    const double m = t.x;
+    sc->iter += 1;
+    if (!(m >= 0.0 && m < 1e300)) {
+      sc->done = 2;
+      return;
+    }
+    if (m < sc->delta2 * sc->maxr2 || m * sc->scale * sc->scale <= sc->tol2 * sc->bnorm2) {
+      sc->need_ru = 1;
+    } else {
+      sc->beta = m / sc->mu;
+      sc->mu = m;
+      sc->maxr2 = fmax(sc->maxr2, m);
+    }
+    if (sc->iter >= sc->maxiter && !sc->need_ru) sc->done = 3;
+  }
+};
+
+// U3: r64 = b - D^dag t in FP64, kept in FP32 in rt at the outgoing scale (so its values
+// stay near unit size whatever the magnitude of b) for the rescale pass; |r64|^2 decides
+// convergence, the new scale (unit rms per component) and beta, which also carries the
+// change of scale of the stored p
+struct EpiResidH {
+  static constexpr bool kReduce = true;
+  const Spin<double>* b;
+  Spin<float>* rt;
+  CGScalars* sc;
+  double inv_ncomp;  // 1 / (4 * points)
+  double inv_so;
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru == 0; }
+  __device__ void begin() { inv_so = 1.0 / sc->scale; }
+  __device__ void operator()(size_t i, const Spin<double>& v, double2& acc) const {
+    const Spin<double> bb = ld(b + i);
+    Spin<double> rr;
+    rr.a = {bb.a.x - v.a.x, bb.a.y - v.a.y};
+    rr.b = {bb.b.x - v.b.x, bb.b.y - v.b.y};
+    Spin<float> rf;
+    rf.a = {float(rr.a.x * inv_so), float(rr.a.y * inv_so)};
+    rf.b = {float(rr.b.x * inv_so), float(rr.b.y * inv_so)};
+    st(rt + i, rf);
+    acc.x += norm2(rr);
+  }
+  __device__ void operator()(double2 t) const {
+    const double m = t.x;
+    sc->need_ru = 0;
+    sc->n_ru += 1;
+    sc->true_r2 = m;
+    if (!(m >= 0.0 && m < 1e300)) {
+      sc->done = 2;
+      return;
+    }
+    if (m <= sc->tol2 * sc->bnorm2) {
+      sc->done = 1;
+      return;
+    }
+    const double so = sc->scale, sn = sqrt(m * inv_ncomp);
+    sc->beta = m / (so * so * sc->mu) * (so / sn);
+    sc->mu = m / (sn * sn);
+    sc->maxr2 = sc->mu;
+    sc->scale = sn;
+    sc->rfac = so / sn;
+    sc->rescale = 1;
     if (sc->iter >= sc->maxiter) sc->done = 3;
   }
 };

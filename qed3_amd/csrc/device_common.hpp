@@ -60,6 +60,30 @@ __device__ __forceinline__ void st(Spin<T>* p, const Spin<T>& v) {
   *p = v;
 }
 
+// One lattice point stored as four IEEE binary16 values (8 B), computed on in FP32.  The
+// half-storage CG keeps its inner fields in this form, scaled to unit rms per component.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4v_t __attribute__((ext_vector_type(4)));
+struct alignas(8) SpinH {
+  half4_t v;
+};
+
+__device__ __forceinline__ Spin<float> from_half(const SpinH& h) {
+  const float4v_t f = __builtin_convertvector(h.v, float4v_t);
+  Spin<float> s;
+  s.a = {f.x, f.y};
+  s.b = {f.z, f.w};
+  return s;
+}
+// round to nearest even; values beyond the binary16 range become +-inf
+__device__ __forceinline__ SpinH to_half(const Spin<float>& s) {
+  const float4v_t f = {s.a.x, s.a.y, s.b.x, s.b.y};
+  SpinH h;
+  h.v = __builtin_convertvector(f, half4_t);
+  return h;
+}
+__device__ __forceinline__ Spin<float> ldh(const SpinH* p) { return from_half(*p); }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -68,10 +68,12 @@ struct CGScalars {
   double maxr2;    // max |r|^2 since the last reliable update
   double bnorm2, tol2, delta2;
   double true_r2;  // last true FP64 |b - A x|^2
+  double scale;    // half-storage CG: true field = scale * stored field (1 otherwise)
+  double rfac;     // half-storage CG: factor from the kept residual to the new stored one
   int done;        // 0 running, 1 converged, 2 breakdown / NaN, 3 maxiter
   int need_ru;     // a reliable update is due in this iteration
   int iter, n_ru, maxiter;
-  int pad;
+  int rescale;     // half-storage CG: the residual is to be re-stored at the new scale
 };
 
 class Dirac {

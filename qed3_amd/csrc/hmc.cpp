@@ -157,6 +157,7 @@ void HMC::update_eta(int k) {
   CGOptions o;
   o.tol = p_.tol;
   o.mixed = p_.mixed;
+  o.half = p_.half;
   const CGResult r = cg_solve(D_, eta_[k], phi_[k], o);
   rep_.cg_solves++;
   rep_.cg_iterations += r.iterations;

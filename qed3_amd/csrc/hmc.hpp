@@ -50,6 +50,7 @@ struct HMCParams {
   int nsteps_inner = 10;
   double tol = 1e-10;
   bool mixed = true;
+  bool half = false;  // binary16-stored inner CG fields
   int seed = 1;
 };
 

@@ -80,6 +80,36 @@ struct SrcPUpdate {
   __device__ Bound bind() const { return Bound{r, p, T(sc->beta)}; }
 };
 
+// The same two sources over half-storage fields (FP32 arithmetic).  SrcPUpdateH rounds
+// the new p to binary16 before the stencil uses it, so D acts on exactly the p that is
+// stored for the x update.
+struct SrcHalf {
+  const SpinH* __restrict__ p;
+  __device__ SrcHalf bind() const { return *this; }
+  __device__ Spin<float> operator()(size_t i) const { return ldh(p + i); }
+};
+
+struct SrcPUpdateH {
+  const SpinH* r;
+  const SpinH* p;
+  const CGScalars* sc;
+  struct Bound {
+    const SpinH* __restrict__ r;
+    const SpinH* __restrict__ p;
+    float beta;
+    __device__ Spin<float> operator()(size_t i) const {
+      Spin<float> a = ldh(r + i);
+      const Spin<float> c = ldh(p + i);
+      a.a.x += beta * c.a.x;
+      a.a.y += beta * c.a.y;
+      a.b.x += beta * c.b.x;
+      a.b.y += beta * c.b.y;
+      return from_half(to_half(a));
+    }
+  };
+  __device__ Bound bind() const { return Bound{r, p, float(sc->beta)}; }
+};
+
 // Epilogues may take the stencil's input value at each output point (its centre) through
 // center(i, v); the others ignore it.
 template <class Epi, typename T>

@@ -23,7 +23,7 @@ def test_cg_dense_known_answer(gpu):
     A = Do.dense()
     b = source(0, 1, 2, seed=5)
     x_ref = np.linalg.solve(A.conj().T @ A, b)
-    for solver in (q.CG_F64, q.CG_MIXED):
+    for solver in (q.CG_F64, q.CG_MIXED, q.CG_HALF):
         x, rep = Dq.solve(b, tol=1e-12, solver=solver)
         assert rep.status == 0 and rep.true_resid <= 1e-12
         assert rel(x, x_ref) < 1e-10
@@ -37,7 +37,7 @@ def test_cg_matches_oracle(gpu, lattice, n, Lt, M5):
     Do, Dq = _pair(lattice, n, Lt, M5)
     b = source(lattice, n, Lt, seed=17)
     x_ref, it_ref, _ = Do.solve(b, tol=1e-13)
-    for solver in (q.CG_F64, q.CG_MIXED):
+    for solver in (q.CG_F64, q.CG_MIXED, q.CG_HALF):
         x, rep = Dq.solve(b, tol=1e-12, solver=solver)
         assert rep.status == 0, rep.status
         assert rep.true_resid <= 1e-12
@@ -71,6 +71,46 @@ def test_cg_zero_rhs_and_errors(gpu):
     rep = Dq.solve_device(q.DeviceArray(b.nbytes), q.DeviceArray.from_numpy(bad), tol=1e-12,
                           raise_on_fail=False)
     assert rep.status == 4  # QED3_ERR_NAN
+
+
+@pytest.mark.gpu
+def test_half_storage_cg_edge_cases(gpu):
+    """Zero source, iteration cap and a non-finite source on the binary16-storage solver."""
+    Do, Dq = _pair(1, 2, 2, 0.1)
+    x, rep = Dq.solve(np.zeros(Dq.N, np.complex128), tol=1e-10, solver=q.CG_HALF)
+    assert rep.status == 0 and np.all(x == 0)
+    b = source(1, 2, 2, seed=1)
+    rep = Dq.solve_device(q.DeviceArray(b.nbytes), q.DeviceArray.from_numpy(b), tol=1e-12,
+                          maxiter=2, solver=q.CG_HALF, raise_on_fail=False)
+    assert rep.status == 3 and rep.iterations == 2
+    bad = b.copy()
+    bad[3] = np.inf
+    rep = Dq.solve_device(q.DeviceArray(b.nbytes), q.DeviceArray.from_numpy(bad), tol=1e-12,
+                          solver=q.CG_HALF, raise_on_fail=False)
+    assert rep.status == 4
+    # a source far outside the binary16 range: the stored fields are scaled to unit rms
+    for s in (1e-30, 1e30):
+        x, rep = Dq.solve(s * b, tol=1e-11, solver=q.CG_HALF)
+        x0, _ = Dq.solve(b, tol=1e-11, solver=q.CG_F64)
+        assert rep.status == 0 and rep.true_resid <= 1e-11
+        assert rel(x / s, x0) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice,n,Lt,M5", [(1, 8, 16, 0.02), (0, 8, 16, 0.02)])
+def test_half_storage_cg_light_mass(gpu, lattice, n, Lt, M5):
+    """Near the chiral limit (cond(D^dag D) ~ 1e4) the binary16-storage solver still reaches
+    the FP64 solution and needs at most a third more iterations than the FP32 one."""
+    print(f"case lattice={lattice} n={n} Lt={Lt} M5={M5}", flush=True)
+    _, Dq = _pair(lattice, n, Lt, M5, kappa=2.0)
+    b = source(lattice, n, Lt, seed=41)
+    x64, r64 = Dq.solve(b, tol=1e-12, solver=q.CG_F64)
+    _, r32 = Dq.solve(b, tol=1e-10, solver=q.CG_MIXED)
+    x16, r16 = Dq.solve(b, tol=1e-10, solver=q.CG_HALF)
+    assert r16.status == 0 and r16.true_resid <= 1e-10
+    assert r16.iterations <= 1.34 * r32.iterations + 10, (r16.iterations, r32.iterations)
+    # |x - x64| / |x64| <= cond(D^dag D) * residual, cond ~ 1e4 here
+    assert rel(x16, x64) < 1e-6, rel(x16, x64)
 
 
 @pytest.mark.gpu
