@@ -2,16 +2,18 @@
 
 Learned workload (the value): dual (trivalent) refined icosahedron n=128 (327,680 sites)
 x Lt=64, antiperiodic in time, r=1, M5=m=0.02, von Mises(kappa=1.4) link angles, Gaussian
-RHS.  Three solvers are timed on the same seeded RHS (one complete solve per step): the
+RHS.  Four solvers are timed on the same seeded RHS (one complete solve per step): the
 flexible CG with the learned preconditioner M = N N^dagger (FP32 vectors, FP64 reliable
 updates; the net trained first, outside the timed region, on a small lattice of the same
 family, dual n=16 x Lt=16, its weights applied unchanged at full size), the
-unpreconditioned mixed CG (FP32 + FP64 reliable updates) and the plain FP64 CG.  `value`
-is the solves/s of the fastest of them, named in config.solver; every method's figures
-and its ratio to plain FP64 CG are in config.methods.
+unpreconditioned mixed CG (FP32 + FP64 reliable updates), the same CG with its inner
+fields stored in binary16 (FP32 arithmetic, FP64 reliable updates) and the plain FP64 CG.
+`value` is the solves/s of the fastest of them, named in config.solver; every method's
+figures and its ratio to plain FP64 CG are in config.methods.  `roofline` is that of the
+stencil kernel of the fastest method (one D apply at its storage precision).
 
-Second workload: simplicial n=128 (163,842 sites) x Lt=64, m=0.05: mixed CG vs plain FP64
-CG.  Prints one JSON line.
+Second workload: simplicial n=128 (163,842 sites) x Lt=64, m=0.05: the three CGs.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -22,12 +24,13 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-# Bytes moved per FP32 D launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
+# Bytes moved per D launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
 # 1 GiB copy kernel) from two rocprofv3 --pmc runs of `bench.py --kernels-only --lattice 1
 # --m 0.02 --tile <block,ts,remap,lds>` per launch shape the autotuner picks; summary in
-# profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (lattice, n, Lt, tile).
-MEASURED_TRAFFIC = {(1, 128, 64, 128, 64, 1, 1): 1.1690e9, (1, 128, 64, 256, 64, 1, 1): 1.1688e9,
-                    (1, 128, 64, 256, 8, 1, 1): 1.4255e9}
+# profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision, lattice, n, Lt, tile).
+MEASURED_TRAFFIC = {("f32", 1, 128, 64, 128, 64, 1, 1): 1.1690e9,
+                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.1688e9,
+                    ("f32", 1, 128, 64, 256, 8, 1, 1): 1.4255e9}
 
 
 def parse():
@@ -116,10 +119,10 @@ def main():
         if a.tile:
             b_, t_, r_, l_ = (int(v) for v in a.tile.split(","))
             D.set_tile(b_, t_, r_, l_, ops=(q.OP_D,), precs=(q.F32,))
-        for prec, nm in ((q.F32, "f32"), (q.F64, "f64")):
+        for prec, nm in ((q.F16, "f16"), (q.F32, "f32"), (q.F64, "f64")):
             for op in (q.OP_D, q.OP_DDAG):
                 ms = D.time_apply(op, prec, warmup=2, reps=10)
-                rb = 4 if prec == q.F32 else 8
+                rb = {q.F16: 2, q.F32: 4, q.F64: 8}[prec]
                 gbs = algorithmic_bytes_D(D.V, D.E, a.Lt, rb) / (ms * 1e-3) / 1e9
                 print(f"# D op={op} {nm}: {ms:.4f} ms  {gbs:.0f} GB/s algorithmic  "
                       f"tile={tile_info(q, D, op, prec)}", file=sys.stderr, flush=True)
@@ -136,7 +139,7 @@ def main():
         return D.solve_device(d_x, rhs[k], tol=a.tol, solver=solver)
 
     results = {}
-    for name, solver in (("mixed", q.CG_MIXED), ("fp64", q.CG_F64)):
+    for name, solver in (("half", q.CG_HALF), ("mixed", q.CG_MIXED), ("fp64", q.CG_F64)):
         if name == "fp64" and a.no_fp64:
             continue
         for k in range(a.warmup):
@@ -179,8 +182,12 @@ def main():
                         f"m={a.m} tol={a.tol} D^dag D CG", "methods": results,
             "speedup_mixed_vs_fp64": (round(results["mixed"]["solves_per_s"]
                                             / results["fp64"]["solves_per_s"], 3)
-                                      if "fp64" in results else None)}
+                                      if "fp64" in results else None),
+            "speedup_half_vs_fp64": (round(results["half"]["solves_per_s"]
+                                           / results["fp64"]["solves_per_s"], 3)
+                                     if "fp64" in results else None)}
     solver_names = {"learned": "flexible CG, M = N N^dag (4-layer net), FP32 + FP64 reliable updates",
+                    "half": "CG, binary16-stored r/p/w (FP32 arithmetic, FP32 x) + FP64 reliable updates",
                     "mixed": "CG, FP32 + FP64 reliable updates", "fp64": "CG, FP64"}
     if True:
         meth = learned["methods"]
@@ -196,10 +203,13 @@ def main():
                   "methods": learned["methods"], "training": learned["training"],
                   "speedup_learned_vs_fp64": learned["speedup_vs_fp64"],
                   "speedup_mixed_vs_fp64": learned["speedup_mixed_vs_fp64"],
+                  "speedup_half_vs_fp64": learned["speedup_half_vs_fp64"],
                   "net_apply": learned["net_apply"], "solution_agreement": learned["agreement"],
                   "second_workload": simp, "hmc": hmc, "batched": batched}
-        roof = learned["roofline"]
+        roof = learned["roofline"]["f16" if winner == "half" else "f32"]
+        roof = dict(roof, other_precision=learned["roofline"]["f32" if winner == "half" else "f16"])
         dtype = ("f32 inner (bf16 net activations) / f64 reliable updates" if winner == "learned"
+                 else "f16 storage, f32 arithmetic / f64 reliable updates" if winner == "half"
                  else "f32 inner / f64 reliable updates" if winner == "mixed" else "f64")
     line = {
         "metric": "CG solves/s to 1e-10 true resid, n=128 Lt=64: learned+FP32 vs plain FP64 CG",
@@ -297,6 +307,9 @@ def learned_leg(a, q, g, with_net=True):
     methods["learned"] = timed_solves(q, a, "learned", run_fcg, rhs, a.warmup, a.learned_steps)
     x_learned = d_x.to_numpy(np.complex128)
     methods["learned"]["fallbacks"] = int(sum(fb for _, fb in stats[a.warmup:]))
+    methods["half"] = timed_solves(q, a, "half", lambda b: D.solve_device(d_x, b, tol=a.tol,
+                                   solver=q.CG_HALF), rhs, a.warmup, a.learned_steps)
+    x_half = d_x.to_numpy(np.complex128)
     methods["mixed"] = timed_solves(q, a, "mixed", lambda b: D.solve_device(d_x, b, tol=a.tol,
                                     solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)
     x_mixed = d_x.to_numpy(np.complex128)
@@ -308,6 +321,7 @@ def learned_leg(a, q, g, with_net=True):
         nrm = np.linalg.norm(x64)
         agreement = {"rhs_seed": 2000 + a.warmup + a.learned_steps - 1,
                      "mixed_vs_fp64": float(np.linalg.norm(x_mixed - x64) / nrm),
+                     "half_vs_fp64": float(np.linalg.norm(x_half - x64) / nrm),
                      "learned_vs_fp64": float(np.linalg.norm(x_learned - x64) / nrm)}
     # one net application N (and N^dag) at full size, device-resident fields
     d_y = q.DeviceArray(D.N * 8)
@@ -345,6 +359,8 @@ def learned_leg(a, q, g, with_net=True):
                                 if fp64 else None),
             "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
                                       if fp64 else None),
+            "speedup_half_vs_fp64": (round(methods["half"]["solves_per_s"] / fp64["solves_per_s"], 3)
+                                     if fp64 else None),
             "net_apply": {"ms": round(ms_net, 4), "ms_adjoint": round(ms_adj, 4),
                           "algorithmic_GB": round(b_net / 1e9, 3),
                           "achieved_GBs": round(b_net / (ms_net * 1e-3) / 1e9, 1)}}
@@ -360,19 +376,25 @@ def dual_context(a, q, g, m, lattice):
 
 
 def stencil_roofline(a, q, D):
-    """The dominant kernel of the CG: the FP32 stencil (one D apply), timed with HIP events
-    on the context's stream, against B_D = 2 B_vec + B_link."""
-    ms32 = D.time_apply(q.OP_D, q.F32, warmup=5, reps=50)
-    b32 = algorithmic_bytes_D(D.V, D.E, a.Lt, 4)
-    gbs = b32 / (ms32 * 1e-3) / 1e9
-    tile = tile_info(q, D, q.OP_D, q.F32)
-    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "traffic": MEASURED_TRAFFIC.get((D.lattice, a.n, a.Lt, tile["block"], tile["ts"],
-                                             tile["xcd_remap"], tile["lds"])),
-            "kernel": ("k_stencil_lds" if tile["lds"] else "k_stencil")
-            + f"<float,{6 if D.lattice == 0 else 3},false,EpiStore> (D, FP32)",
-            "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5), "tile": tile}
+    """The dominant kernel of the CG: the stencil (one D apply) at the inner solver's
+    storage precision, timed with HIP events on the context's stream, against
+    B_D = 2 B_vec + B_link (one real per link at that precision).  Returns both the
+    binary16-storage and the FP32 kernel, keyed "f16" / "f32"."""
+    out = {}
+    for nm, prec, rb, kern in (("f16", q.F16, 2, "EpiStoreH,SrcHalf,half2_t> (D, binary16 storage)"),
+                               ("f32", q.F32, 4, "EpiStore,SrcPlain<float>> (D, FP32)")):
+        ms = D.time_apply(q.OP_D, prec, warmup=5, reps=50)
+        b = algorithmic_bytes_D(D.V, D.E, a.Lt, rb)
+        gbs = b / (ms * 1e-3) / 1e9
+        tile = tile_info(q, D, q.OP_D, prec)
+        out[nm] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(gbs / HBM_PEAK_GBS, 4),
+                   "traffic": MEASURED_TRAFFIC.get((nm, D.lattice, a.n, a.Lt, tile["block"], tile["ts"],
+                                                    tile["xcd_remap"], tile["lds"])),
+                   "kernel": ("k_stencil_lds" if tile["lds"] else "k_stencil")
+                   + f"<float,{6 if D.lattice == 0 else 3},false," + kern,
+                   "bytes_per_launch": b, "ms_per_launch": round(ms, 5), "tile": tile}
+    return out
 
 
 def plain_leg(a, q, g, m, lattice):
@@ -380,8 +402,11 @@ def plain_leg(a, q, g, m, lattice):
     d_x = q.DeviceArray(D.N * 16)
     rhs = [q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=2000 + k))
            for k in range(a.warmup + a.learned_steps)]
-    methods = {"mixed": timed_solves(q, a, "mixed", lambda b: D.solve_device(
-        d_x, b, tol=a.tol, solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)}
+    methods = {"half": timed_solves(q, a, "half", lambda b: D.solve_device(
+        d_x, b, tol=a.tol, solver=q.CG_HALF), rhs, a.warmup, a.learned_steps)}
+    x_half = d_x.to_numpy(np.complex128)
+    methods["mixed"] = timed_solves(q, a, "mixed", lambda b: D.solve_device(
+        d_x, b, tol=a.tol, solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)
     x_mixed = d_x.to_numpy(np.complex128)
     agreement = None
     if not a.no_fp64:
@@ -389,13 +414,16 @@ def plain_leg(a, q, g, m, lattice):
             d_x, b, tol=a.tol, solver=q.CG_F64), rhs, a.warmup, a.learned_steps)
         x64 = d_x.to_numpy(np.complex128)
         agreement = {"rhs_seed": 2000 + a.warmup + a.learned_steps - 1,
-                     "mixed_vs_fp64": float(np.linalg.norm(x_mixed - x64) / np.linalg.norm(x64))}
+                     "mixed_vs_fp64": float(np.linalg.norm(x_mixed - x64) / np.linalg.norm(x64)),
+                     "half_vs_fp64": float(np.linalg.norm(x_half - x64) / np.linalg.norm(x64))}
     fp64 = methods.get("fp64")
     return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG", "agreement": agreement,
             "roofline": stencil_roofline(a, q, D), "sites": D.V, "links": D.E,
             "methods": methods, "training": None, "speedup_vs_fp64": None,
             "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
-                                      if fp64 else None), "net_apply": None}
+                                      if fp64 else None),
+            "speedup_half_vs_fp64": (round(methods["half"]["solves_per_s"] / fp64["solves_per_s"], 3)
+                                     if fp64 else None), "net_apply": None}
 
 
 def hmc_leg(a, q):

@@ -33,7 +33,9 @@ enum {
 enum { QED3_LATTICE_SIMPLICIAL = 0, QED3_LATTICE_DUAL = 1 };
 enum { QED3_BC_ANTIPERIODIC = 0, QED3_BC_PERIODIC = 1 };
 enum { QED3_OP_D = 0, QED3_OP_DDAG = 1, QED3_OP_DDAGD = 2, QED3_OP_DDDAG = 3 };
-enum { QED3_PREC_F64 = 0, QED3_PREC_F32 = 1 };
+/* field precision: complex double, complex float, or binary16 pairs (4 x 16 bit per
+ * point, FP32 arithmetic, binary16 link phases: the half-storage CG's operator) */
+enum { QED3_PREC_F64 = 0, QED3_PREC_F32 = 1, QED3_PREC_F16 = 2 };
 /* FP64 CG; FP32 inner CG with FP64 reliable updates; the same with the inner r, p and w
  * stored in IEEE binary16 (scaled to unit rms at every reliable update) */
 enum { QED3_SOLVER_CG_F64 = 0, QED3_SOLVER_CG_MIXED = 1, QED3_SOLVER_CG_HALF = 2 };
@@ -124,7 +126,8 @@ int qed3_ctx_geometry(const qed3_ctx* ctx, double* kappa, double* kappa_t,
 int qed3_set_gauge(qed3_ctx* ctx, const double* spatial, const double* temporal, int where);
 
 /* out = op(in) (replaces DiracPf::{mult,adj,DHD,DDH}_deviceAsyncLaunch,
- * src/both_3d/dirac_pf.h:66-92).  prec selects complex double or complex float fields. */
+ * src/both_3d/dirac_pf.h:66-92).  prec selects complex double, complex float or binary16
+ * fields (QED3_PREC_*). */
 int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in);
 
 /* ---- CG on D^dagger D (replaces MatPoly::solve, src/both_3d/matpoly.h:204-267) --- */
@@ -313,7 +316,8 @@ int qed3_time_apply(qed3_ctx* ctx, int op, int prec, int warmup, int reps, doubl
 /* Average ms per launch of a plain device copy of `bytes` (read + write): the reference
  * point for HBM bandwidth and for calibrating the FETCH_SIZE/WRITE_SIZE counters. */
 int qed3_time_copy(size_t bytes, int reps, double* ms);
-/* Launch shape the autotuner chose for op (QED3_OP_D / QED3_OP_DDAG) at prec:
+/* Launch shape the autotuner chose for op (QED3_OP_D / QED3_OP_DDAG) at prec (binary16
+ * fields use the FP32 shapes):
  * out[0] threads per block, out[1] time slices per thread, out[2] XCD-aware order,
  * out[3] LDS-staged tile. */
 int qed3_tile_info(const qed3_ctx* ctx, int op, int prec, int* out);

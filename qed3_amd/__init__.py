@@ -17,7 +17,7 @@ from ._lib import CGOpts, CGReport, Params, Qed3Error, check, load
 SIMPLICIAL, DUAL = 0, 1
 ANTIPERIODIC, PERIODIC = 0, 1
 OP_D, OP_DDAG, OP_DDAGD, OP_DDDAG = 0, 1, 2, 3
-F64, F32 = 0, 1
+F64, F32, F16 = 0, 1, 2
 CG_F64, CG_MIXED, CG_HALF = 0, 1, 2
 
 
@@ -198,7 +198,18 @@ class WilsonDirac:
         check(load().qed3_apply(self._h, op, prec, d_out.ptr, d_in.ptr))
 
     def apply(self, op, v: np.ndarray, prec=F64):
-        """op(v) for a host complex vector (round trip through the device)."""
+        """op(v) for a host complex vector (round trip through the device; F16 packs the
+        vector into binary16 (re, im) pairs and returns complex128)."""
+        if prec == F16:
+            v = np.ascontiguousarray(v, np.complex128)
+            assert v.size == self.N
+            h = np.empty(2 * v.size, np.float16)
+            h[0::2], h[1::2] = v.real, v.imag
+            d_in = DeviceArray.from_numpy(h)
+            d_out = DeviceArray(h.nbytes)
+            self.apply_device(op, prec, d_out, d_in)
+            o = d_out.to_numpy(np.float16).astype(np.float64)
+            return o[0::2] + 1j * o[1::2]
         dt = np.complex128 if prec == F64 else np.complex64
         v = np.ascontiguousarray(v, dt)
         assert v.size == self.N

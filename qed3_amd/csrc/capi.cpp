@@ -340,7 +340,8 @@ int qed3_set_gauge(qed3_ctx* ctx, const double* spatial, const double* temporal,
 int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
   if (!ctx || !d_out || !d_in) return fail(QED3_ERR_ARG, "null argument");
   if (op < 0 || op > 3) return fail(QED3_ERR_ARG, "bad op");
-  if (prec != QED3_PREC_F64 && prec != QED3_PREC_F32) return fail(QED3_ERR_ARG, "bad precision");
+  if (prec != QED3_PREC_F64 && prec != QED3_PREC_F32 && prec != QED3_PREC_F16)
+    return fail(QED3_ERR_ARG, "bad precision");
   if (d_out == d_in) return fail(QED3_ERR_ARG, "in-place apply is not supported");
   if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
   return guarded([&]() -> int {
@@ -361,6 +362,20 @@ int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
         D.apply_ddag<T>(o, i, tmp, true);
       QED3_HIP(hipFree(tmp));
     };
+    if (prec == QED3_PREC_F16) {
+      auto* o = static_cast<qed3::SpinH*>(d_out);
+      auto* i = static_cast<const qed3::SpinH*>(d_in);
+      if (op == QED3_OP_D || op == QED3_OP_DDAG) {
+        D.apply_half(op == QED3_OP_DDAG, o, i, true);
+        return QED3_OK;
+      }
+      qed3::SpinH* tmp = nullptr;
+      QED3_HIP(hipMalloc(&tmp, D.points() * sizeof(qed3::SpinH)));
+      D.apply_half(op == QED3_OP_DDAGD ? false : true, tmp, i, false);
+      D.apply_half(op == QED3_OP_DDAGD ? true : false, o, tmp, true);
+      QED3_HIP(hipFree(tmp));
+      return QED3_OK;
+    }
     if (prec == QED3_PREC_F64)
       go(double{});
     else
@@ -832,7 +847,7 @@ int qed3_time_copy(size_t bytes, int reps, double* ms) {
 int qed3_tile_info(const qed3_ctx* ctx, int op, int prec, int* out) {
   if (!ctx || !out) return fail(QED3_ERR_ARG, "null argument");
   const bool dag = op == QED3_OP_DDAG;
-  const qed3::StencilTile t = prec == QED3_PREC_F32 ? ctx->D->tile<float>(dag) : ctx->D->tile<double>(dag);
+  const qed3::StencilTile t = prec != QED3_PREC_F64 ? ctx->D->tile<float>(dag) : ctx->D->tile<double>(dag);
   out[0] = t.block;
   out[1] = t.ts;
   out[2] = t.remap;
@@ -846,7 +861,7 @@ int qed3_set_tile(qed3_ctx* ctx, int op, int prec, const int* in) {
   if (in[0] != 64 && in[0] != 128 && in[0] != 256) return fail(QED3_ERR_ARG, "block must be 64/128/256");
   if (in[1] < 1 || in[1] > 64) return fail(QED3_ERR_ARG, "time tile must be in [1,64]");
   return guarded([&]() -> int {
-    ctx->D->set_tile(prec == QED3_PREC_F32, op == QED3_OP_DDAG,
+    ctx->D->set_tile(prec != QED3_PREC_F64, op == QED3_OP_DDAG,
                      qed3::StencilTile{in[0], in[1], in[2] ? 1 : 0, in[3] ? 1 : 0});
     return QED3_OK;
   });
@@ -863,9 +878,11 @@ int qed3_autotune(qed3_ctx* ctx) {
 int qed3_time_apply(qed3_ctx* ctx, int op, int prec, int warmup, int reps, double* ms) {
   if (!ctx || !ms) return fail(QED3_ERR_ARG, "null argument");
   if (op != QED3_OP_D && op != QED3_OP_DDAG) return fail(QED3_ERR_ARG, "op must be D or DDAG");
+  if (prec != QED3_PREC_F64 && prec != QED3_PREC_F32 && prec != QED3_PREC_F16)
+    return fail(QED3_ERR_ARG, "bad precision");
   if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
   return guarded([&]() -> int {
-    *ms = ctx->D->time_apply(op == QED3_OP_DDAG, prec == QED3_PREC_F32, warmup, reps);
+    *ms = ctx->D->time_apply(op == QED3_OP_DDAG, prec, warmup, reps);
     return QED3_OK;
   });
 }

@@ -324,7 +324,7 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   hipLaunchKernelGGL(k_store_b0, dim3(nbk), dim3(256), 0, st, b, r, n, sc);
   QED3_HIP(hipGetLastError());
 
-  const DiracView<float> vin = D.view<float>();
+  const DiracView<float, half2_t> vin = D.view_half_phases();
   const DiracView<double> v64 = D.view<double>();
   const int nn = D.tables().max_nn;
   const HaloView hvf = D.halo_for<float>(false), hvt = D.halo_for<float>(true);

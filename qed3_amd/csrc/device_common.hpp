@@ -84,6 +84,15 @@ __device__ __forceinline__ SpinH to_half(const Spin<float>& s) {
 }
 __device__ __forceinline__ Spin<float> ldh(const SpinH* p) { return from_half(*p); }
 
+// Link phase exp(i theta) in binary16 (4 B), read as an FP32 complex.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ldu(const double2* p) { return *p; }
+__device__ __forceinline__ float2 ldu(const float2* p) { return *p; }
+__device__ __forceinline__ float2 ldu(const half2_t* p) {
+  const half2_t h = *p;
+  return {float(h.x), float(h.y)};
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -24,6 +24,15 @@ __global__ void k_phases(const double* __restrict__ ang, cplx_t<T>* __restrict__
   }
 }
 
+__global__ void k_phases_half(const double* __restrict__ ang, half2_t* __restrict__ u, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double sn, cs;
+    sincos(ang[i], &sn, &cs);
+    u[i] = half2_t{_Float16(cs), _Float16(sn)};
+  }
+}
+
 // <x,y> = sum conj(x) y over complex double fields of n points.
 __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
                                              const Spin<double>* __restrict__ y, size_t n,
@@ -213,6 +222,7 @@ Dirac::~Dirac() {
                   (void*)d64_.diag, (void*)d64_.kth, (void*)d64_.uS, (void*)d64_.uT,
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
                   (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
+                  (void*)uSh_, (void*)uTh_,
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
@@ -283,8 +293,41 @@ void Dirac::set_gauge(const double* spatial, const double* temporal, bool on_dev
   hipLaunchKernelGGL(k_phases<float>, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, d32_.uS, nS);
   hipLaunchKernelGGL(k_phases<float>, dim3((nT + B - 1) / B), dim3(B), 0, stream_, angT_, d32_.uT, nT);
   QED3_HIP(hipGetLastError());
+  if (uSh_) fill_half_phases();
   QED3_HIP(hipStreamSynchronize(stream_));
   has_gauge_ = true;
+}
+
+void Dirac::fill_half_phases() {
+  const size_t nS = size_t(h_.Lt) * h_.E, nT = size_t(h_.Lt) * h_.V;
+  const int B = 256;
+  hipLaunchKernelGGL(k_phases_half, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, uSh_, nS);
+  hipLaunchKernelGGL(k_phases_half, dim3((nT + B - 1) / B), dim3(B), 0, stream_, angT_, uTh_, nT);
+  QED3_HIP(hipGetLastError());
+}
+
+DiracView<float, half2_t> Dirac::view_half_phases() {
+  if (!uSh_) {
+    uSh_ = dalloc<half2_t>(size_t(h_.Lt) * h_.E);
+    uTh_ = dalloc<half2_t>(size_t(h_.Lt) * h_.V);
+    if (has_gauge_) fill_half_phases();
+  }
+  const DiracView<float> f = view<float>();
+  DiracView<float, half2_t> v;
+  v.nbr = f.nbr;
+  v.lnk = f.lnk;
+  v.hop = f.hop;
+  v.hopd = f.hopd;
+  v.diag = f.diag;
+  v.kth = f.kth;
+  v.uS = uSh_;
+  v.uT = uTh_;
+  v.V = f.V;
+  v.E = f.E;
+  v.Lt = f.Lt;
+  v.r = f.r;
+  v.bc = f.bc;
+  return v;
 }
 
 template <typename T>
@@ -327,6 +370,19 @@ void Dirac::apply(bool dag, Spin<T>* out, const Spin<T>* in, bool sync) {
   if (sync) QED3_HIP(hipStreamSynchronize(stream_));
 }
 
+void Dirac::apply_half(bool dag, SpinH* out, const SpinH* in, bool sync) {
+  const DiracView<float, half2_t> v = view_half_phases();
+  const EpiStoreH e{out};
+  if (dag)
+    launch_stencil_src<float, true>(v, h_.max_nn, SrcHalf{in}, e, tile<float>(true), reducer(), stream_,
+                                    halo_for<float>(true));
+  else
+    launch_stencil_src<float, false>(v, h_.max_nn, SrcHalf{in}, e, tile<float>(false), reducer(), stream_,
+                                     halo_for<float>(false));
+  QED3_HIP(hipGetLastError());
+  if (sync) QED3_HIP(hipStreamSynchronize(stream_));
+}
+
 template <typename T>
 void Dirac::apply_dagd(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync) {
   apply<T>(false, tmp, in, false);
@@ -363,7 +419,7 @@ void Dirac::force(const Spin<double>* eta, double* fs, double* ft) {
   QED3_HIP(hipStreamSynchronize(stream_));
 }
 
-double Dirac::time_apply(bool dag, bool f32, int warmup, int reps) {
+double Dirac::time_apply(bool dag, int prec, int warmup, int reps) {
   const size_t n = points();
   hipEvent_t e0, e1;
   QED3_HIP(hipEventCreate(&e0));
@@ -383,10 +439,23 @@ double Dirac::time_apply(bool dag, bool f32, int warmup, int reps) {
     (void)hipFree(a);
     (void)hipFree(b);
   };
-  if (f32)
+  if (prec == 2) {
+    SpinH* a = dalloc<SpinH>(n);
+    SpinH* b = dalloc<SpinH>(n);
+    QED3_HIP(hipMemsetAsync(a, 0, n * sizeof(SpinH), stream_));
+    for (int i = 0; i < warmup; i++) apply_half(dag, b, a, false);
+    QED3_HIP(hipEventRecord(e0, stream_));
+    for (int i = 0; i < reps; i++) apply_half(dag, b, a, false);
+    QED3_HIP(hipEventRecord(e1, stream_));
+    QED3_HIP(hipEventSynchronize(e1));
+    QED3_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipFree(a);
+    (void)hipFree(b);
+  } else if (prec == 1) {
     run(float{});
-  else
+  } else {
     run(double{});
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return double(ms) / std::max(reps, 1);

@@ -14,8 +14,9 @@
 
 namespace qed3 {
 
-// Device copy of the time-independent tables in one precision.
-template <typename T>
+// Device copy of the time-independent tables in one precision; P is the stored type of
+// the link phases (T's complex, or binary16 pairs for the half-storage solver).
+template <typename T, typename P = cplx_t<T>>
 struct DiracView {
   const int* nbr = nullptr;          // [nslot][V]
   const int* lnk = nullptr;          // [nslot][V] link id or ~id
@@ -23,8 +24,8 @@ struct DiracView {
   const real4_t<T>* hopd = nullptr;  // [nslot][V] for D^dagger
   const T* diag = nullptr;           // [V]
   const T* kth = nullptr;            // [V] kappa_t/2
-  const cplx_t<T>* uS = nullptr;     // [Lt][E] exp(i theta_s(l))
-  const cplx_t<T>* uT = nullptr;     // [Lt][V] exp(i theta^t_s(x))
+  const P* uS = nullptr;             // [Lt][E] exp(i theta_s(l))
+  const P* uT = nullptr;             // [Lt][V] exp(i theta^t_s(x))
   int V = 0, E = 0, Lt = 1;
   T r = 1;
   T bc = -1;  // sign on the time wrap: -1 antiperiodic, +1 periodic
@@ -96,6 +97,9 @@ class Dirac {
 
   template <typename T>
   DiracView<T> view() const;
+  // FP32 tables with binary16 link phases (kept up to date with the gauge field from the
+  // first call on)
+  DiracView<float, half2_t> view_half_phases();
   template <typename T>
   StencilTile tile(bool dag) const;
 
@@ -106,6 +110,8 @@ class Dirac {
   void apply_dagd(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync = true);
   template <typename T>
   void apply_ddag(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync = true);
+  // binary16 fields, FP32 arithmetic, binary16 link phases (the half-storage CG's stencil)
+  void apply_half(bool dag, SpinH* out, const SpinH* in, bool sync = true);
 
   // complex <x,y> in FP64
   void dot(const Spin<double>* x, const Spin<double>* y, double* re, double* im);
@@ -113,7 +119,8 @@ class Dirac {
   // pseudofermion force for all links from eta (FP64)
   void force(const Spin<double>* eta, double* fs, double* ft);
 
-  double time_apply(bool dag, bool f32, int warmup, int reps);
+  // ms per launch of one D or D^dag apply; prec 0 FP64, 1 FP32, 2 binary16 storage
+  double time_apply(bool dag, int prec, int warmup, int reps);
 
   // scratch for reductions
   GridReduce reducer() const { return {partials_, counter_}; }
@@ -142,6 +149,9 @@ class Dirac {
   hipStream_t stream_ = nullptr;
   DeviceTables<double> d64_;
   DeviceTables<float> d32_;
+  half2_t* uSh_ = nullptr;  // [Lt][E] binary16 phases (allocated by view_half_phases)
+  half2_t* uTh_ = nullptr;  // [Lt][V]
+  void fill_half_phases();
   double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)
   double* angT_ = nullptr;  // [Lt][V]
   int* link_ends_ = nullptr;  // [E][4]: x, y, slot of y in x, slot of x in y

@@ -122,8 +122,8 @@ __device__ __forceinline__ void call_center(const Epi&, size_t, const Spin<T>&, 
 
 // The stencil for site x over the time slices [s0, s0+ts), handing every output point
 // to the epilogue (shared by the single-system and the batched kernels).
-template <typename T, int NS, bool DAG, class Epi, class In>
-__device__ __forceinline__ void stencil_site_src(const DiracView<T>& d, const In& in,
+template <typename T, int NS, bool DAG, class Epi, class In, typename P>
+__device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const In& in,
                                                  const Epi& epi, int x, int s0, int ts, double2& acc) {
   using C = cplx_t<T>;
   const int V = d.V, Lt = d.Lt;
@@ -147,18 +147,18 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T>& d, const In
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
     Spin<T> prev = in(size_t(sm) * V + x);
     Spin<T> cur = in(size_t(s0) * V + x);
-    C uprev = d.uT[size_t(sm) * V + x];
+    C uprev = ldu(d.uT + size_t(sm) * V + x);
     for (int s = s0; s < s1; s++) {
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       const Spin<T> next = in(size_t(sp) * V + x);
-      const C ucur = d.uT[size_t(s) * V + x];
+      const C ucur = ldu(d.uT + size_t(s) * V + x);
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
-      const C* uS = d.uS + size_t(s) * d.E;
+      const P* uS = d.uS + size_t(s) * d.E;
       const size_t sV = size_t(s) * V;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
         const int l = lk[j];
-        C u = uS[l >= 0 ? l : ~l];
+        C u = ldu(uS + (l >= 0 ? l : ~l));
         if (l < 0) u.y = -u.y;
         const Spin<T> py = in(sV + nb[j]);
         const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
@@ -198,14 +198,14 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T>& d, const In
   }
 }
 
-template <typename T, int NS, bool DAG, class Epi>
-__device__ __forceinline__ void stencil_site(const DiracView<T>& d, const Spin<T>* __restrict__ in,
+template <typename T, int NS, bool DAG, class Epi, typename P>
+__device__ __forceinline__ void stencil_site(const DiracView<T, P>& d, const Spin<T>* __restrict__ in,
                                              const Epi& epi, int x, int s0, int ts, double2& acc) {
   stencil_site_src<T, NS, DAG>(d, SrcPlain<T>{in}, epi, x, s0, ts, acc);
 }
 
-template <typename T, int NS, bool DAG, class Epi, class Src>
-__global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, Src in, Epi epi, int ts, TileMap tm,
+template <typename T, int NS, bool DAG, class Epi, class Src, typename P = cplx_t<T>>
+__global__ __launch_bounds__(256) void k_stencil(DiracView<T, P> d, Src in, Epi epi, int ts, TileMap tm,
                                                   GridReduce gr) {
   if (epi.skip()) return;  // uniform: solver already finished
   int xr, st;
@@ -222,8 +222,8 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T> d, Src in, Epi epi
 // (the outside neighbours of the range) in LDS with one coalesced load of the range
 // and one gather of the halo, then reads all spatial neighbours from LDS through a
 // 16-bit local index table.  Every spinor is fetched once per block and slice.
-template <typename T, int NS, bool DAG, class Epi, class Src>
-__global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, Src src, Epi epi, int ts, TileMap tm,
+template <typename T, int NS, bool DAG, class Epi, class Src, typename P = cplx_t<T>>
+__global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src, Epi epi, int ts, TileMap tm,
                                                       HaloView hv, GridReduce gr) {
   if (epi.skip()) return;
   const auto in = src.bind();
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, Src src, Ep
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
     prev = in(size_t(sm) * V + x);
     cur = in(size_t(s0) * V + x);
-    uprev = d.uT[size_t(sm) * V + x];
+    uprev = ldu(d.uT + size_t(sm) * V + x);
   }
   for (int s = s0; s < s1; s++) {
     const size_t sV = size_t(s) * V;
@@ -277,13 +277,13 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T> d, Src src, Ep
     if (act) {
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       const Spin<T> next = in(size_t(sp) * V + x);
-      const C ucur = d.uT[size_t(s) * V + x];
+      const C ucur = ldu(d.uT + size_t(s) * V + x);
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
-      const C* uS = d.uS + size_t(s) * d.E;
+      const P* uS = d.uS + size_t(s) * d.E;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
         const int l = lk[j];
-        C u = uS[l >= 0 ? l : ~l];
+        C u = ldu(uS + (l >= 0 ? l : ~l));
         if (l < 0) u.y = -u.y;
         const Spin<T> py = tileS[nb[j]];
         const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
@@ -334,14 +334,24 @@ struct EpiStore {
   __device__ void operator()(double2) const {}
 };
 
+// Epilogue: store in binary16 (half-storage fields)
+struct EpiStoreH {
+  static constexpr bool kReduce = false;
+  SpinH* out;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ void operator()(size_t i, const Spin<float>& v, double2&) const { out[i] = to_half(v); }
+  __device__ void operator()(double2) const {}
+};
+
 template <typename T>
 __device__ __forceinline__ double norm2(const Spin<T>& v) {
   return double(v.a.x) * v.a.x + double(v.a.y) * v.a.y + double(v.b.x) * v.b.x +
          double(v.b.y) * v.b.y;
 }
 
-template <typename T, bool DAG, class Epi, class Src>
-inline void launch_stencil_src(const DiracView<T>& v, int max_nn, const Src& in, const Epi& epi,
+template <typename T, bool DAG, class Epi, class Src, typename P>
+inline void launch_stencil_src(const DiracView<T, P>& v, int max_nn, const Src& in, const Epi& epi,
                                StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv) {
   TileMap tm;
   tm.nxr = (v.V + t.block - 1) / t.block;
@@ -356,22 +366,22 @@ inline void launch_stencil_src(const DiracView<T>& v, int max_nn, const Src& in,
     if (!hv.nbl) throw std::logic_error("LDS stencil launched without halo tables");
     const size_t smem = size_t(t.block + hv.hmax) * sizeof(Spin<T>);
     if (max_nn == 6)
-      hipLaunchKernelGGL((k_stencil_lds<T, 6, DAG, Epi, Src>), grid, dim3(t.block), smem, st, v, in, epi,
+      hipLaunchKernelGGL((k_stencil_lds<T, 6, DAG, Epi, Src, P>), grid, dim3(t.block), smem, st, v, in, epi,
                          t.ts, tm, hv, gr);
     else
-      hipLaunchKernelGGL((k_stencil_lds<T, 3, DAG, Epi, Src>), grid, dim3(t.block), smem, st, v, in, epi,
+      hipLaunchKernelGGL((k_stencil_lds<T, 3, DAG, Epi, Src, P>), grid, dim3(t.block), smem, st, v, in, epi,
                          t.ts, tm, hv, gr);
   } else if (max_nn == 6) {
-    hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi, Src>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
+    hipLaunchKernelGGL((k_stencil<T, 6, DAG, Epi, Src, P>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
   } else {
-    hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi, Src>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
+    hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi, Src, P>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
   }
   if constexpr (Epi::kReduce)
     hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, unsigned(nblk), epi);
 }
 
-template <typename T, bool DAG, class Epi>
-inline void launch_stencil(const DiracView<T>& v, int max_nn, const Spin<T>* in, const Epi& epi,
+template <typename T, bool DAG, class Epi, typename P>
+inline void launch_stencil(const DiracView<T, P>& v, int max_nn, const Spin<T>* in, const Epi& epi,
                            StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv) {
   launch_stencil_src<T, DAG>(v, max_nn, SrcPlain<T>{in}, epi, t, gr, st, hv);
 }

@@ -41,6 +41,32 @@ def test_dirac_f32_matches_oracle(gpu, lattice):
         assert rel(got, ref) < 2e-6, (op, rel(got, ref))
 
 
+def round16(v):
+    """v with real and imaginary parts rounded to binary16 (the F16 kernels' input)."""
+    return v.real.astype(np.float16).astype(np.float64) + 1j * v.imag.astype(np.float16)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_dirac_f16_matches_oracle(gpu, lattice):
+    """binary16 fields and phases, FP32 arithmetic: the FP64 operator on the rounded input
+    to binary16 accuracy (unit roundoff 2^-11); the binary16 phases follow gauge updates."""
+    n, Lt = 4, 4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.05)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.05, at=at)
+    v = round16(source(lattice, n, Lt, seed=9))
+    for seed in (5, 6):
+        sp, tp = links(lattice, n, Lt, seed=seed, kappa=1.4)
+        Do.update(sp, tp)
+        Dq.update(sp, tp)
+        for op in (q.OP_D, q.OP_DDAG, q.OP_DDAGD):
+            ref = Do.apply(op, v)
+            got = Dq.apply(op, v, q.F16)
+            assert rel(got, ref) < 1e-3, (seed, op, rel(got, ref))
+    # the rounding is that of binary16, not of something coarser
+    assert rel(Dq.apply(q.OP_D, v, q.F16), Do.apply(q.OP_D, v)) > 1e-6
+
+
 TILES = [dict(block=b, ts=t, remap=r, lds=l) for (b, t, r, l) in
          [(64, 1, 0, 0), (128, 3, 1, 0), (256, 8, 1, 0), (64, 1, 0, 1), (128, 2, 1, 1),
           (256, 5, 0, 1), (256, 32, 1, 1), (64, 64, 0, 0)]]
@@ -62,7 +88,11 @@ def test_every_launch_shape_matches_oracle(gpu, lattice, tile):
     for op in (q.OP_D, q.OP_DDAG):
         assert rel(Dq.apply(op, v, q.F64), Do.apply(op, v)) < 1e-14
         assert rel(Dq.apply(op, v.astype(np.complex64), q.F32), Do.apply(op, v)) < 2e-6
+        v16 = round16(v)
+        assert rel(Dq.apply(op, v16, q.F16), Do.apply(op, v16)) < 1e-3
     x, rep = Dq.solve(Do.apply(1, v), tol=1e-11, solver=q.CG_MIXED)
+    assert rep.status == 0 and rep.true_resid <= 1e-11
+    x, rep = Dq.solve(Do.apply(1, v), tol=1e-11, solver=q.CG_HALF)
     assert rep.status == 0 and rep.true_resid <= 1e-11
 
 
