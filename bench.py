@@ -429,7 +429,8 @@ def plain_leg(a, q, g, m, lattice):
 
 def hmc_leg(a, q):
     """Full HMC (simplicial n=64, Lt=32, m=0.1, compact U(1) at beta=1.4, tau=1, 20 outer
-    (fermion) steps of hmc_inner gauge steps each, Nf=2 via one pseudofermion, mixed CG):
+    (fermion) steps of hmc_inner gauge steps each, Nf=2 via one pseudofermion, half-storage
+    CG):
     hmc_therm forced-accept trajectories from the von Mises start, then hmc_traj timed
     trajectories with the accept step (seconds per trajectory, dH, acceptance, CG work);
     the pseudofermion force at the same size against the FP64 CPU path (the oracle's local
@@ -446,7 +447,7 @@ def hmc_leg(a, q):
     D.update(sp, tp)
     D.autotune()
     H = q.HMC(D, 1 / 1.4, compact=True, npf=1, tau=1.0, nsteps=a.hmc_nsteps, nsteps_inner=a.hmc_inner,
-              tol=1e-10, solver=q.CG_MIXED, seed=7)
+              tol=1e-10, solver=q.CG_HALF, seed=7)
     for k in range(a.hmc_therm):
         r = H.trajectory(sp, tp, no_reject=True)
         print(f"# hmc thermalisation {k}: dH={r['dH']:.3e}", file=sys.stderr, flush=True)

@@ -288,6 +288,7 @@ std::vector<CGResult> batch_cg(const std::vector<Dirac*>& Ds, const std::vector<
     out[k].reliable_updates = hs[k].n_ru;
     out[k].status = hs[k].done == 1 ? 0 : (hs[k].done == 2 ? 4 : 3);
     out[k].true_resid = true_residual(*Ds[k], xs[k], bs[k]);
+    Ds[k]->release_scratch();  // many contexts: do not keep a residual's temporaries on each
     if (out[k].status == 0 && !(out[k].true_resid <= o.tol * 1.0000001)) out[k].status = 3;
     out[k].seconds = secs;
   }

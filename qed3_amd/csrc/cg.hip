@@ -111,6 +111,7 @@ __global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__
     sc->maxr2 = sc->mu;
     sc->true_r2 = t.x;
     sc->rfac = 1.0;
+    sc->alpha_prev = 0.0;
     sc->rescale = 0;
     sc->gam = 0;
     sc->alpha = 0;
@@ -123,6 +124,37 @@ __global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__
     sc->need_ru = 0;
     sc->done = (t.x == 0.0) ? 1 : (t.x < 1e300 ? 0 : 2);
   });
+}
+
+// Half-storage flush: x64 += scale (x_inner + alpha_prev p_pend), x_inner = 0, where
+// alpha_prev p_pend is the x term an even iteration left pending (alpha_prev = 0 if none)
+__global__ __launch_bounds__(256) void k_flush_h(Spin<double>* x64, Spin<float>* xin,
+                                                 const SpinH* __restrict__ ppend, size_t n,
+                                                 const CGScalars* sc, int always) {
+  if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
+  const double s = sc->scale;
+  const float ap = float(sc->alpha_prev);
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    Spin<float> a = xin[i];
+    if (ap != 0.f) {
+      const Spin<float> pp = ldh(ppend + i);
+      a.a.x += ap * pp.a.x;
+      a.a.y += ap * pp.a.y;
+      a.b.x += ap * pp.b.x;
+      a.b.y += ap * pp.b.y;
+    }
+    Spin<double> xx = x64[i];
+    xx.a.x += s * a.a.x;
+    xx.a.y += s * a.a.y;
+    xx.b.x += s * a.b.x;
+    xx.b.y += s * a.b.y;
+    x64[i] = xx;
+    Spin<float> z;
+    z.a = {0.f, 0.f};
+    z.b = {0.f, 0.f};
+    xin[i] = z;
+  }
 }
 
 // r = b / scale in half (FP64 arithmetic, so any finite b is representable)
@@ -221,21 +253,20 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   hipStream_t st = D.stream();
   auto t0 = std::chrono::steady_clock::now();
   Work<T> W;
-  QED3_HIP(hipMallocAsync((void**)&W.r, n * sizeof(Spin<T>), st));
-  QED3_HIP(hipMallocAsync((void**)&W.p, n * sizeof(Spin<T>), st));
-  QED3_HIP(hipMallocAsync((void**)&W.p2, n * sizeof(Spin<T>), st));
-  QED3_HIP(hipMallocAsync((void**)&W.w, n * sizeof(Spin<T>), st));
+  const size_t fb = n * sizeof(Spin<T>);
+  W.r = static_cast<Spin<T>*>(D.scratch(0, fb));
+  W.p = static_cast<Spin<T>*>(D.scratch(1, fb));
+  W.p2 = static_cast<Spin<T>*>(D.scratch(2, fb));
+  W.w = static_cast<Spin<T>*>(D.scratch(3, fb));
   if (kMixed) {
-    QED3_HIP(hipMallocAsync((void**)&W.xin, n * sizeof(Spin<T>), st));
-    QED3_HIP(hipMallocAsync((void**)&W.t, n * sizeof(Spin<double>), st));
+    W.xin = static_cast<Spin<T>*>(D.scratch(4, fb));
+    W.t = static_cast<Spin<double>*>(D.scratch(5, n * sizeof(Spin<double>)));
   } else {
     W.xin = reinterpret_cast<Spin<T>*>(x);
     W.t = reinterpret_cast<Spin<double>*>(W.w);
   }
-  CGScalars* sc = nullptr;
-  QED3_HIP(hipMallocAsync((void**)&sc, sizeof(CGScalars), st));
-  CGScalars* hsc = nullptr;
-  QED3_HIP(hipHostMalloc((void**)&hsc, sizeof(CGScalars), hipHostMallocDefault));
+  CGScalars* sc = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
+  CGScalars* hsc = D.host_scalars();
 
   const GridReduce gr = D.reducer();
   const int nbk = int(stream_blocks(n));
@@ -278,17 +309,6 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   res.iterations = hsc->iter;
   res.reliable_updates = hsc->n_ru;
   res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
-  (void)hipFreeAsync(W.r, st);
-  (void)hipFreeAsync(W.p, st);
-  (void)hipFreeAsync(W.p2, st);
-  (void)hipFreeAsync(W.w, st);
-  if (kMixed) {
-    (void)hipFreeAsync(W.xin, st);
-    (void)hipFreeAsync(W.t, st);
-  }
-  (void)hipFreeAsync(sc, st);
-  QED3_HIP(hipStreamSynchronize(st));
-  (void)hipHostFree(hsc);
   res.true_resid = true_residual(D, x, b);
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
@@ -302,20 +322,16 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   const size_t n = D.points();
   hipStream_t st = D.stream();
   auto t0 = std::chrono::steady_clock::now();
-  SpinH *r = nullptr, *p = nullptr, *p2 = nullptr, *w = nullptr;
-  Spin<float>*xin = nullptr, *rt = nullptr;
-  Spin<double>* t = nullptr;
-  QED3_HIP(hipMallocAsync((void**)&r, n * sizeof(SpinH), st));
-  QED3_HIP(hipMallocAsync((void**)&p, n * sizeof(SpinH), st));
-  QED3_HIP(hipMallocAsync((void**)&p2, n * sizeof(SpinH), st));
-  QED3_HIP(hipMallocAsync((void**)&w, n * sizeof(SpinH), st));
-  QED3_HIP(hipMallocAsync((void**)&xin, n * sizeof(Spin<float>), st));
-  QED3_HIP(hipMallocAsync((void**)&rt, n * sizeof(Spin<float>), st));
-  QED3_HIP(hipMallocAsync((void**)&t, n * sizeof(Spin<double>), st));
-  CGScalars* sc = nullptr;
-  QED3_HIP(hipMallocAsync((void**)&sc, sizeof(CGScalars), st));
-  CGScalars* hsc = nullptr;
-  QED3_HIP(hipHostMalloc((void**)&hsc, sizeof(CGScalars), hipHostMallocDefault));
+  const size_t hb = n * sizeof(SpinH), fb = n * sizeof(Spin<float>);
+  SpinH* r = static_cast<SpinH*>(D.scratch(0, hb));
+  SpinH* p = static_cast<SpinH*>(D.scratch(1, hb));
+  SpinH* p2 = static_cast<SpinH*>(D.scratch(2, hb));
+  SpinH* w = static_cast<SpinH*>(D.scratch(3, hb));
+  Spin<float>* xin = static_cast<Spin<float>*>(D.scratch(4, fb));
+  Spin<double>* t = static_cast<Spin<double>*>(D.scratch(5, n * sizeof(Spin<double>)));
+  Spin<float>* rt = static_cast<Spin<float>*>(D.scratch(6, fb));
+  CGScalars* sc = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
+  CGScalars* hsc = D.host_scalars();
 
   const GridReduce gr = D.reducer();
   const int nbk = int(stream_blocks(n));
@@ -336,26 +352,28 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
     SpinH* pnew = P[par ^ 1];
     launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{r, pold, sc}, EpiK1PH{w, pnew, sc},
                                      D.tile<float>(false), gr, st, hvf);
-    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, EpiK2H{xin, r, pnew, sc, 0.f},
-                                    D.tile<float>(true), gr, st, hvt);
-    hipLaunchKernelGGL(k_flush<float>, dim3(nbk), dim3(256), 0, st, x, xin, n, sc, 0);
+    // x is updated on odd iterations with both directions (pold is the even one's p, kept
+    // in P[1] until the next K1); P[1] is also the pending direction after an even one
+    if (par)
+      launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, EpiK2H<true>{xin, r, pnew, pold, sc, 0.f, 0.f},
+                                      D.tile<float>(true), gr, st, hvt);
+    else
+      launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, EpiK2H<false>{xin, r, pnew, pold, sc, 0.f, 0.f},
+                                      D.tile<float>(true), gr, st, hvt);
+    hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, P[1], n, sc, 0);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
     launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr,
                                  st, hv64t);
     hipLaunchKernelGGL(k_rescale, dim3(nbk), dim3(256), 0, st, rt, r, n, sc);
   };
   drive_chunks(one_iter, o, n, st, sc, hsc);
-  hipLaunchKernelGGL(k_flush<float>, dim3(nbk), dim3(256), 0, st, x, xin, n, sc, 1);
+  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, P[1], n, sc, 1);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
   CGResult res;
   res.iterations = hsc->iter;
   res.reliable_updates = hsc->n_ru;
   res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
-  for (void* q : {(void*)r, (void*)p, (void*)p2, (void*)w, (void*)xin, (void*)rt, (void*)t, (void*)sc})
-    (void)hipFreeAsync(q, st);
-  QED3_HIP(hipStreamSynchronize(st));
-  (void)hipHostFree(hsc);
   res.true_resid = true_residual(D, x, b);
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
@@ -367,21 +385,15 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
 double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b) {
   const size_t n = D.points();
   hipStream_t st = D.stream();
-  Spin<double>*t1 = nullptr, *t2 = nullptr;
-  double2* out = nullptr;
-  QED3_HIP(hipMallocAsync((void**)&t1, n * sizeof(Spin<double>), st));
-  QED3_HIP(hipMallocAsync((void**)&t2, n * sizeof(Spin<double>), st));
-  QED3_HIP(hipMallocAsync((void**)&out, sizeof(double2), st));
+  Spin<double>* t1 = static_cast<Spin<double>*>(D.scratch(8, n * sizeof(Spin<double>)));
+  Spin<double>* t2 = static_cast<Spin<double>*>(D.scratch(9, n * sizeof(Spin<double>)));
+  double2* out = static_cast<double2*>(D.scratch(10, sizeof(double2)));
   D.apply_dagd<double>(t2, x, t1, false);
   hipLaunchKernelGGL(k_resid_norms, dim3(int(stream_blocks(n))), dim3(256), 0, st, b, t2, n,
                      D.reducer(), out);
   QED3_HIP(hipGetLastError());
   double2 h;
   QED3_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, st));
-  QED3_HIP(hipStreamSynchronize(st));
-  (void)hipFreeAsync(t1, st);
-  (void)hipFreeAsync(t2, st);
-  (void)hipFreeAsync(out, st);
   QED3_HIP(hipStreamSynchronize(st));
   return h.y > 0 ? std::sqrt(h.x / h.y) : 0.0;
 }

@@ -161,35 +161,48 @@ struct EpiK1PH {
   }
 };
 
-// K2: q = D^dag w; x += alpha p (FP32); r -= alpha q (stored in half); |r|^2 in stored
-// units, compared with the tolerance in true units
+// K2: q = D^dag w; r -= alpha q (stored in half); |r|^2 in stored units, compared with the
+// tolerance in true units.  x_inner (FP32) is updated every second iteration only: an even
+// iteration (PAIR = false) leaves alpha p pending (sc->alpha_prev; its p stays in the other
+// search-direction buffer until the next K1 overwrites it), the odd one adds
+// alpha_prev p_prev + alpha p in one read-modify-write of x.  A reliable update or the end
+// of the solve flushes a pending term (k_flush_h).
+template <bool PAIR>
 struct EpiK2H {
   static constexpr bool kReduce = true;
   Spin<float>* x;
   SpinH* r;
   const SpinH* p;
+  const SpinH* pprev;  // PAIR: the previous iteration's p
   CGScalars* sc;
-  float alpha;
+  float alpha, alpha_prev;
   __device__ bool skip() const { return sc->done != 0; }
-  __device__ void begin() { alpha = float(sc->alpha); }
+  __device__ void begin() {
+    alpha = float(sc->alpha);
+    alpha_prev = PAIR ? float(sc->alpha_prev) : 0.f;
+  }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const {
-    const Spin<float> pp = ldh(p + i);
-    Spin<float> xx = ld(x + i), rr = ldh(r + i);
-    xx.a.x += alpha * pp.a.x;
-    xx.a.y += alpha * pp.a.y;
-    xx.b.x += alpha * pp.b.x;
-    xx.b.y += alpha * pp.b.y;
+    Spin<float> rr = ldh(r + i);
+    if constexpr (PAIR) {
+      const Spin<float> pp = ldh(p + i), pq = ldh(pprev + i);
+      Spin<float> xx = ld(x + i);
+      xx.a.x += alpha_prev * pq.a.x + alpha * pp.a.x;
+      xx.a.y += alpha_prev * pq.a.y + alpha * pp.a.y;
+      xx.b.x += alpha_prev * pq.b.x + alpha * pp.b.x;
+      xx.b.y += alpha_prev * pq.b.y + alpha * pp.b.y;
+      st(x + i, xx);
+    }
     rr.a.x -= alpha * q.a.x;
     rr.a.y -= alpha * q.a.y;
     rr.b.x -= alpha * q.b.x;
     rr.b.y -= alpha * q.b.y;
-    st(x + i, xx);
     const SpinH rh = to_half(rr);
     r[i] = rh;
     acc.x += norm2(from_half(rh));
   }
   __device__ void operator()(double2 t) const {
     const double m = t.x;
+    sc->alpha_prev = PAIR ? 0.0 : sc->alpha;
     sc->iter += 1;
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
@@ -235,6 +248,7 @@ struct EpiResidH {
     sc->need_ru = 0;
     sc->n_ru += 1;
     sc->true_r2 = m;
+    sc->alpha_prev = 0.0;  // the flush before this update took the pending x term
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
       return;

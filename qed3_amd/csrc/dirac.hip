@@ -214,10 +214,39 @@ void Dirac::upload_tables(DeviceTables<T>& d) {
   d.uT = dalloc<cplx_t<T>>(size_t(h_.Lt) * h_.V);
 }
 
+void* Dirac::scratch(int slot, size_t bytes) {
+  if (slot < 0 || slot >= kScratchSlots) throw std::logic_error("bad scratch slot");
+  if (scratch_bytes_[slot] < bytes) {
+    QED3_HIP(hipStreamSynchronize(stream_));  // the old buffer may still be in use
+    if (scratch_[slot]) QED3_HIP(hipFree(scratch_[slot]));
+    scratch_[slot] = nullptr;
+    scratch_bytes_[slot] = 0;
+    QED3_HIP(hipMalloc(&scratch_[slot], bytes));
+    scratch_bytes_[slot] = bytes;
+  }
+  return scratch_[slot];
+}
+
+void Dirac::release_scratch() {
+  QED3_HIP(hipStreamSynchronize(stream_));
+  for (int i = 0; i < kScratchSlots; i++) {
+    if (scratch_[i]) QED3_HIP(hipFree(scratch_[i]));
+    scratch_[i] = nullptr;
+    scratch_bytes_[i] = 0;
+  }
+}
+
+CGScalars* Dirac::host_scalars() {
+  if (!hsc_) QED3_HIP(hipHostMalloc((void**)&hsc_, sizeof(CGScalars), hipHostMallocDefault));
+  return hsc_;
+}
+
 Dirac::~Dirac() {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
+  for (void* p : scratch_) f(p);
+  if (hsc_) (void)hipHostFree(hsc_);
   for (void* p : {(void*)d64_.nbr, (void*)d64_.lnk, (void*)d64_.hop, (void*)d64_.hopd,
                   (void*)d64_.diag, (void*)d64_.kth, (void*)d64_.uS, (void*)d64_.uT,
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,

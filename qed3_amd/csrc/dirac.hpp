@@ -71,6 +71,7 @@ struct CGScalars {
   double true_r2;  // last true FP64 |b - A x|^2
   double scale;    // half-storage CG: true field = scale * stored field (1 otherwise)
   double rfac;     // half-storage CG: factor from the kept residual to the new stored one
+  double alpha_prev;  // half-storage CG: alpha of an even iteration whose x update is pending
   int done;        // 0 running, 1 converged, 2 breakdown / NaN, 3 maxiter
   int need_ru;     // a reliable update is due in this iteration
   int iter, n_ru, maxiter;
@@ -124,6 +125,14 @@ class Dirac {
 
   // scratch for reductions
   GridReduce reducer() const { return {partials_, counter_}; }
+  // Device work buffer `slot` of at least `bytes`, kept across calls (grown on demand,
+  // freed with the context), so repeated solves allocate nothing; contents are undefined.
+  // Slots are owned by the CG solvers (cg.hip); calls on one context are sequential.
+  static constexpr int kScratchSlots = 12;
+  void* scratch(int slot, size_t bytes);
+  void release_scratch();
+  // pinned host copy of the solver scalars (same lifetime)
+  CGScalars* host_scalars();
   // LDS halo tables for ranges of R sites (built on first use)
   HaloView halo(int R);
   template <typename T>
@@ -169,6 +178,9 @@ class Dirac {
     int hmax = 0;
   };
   std::map<int, HaloTables> halos_;
+  void* scratch_[kScratchSlots] = {};
+  size_t scratch_bytes_[kScratchSlots] = {};
+  CGScalars* hsc_ = nullptr;
 };
 
 size_t reduce_capacity();  // max blocks a reducing kernel may use

@@ -135,3 +135,22 @@ def test_gauge_covariance_of_solution(gpu):
     Dq.update(sp2, tp2)
     x1, _ = Dq.solve(G * b, tol=1e-12)
     assert rel(np.conj(G) * x1, x0) < 1e-10
+
+
+@pytest.mark.gpu
+def test_half_storage_cg_truncated_iterates(gpu):
+    """Stopped after k iterations (no reliable update yet), the binary16-storage iterate
+    equals the FP32 one to binary16 accuracy for odd and even k: the x term that an even
+    iteration leaves pending is flushed at the end."""
+    _, Dq = _pair(1, 4, 8, 0.05)
+    b = source(1, 4, 8, seed=61)
+    d_b = q.DeviceArray.from_numpy(b)
+    for k in (1, 2, 3, 4, 7):
+        xs = []
+        for solver in (q.CG_MIXED, q.CG_HALF):
+            d_x = q.DeviceArray(b.nbytes)
+            rep = Dq.solve_device(d_x, d_b, tol=1e-14, maxiter=k, solver=solver, delta=1e-6,
+                                  raise_on_fail=False)
+            assert rep.status == 3 and rep.iterations == k and rep.reliable_updates == 0
+            xs.append(d_x.to_numpy(np.complex128))
+        assert rel(xs[1], xs[0]) < 1e-2, (k, rel(xs[1], xs[0]))

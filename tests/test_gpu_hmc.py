@@ -83,3 +83,25 @@ def test_hmc_parallel_host_path_matches_oracle(gpu):
     assert abs(rep["h0"] - ref["h0"]) < 1e-9 * abs(ref["h0"])
     assert abs(rep["dH"] - ref["dH"]) < 1e-9 * max(1.0, abs(ref["h0"]))
     assert rel(u, ref["u"]) < 1e-10 and rel(t, ref["t"]) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", [q.CG_MIXED, q.CG_HALF], ids=["mixed", "half"])
+def test_hmc_with_reduced_precision_solvers_matches_oracle(gpu, solver):
+    """The FP32 and binary16-storage inner solvers stop on the FP64 true residual, so the
+    trajectory matches the oracle's FP64 one to the solver tolerance times the condition
+    number (here ~1e-12 x 1e2)."""
+    lattice, n, Lt, M5, gsq = 0, 8, 16, 0.1, 1 / 1.4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    sp, tp = links(lattice, n, Lt, seed=8, kappa=1.4)
+    Sg = O.GaugeAction(L, Lt, gsq=gsq, at=at, compact=True)
+    ref = O.hmc_trajectory(Do, Sg, seed=21, npf=1, tau=0.2, nsteps=3, nsteps_inner=2,
+                           tol=1e-13, sp=sp, tp=tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, M5=M5, at=at)
+    H = q.HMC(Dq, gsq, compact=True, npf=1, tau=0.2, nsteps=3, nsteps_inner=2, tol=1e-12,
+              solver=solver, seed=21)
+    u, t = sp.copy(), tp.copy()
+    rep = H.trajectory(u, t, no_reject=True)
+    assert abs(rep["h0"] - ref["h0"]) < 1e-8 * abs(ref["h0"])
+    assert abs(rep["dH"] - ref["dH"]) < 1e-7 * max(1.0, abs(ref["h0"]))
+    assert rel(u, ref["u"]) < 1e-8 and rel(t, ref["t"]) < 1e-8
