@@ -270,21 +270,33 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src,
   }
   for (int s = s0; s < s1; s++) {
     const size_t sV = size_t(s) * V;
+    // This slice's loads that do not go through LDS (the forward temporal neighbour and
+    // the link phases) are issued before the barriers, so they are in flight together with
+    // the halo loads instead of after them.
+    Spin<T> next;
+    C ucur;
+    C uu[NS];
+    if (act) {
+      const int sp = (s == Lt - 1) ? 0 : s + 1;
+      next = in(size_t(sp) * V + x);
+      ucur = ldu(d.uT + size_t(s) * V + x);
+      const P* uS = d.uS + size_t(s) * d.E;
+#pragma unroll
+      for (int j = 0; j < NS; j++) {
+        const int l = lk[j];
+        uu[j] = ldu(uS + (l >= 0 ? l : ~l));
+        if (l < 0) uu[j].y = -uu[j].y;
+      }
+    }
     __syncthreads();  // previous slice's LDS reads are done
     if (act) tileS[threadIdx.x] = cur;
     for (int k = threadIdx.x; k < nh; k += R) tileS[R + k] = in(sV + hv.halo[h0 + k]);
     __syncthreads();
     if (act) {
-      const int sp = (s == Lt - 1) ? 0 : s + 1;
-      const Spin<T> next = in(size_t(sp) * V + x);
-      const C ucur = ldu(d.uT + size_t(s) * V + x);
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
-      const P* uS = d.uS + size_t(s) * d.E;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
-        const int l = lk[j];
-        C u = ldu(uS + (l >= 0 ? l : ~l));
-        if (l < 0) u.y = -u.y;
+        const C u = uu[j];
         const Spin<T> py = tileS[nb[j]];
         const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
         const T Cc = hp[j].x, Ss = hp[j].y, A = hp[j].z, B = hp[j].w;
