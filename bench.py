@@ -28,10 +28,10 @@ HBM_PEAK_GBS = 8000.0
 # 1 GiB copy kernel) from two rocprofv3 --pmc runs of `bench.py --kernels-only --lattice 1
 # --m 0.02 --tile <block,ts,remap,lds>` per launch shape the autotuner picks; summary in
 # profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision, lattice, n, Lt, tile).
-MEASURED_TRAFFIC = {("f16", 1, 128, 64, 128, 64, 1, 1): 6.0514e8,
-                    ("f32", 1, 128, 64, 128, 64, 1, 1): 1.1690e9,
-                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.1688e9,
-                    ("f32", 1, 128, 64, 256, 8, 1, 1): 1.4255e9}
+MEASURED_TRAFFIC = {("f16", 1, 128, 64, 128, 64, 1, 1): 6.0730e8,
+                    ("f16", 1, 128, 64, 256, 64, 1, 1): 6.1031e8,
+                    ("f32", 1, 128, 64, 128, 64, 1, 1): 1.3277e9,
+                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.2880e9}
 
 
 def parse():
