@@ -306,7 +306,9 @@ int qed3_hmc_rng_save(const qed3_hmc* h, const char* path);
 int qed3_hmc_rng_load(qed3_hmc* h, const char* path);
 
 /* Pick the stencil launch shape (threads per block, time slices per thread) for this
- * lattice by timing every candidate on the device (project-written autotuner). */
+ * lattice by timing every candidate on the device (project-written autotuner); with a
+ * gauge field set, the FP32 / binary16 shape is then refined on a few half-storage CG
+ * iterations, whose fused kernels prefer other shapes than a bare D apply. */
 int qed3_autotune(qed3_ctx* ctx);
 
 /* ---- timing of the operator kernels (HIP events on the context stream) ------------- */

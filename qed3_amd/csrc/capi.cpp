@@ -871,6 +871,7 @@ int qed3_autotune(qed3_ctx* ctx) {
   if (!ctx) return fail(QED3_ERR_ARG, "null context");
   return guarded([&]() -> int {
     ctx->D->autotune();
+    qed3::tune_cg_tiles(*ctx->D);
     return QED3_OK;
   });
 }

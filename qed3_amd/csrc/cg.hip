@@ -398,6 +398,59 @@ double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b) {
   return h.y > 0 ? std::sqrt(h.x / h.y) : 0.0;
 }
 
+namespace {
+__global__ void k_tune_source(Spin<double>* b, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double t = double(i % 8191);
+    Spin<double> v;
+    v.a = {1.0 + 0.5 * sin(0.37 * t), cos(0.11 * t)};
+    v.b = {sin(0.23 * t), 1.0 - 0.5 * cos(0.53 * t)};
+    b[i] = v;
+  }
+}
+}  // namespace
+
+void tune_cg_tiles(Dirac& D) {
+  if (D.cg_tuned || !D.has_gauge()) return;
+  const size_t n = D.points();
+  hipStream_t st = D.stream();
+  Spin<double>*x = nullptr, *b = nullptr;
+  QED3_HIP(hipMalloc((void**)&x, n * sizeof(Spin<double>)));
+  QED3_HIP(hipMalloc((void**)&b, n * sizeof(Spin<double>)));
+  hipLaunchKernelGGL(k_tune_source, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, b, n);
+  QED3_HIP(hipGetLastError());
+  CGOptions o;
+  o.tol = 1e-30;  // run exactly maxiter iterations
+  o.maxiter = 24;
+  o.mixed = o.half = true;
+  const StencilTile start = D.tile<float>(false);
+  std::vector<StencilTile> cands = {start};
+  for (int block : {128, 256})
+    for (int ts : {8, 16, 32, 64})
+      if (ts <= D.Lt() && !(block == start.block && ts == start.ts && start.remap && start.lds))
+        cands.push_back(StencilTile{block, ts, 1, 1});
+  StencilTile best = start;
+  double best_s = 1e30;
+  for (const StencilTile& t : cands) {
+    D.set_tile(true, false, t);
+    D.set_tile(true, true, t);
+    (void)cg_solve(D, x, b, o);  // warm-up (halo tables of a new range size built here)
+    const auto t0 = std::chrono::steady_clock::now();
+    (void)cg_solve(D, x, b, o);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (s < best_s) {
+      best_s = s;
+      best = t;
+    }
+  }
+  D.set_tile(true, false, best);
+  D.set_tile(true, true, best);
+  (void)hipFree(x);
+  (void)hipFree(b);
+  D.cg_tuned = true;
+}
+
 CGResult cg_solve(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
   if (o.half) return run_cg_half(D, x, b, o);
   if (o.mixed) return run_cg<float>(D, x, b, o);

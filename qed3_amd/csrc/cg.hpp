@@ -51,4 +51,10 @@ CGResult cg_solve(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOpti
 // ||b - D^dag D x|| / ||b|| in FP64
 double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b);
 
+// Refine the FP32 / binary16 launch shape (D and D^dag together) by timing a fixed number
+// of half-storage CG iterations per candidate: the CG kernels carry epilogues the plain
+// D apply timed by Dirac::autotune does not, and prefer other shapes.  Needs a gauge
+// field; runs once per context.
+void tune_cg_tiles(Dirac& D);
+
 }  // namespace qed3

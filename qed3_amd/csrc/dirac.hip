@@ -518,8 +518,8 @@ void Dirac::tune_one(bool dag) {
       QED3_HIP(hipEventSynchronize(e1));
       float ms;
       QED3_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < 0.98f * best_ms) {  // near-ties keep the earlier (smaller) shape, so the
-        best_ms = ms;               // choice does not flip between runs on timing noise
+      if (ms < best_ms) {
+        best_ms = ms;
         best = tile_[f][dag];
       }
     }

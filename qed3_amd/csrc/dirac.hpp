@@ -142,6 +142,7 @@ class Dirac {
   }
   // autotune the stencil launch shapes for this lattice (persisted in memory per shape)
   void autotune();
+  bool cg_tuned = false;  // the FP32 / binary16 shapes were refined on the CG iteration
   void set_tile(bool f32, bool dag, StencilTile t) {
     tile_[f32 ? 1 : 0][dag ? 1 : 0] = t;
     tuned_ = true;

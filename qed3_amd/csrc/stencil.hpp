@@ -241,6 +241,11 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src,
   const int s1 = min(s0 + ts, Lt);
   const int h0 = live ? hv.halo_off[xr] : 0;
   const int nh = live ? hv.halo_off[xr + 1] - h0 : 0;
+  // The halo's site ids are the same for every slice: each thread keeps the first two of
+  // its share in registers (entries threadIdx.x and threadIdx.x + R), so the per-slice
+  // halo gathers do not wait on an index load; any further entries are read in the loop.
+  const int hid0 = (int(threadIdx.x) < nh) ? hv.halo[h0 + threadIdx.x] : -1;
+  const int hid1 = (int(threadIdx.x) + R < nh) ? hv.halo[h0 + threadIdx.x + R] : -1;
   double2 acc = {0.0, 0.0};
   epi.begin();
   int nb[NS], lk[NS];
@@ -290,7 +295,9 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src,
     }
     __syncthreads();  // previous slice's LDS reads are done
     if (act) tileS[threadIdx.x] = cur;
-    for (int k = threadIdx.x; k < nh; k += R) tileS[R + k] = in(sV + hv.halo[h0 + k]);
+    if (hid0 >= 0) tileS[R + threadIdx.x] = in(sV + hid0);
+    if (hid1 >= 0) tileS[2 * R + threadIdx.x] = in(sV + hid1);
+    for (int k = threadIdx.x + 2 * R; k < nh; k += R) tileS[R + k] = in(sV + hv.halo[h0 + k]);
     __syncthreads();
     if (act) {
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);

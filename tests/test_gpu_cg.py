@@ -154,3 +154,20 @@ def test_half_storage_cg_truncated_iterates(gpu):
             assert rep.status == 3 and rep.iterations == k and rep.reliable_updates == 0
             xs.append(d_x.to_numpy(np.complex128))
         assert rel(xs[1], xs[0]) < 1e-2, (k, rel(xs[1], xs[0]))
+
+
+@pytest.mark.gpu
+def test_autotuned_shapes_keep_parity(gpu):
+    """The autotuner (bare-D sweep, then the CG-iteration refinement) only changes launch
+    shapes: the three solvers still reach the oracle's solution."""
+    lattice, n, Lt, M5 = 1, 4, 16, 0.05
+    Do, Dq = _pair(lattice, n, Lt, M5)
+    Dq.autotune()
+    t = Dq.tile(q.OP_D, q.F32)
+    assert t["block"] in (64, 128, 256) and 1 <= t["ts"] <= Lt, t
+    assert Dq.tile(q.OP_DDAG, q.F32) == t  # D and D^dag share the refined shape
+    b = source(lattice, n, Lt, seed=71)
+    x_ref, _, _ = Do.solve(b, tol=1e-13)
+    for solver in (q.CG_F64, q.CG_MIXED, q.CG_HALF):
+        x, rep = Dq.solve(b, tol=1e-12, solver=solver)
+        assert rep.status == 0 and rel(x, x_ref) < 1e-10, (solver, rel(x, x_ref))
