@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__
     sc->maxr2 = sc->mu;
     sc->true_r2 = t.x;
     sc->rfac = 1.0;
-    sc->alpha_prev = 0.0;
+    for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
     sc->rescale = 0;
     sc->gam = 0;
     sc->alpha = 0;
@@ -126,23 +126,29 @@ __global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__
   });
 }
 
-// Half-storage flush: x64 += scale (x_inner + alpha_prev p_pend), x_inner = 0, where
-// alpha_prev p_pend is the x term an even iteration left pending (alpha_prev = 0 if none)
-__global__ __launch_bounds__(256) void k_flush_h(Spin<double>* x64, Spin<float>* xin,
-                                                 const SpinH* __restrict__ ppend, size_t n,
-                                                 const CGScalars* sc, int always) {
+// Half-storage flush: x64 += scale (x_inner + sum_j alpha_pend[j] ring[j]), x_inner = 0,
+// taking the x terms the current group's iterations left pending (alpha_pend[j] = 0 if none)
+struct PendRing {
+  const SpinH* p[kXGroup - 1];
+};
+__global__ __launch_bounds__(256) void k_flush_h(Spin<double>* x64, Spin<float>* xin, PendRing ring,
+                                                 size_t n, const CGScalars* sc, int always) {
   if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
   const double s = sc->scale;
-  const float ap = float(sc->alpha_prev);
+  float ap[kXGroup - 1];
+  for (int j = 0; j < kXGroup - 1; j++) ap[j] = float(sc->alpha_pend[j]);
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
     Spin<float> a = xin[i];
-    if (ap != 0.f) {
-      const Spin<float> pp = ldh(ppend + i);
-      a.a.x += ap * pp.a.x;
-      a.a.y += ap * pp.a.y;
-      a.b.x += ap * pp.b.x;
-      a.b.y += ap * pp.b.y;
+#pragma unroll
+    for (int j = 0; j < kXGroup - 1; j++) {
+      if (ap[j] != 0.f) {
+        const Spin<float> pp = ldh(ring.p[j] + i);
+        a.a.x += ap[j] * pp.a.x;
+        a.a.y += ap[j] * pp.a.y;
+        a.b.x += ap[j] * pp.b.x;
+        a.b.y += ap[j] * pp.b.y;
+      }
     }
     Spin<double> xx = x64[i];
     xx.a.x += s * a.a.x;
@@ -215,19 +221,21 @@ struct Work {
 
 size_t stream_blocks(size_t n) { return std::min<size_t>((n + 255) / 256, 8192); }
 
-// Runs one_iter(parity) in chunks of iterations, captured once as a HIP graph unless
+// Runs one_iter(i), i = 0 .. chunk-1 (the iteration's position in the chunk), in chunks
+// of iterations, captured once as a HIP graph unless
 // disabled, with one host poll of the device scalars per chunk, until the device marks
 // the solve done.  On return hsc holds the final scalars.
 template <class Iter>
 void drive_chunks(const Iter& one_iter, const CGOptions& o, size_t n, hipStream_t st,
                   const CGScalars* sc, CGScalars* hsc) {
   int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
-  chunk += chunk & 1;  // even, so every replay of a captured chunk starts at parity 0
+  chunk = (chunk + kXGroup - 1) / kXGroup * kXGroup;  // every replay starts a new group
+                                                      // (and an even parity)
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   if (o.use_graph && std::getenv("QED3_NO_GRAPH") == nullptr) {
     QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; i++) one_iter(i & 1);
+    for (int i = 0; i < chunk; i++) one_iter(i);
     QED3_HIP(hipStreamEndCapture(st, &graph));
     QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
@@ -236,7 +244,7 @@ void drive_chunks(const Iter& one_iter, const CGOptions& o, size_t n, hipStream_
     if (exec)
       QED3_HIP(hipGraphLaunch(exec, st));
     else
-      for (int i = 0; i < chunk; i++) one_iter(i & 1);
+      for (int i = 0; i < chunk; i++) one_iter(i);
     QED3_HIP(hipGetLastError());
     QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
     QED3_HIP(hipStreamSynchronize(st));
@@ -286,7 +294,8 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   // the other buffer and applies D to it; K2 reads the new p.  p_old is read at halo points
   // of other blocks while the new p is written, hence two buffers alternating by parity.
   Spin<T>* P[2] = {W.p, W.p2};
-  auto one_iter = [&](int par) {
+  auto one_iter = [&](int i) {
+    const int par = i & 1;
     Spin<T>* pold = P[par];
     Spin<T>* pnew = P[par ^ 1];
     EpiK1P<T> k1;
@@ -324,8 +333,10 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   auto t0 = std::chrono::steady_clock::now();
   const size_t hb = n * sizeof(SpinH), fb = n * sizeof(Spin<float>);
   SpinH* r = static_cast<SpinH*>(D.scratch(0, hb));
-  SpinH* p = static_cast<SpinH*>(D.scratch(1, hb));
-  SpinH* p2 = static_cast<SpinH*>(D.scratch(2, hb));
+  // ring of search directions: iteration k writes ring[k % kXGroup] and reads its
+  // predecessor's slot; ring[kXGroup-1] is the zero p_old of the first iteration
+  SpinH* ring[kXGroup] = {static_cast<SpinH*>(D.scratch(1, hb)), static_cast<SpinH*>(D.scratch(2, hb)),
+                          static_cast<SpinH*>(D.scratch(12, hb)), static_cast<SpinH*>(D.scratch(13, hb))};
   SpinH* w = static_cast<SpinH*>(D.scratch(3, hb));
   Spin<float>* xin = static_cast<Spin<float>*>(D.scratch(4, fb));
   Spin<double>* t = static_cast<Spin<double>*>(D.scratch(5, n * sizeof(Spin<double>)));
@@ -335,7 +346,7 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
 
   const GridReduce gr = D.reducer();
   const int nbk = int(stream_blocks(n));
-  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, p, n, sc, gr,
+  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kXGroup - 1], n, sc, gr,
                      o.tol * o.tol, o.delta * o.delta, o.maxiter);
   hipLaunchKernelGGL(k_store_b0, dim3(nbk), dim3(256), 0, st, b, r, n, sc);
   QED3_HIP(hipGetLastError());
@@ -346,28 +357,38 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   const HaloView hvf = D.halo_for<float>(false), hvt = D.halo_for<float>(true);
   const HaloView hv64f = D.halo_for<double>(false), hv64t = D.halo_for<double>(true);
   const double inv_ncomp = 1.0 / (4.0 * double(n));
-  SpinH* P[2] = {p, p2};
-  auto one_iter = [&](int par) {
-    SpinH* pold = P[par];
-    SpinH* pnew = P[par ^ 1];
+  PendRing pend;
+  for (int j = 0; j < kXGroup - 1; j++) pend.p[j] = ring[j];
+  auto k2 = [&](auto tag) {
+    constexpr int POS = decltype(tag)::value;
+    EpiK2H<POS> e{};
+    e.x = xin;
+    e.r = r;
+    for (int j = 0; j < kXGroup; j++) e.ring[j] = ring[j];
+    e.sc = sc;
+    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt);
+  };
+  auto one_iter = [&](int i) {
+    const int pos = i % kXGroup;
+    SpinH* pold = ring[(pos + kXGroup - 1) % kXGroup];
+    SpinH* pnew = ring[pos];
     launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{r, pold, sc}, EpiK1PH{w, pnew, sc},
                                      D.tile<float>(false), gr, st, hvf);
-    // x is updated on odd iterations with both directions (pold is the even one's p, kept
-    // in P[1] until the next K1); P[1] is also the pending direction after an even one
-    if (par)
-      launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, EpiK2H<true>{xin, r, pnew, pold, sc, 0.f, 0.f},
-                                      D.tile<float>(true), gr, st, hvt);
-    else
-      launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, EpiK2H<false>{xin, r, pnew, pold, sc, 0.f, 0.f},
-                                      D.tile<float>(true), gr, st, hvt);
-    hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, P[1], n, sc, 0);
+    static_assert(kXGroup == 4, "one K2 instantiation per group position");
+    switch (pos) {
+      case 0: k2(std::integral_constant<int, 0>{}); break;
+      case 1: k2(std::integral_constant<int, 1>{}); break;
+      case 2: k2(std::integral_constant<int, 2>{}); break;
+      default: k2(std::integral_constant<int, 3>{}); break;
+    }
+    hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 0);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
     launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr,
                                  st, hv64t);
     hipLaunchKernelGGL(k_rescale, dim3(nbk), dim3(256), 0, st, rt, r, n, sc);
   };
   drive_chunks(one_iter, o, n, st, sc, hsc);
-  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, P[1], n, sc, 1);
+  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
   CGResult res;

@@ -162,34 +162,44 @@ struct EpiK1PH {
 };
 
 // K2: q = D^dag w; r -= alpha q (stored in half); |r|^2 in stored units, compared with the
-// tolerance in true units.  x_inner (FP32) is updated every second iteration only: an even
-// iteration (PAIR = false) leaves alpha p pending (sc->alpha_prev; its p stays in the other
-// search-direction buffer until the next K1 overwrites it), the odd one adds
-// alpha_prev p_prev + alpha p in one read-modify-write of x.  A reliable update or the end
-// of the solve flushes a pending term (k_flush_h).
-template <bool PAIR>
+// tolerance in true units.  x_inner (FP32) is updated once per group of kXGroup
+// iterations: the iterations at positions 0 .. kXGroup-2 of a group leave alpha p pending
+// (sc->alpha_pend[pos]; each p stays in its slot of the ring of search-direction buffers
+// until K1 of the next group overwrites it), the last one adds sum_j alpha_pend[j] p_j +
+// alpha p in one read-modify-write of x.  A reliable update or the end of the solve
+// flushes the pending terms (k_flush_h).
+constexpr int kXGroup = 4;
+template <int POS>
 struct EpiK2H {
   static constexpr bool kReduce = true;
+  static constexpr bool kUpdateX = POS == kXGroup - 1;
   Spin<float>* x;
   SpinH* r;
-  const SpinH* p;
-  const SpinH* pprev;  // PAIR: the previous iteration's p
+  const SpinH* ring[kXGroup];  // ring[POS] is this iteration's p
   CGScalars* sc;
-  float alpha, alpha_prev;
+  float alpha;
+  float ap[kXGroup - 1];
   __device__ bool skip() const { return sc->done != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
-    alpha_prev = PAIR ? float(sc->alpha_prev) : 0.f;
+#pragma unroll
+    for (int j = 0; j < kXGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
   }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const {
     Spin<float> rr = ldh(r + i);
-    if constexpr (PAIR) {
-      const Spin<float> pp = ldh(p + i), pq = ldh(pprev + i);
+    if constexpr (kUpdateX) {
       Spin<float> xx = ld(x + i);
-      xx.a.x += alpha_prev * pq.a.x + alpha * pp.a.x;
-      xx.a.y += alpha_prev * pq.a.y + alpha * pp.a.y;
-      xx.b.x += alpha_prev * pq.b.x + alpha * pp.b.x;
-      xx.b.y += alpha_prev * pq.b.y + alpha * pp.b.y;
+#pragma unroll
+      for (int j = 0; j < kXGroup; j++) {
+        const float a = j == kXGroup - 1 ? alpha : ap[j];
+        if (a != 0.f) {  // uniform: a pending term flushed by a reliable update is 0
+          const Spin<float> pp = ldh(ring[j] + i);
+          xx.a.x += a * pp.a.x;
+          xx.a.y += a * pp.a.y;
+          xx.b.x += a * pp.b.x;
+          xx.b.y += a * pp.b.y;
+        }
+      }
       st(x + i, xx);
     }
     rr.a.x -= alpha * q.a.x;
@@ -202,7 +212,11 @@ struct EpiK2H {
   }
   __device__ void operator()(double2 t) const {
     const double m = t.x;
-    sc->alpha_prev = PAIR ? 0.0 : sc->alpha;
+    if constexpr (kUpdateX) {
+      for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
+    } else {
+      sc->alpha_pend[POS] = sc->alpha;
+    }
     sc->iter += 1;
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
@@ -248,7 +262,7 @@ struct EpiResidH {
     sc->need_ru = 0;
     sc->n_ru += 1;
     sc->true_r2 = m;
-    sc->alpha_prev = 0.0;  // the flush before this update took the pending x term
+    for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;  // flushed before this update
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
       return;

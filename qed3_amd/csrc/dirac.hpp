@@ -71,7 +71,8 @@ struct CGScalars {
   double true_r2;  // last true FP64 |b - A x|^2
   double scale;    // half-storage CG: true field = scale * stored field (1 otherwise)
   double rfac;     // half-storage CG: factor from the kept residual to the new stored one
-  double alpha_prev;  // half-storage CG: alpha of an even iteration whose x update is pending
+  double alpha_pend[3];  // half-storage CG: alphas of this group's iterations whose x
+                         // update is pending (0 = none)
   int done;        // 0 running, 1 converged, 2 breakdown / NaN, 3 maxiter
   int need_ru;     // a reliable update is due in this iteration
   int iter, n_ru, maxiter;
@@ -128,7 +129,7 @@ class Dirac {
   // Device work buffer `slot` of at least `bytes`, kept across calls (grown on demand,
   // freed with the context), so repeated solves allocate nothing; contents are undefined.
   // Slots are owned by the CG solvers (cg.hip); calls on one context are sequential.
-  static constexpr int kScratchSlots = 12;
+  static constexpr int kScratchSlots = 14;
   void* scratch(int slot, size_t bytes);
   void release_scratch();
   // pinned host copy of the solver scalars (same lifetime)
