@@ -29,9 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # --m 0.02 --tile <block,ts,remap,lds>` per launch shape the autotuner picks; summary in
 # profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision, lattice, n, Lt, tile).
 MEASURED_TRAFFIC = {("f16", 1, 128, 64, 128, 64, 1, 1): 6.0730e8,
-                    ("f16", 1, 128, 64, 256, 64, 1, 1): 6.1031e8,
+                    ("f16", 1, 128, 64, 256, 64, 1, 1): 6.0736e8,
                     ("f32", 1, 128, 64, 128, 64, 1, 1): 1.3277e9,
-                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.2880e9}
+                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3114e9}
 
 
 def parse():
