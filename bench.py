@@ -27,11 +27,16 @@ HBM_PEAK_GBS = 8000.0
 # Bytes moved per D launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
 # 1 GiB copy kernel) from two rocprofv3 --pmc runs of `bench.py --kernels-only --lattice 1
 # --m 0.02 --tile <block,ts,remap,lds>` per launch shape the autotuner picks; summary in
-# profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision, lattice, n, Lt, tile).
+# profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision or CG kernel,
+# lattice, n, Lt, tile); the CG kernels ("K1-half" ...) from the half-storage solve the same
+# command runs at the pinned shape.
 MEASURED_TRAFFIC = {("f16", 1, 128, 64, 128, 64, 1, 1): 6.0730e8,
                     ("f16", 1, 128, 64, 256, 64, 1, 1): 6.0736e8,
                     ("f32", 1, 128, 64, 128, 64, 1, 1): 1.3277e9,
-                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3114e9}
+                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3114e9,
+                    ("K1-half", 1, 128, 64, 256, 64, 1, 1): 1.0586e9,
+                    ("K2-half", 1, 128, 64, 256, 64, 1, 1): 7.8055e8,
+                    ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 2.1599e9}
 
 
 def parse():
@@ -119,7 +124,7 @@ def main():
     if a.kernels_only:
         if a.tile:
             b_, t_, r_, l_ = (int(v) for v in a.tile.split(","))
-            D.set_tile(b_, t_, r_, l_, ops=(q.OP_D,), precs=(q.F32,))
+            D.set_tile(b_, t_, r_, l_, ops=(q.OP_D, q.OP_DDAG), precs=(q.F32,))
         for prec, nm in ((q.F16, "f16"), (q.F32, "f32"), (q.F64, "f64")):
             for op in (q.OP_D, q.OP_DDAG):
                 ms = D.time_apply(op, prec, warmup=2, reps=10)
@@ -127,6 +132,12 @@ def main():
                 gbs = algorithmic_bytes_D(D.V, D.E, a.Lt, rb) / (ms * 1e-3) / 1e9
                 print(f"# D op={op} {nm}: {ms:.4f} ms  {gbs:.0f} GB/s algorithmic  "
                       f"tile={tile_info(q, D, op, prec)}", file=sys.stderr, flush=True)
+        # one half-storage CG solve at the same shape: its K1 / K2 kernels for the counters
+        d_b = q.DeviceArray.from_numpy(q.random_source(a.lattice, a.n, a.Lt, seed=1000))
+        d_x = q.DeviceArray(D.N * 16)
+        rep, ms = D.solve_kernel_times(d_x, d_b, tol=a.tol, solver=q.CG_HALF)
+        print(f"# half CG: {rep.iterations} its, K1 {ms[0]:.4f} ms, K2 {ms[1]:.4f} ms, "
+              f"K2 with x update {ms[2]:.4f} ms", file=sys.stderr, flush=True)
         gbs, ms = copy_bandwidth(q, 1 << 30, reps=10)
         print(f"# copy 1 GiB: {ms:.4f} ms  {gbs:.0f} GB/s", file=sys.stderr, flush=True)
         return
@@ -207,8 +218,15 @@ def main():
                   "speedup_half_vs_fp64": learned["speedup_half_vs_fp64"],
                   "net_apply": learned["net_apply"], "solution_agreement": learned["agreement"],
                   "second_workload": simp, "hmc": hmc, "batched": batched}
-        roof = learned["roofline"]["f16" if winner == "half" else "f32"]
-        roof = dict(roof, other_precision=learned["roofline"]["f32" if winner == "half" else "f16"])
+        # the dominant kernel of the fastest CG: its K1 (the most time per solve)
+        ck = learned["cg_kernels"]
+        src = winner if winner in ck else "mixed"
+        k1 = ck[src]["K1"]
+        roof = {"bound": "hbm", "achieved": k1["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": k1["frac"], "traffic": k1["traffic"],
+                "kernel": k1["kernel"] + f" (CG K1 of the {src} solver: p = r + beta p_old, w = D p, |w|^2)",
+                "bytes_per_launch": k1["bytes_per_launch"], "ms_per_launch": k1["ms_per_launch"],
+                "tile": ck[src]["tile"], "cg_kernels": ck, "bare_D": learned["roofline"]}
         dtype = ("f32 inner (bf16 net activations) / f64 reliable updates" if winner == "learned"
                  else "f16 storage, f32 arithmetic / f64 reliable updates" if winner == "half"
                  else "f32 inner / f64 reliable updates" if winner == "mixed" else "f64")
@@ -350,6 +368,7 @@ def learned_leg(a, q, g, with_net=True):
     return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG",
             "agreement": agreement,
             "roofline": stencil_roofline(a, q, D),
+            "cg_kernels": cg_kernel_rooflines(a, q, D, d_x, rhs[-1]),
             "sites": D.V, "links": D.E, "methods": methods,
             "training": {"lattice": f"dual n={a.train_n} Lt={a.train_Lt}", "pairs": 3 * 40,
                          "epochs": a.train_epochs, "loss_first": round(hist[0], 4),
@@ -374,6 +393,64 @@ def dual_context(a, q, g, m, lattice):
     D.update(sp, tp)
     D.autotune()
     return D
+
+
+def cg_kernel_bytes(V, E, Lt, solver):
+    """Algorithmic bytes per launch of the CG's two stencil kernels (fields read and written
+    once, one real per link at the inner precision): K1 reads r, p_old and writes p, w;
+    K2 reads w, r (and x, p) and writes r (and x).  Half storage: 4 B per complex,
+    2 B per link, FP32 x read and written by K2 once per four iterations with the four
+    directions of its group."""
+    pts = V * Lt
+    if solver == "half":
+        bv, bl, bx = pts * 2 * 4, (E + V) * Lt * 2, pts * 2 * 8
+        return {"K1": 4 * bv + bl, "K2": 3 * bv + bl, "K2x": 7 * bv + 2 * bx + bl}
+    s_r = 4 if solver == "mixed" else 8
+    bv, bl = pts * 2 * 2 * s_r, (E + V) * Lt * s_r
+    return {"K1": 4 * bv + bl, "K2": 6 * bv + bl, "K2x": 6 * bv + bl}
+
+
+CG_KERNEL_NAMES = {
+    "half": {"K1": "k_stencil_lds<float,NS,false,EpiK1PH,SrcPUpdateH,half2_t>",
+             "K2": "k_stencil_lds<float,NS,true,EpiK2H<0..2>,SrcHalf,half2_t>",
+             "K2x": "k_stencil_lds<float,NS,true,EpiK2H<3>,SrcHalf,half2_t>"},
+    "mixed": {"K1": "k_stencil_lds<float,NS,false,EpiK1P<float>,SrcPUpdate<float>>",
+              "K2": "k_stencil_lds<float,NS,true,EpiK2<float>,SrcPlain<float>>"},
+    "fp64": {"K1": "k_stencil_lds<double,NS,false,EpiK1P<double>,SrcPUpdate<double>>",
+             "K2": "k_stencil_lds<double,NS,true,EpiK2<double>,SrcPlain<double>>"},
+}
+
+
+def cg_kernel_rooflines(a, q, D, d_x, d_b):
+    """The CG's own stencil kernels (fused BLAS-1 epilogues), timed with HIP events around
+    every launch of one solve per solver (direct launches, outside the timed solves),
+    against their algorithmic bytes."""
+    out = {}
+    ns = "6" if D.lattice == 0 else "3"
+    for name, solver in (("half", q.CG_HALF), ("mixed", q.CG_MIXED), ("fp64", q.CG_F64)):
+        if name == "fp64" and a.no_fp64:
+            continue
+        rep, ms = D.solve_kernel_times(d_x, d_b, tol=a.tol, solver=solver)
+        nbytes = cg_kernel_bytes(D.V, D.E, a.Lt, name)
+        prec = q.F64 if name == "fp64" else q.F32
+        tile = tile_info(q, D, q.OP_D, prec)
+        ent = {}
+        for k, t in zip(("K1", "K2", "K2x"), ms):
+            if k not in CG_KERNEL_NAMES[name]:
+                continue
+            gbs = nbytes[k] / (t * 1e-3) / 1e9
+            ent[k] = {"kernel": CG_KERNEL_NAMES[name][k].replace("NS", ns), "ms_per_launch": round(t, 5),
+                      "bytes_per_launch": nbytes[k], "achieved": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "traffic": MEASURED_TRAFFIC.get((f"{k}-{name}", D.lattice, a.n, a.Lt, tile["block"],
+                                                       tile["ts"], tile["xcd_remap"], tile["lds"]))}
+        ent["iterations"] = rep.iterations
+        ent["tile"] = tile
+        out[name] = ent
+        print(f"# {name} CG kernels: " + "  ".join(f"{k} {v['ms_per_launch']:.4f} ms {v['achieved']:.0f} GB/s"
+                                                  for k, v in ent.items() if k.startswith("K")),
+              file=sys.stderr, flush=True)
+    return out
 
 
 def stencil_roofline(a, q, D):
@@ -419,7 +496,8 @@ def plain_leg(a, q, g, m, lattice):
                      "half_vs_fp64": float(np.linalg.norm(x_half - x64) / np.linalg.norm(x64))}
     fp64 = methods.get("fp64")
     return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG", "agreement": agreement,
-            "roofline": stencil_roofline(a, q, D), "sites": D.V, "links": D.E,
+            "roofline": stencil_roofline(a, q, D), "cg_kernels": cg_kernel_rooflines(a, q, D, d_x, rhs[-1]),
+            "sites": D.V, "links": D.E,
             "methods": methods, "training": None, "speedup_vs_fp64": None,
             "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
                                       if fp64 else None),

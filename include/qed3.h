@@ -151,6 +151,15 @@ typedef struct qed3_cg_report {
  * from zero, as in the reference. */
 int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
             qed3_cg_report* rep);
+/* The same solve with direct launches (no graph) and HIP events around the two stencil
+ * kernels of every iteration, for the roofline of the CG's own kernels (no counterpart in
+ * the reference).  ms[0] = mean K1 (D with the search-direction update fused), ms[1] =
+ * mean K2 (D^dag with the residual update fused; for QED3_SOLVER_CG_HALF the iterations
+ * whose x term stays pending), ms[2] = mean K2 that also updates x (half-storage solver,
+ * once per four iterations; = ms[1] for the other solvers); means over the iterations run,
+ * each kernel's reduction finaliser excluded. */
+int qed3_cg_kernel_times(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+                         qed3_cg_report* rep, double* ms);
 
 /* ---- learned preconditioner --------------------------------------------------------
  * No counterpart in the reference (it has no ML part).  A gauge-covariant linear

@@ -236,6 +236,16 @@ class WilsonDirac:
             check(code)
         return rep
 
+    def solve_kernel_times(self, d_x, d_b, tol=1e-10, maxiter=100000, solver=CG_HALF,
+                           delta=0.01):
+        """One solve with HIP-event timing of its stencil kernels: (report, [K1 ms,
+        K2 ms, K2-with-x-update ms]) averaged over the iterations run."""
+        opts = CGOpts(tol, maxiter, solver, delta, 0)
+        rep = CGReport()
+        ms = (C.c_double * 3)()
+        check(load().qed3_cg_kernel_times(self._h, d_x.ptr, d_b.ptr, C.byref(opts), C.byref(rep), ms))
+        return rep, [ms[0], ms[1], ms[2]]
+
     def solve(self, b: np.ndarray, **kw):
         """x with D^dag D x = b (host vectors); returns (x, report)."""
         b = np.ascontiguousarray(b, np.complex128)

@@ -384,8 +384,9 @@ int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
   });
 }
 
-int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
-            qed3_cg_report* rep) {
+namespace {
+int cg_entry(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+             qed3_cg_report* rep, qed3::KernelTimes* times) {
   if (!ctx || !d_x || !d_b || !opts) return fail(QED3_ERR_ARG, "null argument");
   if (d_x == d_b) return fail(QED3_ERR_ARG, "x and b must differ");
   if (!(opts->tol > 0) || opts->maxiter < 1) return fail(QED3_ERR_ARG, "bad tolerance/maxiter");
@@ -401,6 +402,7 @@ int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
     o.half = opts->solver == QED3_SOLVER_CG_HALF;
     o.delta = opts->delta > 0 ? opts->delta : 0.1;
     o.chunk = opts->check_every;
+    o.times = times;
     const qed3::CGResult r = qed3::cg_solve(*ctx->D, static_cast<qed3::Spin<double>*>(d_x),
                                             static_cast<const qed3::Spin<double>*>(d_b), o);
     if (rep) {
@@ -414,6 +416,21 @@ int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
     if (r.status != 0) return fail(QED3_ERR_NOCONV, "CG did not reach the tolerance");
     return QED3_OK;
   });
+}
+}  // namespace
+
+int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+            qed3_cg_report* rep) {
+  return cg_entry(ctx, d_x, d_b, opts, rep, nullptr);
+}
+
+int qed3_cg_kernel_times(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+                         qed3_cg_report* rep, double* ms) {
+  if (!ms) return fail(QED3_ERR_ARG, "null argument");
+  qed3::KernelTimes t;
+  const int rc = cg_entry(ctx, d_x, d_b, opts, rep, &t);
+  for (int k = 0; k < 3; k++) ms[k] = t.ms[k];
+  return rc;
 }
 
 int qed3_net_create(qed3_ctx* ctx, int layers, int seed, qed3_net** out) {

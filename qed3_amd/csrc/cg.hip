@@ -1,5 +1,6 @@
 // Device-driven FP64 and mixed FP32/FP64 CG on D^dagger D (see cg.hpp).
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -221,8 +222,43 @@ struct Work {
 
 size_t stream_blocks(size_t n) { return std::min<size_t>((n + 255) / 256, 8192); }
 
-// Runs one_iter(i), i = 0 .. chunk-1 (the iteration's position in the chunk), in chunks
-// of iterations, captured once as a HIP graph unless
+// Event pairs for KernelTimes: pair(k) hands out a fresh (start, end) pair tagged with
+// kernel class k; finish() averages the pairs of the first `iters` iterations.
+class EventTimer {
+ public:
+  explicit EventTimer(KernelTimes* t) : t_(t) {}
+  ~EventTimer() {
+    for (auto& e : ev_) (void)hipEventDestroy(e.first[0]), (void)hipEventDestroy(e.first[1]);
+  }
+  const hipEvent_t* pair(int cls, int iter) {
+    if (!t_) return nullptr;
+    ev_.emplace_back();
+    auto& e = ev_.back();
+    QED3_HIP(hipEventCreate(&e.first[0]));
+    QED3_HIP(hipEventCreate(&e.first[1]));
+    e.second = {cls, iter};
+    return e.first.data();
+  }
+  void finish(int iters) {
+    if (!t_) return;
+    for (auto& e : ev_) {
+      if (e.second.second >= iters) continue;  // launched after convergence: early exit
+      float ms = 0;
+      QED3_HIP(hipEventElapsedTime(&ms, e.first[0], e.first[1]));
+      t_->ms[e.second.first] += ms;
+      t_->count[e.second.first] += 1;
+    }
+    for (int k = 0; k < 3; k++)
+      if (t_->count[k]) t_->ms[k] /= t_->count[k];
+  }
+
+ private:
+  KernelTimes* t_;
+  std::vector<std::pair<std::array<hipEvent_t, 2>, std::pair<int, int>>> ev_;
+};
+
+// Runs one_iter(i, k), i = 0 .. chunk-1 (the iteration's position in the chunk), k its
+// index in the solve (-1 while a graph is captured), in chunks of iterations, captured once as a HIP graph unless
 // disabled, with one host poll of the device scalars per chunk, until the device marks
 // the solve done.  On return hsc holds the final scalars.
 template <class Iter>
@@ -233,9 +269,9 @@ void drive_chunks(const Iter& one_iter, const CGOptions& o, size_t n, hipStream_
                                                       // (and an even parity)
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  if (o.use_graph && std::getenv("QED3_NO_GRAPH") == nullptr) {
+  if (o.use_graph && !o.times && std::getenv("QED3_NO_GRAPH") == nullptr) {
     QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; i++) one_iter(i);
+    for (int i = 0; i < chunk; i++) one_iter(i, -1);
     QED3_HIP(hipStreamEndCapture(st, &graph));
     QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
@@ -244,7 +280,7 @@ void drive_chunks(const Iter& one_iter, const CGOptions& o, size_t n, hipStream_
     if (exec)
       QED3_HIP(hipGraphLaunch(exec, st));
     else
-      for (int i = 0; i < chunk; i++) one_iter(i);
+      for (int i = 0; i < chunk; i++) one_iter(i, c * chunk + i);
     QED3_HIP(hipGetLastError());
     QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
     QED3_HIP(hipStreamSynchronize(st));
@@ -294,7 +330,8 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   // the other buffer and applies D to it; K2 reads the new p.  p_old is read at halo points
   // of other blocks while the new p is written, hence two buffers alternating by parity.
   Spin<T>* P[2] = {W.p, W.p2};
-  auto one_iter = [&](int i) {
+  EventTimer timer(o.times);
+  auto one_iter = [&](int i, int k) {
     const int par = i & 1;
     Spin<T>* pold = P[par];
     Spin<T>* pnew = P[par ^ 1];
@@ -302,8 +339,10 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
     k1.w = W.w;
     k1.sc = sc;
     k1.pnew = pnew;
-    launch_stencil_src<T, false>(vin, nn, SrcPUpdate<T>{W.r, pold, sc}, k1, D.tile<T>(false), gr, st, hvf);
-    launch_stencil<T, true>(vin, nn, W.w, EpiK2<T>{W.xin, W.r, pnew, sc, T(0)}, D.tile<T>(true), gr, st, hvt);
+    launch_stencil_src<T, false>(vin, nn, SrcPUpdate<T>{W.r, pold, sc}, k1, D.tile<T>(false), gr, st, hvf,
+                                 timer.pair(0, k));
+    launch_stencil_src<T, true>(vin, nn, SrcPlain<T>{W.w}, EpiK2<T>{W.xin, W.r, pnew, sc, T(0)},
+                                D.tile<T>(true), gr, st, hvt, timer.pair(1, k));
     if (kMixed)
       hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 0);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{W.t, sc}, D.tile<double>(false), gr, st, hv64f);
@@ -311,6 +350,8 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   };
 
   drive_chunks(one_iter, o, n, st, sc, hsc);
+  timer.finish(hsc->iter);
+  if (o.times) o.times->ms[2] = o.times->ms[1], o.times->count[2] = o.times->count[1];
   if (kMixed) hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 1);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
@@ -359,27 +400,29 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   const double inv_ncomp = 1.0 / (4.0 * double(n));
   PendRing pend;
   for (int j = 0; j < kXGroup - 1; j++) pend.p[j] = ring[j];
-  auto k2 = [&](auto tag) {
+  EventTimer timer(o.times);
+  auto k2 = [&](auto tag, int k) {
     constexpr int POS = decltype(tag)::value;
     EpiK2H<POS> e{};
     e.x = xin;
     e.r = r;
     for (int j = 0; j < kXGroup; j++) e.ring[j] = ring[j];
     e.sc = sc;
-    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt);
+    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
+                                    timer.pair(EpiK2H<POS>::kUpdateX ? 2 : 1, k));
   };
-  auto one_iter = [&](int i) {
+  auto one_iter = [&](int i, int k) {
     const int pos = i % kXGroup;
     SpinH* pold = ring[(pos + kXGroup - 1) % kXGroup];
     SpinH* pnew = ring[pos];
     launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{r, pold, sc}, EpiK1PH{w, pnew, sc},
-                                     D.tile<float>(false), gr, st, hvf);
+                                     D.tile<float>(false), gr, st, hvf, timer.pair(0, k));
     static_assert(kXGroup == 4, "one K2 instantiation per group position");
     switch (pos) {
-      case 0: k2(std::integral_constant<int, 0>{}); break;
-      case 1: k2(std::integral_constant<int, 1>{}); break;
-      case 2: k2(std::integral_constant<int, 2>{}); break;
-      default: k2(std::integral_constant<int, 3>{}); break;
+      case 0: k2(std::integral_constant<int, 0>{}, k); break;
+      case 1: k2(std::integral_constant<int, 1>{}, k); break;
+      case 2: k2(std::integral_constant<int, 2>{}, k); break;
+      default: k2(std::integral_constant<int, 3>{}, k); break;
     }
     hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 0);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
@@ -388,6 +431,7 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
     hipLaunchKernelGGL(k_rescale, dim3(nbk), dim3(256), 0, st, rt, r, n, sc);
   };
   drive_chunks(one_iter, o, n, st, sc, hsc);
+  timer.finish(hsc->iter);
   hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));

@@ -25,6 +25,17 @@
 
 namespace qed3 {
 
+// Per-kernel timing of the two stencil kernels of every iteration (HIP event pairs on the
+// solver's stream, direct launches instead of graphs).  After the solve, ms[k] is the mean
+// duration of kernel class k over the iterations actually run: 0 = K1 (D with the p
+// update), 1 = K2 (D^dag with the r update; for the half-storage solver only the
+// iterations that leave their x term pending), 2 = K2 with the x update (half-storage
+// solver; = class 1 for the others).
+struct KernelTimes {
+  double ms[3] = {0, 0, 0};
+  int count[3] = {0, 0, 0};
+};
+
 struct CGOptions {
   double tol = 1e-10;
   int maxiter = 100000;
@@ -35,6 +46,7 @@ struct CGOptions {
   bool use_graph = true;  // replay chunks of iterations as a HIP graph (QED3_NO_GRAPH=1 in the
                           // environment launches the same kernels directly, e.g. for profilers)
   int fcg_m = 4;  // directions kept by flexible CG
+  KernelTimes* times = nullptr;  // when set: direct launches, K1/K2 timed with HIP events
 };
 
 struct CGResult {

@@ -369,9 +369,12 @@ __device__ __forceinline__ double norm2(const Spin<T>& v) {
          double(v.b.y) * v.b.y;
 }
 
+// `bracket` (optional, a pair of events) is recorded just before and just after the
+// stencil kernel itself (not its reduction finaliser), for per-kernel timing.
 template <typename T, bool DAG, class Epi, class Src, typename P>
 inline void launch_stencil_src(const DiracView<T, P>& v, int max_nn, const Src& in, const Epi& epi,
-                               StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv) {
+                               StencilTile t, GridReduce gr, hipStream_t st, const HaloView& hv,
+                               const hipEvent_t* bracket = nullptr) {
   TileMap tm;
   tm.nxr = (v.V + t.block - 1) / t.block;
   tm.nxr_pad = (tm.nxr + 7) / 8 * 8;
@@ -381,6 +384,7 @@ inline void launch_stencil_src(const DiracView<T, P>& v, int max_nn, const Src& 
   if (Epi::kReduce && nblk > kReduceCapacity)
     throw std::invalid_argument("stencil grid exceeds the reduction scratch");
   const dim3 grid{static_cast<unsigned>(nblk), 1u, 1u};
+  if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
   if (t.lds) {
     if (!hv.nbl) throw std::logic_error("LDS stencil launched without halo tables");
     const size_t smem = size_t(t.block + hv.hmax) * sizeof(Spin<T>);
@@ -395,6 +399,7 @@ inline void launch_stencil_src(const DiracView<T, P>& v, int max_nn, const Src& 
   } else {
     hipLaunchKernelGGL((k_stencil<T, 3, DAG, Epi, Src, P>), grid, dim3(t.block), 0, st, v, in, epi, t.ts, tm, gr);
   }
+  if (bracket) QED3_HIP(hipEventRecord(bracket[1], st));
   if constexpr (Epi::kReduce)
     hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, unsigned(nblk), epi);
 }

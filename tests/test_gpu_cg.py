@@ -171,3 +171,22 @@ def test_autotuned_shapes_keep_parity(gpu):
     for solver in (q.CG_F64, q.CG_MIXED, q.CG_HALF):
         x, rep = Dq.solve(b, tol=1e-12, solver=solver)
         assert rep.status == 0 and rel(x, x_ref) < 1e-10, (solver, rel(x, x_ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", [q.CG_F64, q.CG_MIXED, q.CG_HALF], ids=["f64", "mixed", "half"])
+def test_kernel_timed_solve_matches(gpu, solver):
+    """The event-timed solve (direct launches) gives the same solution and iteration count
+    as the graph-replayed one, and positive per-kernel times."""
+    lattice, n, Lt, M5 = 0, 4, 8, 0.05
+    Do, Dq = _pair(lattice, n, Lt, M5)
+    b = source(lattice, n, Lt, seed=81)
+    d_b = q.DeviceArray.from_numpy(b)
+    d_x1, d_x2 = q.DeviceArray(b.nbytes), q.DeviceArray(b.nbytes)
+    r1 = Dq.solve_device(d_x1, d_b, tol=1e-12, solver=solver)
+    r2, ms = Dq.solve_kernel_times(d_x2, d_b, tol=1e-12, solver=solver)
+    assert r1.status == 0 and r2.status == 0 and r1.iterations == r2.iterations
+    assert rel(d_x2.to_numpy(np.complex128), d_x1.to_numpy(np.complex128)) < 1e-14
+    assert all(t > 0 for t in ms), ms
+    if solver != q.CG_HALF:
+        assert ms[2] == ms[1]
