@@ -203,7 +203,9 @@ int qed3_fcg(qed3_ctx* ctx, qed3_net* net, void* d_x, const void* d_b, const qed
  * Solves D_k^dag D_k x_k = b_k for nsys contexts (any mix of lattices, refinements, Lt and
  * masses, all on the current device) together: every CG phase is one launch over all
  * systems, per-system scalars and convergence masks, iterations captured in a hipGraph;
- * FP32 inner iteration with FP64 reliable updates (opts->solver is ignored).  The
+ * binary16-storage inner iteration (as QED3_SOLVER_CG_HALF) with FP64 reliable updates and
+ * the true-residual stop (opts->solver is ignored); time tiles from each context's
+ * autotuned FP32 / FP64 shapes, 256-thread LDS-staged blocks.  The
  * reference runs one CG per right-hand side (PseudoFermion::update_eta,
  * pseudofermion.h:51, looped in integrator.h:156-166).  reps[nsys] may be NULL. */
 int qed3_batch_cg(qed3_ctx* const* ctxs, int nsys, void* const* d_x, const void* const* d_b,

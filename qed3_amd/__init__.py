@@ -300,7 +300,7 @@ class WilsonDirac:
 
 def batch_solve(ops, xs, bs, tol=1e-10, maxiter=100000, delta=0.01, check_every=0,
                 raise_on_fail=True):
-    """Batched mixed CG over WilsonDirac systems of any shapes; xs/bs DeviceArrays."""
+    """Batched half-storage CG over WilsonDirac systems of any shapes; xs/bs DeviceArrays."""
     n = len(ops)
     ctxs = (C.c_void_p * n)(*[o._h for o in ops])
     dx = (C.c_void_p * n)(*[x.ptr for x in xs])

@@ -104,34 +104,11 @@ __global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__
     acc.x += norm2(bb);
   }
   const double ncomp = 4.0 * double(n);
-  grid_reduce(acc, gr, [=](double2 t) {
-    const double s = t.x > 0.0 ? sqrt(t.x / ncomp) : 1.0;
-    sc->scale = s;
-    sc->mu = t.x / (s * s);
-    sc->bnorm2 = t.x;
-    sc->maxr2 = sc->mu;
-    sc->true_r2 = t.x;
-    sc->rfac = 1.0;
-    for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
-    sc->rescale = 0;
-    sc->gam = 0;
-    sc->alpha = 0;
-    sc->beta = 0;
-    sc->tol2 = tol2;
-    sc->delta2 = delta2;
-    sc->iter = 0;
-    sc->n_ru = 0;
-    sc->maxiter = maxiter;
-    sc->need_ru = 0;
-    sc->done = (t.x == 0.0) ? 1 : (t.x < 1e300 ? 0 : 2);
-  });
+  grid_reduce(acc, gr, [=](double2 t) { half_start(sc, t.x, ncomp, tol2, delta2, maxiter); });
 }
 
 // Half-storage flush: x64 += scale (x_inner + sum_j alpha_pend[j] ring[j]), x_inner = 0,
 // taking the x terms the current group's iterations left pending (alpha_pend[j] = 0 if none)
-struct PendRing {
-  const SpinH* p[kXGroup - 1];
-};
 __global__ __launch_bounds__(256) void k_flush_h(Spin<double>* x64, Spin<float>* xin, PendRing ring,
                                                  size_t n, const CGScalars* sc, int always) {
   if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
@@ -139,29 +116,8 @@ __global__ __launch_bounds__(256) void k_flush_h(Spin<double>* x64, Spin<float>*
   float ap[kXGroup - 1];
   for (int j = 0; j < kXGroup - 1; j++) ap[j] = float(sc->alpha_pend[j]);
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    Spin<float> a = xin[i];
-#pragma unroll
-    for (int j = 0; j < kXGroup - 1; j++) {
-      if (ap[j] != 0.f) {
-        const Spin<float> pp = ldh(ring.p[j] + i);
-        a.a.x += ap[j] * pp.a.x;
-        a.a.y += ap[j] * pp.a.y;
-        a.b.x += ap[j] * pp.b.x;
-        a.b.y += ap[j] * pp.b.y;
-      }
-    }
-    Spin<double> xx = x64[i];
-    xx.a.x += s * a.a.x;
-    xx.a.y += s * a.a.y;
-    xx.b.x += s * a.b.x;
-    xx.b.y += s * a.b.y;
-    x64[i] = xx;
-    Spin<float> z;
-    z.a = {0.f, 0.f};
-    z.b = {0.f, 0.f};
-    xin[i] = z;
-  }
+       i += size_t(gridDim.x) * blockDim.x)
+    half_flush_point(x64, xin, ring, ap, s, i);
 }
 
 // r = b / scale in half (FP64 arithmetic, so any finite b is representable)
@@ -170,13 +126,8 @@ __global__ __launch_bounds__(256) void k_store_b0(const Spin<double>* __restrict
   if (sc->done != 0) return;
   const double inv = 1.0 / sc->scale;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    const Spin<double> bb = b[i];
-    Spin<float> v;
-    v.a = {float(bb.a.x * inv), float(bb.a.y * inv)};
-    v.b = {float(bb.b.x * inv), float(bb.b.y * inv)};
-    r[i] = to_half(v);
-  }
+       i += size_t(gridDim.x) * blockDim.x)
+    half_store_b(b, r, inv, i);
 }
 
 // r = rfac * rt in half after a reliable update (rt holds the true residual at the
@@ -186,14 +137,8 @@ __global__ __launch_bounds__(256) void k_rescale(const Spin<float>* __restrict__
   if (sc->done != 0 || sc->rescale == 0) return;
   const float inv = float(sc->rfac);
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    Spin<float> v = rt[i];
-    v.a.x *= inv;
-    v.a.y *= inv;
-    v.b.x *= inv;
-    v.b.y *= inv;
-    r[i] = to_half(v);
-  }
+       i += size_t(gridDim.x) * blockDim.x)
+    half_rescale_point(rt, r, inv, i);
 }
 
 __global__ __launch_bounds__(256) void k_resid_norms(const Spin<double>* __restrict__ b,

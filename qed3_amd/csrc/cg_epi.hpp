@@ -282,5 +282,83 @@ struct EpiResidH {
   }
 };
 
+// ---- per-point and scalar steps of the half-storage solver, shared by the single-system
+// and the batched kernels ----------------------------------------------------------------
+
+// solver scalars at the start from |b|^2 (ncomp = 4 * points): the scale makes the stored
+// residual unit-rms per component
+__device__ __forceinline__ void half_start(CGScalars* sc, double b2, double ncomp, double tol2,
+                                           double delta2, int maxiter) {
+  const double s = b2 > 0.0 ? sqrt(b2 / ncomp) : 1.0;
+  sc->scale = s;
+  sc->mu = b2 / (s * s);
+  sc->bnorm2 = b2;
+  sc->maxr2 = sc->mu;
+  sc->true_r2 = b2;
+  sc->rfac = 1.0;
+  for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
+  sc->rescale = 0;
+  sc->gam = 0;
+  sc->alpha = 0;
+  sc->beta = 0;
+  sc->tol2 = tol2;
+  sc->delta2 = delta2;
+  sc->iter = 0;
+  sc->n_ru = 0;
+  sc->maxiter = maxiter;
+  sc->need_ru = 0;
+  sc->done = (b2 == 0.0) ? 1 : (b2 < 1e300 ? 0 : 2);
+}
+
+// the pending directions of the current group (ring slots 0 .. kXGroup-2)
+struct PendRing {
+  const SpinH* p[kXGroup - 1];
+};
+
+// x64 += scale (x_inner + sum_j alpha_pend[j] ring[j]) at point i, x_inner = 0
+__device__ __forceinline__ void half_flush_point(Spin<double>* x64, Spin<float>* xin, const PendRing& ring,
+                                                 const float* ap, double s, size_t i) {
+  Spin<float> a = xin[i];
+#pragma unroll
+  for (int j = 0; j < kXGroup - 1; j++) {
+    if (ap[j] != 0.f) {  // uniform
+      const Spin<float> pp = ldh(ring.p[j] + i);
+      a.a.x += ap[j] * pp.a.x;
+      a.a.y += ap[j] * pp.a.y;
+      a.b.x += ap[j] * pp.b.x;
+      a.b.y += ap[j] * pp.b.y;
+    }
+  }
+  Spin<double> xx = x64[i];
+  xx.a.x += s * a.a.x;
+  xx.a.y += s * a.a.y;
+  xx.b.x += s * a.b.x;
+  xx.b.y += s * a.b.y;
+  x64[i] = xx;
+  Spin<float> z;
+  z.a = {0.f, 0.f};
+  z.b = {0.f, 0.f};
+  xin[i] = z;
+}
+
+// r = b / scale in binary16 at point i (FP64 arithmetic, so any finite b fits)
+__device__ __forceinline__ void half_store_b(const Spin<double>* b, SpinH* r, double inv, size_t i) {
+  const Spin<double> bb = b[i];
+  Spin<float> v;
+  v.a = {float(bb.a.x * inv), float(bb.a.y * inv)};
+  v.b = {float(bb.b.x * inv), float(bb.b.y * inv)};
+  r[i] = to_half(v);
+}
+
+// r = rfac * rt in binary16 at point i (after a reliable update)
+__device__ __forceinline__ void half_rescale_point(const Spin<float>* rt, SpinH* r, float f, size_t i) {
+  Spin<float> v = rt[i];
+  v.a.x *= f;
+  v.a.y *= f;
+  v.b.x *= f;
+  v.b.y *= f;
+  r[i] = to_half(v);
+}
+
 }  // namespace cgepi
 }  // namespace qed3

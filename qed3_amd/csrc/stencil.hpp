@@ -222,18 +222,16 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T, P> d, Src in, Epi 
 // (the outside neighbours of the range) in LDS with one coalesced load of the range
 // and one gather of the halo, then reads all spatial neighbours from LDS through a
 // 16-bit local index table.  Every spinor is fetched once per block and slice.
-template <typename T, int NS, bool DAG, class Epi, class Src, typename P = cplx_t<T>>
-__global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src, Epi epi, int ts, TileMap tm,
-                                                      HaloView hv, GridReduce gr) {
-  if (epi.skip()) return;
-  const auto in = src.bind();
-  extern __shared__ unsigned char smem_raw[];
-  Spin<T>* tileS = reinterpret_cast<Spin<T>*>(smem_raw);
+// The work of one block: site range xr (of nxr; ranges beyond are padding) and time tile
+// st of ts slices, staging through the LDS tile tileS ((blockDim.x + hv.hmax) spinors).
+// Shared by the single-system kernel below and the batched solver.
+template <typename T, int NS, bool DAG, class Epi, class In, typename P>
+__device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, const In& in, const Epi& epi,
+                                                  int ts, int xr, int nxr, int st, const HaloView& hv,
+                                                  Spin<T>* tileS, double2& acc) {
   using C = cplx_t<T>;
   const int V = d.V, Lt = d.Lt, R = blockDim.x;
-  int xr, st;
-  tile_of(tm, blockIdx.x, xr, st);
-  const bool live = xr < tm.nxr;
+  const bool live = xr < nxr;
   const int x0 = xr * R;
   const int x = live ? x0 + threadIdx.x : V;
   const bool act = x < V;
@@ -246,8 +244,6 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src,
   // halo gathers do not wait on an index load; any further entries are read in the loop.
   const int hid0 = (int(threadIdx.x) < nh) ? hv.halo[h0 + threadIdx.x] : -1;
   const int hid1 = (int(threadIdx.x) + R < nh) ? hv.halo[h0 + threadIdx.x + R] : -1;
-  double2 acc = {0.0, 0.0};
-  epi.begin();
   int nb[NS], lk[NS];
   real4_t<T> hp[NS];
   T c = 0, bk0 = 0, bk1 = 0, fw0 = 0, fw1 = 0;
@@ -339,6 +335,20 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src,
       uprev = ucur;
     }
   }
+}
+
+template <typename T, int NS, bool DAG, class Epi, class Src, typename P = cplx_t<T>>
+__global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src, Epi epi, int ts, TileMap tm,
+                                                      HaloView hv, GridReduce gr) {
+  if (epi.skip()) return;
+  const auto in = src.bind();
+  extern __shared__ unsigned char smem_raw[];
+  Spin<T>* tileS = reinterpret_cast<Spin<T>*>(smem_raw);
+  int xr, st;
+  tile_of(tm, blockIdx.x, xr, st);
+  double2 acc = {0.0, 0.0};
+  epi.begin();
+  stencil_lds_block<T, NS, DAG>(d, in, epi, ts, xr, tm.nxr, st, hv, tileS, acc);
   if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
 }
 
