@@ -290,6 +290,16 @@ int qed3_overlap_create(qed3_ctx* ctx, int n, double lambda_min, double lambda_m
                         qed3_overlap** out);
 void qed3_overlap_destroy(qed3_overlap* ov);
 int qed3_overlap_apply(qed3_overlap* ov, int op, void* d_out, const void* d_in);
+/* Overlap pseudofermion force for every link, f_l = -2 Re <eta, dD_ov/dtheta_l eta> with
+ * lambda_max and the Zolotarev coefficients held fixed; d_fs [Lt][E], d_ft [Lt][V] FP64
+ * device arrays.  Replaces Overlap::precalc_grad_deviceAsyncLaunch + the per-link
+ * grad_deviceAsyncLaunch loop of GaugeExt::compute (overlap.h:393-465, gauge_ext.h:204-222). */
+int qed3_overlap_force(qed3_overlap* ov, const void* d_eta, double* d_fs, double* d_ft);
+/* x = (D_ov + D_ov^dag)^{-1} b by FP64 CG from x = 0 (stop: recursive |r| <= tol |b|);
+ * rep gets the iterations and the true residual.  Replaces PseudoFermion<Overlap>::update_eta
+ * (MatPoly::solve on Overlap::DHD_deviceAsyncLaunch, pseudofermion.h:51, overlap.h:364-377). */
+int qed3_overlap_solve(qed3_overlap* ov, void* d_x, const void* d_b, double tol, int maxiter,
+                       qed3_cg_report* rep);
 
 /* ---- checkpoint and resume (hmc.cu:423-447 scans ckpoint_lat.k / ckpoint_rng.k) ------
  * Gauge file: raw native-endian FP64, spatial[Lt][E] then temporal[Lt][V], nothing else

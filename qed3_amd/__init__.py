@@ -571,3 +571,25 @@ class Overlap:
         d_out = DeviceArray(v.nbytes)
         check(load().qed3_overlap_apply(self._h, OP_DDAG if dag else OP_D, d_out.ptr, d_in.ptr))
         return d_out.to_numpy(np.complex128)
+
+    def force(self, eta):
+        """Overlap pseudofermion force -2 Re <eta, dD_ov/dtheta eta> (spatial[Lt][E],
+        temporal[Lt][V]) with lambda_max and the Zolotarev coefficients fixed
+        (Overlap::grad_deviceAsyncLaunch, overlap.h:393-465)."""
+        D = self.D
+        d_eta = DeviceArray.from_numpy(np.ascontiguousarray(eta, np.complex128))
+        d_fs = DeviceArray(8 * D.Lt * D.E)
+        d_ft = DeviceArray(8 * D.Lt * D.V)
+        check(load().qed3_overlap_force(self._h, d_eta.ptr, d_fs.ptr, d_ft.ptr))
+        return (d_fs.to_numpy(np.float64).reshape(D.Lt, D.E),
+                d_ft.to_numpy(np.float64).reshape(D.Lt, D.V))
+
+    def solve(self, b, tol=1e-10, maxiter=10000):
+        """x = (D_ov + D_ov^dag)^{-1} b by FP64 CG (PseudoFermion<Overlap>::update_eta);
+        returns (x, report)."""
+        b = np.ascontiguousarray(b, np.complex128)
+        d_b = DeviceArray.from_numpy(b)
+        d_x = DeviceArray(b.nbytes)
+        rep = CGReport()
+        check(load().qed3_overlap_solve(self._h, d_x.ptr, d_b.ptr, tol, maxiter, C.byref(rep)))
+        return d_x.to_numpy(np.complex128), rep

@@ -106,6 +106,9 @@ SIGNATURES = {
                                       C.POINTER(C.c_void_p)]),
     "qed3_overlap_destroy": (None, [C.c_void_p]),
     "qed3_overlap_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "qed3_overlap_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qed3_overlap_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                     C.c_void_p]),
     "qed3_gauge_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "qed3_gauge_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "qed3_rng_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),

@@ -767,6 +767,34 @@ int qed3_overlap_apply(qed3_overlap* ov, int op, void* d_out, const void* d_in) 
   });
 }
 
+int qed3_overlap_force(qed3_overlap* ov, const void* d_eta, double* d_fs, double* d_ft) {
+  if (!ov || !d_eta || !d_fs || !d_ft || d_fs == d_ft) return fail(QED3_ERR_ARG, "bad argument");
+  if (!ov->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    ov->ov->force(static_cast<const qed3::Spin<double>*>(d_eta), d_fs, d_ft);
+    return QED3_OK;
+  });
+}
+
+int qed3_overlap_solve(qed3_overlap* ov, void* d_x, const void* d_b, double tol, int maxiter,
+                       qed3_cg_report* rep) {
+  if (!ov || !d_x || !d_b || d_x == d_b || !(tol > 0) || maxiter < 0) return fail(QED3_ERR_ARG, "bad argument");
+  if (!ov->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    const qed3::CGResult r = ov->ov->solve_dhd(static_cast<qed3::Spin<double>*>(d_x),
+                                               static_cast<const qed3::Spin<double>*>(d_b), tol, maxiter);
+    if (rep) {
+      rep->status = r.status == 0 ? QED3_OK : QED3_ERR_NOCONV;
+      rep->iterations = r.iterations;
+      rep->reliable_updates = 0;
+      rep->true_resid = r.true_resid;
+      rep->seconds = r.seconds;
+    }
+    if (r.status != 0) return fail(QED3_ERR_NOCONV, "overlap CG did not reach the tolerance");
+    return QED3_OK;
+  });
+}
+
 int qed3_gauge_write(const char* path, int Lt, int E, int V, const double* spatial,
                      const double* temporal) {
   if (!path || !spatial || !temporal || Lt < 1 || E < 1 || V < 1) return fail(QED3_ERR_ARG, "bad argument");

@@ -48,16 +48,14 @@ __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
   grid_reduce(acc, gr, [out](double2 t) { *out = t; });
 }
 
-// Pseudofermion force, one thread per spatial link (s,l) followed by one per temporal
-// link (s,x):  F = -2 Re <w, dD/dtheta eta> with w = D eta  (dirac_pf.h:107-121,
-// with the two derivative blocks of dirac_ext.h:233-269 and :272-302).
-__global__ __launch_bounds__(256) void k_force(DiracView<double> d, const int4* __restrict__ ends,
-                                               const Spin<double>* __restrict__ eta,
-                                               const Spin<double>* __restrict__ w,
-                                               double* __restrict__ fs, double* __restrict__ ft) {
+// Re <w, dD/dtheta_i eta> for link i: spatial links (s,l) first, then temporal links (s,x)
+// (the two derivative blocks of dirac_ext.h:233-269 for a spatial link, :272-302 for a
+// temporal one).  i < Lt (E + V).
+__device__ __forceinline__ double link_bilinear(const DiracView<double>& d, const int4* __restrict__ ends,
+                                                size_t i, const Spin<double>* __restrict__ w,
+                                                const Spin<double>* __restrict__ eta) {
   using C = double2;
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t nS = size_t(d.Lt) * d.E, nT = size_t(d.Lt) * d.V;
+  const size_t nS = size_t(d.Lt) * d.E;
   const double r = d.r;
   if (i < nS) {
     const int s = int(i / d.E), l = int(i % d.E);
@@ -81,30 +79,60 @@ __global__ __launch_bounds__(256) void k_force(DiracView<double> d, const int4* 
     // Re <w, i v> = -Im <w, v>
     const C t1 = cadd<double>(cconjmul<double>(wx.a, a0), cconjmul<double>(wx.b, a1));
     const C t2 = cadd<double>(cconjmul<double>(wy.a, b0), cconjmul<double>(wy.b, b1));
-    const double re = -t1.y + t2.y;
-    fs[i] = -2.0 * re;
-  } else if (i < nS + nT) {
-    const size_t k = i - nS;
-    const int s = int(k / d.V), x = int(k % d.V);
-    const int sp = (s == d.Lt - 1) ? 0 : s + 1;
-    const double sg = (s == d.Lt - 1) ? double(d.bc) : 1.0;
-    const double kh = d.kth[x];
-    const double m0 = kh * (-r - 1.0), m1 = kh * (-r + 1.0);  // T- = diag(m0,m1), T+ = diag(m1,m0)
-    const C u = d.uT[k];
-    const size_t o0 = size_t(s) * d.V + x, o1 = size_t(sp) * d.V + x;
-    const Spin<double> e0 = eta[o0], e1 = eta[o1], w0 = w[o0], w1 = w[o1];
-    // dD[s+1 <- s] = -i sg T- e^{-i theta};  dD[s <- s+1] = +i sg T+ e^{+i theta}
-    const C uc = {u.x, -u.y};
-    const C p0 = cmul<double>(uc, e0.a), p1 = cmul<double>(uc, e0.b);
-    const C q0 = cmul<double>(u, e1.a), q1 = cmul<double>(u, e1.b);
-    const C tP = cadd<double>(cconjmul<double>(w1.a, cscale<double>(m0, p0)),
-                              cconjmul<double>(w1.b, cscale<double>(m1, p1)));
-    const C tM = cadd<double>(cconjmul<double>(w0.a, cscale<double>(m1, q0)),
-                              cconjmul<double>(w0.b, cscale<double>(m0, q1)));
-    // Re<w, -i sg tP> = sg Im(tP);  Re<w, i sg tM> = -sg Im(tM)
-    const double re = sg * (tP.y - tM.y);
-    ft[k] = -2.0 * re;
+    return -t1.y + t2.y;
   }
+  const size_t k = i - nS;
+  const int s = int(k / d.V), x = int(k % d.V);
+  const int sp = (s == d.Lt - 1) ? 0 : s + 1;
+  const double sg = (s == d.Lt - 1) ? double(d.bc) : 1.0;
+  const double kh = d.kth[x];
+  const double m0 = kh * (-r - 1.0), m1 = kh * (-r + 1.0);  // T- = diag(m0,m1), T+ = diag(m1,m0)
+  const C u = d.uT[k];
+  const size_t o0 = size_t(s) * d.V + x, o1 = size_t(sp) * d.V + x;
+  const Spin<double> e0 = eta[o0], e1 = eta[o1], w0 = w[o0], w1 = w[o1];
+  // dD[s+1 <- s] = -i sg T- e^{-i theta};  dD[s <- s+1] = +i sg T+ e^{+i theta}
+  const C uc = {u.x, -u.y};
+  const C p0 = cmul<double>(uc, e0.a), p1 = cmul<double>(uc, e0.b);
+  const C q0 = cmul<double>(u, e1.a), q1 = cmul<double>(u, e1.b);
+  const C tP = cadd<double>(cconjmul<double>(w1.a, cscale<double>(m0, p0)),
+                            cconjmul<double>(w1.b, cscale<double>(m1, p1)));
+  const C tM = cadd<double>(cconjmul<double>(w0.a, cscale<double>(m1, q0)),
+                            cconjmul<double>(w0.b, cscale<double>(m0, q1)));
+  // Re<w, -i sg tP> = sg Im(tP);  Re<w, i sg tM> = -sg Im(tM)
+  return sg * (tP.y - tM.y);
+}
+
+// Pseudofermion force, one thread per spatial link (s,l) followed by one per temporal
+// link (s,x):  F = -2 Re <w, dD/dtheta eta> with w = D eta  (dirac_pf.h:107-121).
+__global__ __launch_bounds__(256) void k_force(DiracView<double> d, const int4* __restrict__ ends,
+                                               const Spin<double>* __restrict__ eta,
+                                               const Spin<double>* __restrict__ w,
+                                               double* __restrict__ fs, double* __restrict__ ft) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t nS = size_t(d.Lt) * d.E, nT = size_t(d.Lt) * d.V;
+  if (i >= nS + nT) return;
+  const double f = -2.0 * link_bilinear(d, ends, i, w, eta);
+  if (i < nS)
+    fs[i] = f;
+  else
+    ft[i - nS] = f;
+}
+
+// f_i (+)= c (Re <u1, dD/dtheta_i v1> + Re <u2, dD/dtheta_i v2>) for every link; the second
+// pair is skipped when u2 is null, f is overwritten when init.
+__global__ __launch_bounds__(256) void k_force_acc(DiracView<double> d, const int4* __restrict__ ends,
+                                                   const Spin<double>* __restrict__ u1,
+                                                   const Spin<double>* __restrict__ v1,
+                                                   const Spin<double>* __restrict__ u2,
+                                                   const Spin<double>* __restrict__ v2, double c, int init,
+                                                   double* __restrict__ fs, double* __restrict__ ft) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t nS = size_t(d.Lt) * d.E, nT = size_t(d.Lt) * d.V;
+  if (i >= nS + nT) return;
+  double t = link_bilinear(d, ends, i, u1, v1);
+  if (u2) t += link_bilinear(d, ends, i, u2, v2);
+  double* f = i < nS ? fs + i : ft + (i - nS);
+  *f = init ? c * t : *f + c * t;
 }
 
 template <typename T>
@@ -446,6 +474,16 @@ void Dirac::force(const Spin<double>* eta, double* fs, double* ft) {
                      reinterpret_cast<const int4*>(link_ends_), eta, tmp64_, fs, ft);
   QED3_HIP(hipGetLastError());
   QED3_HIP(hipStreamSynchronize(stream_));
+}
+
+void Dirac::force_bilinear(const Spin<double>* u1, const Spin<double>* v1, const Spin<double>* u2,
+                           const Spin<double>* v2, double c, bool init, double* fs, double* ft) {
+  if ((u2 == nullptr) != (v2 == nullptr)) throw std::invalid_argument("force_bilinear: second pair incomplete");
+  const size_t n = size_t(h_.Lt) * (h_.E + h_.V);
+  const int B = 256;
+  hipLaunchKernelGGL(k_force_acc, dim3((n + B - 1) / B), dim3(B), 0, stream_, view<double>(),
+                     reinterpret_cast<const int4*>(link_ends_), u1, v1, u2, v2, c, init ? 1 : 0, fs, ft);
+  QED3_HIP(hipGetLastError());
 }
 
 double Dirac::time_apply(bool dag, int prec, int warmup, int reps) {

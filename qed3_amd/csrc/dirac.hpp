@@ -120,6 +120,11 @@ class Dirac {
 
   // pseudofermion force for all links from eta (FP64)
   void force(const Spin<double>* eta, double* fs, double* ft);
+  // f_l (+)= c (Re <u1, dD/dtheta_l v1> + Re <u2, dD/dtheta_l v2>) for every link, in the
+  // force layout (spatial [Lt][E], temporal [Lt][V]); (u2, v2) may both be null; f is
+  // overwritten when init.  Asynchronous on the context's stream.
+  void force_bilinear(const Spin<double>* u1, const Spin<double>* v1, const Spin<double>* u2,
+                      const Spin<double>* v2, double c, bool init, double* fs, double* ft);
 
   // ms per launch of one D or D^dag apply; prec 0 FP64, 1 FP32, 2 binary16 storage
   double time_apply(bool dag, int prec, int warmup, int reps);

@@ -335,6 +335,7 @@ Overlap::Overlap(Dirac& D, int n, double lambda_min, double lambda_max, double t
 
 Overlap::~Overlap() {
   for (auto* p : sol_) (void)hipFree(p);
+  for (auto* p : ysol_) (void)hipFree(p);
   (void)hipFree(y_);
   (void)hipFree(t_);
 }
@@ -381,6 +382,120 @@ void Overlap::apply(bool dag, Spin<double>* out, const Spin<double>* in) {
   }
   QED3_HIP(hipGetLastError());
   QED3_HIP(hipStreamSynchronize(st));
+}
+
+void Overlap::force(const Spin<double>* eta, double* fs, double* ft) {
+  const size_t n = D_.points();
+  hipStream_t st = D_.stream();
+  const unsigned nb = unsigned(std::min<size_t>((n + 255) / 256, 4096));
+  const int ns = int(sol_.size());
+  if (ysol_.empty()) {
+    ysol_.resize(ns);
+    for (auto& p : ysol_) QED3_HIP(hipMalloc(&p, n * sizeof(Spin<double>)));
+  }
+  // R_m v = lmax^2 (D^dag D + lmax^2 sigma_m)^{-1} v: the solver's x_m are Z_m / lmax^2, Y_m / lmax^2
+  std::vector<double> sh = z_.shifts();
+  for (double& s : sh) s *= lmax_ * lmax_;
+  D_.apply<double>(true, t_, eta, false);
+  hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, st, y_, t_, 1.0 / lmax_, n);  // D^dag eta / lmax
+  CGOptions o;
+  o.tol = tol_;
+  o.mixed = false;
+  auto check = [](const std::vector<CGResult>& r) {
+    for (const auto& ri : r)
+      if (ri.status != 0) throw std::runtime_error("overlap force: shifted solve did not converge");
+  };
+  check(multishift_cg(D_, sh, sol_, eta, o));
+  check(multishift_cg(D_, sh, ysol_, y_, o));
+  const double l2 = lmax_ * lmax_, c0 = -2.0 * z_.C / lmax_;
+  // Z_0 = eta + sum_m A_m lmax^2 x_m into t_, then f = c0 Re <eta, dD Z_0>
+  ShiftSet S;
+  S.n = ns;
+  for (int m = 0; m < ns; m++) {
+    S.x[m] = sol_[m];
+    S.p[m] = nullptr;
+    S.a[m] = z_.A[m + 1] * l2;
+    S.b[m] = 0;
+  }
+  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, st, t_, static_cast<const Spin<double>*>(nullptr), 0.0,
+                     eta, 1.0, S, n);
+  D_.force_bilinear(eta, t_, nullptr, nullptr, c0, true, fs, ft);
+  // pole terms: c0 (-A_m) lmax^3 [Re <D xZ_m, dD xY_m> + Re <D xY_m, dD xZ_m>]
+  for (int m = 0; m < ns; m++) {
+    D_.apply<double>(false, t_, sol_[m], false);
+    D_.apply<double>(false, y_, ysol_[m], false);
+    D_.force_bilinear(t_, ysol_[m], y_, sol_[m], -c0 * z_.A[m + 1] * l2 * lmax_, false, fs, ft);
+  }
+  QED3_HIP(hipGetLastError());
+  QED3_HIP(hipStreamSynchronize(st));
+}
+
+void Overlap::apply_dhd(Spin<double>* out, const Spin<double>* in, Spin<double>* tmp) {
+  const size_t n = D_.points();
+  const unsigned nb = unsigned(std::min<size_t>((n + 255) / 256, 4096));
+  apply(false, out, in);
+  apply(true, tmp, in);
+  ShiftSet none;
+  none.n = 0;
+  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, D_.stream(), out, out, 1.0, tmp, 1.0, none, n);
+  QED3_HIP(hipGetLastError());
+}
+
+CGResult Overlap::solve_dhd(Spin<double>* x, const Spin<double>* b, double tol, int maxiter) {
+  if (!(tol > 0) || maxiter < 0) throw std::invalid_argument("solve_dhd: tol > 0, maxiter >= 0");
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t n = D_.points();
+  hipStream_t st = D_.stream();
+  const unsigned nb = unsigned(std::min<size_t>((n + 255) / 256, 4096));
+  Spin<double>* r = dnew<Spin<double>>(n, st);
+  Spin<double>* p = dnew<Spin<double>>(n, st);
+  Spin<double>* q = dnew<Spin<double>>(n, st);
+  Spin<double>* tmp = dnew<Spin<double>>(n, st);
+  double2* partials = dnew<double2>(nb, st);
+  double2* out = dnew<double2>(1, st);
+  double2* host = nullptr;
+  QED3_HIP(hipHostMalloc((void**)&host, sizeof(double2), hipHostMallocDefault));
+  Reducer red{partials, out, host, nb, st};
+  const size_t bytes = n * sizeof(Spin<double>);
+  ShiftSet none;
+  none.n = 0;
+  QED3_HIP(hipMemsetAsync(x, 0, bytes, st));
+  QED3_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, st));
+  QED3_HIP(hipMemcpyAsync(p, b, bytes, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_pair_norms, dim3(nb), dim3(256), 0, st, b, b, n, partials);
+  const double bn2 = red.get().x;
+  double rr = bn2;
+  CGResult res;
+  int it = 0;
+  while (bn2 > 0 && rr > tol * tol * bn2 && it < maxiter) {
+    apply_dhd(q, p, tmp);
+    double pq, im;
+    D_.dot(p, q, &pq, &im);
+    if (!(pq > 0) || !std::isfinite(pq)) {
+      res.status = 4;
+      break;
+    }
+    const double alpha = rr / pq;
+    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, st, x, x, 1.0, p, alpha, none, n);
+    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, st, r, r, 1.0, q, -alpha, none, n);
+    hipLaunchKernelGGL(k_pair_norms, dim3(nb), dim3(256), 0, st, r, r, n, partials);
+    const double rr_new = red.get().x;
+    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, st, p, r, 1.0, p, rr_new / rr, none, n);
+    rr = rr_new;
+    it++;
+  }
+  // true residual |b - A x| / |b|
+  apply_dhd(q, x, tmp);
+  hipLaunchKernelGGL(k_shift_resid, dim3(nb), dim3(256), 0, st, b, q, x, 0.0, n, partials);
+  const double tr = red.get().x;
+  res.iterations = it;
+  res.true_resid = bn2 > 0 ? std::sqrt(tr / bn2) : 0.0;
+  if (res.status == 0 && !(rr <= tol * tol * bn2)) res.status = 3;
+  for (void* ptr : {(void*)r, (void*)p, (void*)q, (void*)tmp, (void*)partials, (void*)out}) (void)hipFreeAsync(ptr, st);
+  QED3_HIP(hipStreamSynchronize(st));
+  (void)hipHostFree(host);
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
 }
 
 }  // namespace qed3

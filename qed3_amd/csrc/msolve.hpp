@@ -36,6 +36,20 @@ class Overlap {
   Overlap(const Overlap&) = delete;
   Overlap& operator=(const Overlap&) = delete;
   void apply(bool dag, Spin<double>* out, const Spin<double>* in);
+  // Pseudofermion force of the overlap fermion for every link, f_l = -2 Re <eta, dD_ov/dtheta_l eta>
+  // with lambda_max and the Zolotarev coefficients held fixed (Overlap::precalc_grad and
+  // grad_deviceAsyncLaunch, overlap.h:393-465, for all links at once):
+  //   f_l = -(2C/lmax) { Re <eta, dD Z_0> - sum_m A_m [Re <D Z_m / lmax, dD Y_m>
+  //                                                  + Re <D Y_m / lmax, dD Z_m>] },
+  //   Z_m = R_m eta, Y_m = R_m D^dag eta / lmax, R_m = (D^dag D / lmax^2 + sigma_m)^{-1},
+  //   Z_0 = eta + sum_m A_m Z_m.
+  // fs [Lt][E], ft [Lt][V] device arrays.  Two multi-shift solves (to the overlap's tol).
+  void force(const Spin<double>* eta, double* fs, double* ft);
+  // CG on D_ov + D_ov^dag (= D_ov^dag D_ov for an exactly unitary sign function), x from 0,
+  // to |b - A x| <= tol |b| on the recursive residual, then the true residual is reported
+  // (PseudoFermion<Overlap>::update_eta: MatPoly::solve on DHD_deviceAsyncLaunch,
+  // pseudofermion.h:51, overlap.h:364-377, matpoly.h:204-267).  FP64.
+  CGResult solve_dhd(Spin<double>* x, const Spin<double>* b, double tol, int maxiter);
   const Zolotarev& zolotarev() const { return z_; }
   long last_iterations() const { return iters_; }
 
@@ -44,6 +58,8 @@ class Overlap {
   Zolotarev z_;
   double lmax_, tol_;
   std::vector<Spin<double>*> sol_;
+  std::vector<Spin<double>*> ysol_;  // second set of shifted solutions (force only)
+  void apply_dhd(Spin<double>* out, const Spin<double>* in, Spin<double>* tmp);
   Spin<double>* y_ = nullptr;
   Spin<double>* t_ = nullptr;
   long iters_ = 0;

@@ -69,6 +69,8 @@ def test_argument_errors_of_the_host_entry_points():
     assert lib.qed3_plaquette_correlator(g, C.byref(p), buf, buf) == 1
     lib.qed3_geom_destroy(g)
     assert lib.qed3_overlap_apply(None, 0, None, None) == 1
+    assert lib.qed3_overlap_force(None, None, None, None) == 1
+    assert lib.qed3_overlap_solve(None, None, None, 1e-10, 10, None) == 1
     assert lib.qed3_multishift_cg(None, 1, None, None, None, None, None) == 1
 
 

@@ -116,3 +116,77 @@ def test_overlap_matches_dense(gpu, lattice):
         assert rel(got, Dex @ v) < 4 * z["Delta"] + 1e-9
         assert rel(ov.apply(v, dag=True), Dzol.conj().T @ v) < 1e-9
         assert abs(np.linalg.norm(got - v) / np.linalg.norm(v) - 1) < 4 * z["Delta"] + 1e-9
+
+
+def _dense_zolotarev(Dd, lmax, z):
+    """The dense D_ov of the Zolotarev partial fractions at fixed lambda_max and k."""
+    N = Dd.shape[0]
+    H = Dd.conj().T @ Dd / lmax ** 2
+    R = np.eye(N, dtype=complex)
+    for m in range(1, z["A"].size):
+        R += z["A"][m] * np.linalg.inv(H + z["shifts"][m - 1] * np.eye(N))
+    return np.eye(N) + z["C"] / lmax * Dd @ R
+
+
+def _overlap_pair(lattice, n, Lt, nz, seed, tol):
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=-0.5)
+    sp, tp = links(lattice, n, Lt, seed=seed)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=-0.5, at=at)
+    Dq.update(sp, tp)
+    N = 2 * Lt * Dq.V
+    Dd = _dense(Do, N, 0)
+    ev = np.linalg.eigvalsh(Dd.conj().T @ Dd)
+    lmin, lmax = np.sqrt(ev[0]) * 0.999, np.sqrt(ev[-1]) * 1.001
+    ov = q.Overlap(Dq, n=nz, lambda_min=lmin, lambda_max=lmax, tol=tol)
+    return Do, Dq, ov, q.zolotarev(nz, lmin / lmax), sp, tp, N, Dd
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_overlap_force_is_the_derivative_of_the_dense_overlap(gpu, lattice):
+    """f_l = -2 d/dtheta_l Re <eta, D_ov eta> for every spatial and temporal link, against
+    central differences (eps = 1e-5) of the dense Zolotarev operator with lambda_max and the
+    coefficients held fixed, as the reference's grad does (overlap.h:393-465)."""
+    n, Lt, nz = 1, 4, 13
+    Do, Dq, ov, z, sp, tp, N, _ = _overlap_pair(lattice, n, Lt, nz, seed=5, tol=1e-13)
+    lmax = ov.lambda_max
+    eta = source(lattice, n, Lt, seed=3)
+    fs, ft = ov.force(eta)
+
+    def g(sp2, tp2):
+        Do.update(sp2, tp2)
+        return np.real(np.vdot(eta, _dense_zolotarev(_dense(Do, N, 0), lmax, z) @ eta))
+
+    eps = 1e-5
+    ref_s, ref_t = np.zeros_like(sp), np.zeros_like(tp)
+    for ref, base, spatial in ((ref_s, sp, True), (ref_t, tp, False)):
+        for idx in np.ndindex(base.shape):
+            up, dn = base.copy(), base.copy()
+            up[idx] += eps
+            dn[idx] -= eps
+            gp = g(up, tp) if spatial else g(sp, up)
+            gm = g(dn, tp) if spatial else g(sp, dn)
+            ref[idx] = -2.0 * (gp - gm) / (2 * eps)
+    assert rel(fs, ref_s) < 1e-7, rel(fs, ref_s)
+    assert rel(ft, ref_t) < 1e-7, rel(ft, ref_t)
+    # the pole terms matter: without them the force is far off
+    Do.update(sp, tp)
+    assert rel(fs, Dq.force(eta)[0]) > 1e-2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_overlap_pseudofermion_solve_matches_dense(gpu, lattice):
+    """eta = (D_ov + D_ov^dag)^{-1} phi (PseudoFermion<Overlap>::update_eta) against the
+    dense solve, and the true residual the report states."""
+    n, Lt, nz = 2, 4, 13
+    Do, Dq, ov, z, sp, tp, N, Dd = _overlap_pair(lattice, n, Lt, nz, seed=8, tol=1e-13)
+    Dz = _dense_zolotarev(Dd, ov.lambda_max, z)
+    A = Dz + Dz.conj().T
+    b = source(lattice, n, Lt, seed=12)
+    x, rep = ov.solve(b, tol=1e-11)
+    ref = np.linalg.solve(A, b)
+    assert rel(x, ref) < 1e-9, rel(x, ref)
+    assert rep.status == 0 and rep.true_resid <= 1e-11 and rep.iterations > 0
+    assert abs(rep.true_resid - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) < 1e-12
