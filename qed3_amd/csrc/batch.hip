@@ -331,8 +331,8 @@ std::vector<CGResult> batch_launches(const std::vector<Dirac*>& Ds, const std::v
     S.nsblk = int(std::min<size_t>((S.n + kBlock - 1) / kBlock, 2048));
     g.tots += S.nsblk;
     maxtot = std::max({maxtot, g.tot32, g.tot64, g.tots});
-    smem32 = std::max(smem32, size_t(kBlock + S.hv.hmax) * sizeof(Spin<float>));
-    smem64 = std::max(smem64, size_t(kBlock + S.hv.hmax) * sizeof(Spin<double>));
+    smem32 = std::max(smem32, lds_bytes<float>(kBlock, S.hv.hmax));
+    smem64 = std::max(smem64, lds_bytes<double>(kBlock, S.hv.hmax));
     const size_t hb = S.n * sizeof(SpinH), fb = S.n * sizeof(Spin<float>);
     S.r = static_cast<SpinH*>(dalloc(hb));
     S.w = static_cast<SpinH*>(dalloc(hb));

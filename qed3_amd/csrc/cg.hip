@@ -19,8 +19,7 @@ using namespace cgepi;
 template <typename T>
 __global__ __launch_bounds__(256) void k_init(const Spin<double>* __restrict__ b, Spin<double>* x64,
                                               Spin<T>* xin, Spin<T>* r, Spin<T>* p, size_t n,
-                                              CGScalars* sc, GridReduce gr, double tol2,
-                                              double delta2, int maxiter) {
+                                              GridReduce gr) {
   double2 acc = {0.0, 0.0};
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
@@ -42,7 +41,16 @@ __global__ __launch_bounds__(256) void k_init(const Spin<double>* __restrict__ b
     }
     acc.x += norm2(bb);
   }
-  grid_reduce(acc, gr, [=](double2 t) {
+  store_partial(acc, gr.partials, blockIdx.x);
+}
+
+// finaliser of k_init: solver scalars from |b|^2
+struct FinStart {
+  CGScalars* sc;
+  double tol2, delta2;
+  int maxiter;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(double2 t) const {
     sc->mu = t.x;
     sc->bnorm2 = t.x;
     sc->maxr2 = t.x;
@@ -59,8 +67,8 @@ __global__ __launch_bounds__(256) void k_init(const Spin<double>* __restrict__ b
     sc->maxiter = maxiter;
     sc->need_ru = 0;
     sc->done = (t.x == 0.0) ? 1 : (t.x < 1e300 ? 0 : 2);
-  });
-}
+  }
+};
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_flush(Spin<double>* x64, Spin<T>* xin, size_t n,
@@ -86,8 +94,7 @@ __global__ __launch_bounds__(256) void k_flush(Spin<double>* x64, Spin<T>* xin, 
 // Half-storage start: x = 0, p_old = 0, |b|^2; the scale makes the stored residual
 // unit-rms per component (k_store_b0 then stores r = b / scale).
 __global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__ b, Spin<double>* x64,
-                                                Spin<float>* xin, SpinH* p, size_t n, CGScalars* sc,
-                                                GridReduce gr, double tol2, double delta2, int maxiter) {
+                                                Spin<float>* xin, SpinH* p, size_t n, GridReduce gr) {
   double2 acc = {0.0, 0.0};
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
@@ -103,9 +110,17 @@ __global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__
     x64[i] = z;
     acc.x += norm2(bb);
   }
-  const double ncomp = 4.0 * double(n);
-  grid_reduce(acc, gr, [=](double2 t) { half_start(sc, t.x, ncomp, tol2, delta2, maxiter); });
+  store_partial(acc, gr.partials, blockIdx.x);
 }
+
+// finaliser of k_init_h
+struct FinStartH {
+  CGScalars* sc;
+  double ncomp, tol2, delta2;
+  int maxiter;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(double2 t) const { half_start(sc, t.x, ncomp, tol2, delta2, maxiter); }
+};
 
 // Half-storage flush: x64 += scale (x_inner + sum_j alpha_pend[j] ring[j]), x_inner = 0,
 // taking the x terms the current group's iterations left pending (alpha_pend[j] = 0 if none)
@@ -143,7 +158,7 @@ __global__ __launch_bounds__(256) void k_rescale(const Spin<float>* __restrict__
 
 __global__ __launch_bounds__(256) void k_resid_norms(const Spin<double>* __restrict__ b,
                                                      const Spin<double>* __restrict__ ax, size_t n,
-                                                     GridReduce gr, double2* out) {
+                                                     GridReduce gr) {
   double2 acc = {0.0, 0.0};
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
@@ -152,8 +167,14 @@ __global__ __launch_bounds__(256) void k_resid_norms(const Spin<double>* __restr
     acc.x += d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
     acc.y += norm2(bb);
   }
-  grid_reduce(acc, gr, [out](double2 t) { *out = t; });
+  store_partial(acc, gr.partials, blockIdx.x);
 }
+
+struct FinOut {
+  double2* out;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(double2 t) const { *out = t; }
+};
 
 template <typename T>
 struct Work {
@@ -166,6 +187,14 @@ struct Work {
 };
 
 size_t stream_blocks(size_t n) { return std::min<size_t>((n + 255) / 256, 8192); }
+
+// The FP64 true residual of the returned x.  A solve that converged did so in a reliable
+// update, which computed |b - D^dag D x|^2 in FP64 for this same x (every pending inner
+// term was flushed before it), so that value is reported; otherwise it is recomputed.
+double final_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b, const CGScalars& h) {
+  if (h.done == 1 && h.n_ru > 0 && h.bnorm2 > 0) return std::sqrt(h.true_r2 / h.bnorm2);
+  return true_residual(D, x, b);
+}
 
 // Event pairs for KernelTimes: pair(k) hands out a fresh (start, end) pair tagged with
 // kernel class k; finish() averages the pairs of the first `iters` iterations.
@@ -261,8 +290,9 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   const int nbk = int(stream_blocks(n));
   const double tol2 = o.tol * o.tol;
   const double delta2 = kMixed ? o.delta * o.delta : 0.0;
-  hipLaunchKernelGGL(k_init<T>, dim3(nbk), dim3(256), 0, st, b, x, W.xin, W.r, W.p, n, sc, gr,
-                     tol2, delta2, o.maxiter);
+  hipLaunchKernelGGL(k_init<T>, dim3(nbk), dim3(256), 0, st, b, x, W.xin, W.r, W.p, n, gr);
+  hipLaunchKernelGGL(k_finalize<FinStart>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
+                     FinStart{sc, tol2, delta2, o.maxiter});
   QED3_HIP(hipGetLastError());
 
   const DiracView<T> vin = D.view<T>();
@@ -304,7 +334,7 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
   res.iterations = hsc->iter;
   res.reliable_updates = hsc->n_ru;
   res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
-  res.true_resid = true_residual(D, x, b);
+  res.true_resid = final_residual(D, x, b, *hsc);
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
   return res;
@@ -332,8 +362,9 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
 
   const GridReduce gr = D.reducer();
   const int nbk = int(stream_blocks(n));
-  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kXGroup - 1], n, sc, gr,
-                     o.tol * o.tol, o.delta * o.delta, o.maxiter);
+  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kXGroup - 1], n, gr);
+  hipLaunchKernelGGL(k_finalize<FinStartH>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
+                     FinStartH{sc, 4.0 * double(n), o.tol * o.tol, o.delta * o.delta, o.maxiter});
   hipLaunchKernelGGL(k_store_b0, dim3(nbk), dim3(256), 0, st, b, r, n, sc);
   QED3_HIP(hipGetLastError());
 
@@ -384,7 +415,7 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   res.iterations = hsc->iter;
   res.reliable_updates = hsc->n_ru;
   res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
-  res.true_resid = true_residual(D, x, b);
+  res.true_resid = final_residual(D, x, b, *hsc);
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
   return res;
@@ -399,8 +430,9 @@ double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b) {
   Spin<double>* t2 = static_cast<Spin<double>*>(D.scratch(9, n * sizeof(Spin<double>)));
   double2* out = static_cast<double2*>(D.scratch(10, sizeof(double2)));
   D.apply_dagd<double>(t2, x, t1, false);
-  hipLaunchKernelGGL(k_resid_norms, dim3(int(stream_blocks(n))), dim3(256), 0, st, b, t2, n,
-                     D.reducer(), out);
+  const unsigned nbk = unsigned(stream_blocks(n));
+  hipLaunchKernelGGL(k_resid_norms, dim3(nbk), dim3(256), 0, st, b, t2, n, D.reducer());
+  hipLaunchKernelGGL(k_finalize<FinOut>, dim3(1), dim3(256), 0, st, D.reducer().partials, nbk, FinOut{out});
   QED3_HIP(hipGetLastError());
   double2 h;
   QED3_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, st));

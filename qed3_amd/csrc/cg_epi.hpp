@@ -151,10 +151,10 @@ struct EpiK1PH {
   __device__ void begin() {}
   __device__ void operator()(size_t i, const Spin<float>& v, double2& acc) const {
     const SpinH h = to_half(v);
-    w[i] = h;
+    sth_nt(w + i, h);
     acc.x += norm2(from_half(h));
   }
-  __device__ void center(size_t i, const Spin<float>& v) const { pnew[i] = to_half(v); }
+  __device__ void center(size_t i, const Spin<float>& v) const { sth_nt(pnew + i, to_half(v)); }
   __device__ void operator()(double2 t) const {
     sc->rescale = 0;
     k1_finish(sc, t.x);
@@ -207,7 +207,7 @@ struct EpiK2H {
     rr.b.x -= alpha * q.b.x;
     rr.b.y -= alpha * q.b.y;
     const SpinH rh = to_half(rr);
-    r[i] = rh;
+    sth_nt(r + i, rh);
     acc.x += norm2(from_half(rh));
   }
   __device__ void operator()(double2 t) const {

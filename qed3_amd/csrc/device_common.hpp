@@ -83,6 +83,15 @@ __device__ __forceinline__ SpinH to_half(const Spin<float>& s) {
   return h;
 }
 __device__ __forceinline__ Spin<float> ldh(const SpinH* p) { return from_half(*p); }
+// Streaming store of a binary16 spinor (the CG's p, w and r: written once per kernel and
+// read by the next one), marked non-temporal so it does not displace the halo and
+// neighbour spinors that the stencil re-reads from L2.
+__device__ __forceinline__ void sth_nt(SpinH* p, const SpinH& h) {
+  unsigned long long v;
+  __builtin_memcpy(&v, &h, 8);
+  __builtin_nontemporal_store(v, reinterpret_cast<unsigned long long*>(p));
+}
+
 
 // Link phase exp(i theta) in binary16 (4 B), read as an FP32 complex.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));

@@ -36,7 +36,7 @@ __global__ void k_phases_half(const double* __restrict__ ang, half2_t* __restric
 // <x,y> = sum conj(x) y over complex double fields of n points.
 __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
                                              const Spin<double>* __restrict__ y, size_t n,
-                                             GridReduce gr, double2* out) {
+                                             GridReduce gr) {
   double2 acc = {0.0, 0.0};
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
@@ -45,8 +45,14 @@ __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
     acc.x += p.x + q.x;
     acc.y += p.y + q.y;
   }
-  grid_reduce(acc, gr, [out](double2 t) { *out = t; });
+  store_partial(acc, gr.partials, blockIdx.x);
 }
+
+struct FinDot {
+  double2* out;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(double2 t) const { *out = t; }
+};
 
 // Re <w, dD/dtheta_i eta> for link i: spatial links (s,l) first, then temporal links (s,x)
 // (the two derivative blocks of dirac_ext.h:233-269 for a spatial link, :272-302 for a
@@ -456,7 +462,9 @@ void Dirac::dot(const Spin<double>* x, const Spin<double>* y, double* re, double
   const size_t n = points();
   const int B = 256;
   const size_t nb = std::min<size_t>((n + B - 1) / B, 4096);
-  hipLaunchKernelGGL(k_dot, dim3(std::max<size_t>(nb, 1)), dim3(B), 0, stream_, x, y, n, reducer(), dot_out_);
+  const unsigned nbl = unsigned(std::max<size_t>(nb, 1));
+  hipLaunchKernelGGL(k_dot, dim3(nbl), dim3(B), 0, stream_, x, y, n, reducer());
+  hipLaunchKernelGGL(k_finalize<FinDot>, dim3(1), dim3(256), 0, stream_, reducer().partials, nbl, FinDot{dot_out_});
   QED3_HIP(hipGetLastError());
   double2 h;
   QED3_HIP(hipMemcpyAsync(&h, dot_out_, sizeof(h), hipMemcpyDeviceToHost, stream_));

@@ -223,8 +223,17 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T, P> d, Src in, Epi 
 // and one gather of the halo, then reads all spatial neighbours from LDS through a
 // 16-bit local index table.  Every spinor is fetched once per block and slice.
 // The work of one block: site range xr (of nxr; ranges beyond are padding) and time tile
-// st of ts slices, staging through the LDS tile tileS ((blockDim.x + hv.hmax) spinors).
+// st of ts slices, staging through the LDS tiles tileS (kLdsBuffers tiles of
+// blockDim.x + hv.hmax spinors each).  Slices alternate between two tiles, so one barrier
+// per slice suffices: slice s writes the tile slice s-2 read, and every thread finished
+// slice s-2's reads before it reached slice s-1's barrier.
 // Shared by the single-system kernel below and the batched solver.
+constexpr int kLdsBuffers = 2;
+template <typename T>
+__host__ __device__ constexpr size_t lds_bytes(int block, int hmax) {
+  return size_t(kLdsBuffers) * size_t(block + hmax) * sizeof(Spin<T>);
+}
+
 template <typename T, int NS, bool DAG, class Epi, class In, typename P>
 __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, const In& in, const Epi& epi,
                                                   int ts, int xr, int nxr, int st, const HaloView& hv,
@@ -289,18 +298,18 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
         if (l < 0) uu[j].y = -uu[j].y;
       }
     }
-    __syncthreads();  // previous slice's LDS reads are done
-    if (act) tileS[threadIdx.x] = cur;
-    if (hid0 >= 0) tileS[R + threadIdx.x] = in(sV + hid0);
-    if (hid1 >= 0) tileS[2 * R + threadIdx.x] = in(sV + hid1);
-    for (int k = threadIdx.x + 2 * R; k < nh; k += R) tileS[R + k] = in(sV + hv.halo[h0 + k]);
+    Spin<T>* tb = tileS + (s & 1) * (R + hv.hmax);
+    if (act) tb[threadIdx.x] = cur;
+    if (hid0 >= 0) tb[R + threadIdx.x] = in(sV + hid0);
+    if (hid1 >= 0) tb[2 * R + threadIdx.x] = in(sV + hid1);
+    for (int k = threadIdx.x + 2 * R; k < nh; k += R) tb[R + k] = in(sV + hv.halo[h0 + k]);
     __syncthreads();
     if (act) {
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
 #pragma unroll
       for (int j = 0; j < NS; j++) {
         const C u = uu[j];
-        const Spin<T> py = tileS[nb[j]];
+        const Spin<T> py = tb[nb[j]];
         const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
         const T Cc = hp[j].x, Ss = hp[j].y, A = hp[j].z, B = hp[j].w;
         const T rC = d.r * Cc, rS = d.r * Ss;
@@ -335,6 +344,7 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
       uprev = ucur;
     }
   }
+  __syncthreads();  // every read of the tiles is done: the caller may stage its next block
 }
 
 template <typename T, int NS, bool DAG, class Epi, class Src, typename P = cplx_t<T>>
@@ -397,7 +407,7 @@ inline void launch_stencil_src(const DiracView<T, P>& v, int max_nn, const Src& 
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
   if (t.lds) {
     if (!hv.nbl) throw std::logic_error("LDS stencil launched without halo tables");
-    const size_t smem = size_t(t.block + hv.hmax) * sizeof(Spin<T>);
+    const size_t smem = lds_bytes<T>(t.block, hv.hmax);
     if (max_nn == 6)
       hipLaunchKernelGGL((k_stencil_lds<T, 6, DAG, Epi, Src, P>), grid, dim3(t.block), smem, st, v, in, epi,
                          t.ts, tm, hv, gr);
