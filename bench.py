@@ -411,9 +411,9 @@ def cg_kernel_bytes(V, E, Lt, solver):
 
 
 CG_KERNEL_NAMES = {
-    "half": {"K1": "k_stencil_lds<float,NS,false,EpiK1PH,SrcPUpdateH,half2_t>",
-             "K2": "k_stencil_lds<float,NS,true,EpiK2H<0..2>,SrcHalf,half2_t>",
-             "K2x": "k_stencil_lds<float,NS,true,EpiK2H<3>,SrcHalf,half2_t>"},
+    "half": {"K1": "k_stencil_lds<float,NS,false,EpiK1PH,SrcPUpdateH,phase16_t>",
+             "K2": "k_stencil_lds<float,NS,true,EpiK2H<0..2>,SrcHalf,phase16_t>",
+             "K2x": "k_stencil_lds<float,NS,true,EpiK2H<3>,SrcHalf,phase16_t>"},
     "mixed": {"K1": "k_stencil_lds<float,NS,false,EpiK1P<float>,SrcPUpdate<float>>",
               "K2": "k_stencil_lds<float,NS,true,EpiK2<float>,SrcPlain<float>>"},
     "fp64": {"K1": "k_stencil_lds<double,NS,false,EpiK1P<double>,SrcPUpdate<double>>",
@@ -459,7 +459,7 @@ def stencil_roofline(a, q, D):
     B_D = 2 B_vec + B_link (one real per link at that precision).  Returns both the
     binary16-storage and the FP32 kernel, keyed "f16" / "f32"."""
     out = {}
-    for nm, prec, rb, kern in (("f16", q.F16, 2, "EpiStoreH,SrcHalf,half2_t> (D, binary16 storage)"),
+    for nm, prec, rb, kern in (("f16", q.F16, 2, "EpiStoreH,SrcHalf,phase16_t> (D, binary16 storage)"),
                                ("f32", q.F32, 4, "EpiStore,SrcPlain<float>> (D, FP32)")):
         ms = D.time_apply(q.OP_D, prec, warmup=5, reps=50)
         b = algorithmic_bytes_D(D.V, D.E, a.Lt, rb)

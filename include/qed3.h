@@ -34,7 +34,7 @@ enum { QED3_LATTICE_SIMPLICIAL = 0, QED3_LATTICE_DUAL = 1 };
 enum { QED3_BC_ANTIPERIODIC = 0, QED3_BC_PERIODIC = 1 };
 enum { QED3_OP_D = 0, QED3_OP_DDAG = 1, QED3_OP_DDAGD = 2, QED3_OP_DDDAG = 3 };
 /* field precision: complex double, complex float, or binary16 pairs (4 x 16 bit per
- * point, FP32 arithmetic, binary16 link phases: the half-storage CG's operator) */
+ * point, FP32 arithmetic, 16-bit fixed-point link angles: the half-storage CG's operator) */
 enum { QED3_PREC_F64 = 0, QED3_PREC_F32 = 1, QED3_PREC_F16 = 2 };
 /* FP64 CG; FP32 inner CG with FP64 reliable updates; the same with the inner r, p and w
  * stored in IEEE binary16 (scaled to unit rms at every reliable update) */

@@ -19,7 +19,7 @@ constexpr long kLargeBlocks = 1024;  // 4 blocks per CU of a 256-CU part
 // One system of the batch: its operator views, LDS halo tables for ranges of kBlock
 // sites, its blocks in the stencil grids and the streaming grids, and its fields.
 struct Sys {
-  DiracView<float, half2_t> vh;
+  DiracView<float, phase16_t> vh;
   DiracView<double> v64;
   HaloView hv;
   int ns;                  // neighbour slots (3 dual, 6 simplicial)

@@ -185,8 +185,14 @@ struct EpiK2H {
 #pragma unroll
     for (int j = 0; j < kXGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
   }
+  // r is loaded ahead of the stencil (2 registers held across the slice); x and the group's
+  // search directions are read in the epilogue: holding them too measured slower
+  __device__ SpinH prefetch(size_t i) const { return r[i]; }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const {
-    Spin<float> rr = ldh(r + i);
+    (*this)(i, q, acc, prefetch(i));
+  }
+  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const SpinH& rpre) const {
+    Spin<float> rr = from_half(rpre);
     if constexpr (kUpdateX) {
       Spin<float> xx = ld(x + i);
 #pragma unroll

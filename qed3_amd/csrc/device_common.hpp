@@ -92,14 +92,18 @@ __device__ __forceinline__ void sth_nt(SpinH* p, const SpinH& h) {
   __builtin_nontemporal_store(v, reinterpret_cast<unsigned long long*>(p));
 }
 
-
-// Link phase exp(i theta) in binary16 (4 B), read as an FP32 complex.
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+// Link phases exp(i theta): stored as complex in the FP64 and FP32 tables.
 __device__ __forceinline__ double2 ldu(const double2* p) { return *p; }
 __device__ __forceinline__ float2 ldu(const float2* p) { return *p; }
-__device__ __forceinline__ float2 ldu(const half2_t* p) {
-  const half2_t h = *p;
-  return {float(h.x), float(h.y)};
+// Link phase of the half-storage solver as a 16-bit fixed-point angle
+// (theta = 2 pi v / 65536, 2 B per link, uniform resolution 9.6e-5 rad); the hardware sine
+// and cosine take the angle in revolutions.
+struct phase16_t {
+  short v;
+};
+__device__ __forceinline__ float2 ldu(const phase16_t* p) {
+  const float rev = float(p->v) * (1.0f / 65536.0f);
+  return {__builtin_amdgcn_cosf(rev), __builtin_amdgcn_sinf(rev)};
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -24,12 +24,12 @@ __global__ void k_phases(const double* __restrict__ ang, cplx_t<T>* __restrict__
   }
 }
 
-__global__ void k_phases_half(const double* __restrict__ ang, half2_t* __restrict__ u, size_t n) {
+__global__ void k_phases_half(const double* __restrict__ ang, phase16_t* __restrict__ u, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) {
-    double sn, cs;
-    sincos(ang[i], &sn, &cs);
-    u[i] = half2_t{_Float16(cs), _Float16(sn)};
+    // 16-bit fixed-point angle in units of 2 pi / 65536, wrapped into [-pi, pi)
+    const long long k = llrint(ang[i] * (65536.0 / (2.0 * M_PI)));
+    u[i] = phase16_t{static_cast<short>(static_cast<unsigned short>(k & 0xFFFF))};
   }
 }
 
@@ -369,14 +369,14 @@ void Dirac::fill_half_phases() {
   QED3_HIP(hipGetLastError());
 }
 
-DiracView<float, half2_t> Dirac::view_half_phases() {
+DiracView<float, phase16_t> Dirac::view_half_phases() {
   if (!uSh_) {
-    uSh_ = dalloc<half2_t>(size_t(h_.Lt) * h_.E);
-    uTh_ = dalloc<half2_t>(size_t(h_.Lt) * h_.V);
+    uSh_ = dalloc<phase16_t>(size_t(h_.Lt) * h_.E);
+    uTh_ = dalloc<phase16_t>(size_t(h_.Lt) * h_.V);
     if (has_gauge_) fill_half_phases();
   }
   const DiracView<float> f = view<float>();
-  DiracView<float, half2_t> v;
+  DiracView<float, phase16_t> v;
   v.nbr = f.nbr;
   v.lnk = f.lnk;
   v.hop = f.hop;
@@ -434,7 +434,7 @@ void Dirac::apply(bool dag, Spin<T>* out, const Spin<T>* in, bool sync) {
 }
 
 void Dirac::apply_half(bool dag, SpinH* out, const SpinH* in, bool sync) {
-  const DiracView<float, half2_t> v = view_half_phases();
+  const DiracView<float, phase16_t> v = view_half_phases();
   const EpiStoreH e{out};
   if (dag)
     launch_stencil_src<float, true>(v, h_.max_nn, SrcHalf{in}, e, tile<float>(true), reducer(), stream_,

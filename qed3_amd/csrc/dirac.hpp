@@ -99,9 +99,9 @@ class Dirac {
 
   template <typename T>
   DiracView<T> view() const;
-  // FP32 tables with binary16 link phases (kept up to date with the gauge field from the
-  // first call on)
-  DiracView<float, half2_t> view_half_phases();
+  // FP32 tables with 16-bit fixed-point link angles (kept up to date with the gauge field
+  // from the first call on)
+  DiracView<float, phase16_t> view_half_phases();
   template <typename T>
   StencilTile tile(bool dag) const;
 
@@ -112,7 +112,7 @@ class Dirac {
   void apply_dagd(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync = true);
   template <typename T>
   void apply_ddag(Spin<T>* out, const Spin<T>* in, Spin<T>* tmp, bool sync = true);
-  // binary16 fields, FP32 arithmetic, binary16 link phases (the half-storage CG's stencil)
+  // binary16 fields, FP32 arithmetic, 16-bit link angles (the half-storage CG's stencil)
   void apply_half(bool dag, SpinH* out, const SpinH* in, bool sync = true);
 
   // complex <x,y> in FP64
@@ -165,8 +165,8 @@ class Dirac {
   hipStream_t stream_ = nullptr;
   DeviceTables<double> d64_;
   DeviceTables<float> d32_;
-  half2_t* uSh_ = nullptr;  // [Lt][E] binary16 phases (allocated by view_half_phases)
-  half2_t* uTh_ = nullptr;  // [Lt][V]
+  phase16_t* uSh_ = nullptr;  // [Lt][E] 16-bit fixed-point angles (allocated by view_half_phases)
+  phase16_t* uTh_ = nullptr;  // [Lt][V]
   void fill_half_phases();
   double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)
   double* angT_ = nullptr;  // [Lt][V]

@@ -19,6 +19,9 @@
 // launch_stencil follows them with a one-block finaliser kernel that sums the partials
 // in block order and runs the epilogue's finaliser (no device-scope fences).
 #pragma once
+#include <type_traits>
+#include <utility>
+
 #include "device_common.hpp"
 #include "dirac.hpp"
 
@@ -110,6 +113,28 @@ struct SrcPUpdateH {
   __device__ Bound bind() const { return Bound{r, p, float(sc->beta)}; }
 };
 
+// Epilogues may load what they need at an output point early through pre = prefetch(i),
+// which the stencil issues with the slice's other loads (before the LDS barrier, so the
+// latency overlaps the halo staging) and hands back as operator()(i, v, acc, pre).
+template <class E, class = void>
+struct EpiPre {
+  struct type {};
+  static constexpr bool value = false;
+};
+template <class E>
+struct EpiPre<E, std::void_t<decltype(std::declval<const E&>().prefetch(size_t(0)))>> {
+  using type = decltype(std::declval<const E&>().prefetch(size_t(0)));
+  static constexpr bool value = true;
+};
+template <class Epi, typename T>
+__device__ __forceinline__ void call_epi(const Epi& e, size_t i, const Spin<T>& v, double2& acc,
+                                         const typename EpiPre<Epi>::type& pre) {
+  if constexpr (EpiPre<Epi>::value)
+    e(i, v, acc, pre);
+  else
+    e(i, v, acc);
+}
+
 // Epilogues may take the stencil's input value at each output point (its centre) through
 // center(i, v); the others ignore it.
 template <class Epi, typename T>
@@ -152,6 +177,8 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       const Spin<T> next = in(size_t(sp) * V + x);
       const C ucur = ldu(d.uT + size_t(s) * V + x);
+      typename EpiPre<Epi>::type pre{};
+      if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(size_t(s) * V + x);
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
       const P* uS = d.uS + size_t(s) * d.E;
       const size_t sV = size_t(s) * V;
@@ -189,7 +216,7 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const
       Spin<T> o;
       o.a = o0;
       o.b = o1;
-      epi(size_t(s) * V + x, o, acc);
+      call_epi(epi, size_t(s) * V + x, o, acc, pre);
       call_center(epi, size_t(s) * V + x, cur, 0);
       prev = cur;
       cur = next;
@@ -286,9 +313,11 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
     Spin<T> next;
     C ucur;
     C uu[NS];
+    typename EpiPre<Epi>::type pre{};
     if (act) {
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       next = in(size_t(sp) * V + x);
+      if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(sV + x);
       ucur = ldu(d.uT + size_t(s) * V + x);
       const P* uS = d.uS + size_t(s) * d.E;
 #pragma unroll
@@ -337,7 +366,7 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
       Spin<T> o;
       o.a = o0;
       o.b = o1;
-      epi(size_t(s) * V + x, o, acc);
+      call_epi(epi, size_t(s) * V + x, o, acc, pre);
       call_center(epi, size_t(s) * V + x, cur, 0);
       prev = cur;
       cur = next;

@@ -49,8 +49,9 @@ def round16(v):
 @pytest.mark.gpu
 @pytest.mark.parametrize("lattice", [0, 1])
 def test_dirac_f16_matches_oracle(gpu, lattice):
-    """binary16 fields and phases, FP32 arithmetic: the FP64 operator on the rounded input
-    to binary16 accuracy (unit roundoff 2^-11); the binary16 phases follow gauge updates."""
+    """binary16 fields, 16-bit fixed-point link angles, FP32 arithmetic: the FP64 operator
+    on the rounded input to binary16 accuracy (unit roundoff 2^-11; the angle step 2 pi/65536
+    is finer); the stored angles follow gauge updates."""
     n, Lt = 4, 4
     L, Do, at = oracle_setup(lattice, n, Lt, M5=0.05)
     Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.05, at=at)
