@@ -185,11 +185,16 @@ struct EpiK2H {
 #pragma unroll
     for (int j = 0; j < kXGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
   }
-  // r is loaded ahead of the stencil (2 registers held across the slice); x and the group's
-  // search directions are read in the epilogue: holding them too measured slower
-  __device__ SpinH prefetch(size_t i) const { return r[i]; }
+  // Positions 0..kXGroup-2 load r ahead of the stencil (2 registers held across the slice).
+  // The x-updating position loads everything in the epilogue: holding r there too took the
+  // kernel from 94 to 98 VGPRs and from 5 to 4 waves per SIMD, which with one block per
+  // 256-site range and 64 slices leaves a second round of blocks (0.55 vs 0.37-0.39 ms).
+  template <int P = POS, std::enable_if_t<P != kXGroup - 1, int> = 0>
+  __device__ SpinH prefetch(size_t i) const {
+    return r[i];
+  }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const {
-    (*this)(i, q, acc, prefetch(i));
+    (*this)(i, q, acc, r[i]);
   }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const SpinH& rpre) const {
     Spin<float> rr = from_half(rpre);
