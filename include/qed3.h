@@ -267,6 +267,15 @@ void qed3_hmc_destroy(qed3_hmc* h);
  * in place by the new field when accepted (or always when no_reject != 0). */
 int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_reject,
                         qed3_hmc_report* rep);
+/* Overlap fermions in the HMC (hmc.cu:155-186 with IS_OVERLAP, Fermion =
+ * Overlap<WilsonDirac>): phi = D_ov^dag xi, eta = (D_ov + D_ov^dag)^{-1} phi, the overlap
+ * force, all through ov (created on the HMC's context; it must outlive its use here).
+ * refresh != 0 recomputes the spectral bounds and Zolotarev coefficients after every gauge
+ * update as Overlap::update does (overlap.h:200-209); 0 keeps ov's, so H is a smooth
+ * function of the links.  ov = NULL returns to Wilson fermions.  (qed3_overlap: the
+ * overlap operator handle of the section below.) */
+typedef struct qed3_overlap qed3_overlap;
+int qed3_hmc_set_overlap(qed3_hmc* h, qed3_overlap* ov, int refresh);
 
 /* ---- multi-shift CG and the overlap operator (overlap.h:7-465) --------------------
  * Zolotarev coefficients of the overlap's sign approximation (Zolotarev, overlap.h:7-135):
@@ -285,7 +294,6 @@ int qed3_spectral_bounds(qed3_ctx* ctx, double tol, int maxiter, double* mu_min,
 /* Overlap operator D_ov = 1 + X eps(X^dag X), X = D / lambda_max, k = lambda_min/lambda_max,
  * Zolotarev degree n (<= 65), shifted solves to tol (Overlap, overlap.h:137-361);
  * op = QED3_OP_D applies D_ov, QED3_OP_DDAG its adjoint.  FP64 device fields. */
-typedef struct qed3_overlap qed3_overlap;
 int qed3_overlap_create(qed3_ctx* ctx, int n, double lambda_min, double lambda_max, double tol,
                         qed3_overlap** out);
 void qed3_overlap_destroy(qed3_overlap* ov);

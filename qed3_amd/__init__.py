@@ -426,6 +426,14 @@ class HMC:
         except Exception:
             pass
 
+    def set_overlap(self, ov, refresh=False):
+        """Overlap fermions (hmc.cu with IS_OVERLAP): pseudofermions, solves and forces of
+        `ov` (an Overlap on this handle's WilsonDirac); refresh recomputes its spectral
+        bounds and Zolotarev coefficients after every gauge update as Overlap::update does.
+        None returns to Wilson fermions."""
+        check(load().qed3_hmc_set_overlap(self._h, ov._h if ov is not None else None, int(refresh)))
+        self._ov = ov  # keeps the overlap handle alive while the HMC uses it
+
     def rng_save(self, path):
         check(load().qed3_hmc_rng_save(self._h, str(path).encode()))
 

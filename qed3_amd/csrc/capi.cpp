@@ -664,6 +664,15 @@ int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
 
 void qed3_hmc_destroy(qed3_hmc* h) { delete h; }
 
+int qed3_hmc_set_overlap(qed3_hmc* h, qed3_overlap* ov, int refresh) {
+  if (!h) return fail(QED3_ERR_ARG, "null argument");
+  if (ov && ov->ctx != h->ctx) return fail(QED3_ERR_ARG, "overlap handle of another context");
+  return guarded([&]() -> int {
+    h->hmc->set_overlap(ov ? ov->ov.get() : nullptr, refresh != 0);
+    return QED3_OK;
+  });
+}
+
 int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_reject,
                         qed3_hmc_report* rep) {
   if (!h || !spatial || !temporal) return fail(QED3_ERR_ARG, "null argument");

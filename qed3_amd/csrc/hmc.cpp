@@ -158,7 +158,8 @@ void HMC::update_eta(int k) {
   o.tol = p_.tol;
   o.mixed = p_.mixed;
   o.half = p_.half;
-  const CGResult r = cg_solve(D_, eta_[k], phi_[k], o);
+  // Wilson: CG on D^dag D; overlap: FP64 CG on D_ov + D_ov^dag (PseudoFermion::update_eta)
+  const CGResult r = ov_ ? ov_->solve_dhd(eta_[k], phi_[k], p_.tol, o.maxiter) : cg_solve(D_, eta_[k], phi_[k], o);
   rep_.cg_solves++;
   rep_.cg_iterations += r.iterations;
   rep_.max_resid = std::max(rep_.max_resid, r.true_resid);
@@ -183,8 +184,22 @@ void HMC::pf_gen(int k) {
     }
   });
   QED3_HIP(hipMemcpy(xi_, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice));
-  D_.apply<double>(true, phi_[k], xi_, true);
+  if (ov_)
+    ov_->apply(true, phi_[k], xi_);
+  else
+    D_.apply<double>(true, phi_[k], xi_, true);
   update_eta(k);
+}
+
+void HMC::fermion_force(int k) {
+  if (ov_)
+    ov_->force(eta_[k], dfs_, dft_);
+  else
+    D_.force(eta_[k], dfs_, dft_);
+}
+
+void HMC::gauge_changed() {
+  if (ov_ && ov_refresh_) ov_->refresh_bounds();
 }
 
 double HMC::Sf(int k) {  // Re <phi, eta> (pseudofermion.h:53-58)
@@ -234,6 +249,7 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
   QED3_HIP(hipMemcpyAsync(du_, u, nS * sizeof(double), hipMemcpyHostToDevice, st));
   QED3_HIP(hipMemcpyAsync(dt_, t, nT * sizeof(double), hipMemcpyHostToDevice, st));
   D_.set_gauge(du_, dt_, true);
+  gauge_changed();
   t_rng += since(ta);
   ta = now();
   for (int k = 0; k < p_.npf; k++) pf_gen(k);
@@ -251,7 +267,7 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
   auto gauge_force = [&] { dg_->force(du_, dt_, dgs_, dgt_, st); };
   ta = now();
   for (int k = 0; k < p_.npf; k++) {
-    D_.force(eta_[k], dfs_, dft_);
+    fermion_force(k);
     kick(dfs_, dft_, -0.5 * tau);
   }
   t_fermion += since(ta);
@@ -266,11 +282,12 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
       kick(dgs_, dgt_, (ni != p_.nsteps_inner - 1) ? -1.0 * tau_in : -0.5 * tau_in);
     }
     D_.set_gauge(du_, dt_, true);
+    gauge_changed();
     t_gauge += since(ta);
     ta = now();
     for (int k = 0; k < p_.npf; k++) {
       update_eta(k);
-      D_.force(eta_[k], dfs_, dft_);
+      fermion_force(k);
       kick(dfs_, dft_, (n != p_.nsteps - 1) ? -1.0 * tau : -0.5 * tau);
     }
     t_fermion += since(ta);
@@ -293,6 +310,7 @@ HMCReport HMC::trajectory(double* u, double* t, bool no_reject) {
     QED3_HIP(hipStreamSynchronize(st));
   } else {
     D_.set_gauge(u, t, false);
+    gauge_changed();
   }
   rep_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return rep_;

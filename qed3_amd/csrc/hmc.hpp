@@ -12,6 +12,7 @@
 #include "cg.hpp"
 #include "dirac.hpp"
 #include "gauge_dev.hpp"
+#include "msolve.hpp"
 #include "rng.hpp"
 
 namespace qed3 {
@@ -69,6 +70,15 @@ class HMC {
   ~HMC();
   // one trajectory; u/t (host, gauge layout) are replaced by the new field on accept
   HMCReport trajectory(double* u, double* t, bool no_reject = false);
+  // Overlap fermions (Fermion = Overlap<WilsonDirac>, hmc.cu:155-186) on the same context:
+  // phi = D_ov^dag xi, eta = (D_ov + D_ov^dag)^{-1} phi, the overlap force.  With refresh the
+  // spectral bounds and Zolotarev coefficients are recomputed after every gauge update, as
+  // Overlap::update does in the reference; without, they stay those of the overlap
+  // handle, so H is a smooth function of the links.  ov = nullptr returns to Wilson fermions.
+  void set_overlap(Overlap* ov, bool refresh) {
+    ov_ = ov;
+    ov_refresh_ = refresh;
+  }
   ParallelRng& rng() { return *rng_; }
   const ParallelRng& rng() const { return *rng_; }
 
@@ -78,6 +88,8 @@ class HMC {
   void update_eta(int k);
   double Sf(int k);
   double H();
+  void fermion_force(int k);   // dfs_, dft_ = the force of pseudofermion k
+  void gauge_changed();        // after D_.set_gauge: refresh the overlap bounds if asked
 
   Dirac& D_;
   HMCParams p_;
@@ -93,6 +105,8 @@ class HMC {
   double *du_ = nullptr, *dt_ = nullptr, *dpu_ = nullptr, *dpt_ = nullptr;
   double *dgs_ = nullptr, *dgt_ = nullptr;
   HMCReport rep_;
+  Overlap* ov_ = nullptr;
+  bool ov_refresh_ = false;
 };
 
 }  // namespace qed3

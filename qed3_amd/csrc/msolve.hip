@@ -498,4 +498,13 @@ CGResult Overlap::solve_dhd(Spin<double>* x, const Spin<double>* b, double tol, 
   return res;
 }
 
+void Overlap::refresh_bounds(double tol, int maxiter) {
+  double mu_min = 0, mu_max = 0;
+  spectral_bounds(D_, tol, maxiter, &mu_min, &mu_max);
+  const double lmin = std::sqrt((1.0 - 100.0 * tol) * mu_min), lmax = std::sqrt((1.0 + 100.0 * tol) * mu_max);
+  if (!(lmin > 0 && lmax > lmin)) throw std::runtime_error("overlap: bad spectral bounds");
+  z_.set(lmin / lmax);
+  lmax_ = lmax;
+}
+
 }  // namespace qed3

@@ -50,6 +50,11 @@ class Overlap {
   // (PseudoFermion<Overlap>::update_eta: MatPoly::solve on DHD_deviceAsyncLaunch,
   // pseudofermion.h:51, overlap.h:364-377, matpoly.h:204-267).  FP64.
   CGResult solve_dhd(Spin<double>* x, const Spin<double>* b, double tol, int maxiter);
+  // Spectral bounds of the current D^dag D and the Zolotarev coefficients recomputed
+  // (Overlap::update -> compute_lambda_max, Zolotarev::update; overlap.h:200-209, 217-306):
+  // lambda_min = sqrt((1 - 100 tol) mu_min), lambda_max = sqrt((1 + 100 tol) mu_max).
+  void refresh_bounds(double tol = 1e-8, int maxiter = 2000);
+  double lambda_max() const { return lmax_; }
   const Zolotarev& zolotarev() const { return z_; }
   long last_iterations() const { return iters_; }
 

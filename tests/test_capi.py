@@ -71,6 +71,7 @@ def test_argument_errors_of_the_host_entry_points():
     assert lib.qed3_overlap_apply(None, 0, None, None) == 1
     assert lib.qed3_overlap_force(None, None, None, None) == 1
     assert lib.qed3_overlap_solve(None, None, None, 1e-10, 10, None) == 1
+    assert lib.qed3_hmc_set_overlap(None, None, 0) == 1
     assert lib.qed3_multishift_cg(None, 1, None, None, None, None, None) == 1
 
 

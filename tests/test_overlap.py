@@ -190,3 +190,57 @@ def test_overlap_pseudofermion_solve_matches_dense(gpu, lattice):
     assert rel(x, ref) < 1e-9, rel(x, ref)
     assert rep.status == 0 and rep.true_resid <= 1e-11 and rep.iterations > 0
     assert abs(rep.true_resid - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) < 1e-12
+
+
+def _overlap_hmc_setup(kappa=2.0):
+    lattice, n, Lt, M5 = 0, 2, 4, -0.5
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    sp, tp = links(lattice, n, Lt, seed=5, kappa=kappa)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=M5, at=at)
+    Dq.update(sp, tp)
+    mu_min, mu_max = q.spectral_bounds(Dq, tol=1e-10, maxiter=20000)
+    ov = q.Overlap(Dq, n=13, lambda_min=np.sqrt((1 - 1e-6) * mu_min),
+                   lambda_max=np.sqrt((1 + 1e-6) * mu_max), tol=1e-12)
+    return Dq, ov, sp, tp
+
+
+@pytest.mark.gpu
+def test_overlap_hmc_dH_scales_as_tau_squared(gpu):
+    """HMC with overlap fermions (hmc.cu with IS_OVERLAP) at fixed Zolotarev coefficients:
+    H is then a smooth function of the links and the overlap force its gradient, so the
+    leapfrog's dH falls as tau^2 (ratio 4 when the step count doubles)."""
+    Dq, ov, sp, tp = _overlap_hmc_setup()
+    dh = []
+    for nsteps in (4, 8):
+        H = q.HMC(Dq, 1 / 2.0, compact=True, npf=1, tau=0.2, nsteps=nsteps, nsteps_inner=8,
+                  tol=1e-11, solver=q.CG_F64, seed=7)
+        H.set_overlap(ov, refresh=False)
+        rep = H.trajectory(sp.copy(), tp.copy(), no_reject=True)
+        assert rep["cg_solves"] == nsteps + 1 and rep["max_resid"] <= 1e-11
+        dh.append(abs(rep["dH"]))
+    ratio = dh[0] / dh[1]
+    assert 2.5 < ratio < 6.0, (dh, ratio)
+
+
+@pytest.mark.gpu
+def test_overlap_hmc_with_refreshed_bounds_and_back_to_wilson(gpu):
+    """With refresh the bounds and coefficients follow every gauge update (Overlap::update);
+    the trajectory runs and gives a finite dH.  set_overlap(None) returns the handle to
+    Wilson fermions: the same trajectory as a fresh Wilson HMC."""
+    Dq, ov, sp, tp = _overlap_hmc_setup()
+    H = q.HMC(Dq, 1 / 2.0, compact=True, npf=1, tau=0.2, nsteps=3, nsteps_inner=4,
+              tol=1e-11, solver=q.CG_F64, seed=9)
+    H.set_overlap(ov, refresh=True)
+    rep = H.trajectory(sp.copy(), tp.copy(), no_reject=True)
+    assert np.isfinite(rep["dH"]) and rep["max_resid"] <= 1e-11
+    H2 = q.HMC(Dq, 1 / 2.0, compact=True, npf=1, tau=0.2, nsteps=3, nsteps_inner=4,
+               tol=1e-12, solver=q.CG_F64, seed=9)
+    H2.set_overlap(ov)
+    H2.set_overlap(None)
+    H3 = q.HMC(Dq, 1 / 2.0, compact=True, npf=1, tau=0.2, nsteps=3, nsteps_inner=4,
+               tol=1e-12, solver=q.CG_F64, seed=9)
+    u2, t2, u3, t3 = sp.copy(), tp.copy(), sp.copy(), tp.copy()
+    r2 = H2.trajectory(u2, t2, no_reject=True)
+    r3 = H3.trajectory(u3, t3, no_reject=True)
+    assert abs(r2["dH"] - r3["dH"]) <= 1e-9 * max(1.0, abs(r3["dH"]))
+    assert np.allclose(u2, u3, rtol=0, atol=1e-12) and np.allclose(t2, t3, rtol=0, atol=1e-12)
