@@ -30,13 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision or CG kernel,
 # lattice, n, Lt, tile); the CG kernels ("K1-half" ...) from the half-storage solve the same
 # command runs at the pinned shape.
-MEASURED_TRAFFIC = {("f16", 1, 128, 64, 128, 64, 1, 1): 6.0730e8,
-                    ("f16", 1, 128, 64, 256, 64, 1, 1): 6.0736e8,
-                    ("f32", 1, 128, 64, 128, 64, 1, 1): 1.3277e9,
-                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3114e9,
-                    ("K1-half", 1, 128, 64, 256, 64, 1, 1): 1.0586e9,
-                    ("K2-half", 1, 128, 64, 256, 64, 1, 1): 7.8055e8,
-                    ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 2.1599e9}
+MEASURED_TRAFFIC = {("f16", 1, 128, 64, 256, 64, 1, 1): 5.0329e8,
+                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3096e9,
+                    ("K1-half", 1, 128, 64, 256, 64, 1, 1): 9.1420e8,
+                    ("K2-half", 1, 128, 64, 256, 64, 1, 1): 6.8241e8,
+                    ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 2.0112e9}
 
 
 def parse():
@@ -56,6 +54,7 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-learned", action="store_true")
     p.add_argument("--no-hmc", action="store_true")
+    p.add_argument("--no-overlap", action="store_true")
     p.add_argument("--hmc-traj", type=int, default=3)
     p.add_argument("--hmc-therm", type=int, default=6)
     p.add_argument("--hmc-inner", type=int, default=50)
@@ -188,6 +187,7 @@ def main():
     del D, d_x, rhs
     learned = learned_leg(a, q, g, with_net=not a.no_learned)
     hmc = None if a.no_hmc else hmc_leg(a, q)
+    overlap = None if a.no_overlap else overlap_leg(q)
     batched = batched_leg(a, q) if a.batched else None
 
     simp = {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
@@ -217,7 +217,7 @@ def main():
                   "speedup_mixed_vs_fp64": learned["speedup_mixed_vs_fp64"],
                   "speedup_half_vs_fp64": learned["speedup_half_vs_fp64"],
                   "net_apply": learned["net_apply"], "solution_agreement": learned["agreement"],
-                  "second_workload": simp, "hmc": hmc, "batched": batched}
+                  "second_workload": simp, "hmc": hmc, "overlap": overlap, "batched": batched}
         # the dominant kernel of the fastest CG: its K1 (the most time per solve)
         ck = learned["cg_kernels"]
         src = winner if winner in ck else "mixed"
@@ -503,6 +503,61 @@ def plain_leg(a, q, g, m, lattice):
                                       if fp64 else None),
             "speedup_half_vs_fp64": (round(methods["half"]["solves_per_s"] / fp64["solves_per_s"], 3)
                                      if fp64 else None), "net_apply": None}
+
+
+def overlap_leg(q, n=16, Lt=16, M5=-0.5, degree=21, tol=1e-10):
+    """Overlap fermion pieces (overlap.h:137-465) on simplicial n=16 x Lt=16 with the
+    Wilson kernel at M5 = -0.5: spectral bounds of D^dag D, one D_ov apply, the
+    pseudofermion force of every link, the CG on D_ov + D_ov^dag.  The force is checked
+    at this size against a central difference of Re <eta, D_ov eta> along a random link
+    direction (lambda_max and the Zolotarev coefficients fixed, eps = 1e-5)."""
+    g = q.Geometry(n)
+    at = 0.5 * mean_ell(g, 0)
+    D = q.WilsonDirac(0, n, Lt, r=1.0, M5=M5, at=at)
+    sp, tp = q.random_gauge(0, n, Lt, seed=77, kappa=1.4)
+    D.update(sp, tp)
+    t0 = time.time()
+    mu_min, mu_max = q.spectral_bounds(D, tol=1e-8, maxiter=2000)
+    t_bounds = time.time() - t0
+    ov = q.Overlap(D, n=degree, lambda_min=np.sqrt((1 - 1e-6) * mu_min),
+                   lambda_max=np.sqrt((1 + 1e-6) * mu_max), tol=tol)
+    z = q.zolotarev(degree, ov.lambda_min / ov.lambda_max)
+    eta = q.random_source(0, n, Lt, seed=3)
+    ov.apply(eta)  # warm-up
+    t0 = time.time()
+    w = ov.apply(eta)
+    t_apply = time.time() - t0
+    t0 = time.time()
+    fs, ft = ov.force(eta)
+    t_force = time.time() - t0
+    rng = np.random.default_rng(5)
+    vs, vt = rng.standard_normal(sp.shape), rng.standard_normal(tp.shape)
+    eps = 1e-5
+
+    def g_of(sgn):
+        D.update(sp + sgn * eps * vs, tp + sgn * eps * vt)
+        return float(np.real(np.vdot(eta, ov.apply(eta))))
+
+    fd = -2.0 * (g_of(1) - g_of(-1)) / (2 * eps)
+    D.update(sp, tp)
+    dirf = float(np.sum(fs * vs) + np.sum(ft * vt))
+    phi = q.random_source(0, n, Lt, seed=4)
+    t0 = time.time()
+    _, rep = ov.solve(phi, tol=tol)
+    t_solve = time.time() - t0
+    print(f"# overlap n={n} Lt={Lt}: k={ov.lambda_min / ov.lambda_max:.3e} apply {1e3 * t_apply:.1f} ms "
+          f"force {1e3 * t_force:.1f} ms solve {t_solve:.2f}s ({rep.iterations} its)",
+          file=sys.stderr, flush=True)
+    return {"workload": f"simplicial n={n} Lt={Lt} M5={M5} kappa=1.4, Zolotarev degree {degree}, "
+                        f"shifted solves to {tol}",
+            "points": D.V * Lt, "mu_min": mu_min, "mu_max": mu_max,
+            "k": ov.lambda_min / ov.lambda_max, "zolotarev_delta": z["Delta"],
+            "bounds_s": round(t_bounds, 3), "apply_ms": round(1e3 * t_apply, 2),
+            "unitarity_dev": float(abs(np.linalg.norm(w - eta) / np.linalg.norm(eta) - 1)),
+            "force_ms": round(1e3 * t_force, 2),
+            "force_vs_central_difference": abs(dirf - fd) / abs(fd),
+            "solve_s": round(t_solve, 3), "solve_iterations": rep.iterations,
+            "solve_true_resid": rep.true_resid}
 
 
 def hmc_leg(a, q):
