@@ -146,15 +146,26 @@ __global__ __launch_bounds__(256) void k_store_b0(const Spin<double>* __restrict
     half_store_b(b, r, inv, i);
 }
 
+struct RingAll {
+  SpinH* p[kXGroup];
+};
+
 // r = rfac * rt in half after a reliable update (rt holds the true residual at the
-// outgoing scale, rfac = old scale / new scale)
-__global__ __launch_bounds__(256) void k_rescale(const Spin<float>* __restrict__ rt, SpinH* r, size_t n,
-                                                 const CGScalars* sc) {
+// outgoing scale, rfac = old scale / new scale).  With copy_last the update ran at the end
+// of a chunk, so the next iteration starts a group (position 0, reading its p_old from the
+// last slot): the direction of the last iteration run is copied there.
+__global__ __launch_bounds__(256) void k_rescale(const Spin<float>* __restrict__ rt, SpinH* r, RingAll ring,
+                                                 int copy_last, size_t n, const CGScalars* sc) {
   if (sc->done != 0 || sc->rescale == 0) return;
   const float inv = float(sc->rfac);
+  const int last = sc->last_pos;
+  const SpinH* src = (copy_last && last != kXGroup - 1) ? ring.p[last] : nullptr;
+  SpinH* dst = ring.p[kXGroup - 1];
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x)
+       i += size_t(gridDim.x) * blockDim.x) {
     half_rescale_point(rt, r, inv, i);
+    if (src) dst[i] = src[i];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_resid_norms(const Spin<double>* __restrict__ b,
@@ -237,11 +248,11 @@ class EventTimer {
 };
 
 // Runs one_iter(i, k), i = 0 .. chunk-1 (the iteration's position in the chunk), k its
-// index in the solve (-1 while a graph is captured), in chunks of iterations, captured once as a HIP graph unless
+// index in the solve (-1 while a graph is captured), then tail(), in chunks of iterations, captured once as a HIP graph unless
 // disabled, with one host poll of the device scalars per chunk, until the device marks
 // the solve done.  On return hsc holds the final scalars.
-template <class Iter>
-void drive_chunks(const Iter& one_iter, const CGOptions& o, size_t n, hipStream_t st,
+template <class Iter, class Tail>
+void drive_chunks(const Iter& one_iter, const Tail& tail, const CGOptions& o, size_t n, hipStream_t st,
                   const CGScalars* sc, CGScalars* hsc) {
   int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
   chunk = (chunk + kXGroup - 1) / kXGroup * kXGroup;  // every replay starts a new group
@@ -251,15 +262,20 @@ void drive_chunks(const Iter& one_iter, const CGOptions& o, size_t n, hipStream_
   if (o.use_graph && !o.times && std::getenv("QED3_NO_GRAPH") == nullptr) {
     QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < chunk; i++) one_iter(i, -1);
+    tail();
     QED3_HIP(hipStreamEndCapture(st, &graph));
     QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
-  const int max_chunks = o.maxiter / chunk + 4;
+  // every chunk runs at least one iteration (a reliable update ends each chunk), so the
+  // device's iteration cap ends the solve within this many chunks
+  const int max_chunks = o.maxiter + 4;
   for (int c = 0; c < max_chunks; c++) {
-    if (exec)
+    if (exec) {
       QED3_HIP(hipGraphLaunch(exec, st));
-    else
+    } else {
       for (int i = 0; i < chunk; i++) one_iter(i, c * chunk + i);
+      tail();
+    }
     QED3_HIP(hipGetLastError());
     QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
     QED3_HIP(hipStreamSynchronize(st));
@@ -330,7 +346,7 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
     launch_stencil<double, true>(v64, nn, W.t, EpiResid<T>{b, W.r, sc}, D.tile<double>(true), gr, st, hv64t);
   };
 
-  drive_chunks(one_iter, o, n, st, sc, hsc);
+  drive_chunks(one_iter, [] {}, o, n, st, sc, hsc);
   timer.finish(hsc->iter);
   if (o.times) o.times->ms[2] = o.times->ms[1], o.times->count[2] = o.times->count[1];
   if (kMixed) hipLaunchKernelGGL(k_flush<T>, dim3(nbk), dim3(256), 0, st, x, W.xin, n, sc, 1);
@@ -394,6 +410,22 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
     launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
                                     timer.pair(EpiK2H<POS>::kUpdateX ? 2 : 1, k));
   };
+  // The reliable update (flush the pending x terms, FP64 r = b - D^dag D x, re-store r at the
+  // new scale); its kernels exit at once unless an update is due.  It runs once per chunk of
+  // iterations: an iteration that asks for it makes the rest of the chunk's iterations exit
+  // at once (EpiK1PH/EpiK2H::skip), and the next chunk starts a group with the last
+  // direction copied into the group's p_old slot.  With kernel timing (direct launches whose
+  // events must each bracket a real iteration) it runs after every iteration instead.
+  const bool ru_each = o.times != nullptr;
+  RingAll ring_all;
+  for (int j = 0; j < kXGroup; j++) ring_all.p[j] = ring[j];
+  auto reliable_update = [&](int copy_last) {
+    hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0);
+    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
+    launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr,
+                                 st, hv64t);
+    hipLaunchKernelGGL(k_rescale, dim3(ngate), dim3(256), 0, st, rt, r, ring_all, copy_last, n, sc);
+  };
   auto one_iter = [&](int i, int k) {
     const int pos = i % kXGroup;
     SpinH* pold = ring[(pos + kXGroup - 1) % kXGroup];
@@ -407,13 +439,12 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
       case 2: k2(std::integral_constant<int, 2>{}, k); break;
       default: k2(std::integral_constant<int, 3>{}, k); break;
     }
-    hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0);
-    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
-    launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr,
-                                 st, hv64t);
-    hipLaunchKernelGGL(k_rescale, dim3(ngate), dim3(256), 0, st, rt, r, n, sc);
+    if (ru_each) reliable_update(0);
   };
-  drive_chunks(one_iter, o, n, st, sc, hsc);
+  if (ru_each)
+    drive_chunks(one_iter, [] {}, o, n, st, sc, hsc);
+  else
+    drive_chunks(one_iter, [&] { reliable_update(1); }, o, n, st, sc, hsc);
   timer.finish(hsc->iter);
   hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));

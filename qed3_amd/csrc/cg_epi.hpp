@@ -147,7 +147,9 @@ struct EpiK1PH {
   SpinH* w;
   SpinH* pnew;
   CGScalars* sc;
-  __device__ bool skip() const { return sc->done != 0; }
+  // an iteration after one that asked for a reliable update waits for it (the single-system
+  // solver runs the update once per chunk of iterations)
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {}
   __device__ void operator()(size_t i, const Spin<float>& v, double2& acc) const {
     const SpinH h = to_half(v);
@@ -179,7 +181,7 @@ struct EpiK2H {
   CGScalars* sc;
   float alpha;
   float ap[kXGroup - 1];
-  __device__ bool skip() const { return sc->done != 0; }
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
 #pragma unroll
@@ -228,6 +230,7 @@ struct EpiK2H {
     } else {
       sc->alpha_pend[POS] = sc->alpha;
     }
+    sc->last_pos = POS;
     sc->iter += 1;
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
@@ -309,6 +312,7 @@ __device__ __forceinline__ void half_start(CGScalars* sc, double b2, double ncom
   sc->rfac = 1.0;
   for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
   sc->rescale = 0;
+  sc->last_pos = kXGroup - 1;
   sc->gam = 0;
   sc->alpha = 0;
   sc->beta = 0;

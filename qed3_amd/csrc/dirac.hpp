@@ -77,6 +77,7 @@ struct CGScalars {
   int need_ru;     // a reliable update is due in this iteration
   int iter, n_ru, maxiter;
   int rescale;     // half-storage CG: the residual is to be re-stored at the new scale
+  int last_pos;    // half-storage CG: group position of the last iteration run
 };
 
 class Dirac {
