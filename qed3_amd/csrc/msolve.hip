@@ -2,19 +2,20 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <random>
 #include <stdexcept>
 
 #include "msolve.hpp"
+#include "stencil.hpp"
 
 namespace qed3 {
 namespace {
 
+// fields y_k with coefficients a_k for k_combine
 struct ShiftSet {
   Spin<double>* x[kMaxShifts];
-  Spin<double>* p[kMaxShifts];
-  double a[kMaxShifts];  // alpha_i (kernel A) or zeta_i (kernel B)
-  double b[kMaxShifts];  // beta_i (kernel B)
+  double a[kMaxShifts];
   int n;
 };
 
@@ -46,10 +47,77 @@ __global__ __launch_bounds__(256) void k_pair_norms(const Spin<double>* __restri
   store_partial(acc, partials, blockIdx.x);
 }
 
-// r -= alpha (q + s0 p); x_i += alpha_i p_i; partial |r|^2
-__global__ __launch_bounds__(256) void k_ms_a(Spin<double>* r, const Spin<double>* __restrict__ q,
-                                              const Spin<double>* __restrict__ p, double alpha,
-                                              double s0, ShiftSet S, size_t n, double2* partials) {
+// ---- device-driven multi-shift CG: every scalar recursion runs in one-block finalisers, so a
+// chunk of iterations is one graph replay and the host polls once per chunk ----------------
+struct MSScalars {
+  double rho, bn2, alpha, beta, a_prev, b_prev, s0, tol2;
+  double shift[kMaxShifts], z[kMaxShifts], zo[kMaxShifts], zn[kMaxShifts], ai[kMaxShifts], bi[kMaxShifts];
+  int active[kMaxShifts];  // still iterating
+  int now[kMaxShifts];     // active in the current iteration (what K_a and K_b update)
+  int iters[kMaxShifts];
+  int ns, it, maxiter;
+  int done;  // 0 running, 1 every shift converged, 3 iteration cap, 4 breakdown
+};
+
+struct ShiftPtrs {
+  Spin<double>* x[kMaxShifts];
+  Spin<double>* p[kMaxShifts];
+  int n;
+};
+
+// D or D^dag into out, skipped once the solve is done
+struct EpiStoreMS {
+  static constexpr bool kReduce = false;
+  Spin<double>* out;
+  const MSScalars* sc;
+  __device__ bool skip() const { return sc->done != 0; }
+  __device__ void begin() {}
+  __device__ void operator()(size_t i, const Spin<double>& v, double2&) const { out[i] = v; }
+  __device__ void operator()(double2) const {}
+};
+
+// partials of (|w|^2, |p|^2)
+__global__ __launch_bounds__(256) void k_msd_norms(const Spin<double>* __restrict__ w,
+                                                   const Spin<double>* __restrict__ p, size_t n,
+                                                   double2* partials, const MSScalars* sc) {
+  if (sc->done != 0) return;
+  double2 acc = {0.0, 0.0};
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    acc.x += nrm2(w[i]);
+    acc.y += nrm2(p[i]);
+  }
+  store_partial(acc, partials, blockIdx.x);
+}
+
+// alpha = rho / <p, (A + s0) p>; zeta_i and alpha_i of every active shift (Jegerlehner)
+struct FinMSAlpha {
+  MSScalars* sc;
+  __device__ bool skip() const { return sc->done != 0; }
+  __device__ void operator()(double2 t) const {
+    const double pq = t.x + sc->s0 * t.y;
+    if (!(pq > 0.0) || !(pq < 1e300)) {
+      sc->done = 4;
+      return;
+    }
+    const double alpha = sc->rho / pq, ap = sc->a_prev, bp = sc->b_prev;
+    sc->alpha = alpha;
+    for (int i = 0; i < sc->ns; i++) {
+      sc->now[i] = sc->active[i];
+      if (!sc->active[i]) continue;
+      const double ds = sc->shift[i] - sc->s0, z = sc->z[i], zo = sc->zo[i];
+      const double zn = z * zo * ap / (alpha * bp * (zo - z) + zo * ap * (1.0 + ds * alpha));
+      sc->zn[i] = zn;
+      sc->ai[i] = alpha * zn / z;
+    }
+  }
+};
+
+// r -= alpha (q + s0 p); x_i += alpha_i p_i for the shifts active now; partial |r|^2
+__global__ __launch_bounds__(256) void k_msd_a(Spin<double>* r, const Spin<double>* __restrict__ q,
+                                               const Spin<double>* __restrict__ p, ShiftPtrs S, size_t n,
+                                               double2* partials, const MSScalars* sc) {
+  if (sc->done != 0) return;
+  const double alpha = sc->alpha, s0 = sc->s0;
   double2 acc = {0.0, 0.0};
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     const Spin<double> pp = p[i], qq = q[i];
@@ -59,28 +127,64 @@ __global__ __launch_bounds__(256) void k_ms_a(Spin<double>* r, const Spin<double
     r[i] = rr;
     acc.x += nrm2(rr);
     for (int k = 0; k < S.n; k++) {
+      if (!sc->now[k]) continue;  // uniform
       Spin<double> xv = S.x[k][i];
-      axpy_(xv, S.a[k], S.p[k][i]);
+      axpy_(xv, sc->ai[k], S.p[k][i]);
       S.x[k][i] = xv;
     }
   }
   store_partial(acc, partials, blockIdx.x);
 }
 
-// p_i = zeta_i r + beta_i p_i; p = r + beta p
-__global__ __launch_bounds__(256) void k_ms_b(const Spin<double>* __restrict__ r, Spin<double>* p,
-                                              double beta, ShiftSet S, size_t n) {
+// beta, beta_i; then the shifts' zeta history, iteration counts and convergence
+// (|zeta_i| |r| <= tol |b|), the iteration cap
+struct FinMSBeta {
+  MSScalars* sc;
+  __device__ bool skip() const { return sc->done != 0; }
+  __device__ void operator()(double2 t) const {
+    const double rho_new = t.x, beta = rho_new / sc->rho;
+    sc->beta = beta;
+    sc->it += 1;
+    int any = 0;
+    for (int i = 0; i < sc->ns; i++) {
+      if (!sc->active[i]) continue;
+      const double z = sc->z[i], zn = sc->zn[i];
+      sc->bi[i] = beta * (zn / z) * (zn / z);
+      sc->zo[i] = z;
+      sc->z[i] = zn;
+      sc->iters[i] = sc->it;
+      if (fabs(zn) * sqrt(rho_new) <= sqrt(sc->tol2 * sc->bn2))
+        sc->active[i] = 0;
+      else
+        any = 1;
+    }
+    sc->a_prev = sc->alpha;
+    sc->b_prev = beta;
+    sc->rho = rho_new;
+    if (!any)
+      sc->done = 1;
+    else if (sc->it >= sc->maxiter)
+      sc->done = 3;
+  }
+};
+
+// p = r + beta p; p_i = zeta_i r + beta_i p_i for the shifts active now
+__global__ __launch_bounds__(256) void k_msd_b(const Spin<double>* __restrict__ r, Spin<double>* p, ShiftPtrs S,
+                                               size_t n, const MSScalars* sc) {
+  if (sc->done != 0) return;
+  const double beta = sc->beta;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     const Spin<double> rr = r[i];
-    Spin<double> pp = p[i];
     Spin<double> v = rr;
-    axpy_(v, beta, pp);
+    axpy_(v, beta, p[i]);
     p[i] = v;
     for (int k = 0; k < S.n; k++) {
+      if (!sc->now[k]) continue;  // uniform
+      const double a = sc->zn[k], bb = sc->bi[k];
       const Spin<double> pk = S.p[k][i];
       Spin<double> o;
-      o.a = {S.a[k] * rr.a.x + S.b[k] * pk.a.x, S.a[k] * rr.a.y + S.b[k] * pk.a.y};
-      o.b = {S.a[k] * rr.b.x + S.b[k] * pk.b.x, S.a[k] * rr.b.y + S.b[k] * pk.b.y};
+      o.a = {a * rr.a.x + bb * pk.a.x, a * rr.a.y + bb * pk.a.y};
+      o.b = {a * rr.b.x + bb * pk.b.x, a * rr.b.y + bb * pk.b.y};
       S.p[k][i] = o;
     }
   }
@@ -188,60 +292,72 @@ std::vector<CGResult> multishift_cg(Dirac& D, const std::vector<double>& shifts,
     QED3_HIP(hipMemcpyAsync(ps[i], b, bytes, hipMemcpyDeviceToDevice, st));
   }
   hipLaunchKernelGGL(k_pair_norms, dim3(nb), dim3(256), 0, st, b, b, n, partials);
-  double rho = red.get().x;
-  const double bn2 = rho;
-  std::vector<double> z(ns, 1.0), zo(ns, 1.0), zn(ns, 1.0);
-  std::vector<int> active(ns, bn2 > 0 ? 1 : 0), iters(ns, 0);
-  double a_prev = 1.0, b_prev = 0.0;
-  int it = 0;
-  auto pack = [&](ShiftSet& S, const std::vector<double>& va, const std::vector<double>* vb) {
-    S.n = 0;
-    for (int i = 0; i < ns; i++) {
-      if (!active[i]) continue;
-      S.x[S.n] = x[i];
-      S.p[S.n] = ps[i];
-      S.a[S.n] = va[i];
-      S.b[S.n] = vb ? (*vb)[i] : 0.0;
-      S.n++;
-    }
-  };
-  std::vector<double> ai(ns), bi(ns);
-  while (it < o.maxiter && std::any_of(active.begin(), active.end(), [](int a) { return a != 0; })) {
-    D.apply<double>(false, w, p, false);
-    D.apply<double>(true, q, w, false);
-    hipLaunchKernelGGL(k_pair_norms, dim3(nb), dim3(256), 0, st, w, p, n, partials);
-    const double2 wp = red.get();
-    const double pq = wp.x + s0 * wp.y;
-    if (!(pq > 0) || !std::isfinite(pq)) break;
-    const double alpha = rho / pq;
-    for (int i = 0; i < ns; i++) {
-      if (!active[i]) continue;
-      const double ds = shifts[i] - s0;
-      zn[i] = z[i] * zo[i] * a_prev / (alpha * b_prev * (zo[i] - z[i]) + zo[i] * a_prev * (1.0 + ds * alpha));
-      ai[i] = alpha * zn[i] / z[i];
-    }
-    ShiftSet S;
-    pack(S, ai, nullptr);
-    hipLaunchKernelGGL(k_ms_a, dim3(nb), dim3(256), 0, st, r, q, p, alpha, s0, S, n, partials);
-    const double rho_new = red.get().x;
-    const double beta = rho_new / rho;
-    for (int i = 0; i < ns; i++)
-      if (active[i]) bi[i] = beta * (zn[i] / z[i]) * (zn[i] / z[i]);
-    pack(S, zn, &bi);
-    hipLaunchKernelGGL(k_ms_b, dim3(nb), dim3(256), 0, st, r, p, beta, S, n);
-    QED3_HIP(hipGetLastError());
-    it++;
-    for (int i = 0; i < ns; i++) {
-      if (!active[i]) continue;
-      zo[i] = z[i];
-      z[i] = zn[i];
-      iters[i] = it;
-      if (std::abs(z[i]) * std::sqrt(rho_new) <= o.tol * std::sqrt(bn2)) active[i] = 0;
-    }
-    a_prev = alpha;
-    b_prev = beta;
-    rho = rho_new;
+  const double bn2 = red.get().x;
+  MSScalars h{};
+  h.rho = bn2;
+  h.bn2 = bn2;
+  h.a_prev = 1.0;
+  h.b_prev = 0.0;
+  h.s0 = s0;
+  h.tol2 = o.tol * o.tol;
+  h.ns = ns;
+  h.maxiter = o.maxiter;
+  for (int i = 0; i < ns; i++) {
+    h.shift[i] = shifts[i];
+    h.z[i] = h.zo[i] = h.zn[i] = 1.0;
+    h.active[i] = bn2 > 0 ? 1 : 0;
   }
+  h.done = (bn2 > 0 && o.maxiter > 0) ? 0 : (bn2 > 0 ? 3 : 1);
+  MSScalars* sc = dnew<MSScalars>(1, st);
+  MSScalars* hs = nullptr;
+  QED3_HIP(hipHostMalloc((void**)&hs, sizeof(MSScalars), hipHostMallocDefault));
+  *hs = h;
+  QED3_HIP(hipMemcpyAsync(sc, hs, sizeof(MSScalars), hipMemcpyHostToDevice, st));
+  ShiftPtrs S;
+  S.n = ns;
+  for (int i = 0; i < ns; i++) {
+    S.x[i] = x[i];
+    S.p[i] = ps[i];
+  }
+  const DiracView<double> v64 = D.view<double>();
+  const int nn = D.tables().max_nn;
+  const HaloView hvf = D.halo_for<double>(false), hvt = D.halo_for<double>(true);
+  const GridReduce gr = D.reducer();
+  auto one_iter = [&] {
+    launch_stencil<double, false>(v64, nn, p, EpiStoreMS{w, sc}, D.tile<double>(false), gr, st, hvf);
+    launch_stencil<double, true>(v64, nn, w, EpiStoreMS{q, sc}, D.tile<double>(true), gr, st, hvt);
+    hipLaunchKernelGGL(k_msd_norms, dim3(nb), dim3(256), 0, st, w, p, n, partials, sc);
+    hipLaunchKernelGGL(k_finalize<FinMSAlpha>, dim3(1), dim3(256), 0, st, partials, nb, FinMSAlpha{sc});
+    hipLaunchKernelGGL(k_msd_a, dim3(nb), dim3(256), 0, st, r, q, p, S, n, partials, sc);
+    hipLaunchKernelGGL(k_finalize<FinMSBeta>, dim3(1), dim3(256), 0, st, partials, nb, FinMSBeta{sc});
+    hipLaunchKernelGGL(k_msd_b, dim3(nb), dim3(256), 0, st, r, p, S, n, sc);
+  };
+  // chunks of iterations replayed as one graph; the kernels of iterations after the end
+  // exit at once
+  const int chunk = 8;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (std::getenv("QED3_NO_GRAPH") == nullptr) {
+    QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int c = 0; c < chunk; c++) one_iter();
+    QED3_HIP(hipStreamEndCapture(st, &graph));
+    QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  for (int c = 0; c <= o.maxiter / chunk + 1; c++) {
+    QED3_HIP(hipMemcpyAsync(hs, sc, sizeof(MSScalars), hipMemcpyDeviceToHost, st));
+    QED3_HIP(hipStreamSynchronize(st));
+    if (hs->done != 0) break;
+    if (exec)
+      QED3_HIP(hipGraphLaunch(exec, st));
+    else
+      for (int k = 0; k < chunk; k++) one_iter();
+    QED3_HIP(hipGetLastError());
+  }
+  QED3_HIP(hipMemcpyAsync(hs, sc, sizeof(MSScalars), hipMemcpyDeviceToHost, st));
+  QED3_HIP(hipStreamSynchronize(st));
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  std::vector<int> iters(hs->iters, hs->iters + ns);
   // FP64 true residual per shift
   std::vector<CGResult> res(ns);
   for (int i = 0; i < ns; i++) {
@@ -253,11 +369,12 @@ std::vector<CGResult> multishift_cg(Dirac& D, const std::vector<double>& shifts,
     res[i].true_resid = bn2 > 0 ? std::sqrt(rr / bn2) : 0.0;
     res[i].status = res[i].true_resid <= o.tol * 1.0000001 ? 0 : 3;
   }
-  for (void* ptr : {(void*)r, (void*)p, (void*)w, (void*)q, (void*)partials, (void*)out})
+  for (void* ptr : {(void*)r, (void*)p, (void*)w, (void*)q, (void*)partials, (void*)out, (void*)sc})
     (void)hipFreeAsync(ptr, st);
   for (auto* ptr : ps) (void)hipFreeAsync(ptr, st);
   QED3_HIP(hipStreamSynchronize(st));
   (void)hipHostFree(host);
+  (void)hipHostFree(hs);
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (auto& r2 : res) r2.seconds = sec;
   return res;
@@ -364,9 +481,7 @@ void Overlap::apply(bool dag, Spin<double>* out, const Spin<double>* in) {
   S.n = int(sol_.size());
   for (int m = 0; m < S.n; m++) {
     S.x[m] = sol_[m];
-    S.p[m] = nullptr;
     S.a[m] = z_.A[m + 1] * lmax_ * lmax_;
-    S.b[m] = 0;
   }
   if (!dag) {
     // t = psi + sum A_m Z_m ; out = psi + (C/lmax) D t
@@ -413,9 +528,7 @@ void Overlap::force(const Spin<double>* eta, double* fs, double* ft) {
   S.n = ns;
   for (int m = 0; m < ns; m++) {
     S.x[m] = sol_[m];
-    S.p[m] = nullptr;
     S.a[m] = z_.A[m + 1] * l2;
-    S.b[m] = 0;
   }
   hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, st, t_, static_cast<const Spin<double>*>(nullptr), 0.0,
                      eta, 1.0, S, n);
