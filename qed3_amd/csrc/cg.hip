@@ -511,6 +511,7 @@ void tune_cg_tiles(Dirac& D) {
   QED3_HIP(hipGetLastError());
   CGOptions o;
   o.tol = 1e-10;
+  o.maxiter = 400;  // bounds the tuning time on badly conditioned operators
   o.mixed = o.half = true;
   // The first candidate, which the others must beat by 1%, is the whole time extent per
   // block (up to 64 slices): then every block of an XCD steps through the same slice and the
