@@ -106,7 +106,7 @@ def algorithmic_bytes_D(V, E, Lt, real_bytes):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "qed3_amd runs on one GPU; --gpus must be 1"}))
         sys.exit(1)
     import qed3_amd as q
 
