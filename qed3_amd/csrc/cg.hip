@@ -126,24 +126,25 @@ struct FinStartH {
 // Half-storage flush: x64 += scale (x_inner + sum_j alpha_pend[j] ring[j]), x_inner = 0,
 // taking the x terms the current group's iterations left pending (alpha_pend[j] = 0 if none)
 __global__ __launch_bounds__(256) void k_flush_h(Spin<double>* x64, Spin<float>* xin, PendRing ring,
-                                                 size_t n, const CGScalars* sc, int always) {
+                                                 size_t n, const CGScalars* sc, int always,
+                                                 const phase16_t* rot) {
   if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
   const double s = sc->scale;
   float ap[kXGroup - 1];
   for (int j = 0; j < kXGroup - 1; j++) ap[j] = float(sc->alpha_pend[j]);
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x)
-    half_flush_point(x64, xin, ring, ap, s, i);
+    half_flush_point(x64, xin, ring, ap, s, i, rot);
 }
 
 // r = b / scale in half (FP64 arithmetic, so any finite b is representable)
 __global__ __launch_bounds__(256) void k_store_b0(const Spin<double>* __restrict__ b, SpinH* r, size_t n,
-                                                  const CGScalars* sc) {
+                                                  const CGScalars* sc, const phase16_t* rot) {
   if (sc->done != 0) return;
   const double inv = 1.0 / sc->scale;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x)
-    half_store_b(b, r, inv, i);
+    half_store_b(b, r, inv, i, rot);
 }
 
 struct RingAll {
@@ -155,7 +156,8 @@ struct RingAll {
 // of a chunk, so the next iteration starts a group (position 0, reading its p_old from the
 // last slot): the direction of the last iteration run is copied there.
 __global__ __launch_bounds__(256) void k_rescale(const Spin<float>* __restrict__ rt, SpinH* r, RingAll ring,
-                                                 int copy_last, size_t n, const CGScalars* sc) {
+                                                 int copy_last, size_t n, const CGScalars* sc,
+                                                 const phase16_t* rot) {
   if (sc->done != 0 || sc->rescale == 0) return;
   const float inv = float(sc->rfac);
   const int last = sc->last_pos;
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(256) void k_rescale(const Spin<float>* __restrict__
   SpinH* dst = ring.p[kXGroup - 1];
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
-    half_rescale_point(rt, r, inv, i);
+    half_rescale_point(rt, r, inv, i, rot);
     if (src) dst[i] = src[i];
   }
 }
@@ -388,10 +390,13 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kXGroup - 1], n, gr);
   hipLaunchKernelGGL(k_finalize<FinStartH>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
                      FinStartH{sc, 4.0 * double(n), o.tol * o.tol, o.delta * o.delta, o.maxiter});
-  hipLaunchKernelGGL(k_store_b0, dim3(nbk), dim3(256), 0, st, b, r, n, sc);
+  // the inner fields live in the temporal gauge (exp(i phi) at every point): b is rotated
+  // into it here, the true residual at every reliable update, and x out of it at each flush
+  const DiracView<float, phase16_t> vin = D.view_half_temporal_gauge();
+  const phase16_t* rot = D.half_gauge_rotation();
+  hipLaunchKernelGGL(k_store_b0, dim3(nbk), dim3(256), 0, st, b, r, n, sc, rot);
   QED3_HIP(hipGetLastError());
 
-  const DiracView<float, phase16_t> vin = D.view_half_phases();
   const DiracView<double> v64 = D.view<double>();
   const int nn = D.tables().max_nn;
   const HaloView hvf = D.halo_for<float>(false), hvt = D.halo_for<float>(true);
@@ -420,11 +425,11 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   RingAll ring_all;
   for (int j = 0; j < kXGroup; j++) ring_all.p[j] = ring[j];
   auto reliable_update = [&](int copy_last) {
-    hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0);
+    hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0, rot);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
     launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr,
                                  st, hv64t);
-    hipLaunchKernelGGL(k_rescale, dim3(ngate), dim3(256), 0, st, rt, r, ring_all, copy_last, n, sc);
+    hipLaunchKernelGGL(k_rescale, dim3(ngate), dim3(256), 0, st, rt, r, ring_all, copy_last, n, sc, rot);
   };
   auto one_iter = [&](int i, int k) {
     const int pos = i % kXGroup;
@@ -446,7 +451,7 @@ CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGO
   else
     drive_chunks(one_iter, [&] { reliable_update(1); }, o, n, st, sc, hsc);
   timer.finish(hsc->iter);
-  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1);
+  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1, rot);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
   CGResult res;

@@ -330,9 +330,20 @@ struct PendRing {
   const SpinH* p[kXGroup - 1];
 };
 
-// x64 += scale (x_inner + sum_j alpha_pend[j] ring[j]) at point i, x_inner = 0
+// The inner fields of the single-system half-storage solver live in the temporal gauge:
+// field' = exp(i phi) field at every point (rot = phi as 16-bit angles; null: no rotation).
+__device__ __forceinline__ void rotate(Spin<float>& v, const phase16_t* rot, size_t i, bool inverse) {
+  if (!rot) return;
+  float2 u = ldu(rot + i);
+  if (inverse) u.y = -u.y;
+  v.a = cmul<float>(u, v.a);
+  v.b = cmul<float>(u, v.b);
+}
+
+// x64 += scale exp(-i phi) (x_inner + sum_j alpha_pend[j] ring[j]) at point i, x_inner = 0
 __device__ __forceinline__ void half_flush_point(Spin<double>* x64, Spin<float>* xin, const PendRing& ring,
-                                                 const float* ap, double s, size_t i) {
+                                                 const float* ap, double s, size_t i,
+                                                 const phase16_t* rot = nullptr) {
   Spin<float> a = xin[i];
 #pragma unroll
   for (int j = 0; j < kXGroup - 1; j++) {
@@ -344,6 +355,7 @@ __device__ __forceinline__ void half_flush_point(Spin<double>* x64, Spin<float>*
       a.b.y += ap[j] * pp.b.y;
     }
   }
+  rotate(a, rot, i, true);
   Spin<double> xx = x64[i];
   xx.a.x += s * a.a.x;
   xx.a.y += s * a.a.y;
@@ -356,22 +368,26 @@ __device__ __forceinline__ void half_flush_point(Spin<double>* x64, Spin<float>*
   xin[i] = z;
 }
 
-// r = b / scale in binary16 at point i (FP64 arithmetic, so any finite b fits)
-__device__ __forceinline__ void half_store_b(const Spin<double>* b, SpinH* r, double inv, size_t i) {
+// r = exp(i phi) b / scale in binary16 at point i (FP64 scaling, so any finite b fits)
+__device__ __forceinline__ void half_store_b(const Spin<double>* b, SpinH* r, double inv, size_t i,
+                                             const phase16_t* rot = nullptr) {
   const Spin<double> bb = b[i];
   Spin<float> v;
   v.a = {float(bb.a.x * inv), float(bb.a.y * inv)};
   v.b = {float(bb.b.x * inv), float(bb.b.y * inv)};
+  rotate(v, rot, i, false);
   r[i] = to_half(v);
 }
 
-// r = rfac * rt in binary16 at point i (after a reliable update)
-__device__ __forceinline__ void half_rescale_point(const Spin<float>* rt, SpinH* r, float f, size_t i) {
+// r = exp(i phi) rfac rt in binary16 at point i (after a reliable update)
+__device__ __forceinline__ void half_rescale_point(const Spin<float>* rt, SpinH* r, float f, size_t i,
+                                                   const phase16_t* rot = nullptr) {
   Spin<float> v = rt[i];
   v.a.x *= f;
   v.a.y *= f;
   v.b.x *= f;
   v.b.y *= f;
+  rotate(v, rot, i, false);
   r[i] = to_half(v);
 }
 

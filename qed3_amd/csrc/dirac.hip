@@ -24,13 +24,41 @@ __global__ void k_phases(const double* __restrict__ ang, cplx_t<T>* __restrict__
   }
 }
 
+// 16-bit fixed-point angle in units of 2 pi / 65536, wrapped into [-pi, pi)
+__device__ __forceinline__ phase16_t angle16(double a) {
+  const long long k = llrint(a * (65536.0 / (2.0 * M_PI)));
+  return phase16_t{static_cast<short>(static_cast<unsigned short>(k & 0xFFFF))};
+}
+
 __global__ void k_phases_half(const double* __restrict__ ang, phase16_t* __restrict__ u, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) {
-    // 16-bit fixed-point angle in units of 2 pi / 65536, wrapped into [-pi, pi)
-    const long long k = llrint(ang[i] * (65536.0 / (2.0 * M_PI)));
-    u[i] = phase16_t{static_cast<short>(static_cast<unsigned short>(k & 0xFFFF))};
+  if (i < n) u[i] = angle16(ang[i]);
+}
+
+// Temporal gauge, per site: phi_0 = 0, phi_{s+1} = phi_s + theta^t_s (FP64 and 16-bit), and
+// the wrap link's angle phi_{Lt-1} + theta^t_{Lt-1} = the sum of the site's temporal angles.
+__global__ void k_tg_sites(const double* __restrict__ angT, double* __restrict__ phi64,
+                           phase16_t* __restrict__ rot, phase16_t* __restrict__ poly, int V, int Lt) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= V) return;
+  double phi = 0.0;
+  for (int s = 0; s < Lt; s++) {
+    const size_t i = size_t(s) * V + x;
+    phi64[i] = phi;
+    rot[i] = angle16(phi);
+    phi += angT[i];
   }
+  poly[x] = angle16(phi);
+}
+
+// Temporal gauge, per spatial link l = (x -> y) at slice s: theta_l + phi_s(x) - phi_s(y).
+__global__ void k_tg_links(const double* __restrict__ angS, const double* __restrict__ phi64,
+                           const int4* __restrict__ ends, phase16_t* __restrict__ out, int E, int V, int Lt) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(Lt) * E) return;
+  const int s = int(i / E), l = int(i % E);
+  const int4 e = ends[l];
+  out[i] = angle16(angS[i] + phi64[size_t(s) * V + e.x] - phi64[size_t(s) * V + e.y]);
 }
 
 // <x,y> = sum conj(x) y over complex double fields of n points.
@@ -285,7 +313,7 @@ Dirac::~Dirac() {
                   (void*)d64_.diag, (void*)d64_.kth, (void*)d64_.uS, (void*)d64_.uT,
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
                   (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
-                  (void*)uSh_, (void*)uTh_,
+                  (void*)uSh_, (void*)uTh_, (void*)uStg_, (void*)uPolyH_, (void*)rotH_, (void*)phi64_,
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
@@ -357,6 +385,7 @@ void Dirac::set_gauge(const double* spatial, const double* temporal, bool on_dev
   hipLaunchKernelGGL(k_phases<float>, dim3((nT + B - 1) / B), dim3(B), 0, stream_, angT_, d32_.uT, nT);
   QED3_HIP(hipGetLastError());
   if (uSh_) fill_half_phases();
+  if (uStg_) fill_temporal_gauge();
   QED3_HIP(hipStreamSynchronize(stream_));
   has_gauge_ = true;
 }
@@ -367,6 +396,31 @@ void Dirac::fill_half_phases() {
   hipLaunchKernelGGL(k_phases_half, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, uSh_, nS);
   hipLaunchKernelGGL(k_phases_half, dim3((nT + B - 1) / B), dim3(B), 0, stream_, angT_, uTh_, nT);
   QED3_HIP(hipGetLastError());
+}
+
+void Dirac::fill_temporal_gauge() {
+  const int B = 256;
+  hipLaunchKernelGGL(k_tg_sites, dim3((h_.V + B - 1) / B), dim3(B), 0, stream_, angT_, phi64_, rotH_, uPolyH_,
+                     h_.V, h_.Lt);
+  const size_t nS = size_t(h_.Lt) * h_.E;
+  hipLaunchKernelGGL(k_tg_links, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, phi64_,
+                     reinterpret_cast<const int4*>(link_ends_), uStg_, h_.E, h_.V, h_.Lt);
+  QED3_HIP(hipGetLastError());
+}
+
+DiracView<float, phase16_t> Dirac::view_half_temporal_gauge() {
+  DiracView<float, phase16_t> v = view_half_phases();
+  if (!uStg_) {
+    uStg_ = dalloc<phase16_t>(size_t(h_.Lt) * h_.E);
+    uPolyH_ = dalloc<phase16_t>(size_t(h_.V));
+    rotH_ = dalloc<phase16_t>(size_t(h_.Lt) * h_.V);
+    phi64_ = dalloc<double>(size_t(h_.Lt) * h_.V);
+    if (has_gauge_) fill_temporal_gauge();
+  }
+  v.uS = uStg_;
+  v.uT = nullptr;
+  v.uPoly = uPolyH_;
+  return v;
 }
 
 DiracView<float, phase16_t> Dirac::view_half_phases() {

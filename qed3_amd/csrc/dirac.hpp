@@ -26,6 +26,9 @@ struct DiracView {
   const T* kth = nullptr;            // [V] kappa_t/2
   const P* uS = nullptr;             // [Lt][E] exp(i theta_s(l))
   const P* uT = nullptr;             // [Lt][V] exp(i theta^t_s(x))
+  // Temporal gauge (set instead of uT): every temporal link is 1 except those from the last
+  // slice to the first, whose phase is the product of the site's temporal phases ([V]).
+  const P* uPoly = nullptr;
   int V = 0, E = 0, Lt = 1;
   T r = 1;
   T bc = -1;  // sign on the time wrap: -1 antiperiodic, +1 periodic
@@ -103,6 +106,13 @@ class Dirac {
   // FP32 tables with 16-bit fixed-point link angles (kept up to date with the gauge field
   // from the first call on)
   DiracView<float, phase16_t> view_half_phases();
+  // The same operator in the temporal gauge for the half-storage solver: with
+  // psi'(s,x) = exp(i phi_s(x)) psi(s,x), phi_0 = 0, phi_{s+1} = phi_s + theta^t_s(x), the
+  // spatial angles become theta_l + phi_s(x_l) - phi_s(y_l) and the temporal links 1 but at
+  // the wrap (kept up to date with the gauge field from the first call on).
+  DiracView<float, phase16_t> view_half_temporal_gauge();
+  // phi_s(x) as 16-bit angles, [Lt][V] (valid once view_half_temporal_gauge was called)
+  const phase16_t* half_gauge_rotation() const { return rotH_; }
   template <typename T>
   StencilTile tile(bool dag) const;
 
@@ -169,6 +179,11 @@ class Dirac {
   phase16_t* uSh_ = nullptr;  // [Lt][E] 16-bit fixed-point angles (allocated by view_half_phases)
   phase16_t* uTh_ = nullptr;  // [Lt][V]
   void fill_half_phases();
+  phase16_t* uStg_ = nullptr;   // [Lt][E] spatial angles in the temporal gauge
+  phase16_t* uPolyH_ = nullptr; // [V] the wrap link's angle in the temporal gauge
+  phase16_t* rotH_ = nullptr;   // [Lt][V] phi_s(x)
+  double* phi64_ = nullptr;     // [Lt][V] phi_s(x) in FP64 while the tables are built
+  void fill_temporal_gauge();
   double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)
   double* angT_ = nullptr;  // [Lt][V]
   int* link_ends_ = nullptr;  // [E][4]: x, y, slot of y in x, slot of x in y

@@ -113,6 +113,14 @@ struct SrcPUpdateH {
   __device__ Bound bind() const { return Bound{r, p, float(sc->beta)}; }
 };
 
+// Phase of the temporal link (s, x) -> (s+1, x): from the table, or in the temporal gauge
+// 1 except at the wrap (s = Lt-1).  The branch is uniform (a kernel argument and the slice).
+template <typename T, typename P>
+__device__ __forceinline__ cplx_t<T> temporal_phase(const DiracView<T, P>& d, int s, int x) {
+  if (d.uPoly) return s == d.Lt - 1 ? ldu(d.uPoly + x) : cplx_t<T>{T(1), T(0)};
+  return ldu(d.uT + size_t(s) * d.V + x);
+}
+
 // Epilogues may load what they need at an output point early through pre = prefetch(i),
 // which the stencil issues with the slice's other loads (before the LDS barrier, so the
 // latency overlaps the halo staging) and hands back as operator()(i, v, acc, pre).
@@ -172,11 +180,11 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
     Spin<T> prev = in(size_t(sm) * V + x);
     Spin<T> cur = in(size_t(s0) * V + x);
-    C uprev = ldu(d.uT + size_t(sm) * V + x);
+    C uprev = temporal_phase(d, sm, x);
     for (int s = s0; s < s1; s++) {
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       const Spin<T> next = in(size_t(sp) * V + x);
-      const C ucur = ldu(d.uT + size_t(s) * V + x);
+      const C ucur = temporal_phase(d, s, x);
       typename EpiPre<Epi>::type pre{};
       if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(size_t(s) * V + x);
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
@@ -303,7 +311,7 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
     prev = in(size_t(sm) * V + x);
     cur = in(size_t(s0) * V + x);
-    uprev = ldu(d.uT + size_t(sm) * V + x);
+    uprev = temporal_phase(d, sm, x);
   }
   for (int s = s0; s < s1; s++) {
     const size_t sV = size_t(s) * V;
@@ -318,7 +326,7 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       next = in(size_t(sp) * V + x);
       if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(sV + x);
-      ucur = ldu(d.uT + size_t(s) * V + x);
+      ucur = temporal_phase(d, s, x);
       const P* uS = d.uS + size_t(s) * d.E;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
