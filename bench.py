@@ -30,11 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision or CG kernel,
 # lattice, n, Lt, tile); the CG kernels ("K1-half" ...) from the half-storage solve the same
 # command runs at the pinned shape.
-MEASURED_TRAFFIC = {("f16", 1, 128, 64, 256, 64, 1, 1): 5.0329e8,
-                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3096e9,
-                    ("K1-half", 1, 128, 64, 256, 64, 1, 1): 9.1420e8,
-                    ("K2-half", 1, 128, 64, 256, 64, 1, 1): 6.8241e8,
-                    ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 2.0112e9}
+MEASURED_TRAFFIC = {("f16", 1, 128, 64, 256, 64, 1, 1): 4.9646e8,
+                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.2992e9,
+                    ("K1-half", 1, 128, 64, 256, 64, 1, 1): 8.5079e8,
+                    ("K2-half", 1, 128, 64, 256, 64, 1, 1): 6.3056e8,
+                    ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 1.9851e9}
 
 
 def parse():
@@ -400,10 +400,12 @@ def cg_kernel_bytes(V, E, Lt, solver):
     once, one real per link at the inner precision): K1 reads r, p_old and writes p, w;
     K2 reads w, r (and x, p) and writes r (and x).  Half storage: 4 B per complex,
     2 B per link, FP32 x read and written by K2 once per four iterations with the four
-    directions of its group."""
+    directions of its group; its fields live in the temporal gauge, where the temporal
+    links are 1 except one wrap phase per site, so it reads the E Lt spatial links and
+    V wrap phases only."""
     pts = V * Lt
     if solver == "half":
-        bv, bl, bx = pts * 2 * 4, (E + V) * Lt * 2, pts * 2 * 8
+        bv, bl, bx = pts * 2 * 4, (E * Lt + V) * 2, pts * 2 * 8
         return {"K1": 4 * bv + bl, "K2": 3 * bv + bl, "K2x": 7 * bv + 2 * bx + bl}
     s_r = 4 if solver == "mixed" else 8
     bv, bl = pts * 2 * 2 * s_r, (E + V) * Lt * s_r

@@ -124,7 +124,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencil(const Sys* __restrict_
     auto run = [&](const auto v, const auto& in, auto tag) {
       using T = decltype(tag);
       Spin<T>* tile = reinterpret_cast<Spin<T>*>(smem_raw);
-      stencil_lds_block<T, NS, DAG>(v, in, epi, ts, xr, nxr, st, hv, tile, acc);
+      with_gauge(v, [&](auto tg) {
+        stencil_lds_block<T, NS, DAG, decltype(tg)::value>(v, in, epi, ts, xr, nxr, st, hv, tile, acc);
+      });
     };
     if constexpr (PH == kK1)
       run(S.vh, SrcPUpdateH{S.r, S.ring[(POS + kXGroup - 1) % kXGroup], S.sc}.bind(), float{});

@@ -113,12 +113,30 @@ struct SrcPUpdateH {
   __device__ Bound bind() const { return Bound{r, p, float(sc->beta)}; }
 };
 
-// Phase of the temporal link (s, x) -> (s+1, x): from the table, or in the temporal gauge
-// 1 except at the wrap (s = Lt-1).  The branch is uniform (a kernel argument and the slice).
-template <typename T, typename P>
+// Phase of the temporal link (s, x) -> (s+1, x): from the table, or, for a view in the
+// temporal gauge (TG, d.uPoly set), 1 except at the wrap (s = Lt-1).  TG is a template
+// argument so that neither form carries the other's branch and loads inside the slice loop.
+template <bool TG, typename T, typename P>
 __device__ __forceinline__ cplx_t<T> temporal_phase(const DiracView<T, P>& d, int s, int x) {
-  if (d.uPoly) return s == d.Lt - 1 ? ldu(d.uPoly + x) : cplx_t<T>{T(1), T(0)};
-  return ldu(d.uT + size_t(s) * d.V + x);
+  if constexpr (TG)
+    return s == d.Lt - 1 ? ldu(d.uPoly + x) : cplx_t<T>{T(1), T(0)};
+  else
+    return ldu(d.uT + size_t(s) * d.V + x);
+}
+
+// Calls body(std::bool_constant<TG>) with TG = whether the view is in the temporal gauge,
+// decided once per block (a uniform branch on a kernel argument).  Only the 16-bit-angle
+// views of the half-storage solver have a temporal-gauge form.
+template <typename T, typename P, class Body>
+__device__ __forceinline__ void with_gauge(const DiracView<T, P>& d, const Body& body) {
+  if constexpr (std::is_same<P, phase16_t>::value) {
+    if (d.uPoly)
+      body(std::true_type{});
+    else
+      body(std::false_type{});
+  } else {
+    body(std::false_type{});
+  }
 }
 
 // Epilogues may load what they need at an output point early through pre = prefetch(i),
@@ -155,7 +173,7 @@ __device__ __forceinline__ void call_center(const Epi&, size_t, const Spin<T>&, 
 
 // The stencil for site x over the time slices [s0, s0+ts), handing every output point
 // to the epilogue (shared by the single-system and the batched kernels).
-template <typename T, int NS, bool DAG, class Epi, class In, typename P>
+template <typename T, int NS, bool DAG, bool TG, class Epi, class In, typename P>
 __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const In& in,
                                                  const Epi& epi, int x, int s0, int ts, double2& acc) {
   using C = cplx_t<T>;
@@ -180,11 +198,11 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
     Spin<T> prev = in(size_t(sm) * V + x);
     Spin<T> cur = in(size_t(s0) * V + x);
-    C uprev = temporal_phase(d, sm, x);
+    C uprev = temporal_phase<TG>(d, sm, x);
     for (int s = s0; s < s1; s++) {
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       const Spin<T> next = in(size_t(sp) * V + x);
-      const C ucur = temporal_phase(d, s, x);
+      const C ucur = temporal_phase<TG>(d, s, x);
       typename EpiPre<Epi>::type pre{};
       if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(size_t(s) * V + x);
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
@@ -236,7 +254,9 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const
 template <typename T, int NS, bool DAG, class Epi, typename P>
 __device__ __forceinline__ void stencil_site(const DiracView<T, P>& d, const Spin<T>* __restrict__ in,
                                              const Epi& epi, int x, int s0, int ts, double2& acc) {
-  stencil_site_src<T, NS, DAG>(d, SrcPlain<T>{in}, epi, x, s0, ts, acc);
+  with_gauge(d, [&](auto tg) {
+    stencil_site_src<T, NS, DAG, decltype(tg)::value>(d, SrcPlain<T>{in}, epi, x, s0, ts, acc);
+  });
 }
 
 template <typename T, int NS, bool DAG, class Epi, class Src, typename P = cplx_t<T>>
@@ -248,7 +268,10 @@ __global__ __launch_bounds__(256) void k_stencil(DiracView<T, P> d, Src in, Epi 
   const int x = (xr < tm.nxr) ? xr * blockDim.x + threadIdx.x : d.V;
   double2 acc = {0.0, 0.0};
   epi.begin();
-  stencil_site_src<T, NS, DAG>(d, in.bind(), epi, x, st * ts, ts, acc);
+  const auto bound = in.bind();
+  with_gauge(d, [&](auto tg) {
+    stencil_site_src<T, NS, DAG, decltype(tg)::value>(d, bound, epi, x, st * ts, ts, acc);
+  });
   if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
 }
 
@@ -269,7 +292,7 @@ __host__ __device__ constexpr size_t lds_bytes(int block, int hmax) {
   return size_t(kLdsBuffers) * size_t(block + hmax) * sizeof(Spin<T>);
 }
 
-template <typename T, int NS, bool DAG, class Epi, class In, typename P>
+template <typename T, int NS, bool DAG, bool TG, class Epi, class In, typename P>
 __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, const In& in, const Epi& epi,
                                                   int ts, int xr, int nxr, int st, const HaloView& hv,
                                                   Spin<T>* tileS, double2& acc) {
@@ -311,7 +334,7 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
     const int sm = (s0 == 0) ? Lt - 1 : s0 - 1;
     prev = in(size_t(sm) * V + x);
     cur = in(size_t(s0) * V + x);
-    uprev = temporal_phase(d, sm, x);
+    uprev = temporal_phase<TG>(d, sm, x);
   }
   for (int s = s0; s < s1; s++) {
     const size_t sV = size_t(s) * V;
@@ -326,7 +349,7 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
       const int sp = (s == Lt - 1) ? 0 : s + 1;
       next = in(size_t(sp) * V + x);
       if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(sV + x);
-      ucur = temporal_phase(d, s, x);
+      ucur = temporal_phase<TG>(d, s, x);
       const P* uS = d.uS + size_t(s) * d.E;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
@@ -395,7 +418,9 @@ __global__ __launch_bounds__(256) void k_stencil_lds(DiracView<T, P> d, Src src,
   tile_of(tm, blockIdx.x, xr, st);
   double2 acc = {0.0, 0.0};
   epi.begin();
-  stencil_lds_block<T, NS, DAG>(d, in, epi, ts, xr, tm.nxr, st, hv, tileS, acc);
+  with_gauge(d, [&](auto tg) {
+    stencil_lds_block<T, NS, DAG, decltype(tg)::value>(d, in, epi, ts, xr, tm.nxr, st, hv, tileS, acc);
+  });
   if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
 }
 
