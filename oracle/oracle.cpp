@@ -968,8 +968,6 @@ struct Net {
   }
 };
 
-size_t net_params(int C, int L) { return 20 * size_t(C) + size_t(L - 2) * 10 * C * C; }
-
 Net make_net(const Dirac* D, int C, int L, int B, const double* params, double skip) {
   Net n;
   n.D = D;

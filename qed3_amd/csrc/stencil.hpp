@@ -251,14 +251,6 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const
   }
 }
 
-template <typename T, int NS, bool DAG, class Epi, typename P>
-__device__ __forceinline__ void stencil_site(const DiracView<T, P>& d, const Spin<T>* __restrict__ in,
-                                             const Epi& epi, int x, int s0, int ts, double2& acc) {
-  with_gauge(d, [&](auto tg) {
-    stencil_site_src<T, NS, DAG, decltype(tg)::value>(d, SrcPlain<T>{in}, epi, x, s0, ts, acc);
-  });
-}
-
 template <typename T, int NS, bool DAG, class Epi, class Src, typename P = cplx_t<T>>
 __global__ __launch_bounds__(256) void k_stencil(DiracView<T, P> d, Src in, Epi epi, int ts, TileMap tm,
                                                   GridReduce gr) {
