@@ -199,6 +199,50 @@ int qed3_net_train(qed3_net* net, const void* d_r, const void* d_e, int npairs, 
 int qed3_fcg(qed3_ctx* ctx, qed3_net* net, void* d_x, const void* d_b, const qed3_cg_opts* opts,
              qed3_cg_report* rep, int* precond_applies, int* used_fallback);
 
+/* ---- time-exact learned preconditioner and flexible CG (the learned method) ---------
+ * No counterpart in the reference (it has no preconditioner, MatPoly::solve is plain CG,
+ * matpoly.h:204-267).  For r = 1 the operator splits as D = T - S with T = c_x + the
+ * temporal hops with the time wrap left out (bidiagonal in time per site and spin row:
+ * the hops of dirac_ext.h:158-187 are spin projectors) and S = -(spatial hops + the wrap
+ * hop from slice Lt-1 to 0).  The
+ * network, a gauge-covariant linear message-passing net on the lattice graph, is
+ *   K v = sum_{j<layers} c_j (T^-1 S_th)^j T^-1 v
+ * evaluated in Horner form (one launch per layer: an exact first-order recurrence along
+ * each site's time column plus one spatial transport step), S_th = S with its same-spin
+ * spatial entries of spin row a scaled by A_a and its spin-flip ones by B_a.  Defaults
+ * c_j = A_a = B_a = 1 give the Neumann series of D^-1; training adjusts them.
+ * Parameters (complex, interleaved doubles): c[0..layers), A0, A1, B0, B1. */
+typedef struct qed3_precond qed3_precond;
+/* needs the context's r = 1 (QED3_ERR_ARG otherwise); layers in [1,16] */
+int qed3_precond_create(qed3_ctx* ctx, int layers, qed3_precond** out);
+void qed3_precond_destroy(qed3_precond* pc);
+int qed3_precond_nparams(const qed3_precond* pc, int* n);
+int qed3_precond_get_params(const qed3_precond* pc, double* p);
+int qed3_precond_set_params(qed3_precond* pc, const double* p);
+/* d_out = K d_in (op = QED3_OP_D) or K^dagger d_in (op = QED3_OP_DDAG); complex double
+ * device fields, FP32 arithmetic on the context's link phases */
+int qed3_precond_apply(qed3_precond* pc, int op, void* d_out, const void* d_in);
+/* loss = mean_b |K v_b - e_b|^2 / |e_b|^2 over `batch` consecutive device field pairs and
+ * its gradient d loss / d Re p + i d loss / d Im p (2 * nparams doubles), forward and
+ * backward on the GPU */
+int qed3_precond_loss_grad(qed3_precond* pc, const void* d_v, const void* d_e, int batch,
+                           double* loss, double* grad);
+/* Adam over npairs (v, e) device pairs in mini-batches; history[epochs] = mean loss */
+int qed3_precond_train(qed3_precond* pc, const void* d_v, const void* d_e, int npairs, int batch,
+                       int epochs, double lr, double* history);
+/* Flexible CG (FCG(1)) on D^dag D preconditioned by M = K K^dagger: binary16-stored
+ * fields with FP32 arithmetic, FP64 reliable updates every time |r| drops by
+ * opts->delta, convergence on the FP64 true residual (opts->solver is ignored).
+ * Replaces MatPoly::solve (matpoly.h:204-267) like qed3_cg. */
+int qed3_pcg(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+             qed3_cg_report* rep);
+/* The same solve with direct launches and HIP events around its kernels:
+ * ms[0] K1, ms[1] K2 with the x term pending, ms[2] K2 updating x, ms[3] a first layer of
+ * the network (recurrence only), ms[4] a transport layer of K^dagger, ms[5] a transport
+ * layer of K (means per launch over the iterations run) */
+int qed3_pcg_kernel_times(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b,
+                          const qed3_cg_opts* opts, qed3_cg_report* rep, double* ms);
+
 /* ---- batched CG ----------------------------------------------------------------------
  * Solves D_k^dag D_k x_k = b_k for nsys contexts (any mix of lattices, refinements, Lt and
  * masses, all on the current device) together: every CG phase is one launch over all

@@ -42,6 +42,9 @@ def load():
             "oracle_apply": (None, [vp, C.c_int, dp, dp]),
             "oracle_cg": (C.c_int, [vp, dp, dp, C.c_double, C.c_int, dp]),
             "oracle_dense": (None, [vp, dp]),
+            "oracle_csr": (None, [vp, C.POINTER(C.c_long), ip, dp]),
+            "oracle_horner_apply": (None, [vp, C.c_int, dp, C.c_int, dp, dp]),
+            "oracle_horner_loss_grad": (C.c_double, [vp, C.c_int, dp, C.c_int, dp, dp, dp]),
             "oracle_force": (None, [vp, dp, C.c_int, dp, dp]),
             "oracle_gauge_action_create": (vp, [vp, C.c_int, C.c_double, C.c_double, C.c_int]),
             "oracle_gauge_action_destroy": (None, [vp]),
@@ -139,6 +142,17 @@ class Dirac:
                               maxiter, C.byref(mu))
         return x, it, mu.value
 
+    def csr(self):
+        """scipy.sparse CSR matrix of D in the reference's row order."""
+        import scipy.sparse as sps
+        nnz = self.nnz()
+        rows = np.zeros(self.N + 1, np.int64)
+        cols = np.zeros(nnz, np.int32)
+        vals = np.zeros(nnz, np.complex128)
+        load().oracle_csr(self.h, rows.ctypes.data_as(C.POINTER(C.c_long)), _ip(cols),
+                          _dp(vals.view(np.float64)))
+        return sps.csr_matrix((vals, cols, rows), shape=(self.N, self.N))
+
     def dense(self):
         out = np.zeros((self.N, self.N), np.complex128)
         load().oracle_dense(self.h, _dp(out.view(np.float64)))
@@ -197,6 +211,28 @@ def mpnn_loss_grad(dirac: "Dirac", C_, L_, params, r, e, skip=0.0):
     loss = load().oracle_mpnn_loss_grad(dirac.h, C_, L_, B, _dp(p.view(np.float64)), float(skip),
                                         _dp(r.view(np.float64)), _dp(e.view(np.float64)),
                                         _dp(g.view(np.float64)))
+    return loss, g
+
+
+def horner_apply(dirac: "Dirac", layers, params, v, adjoint=False):
+    """K v (or K^dagger v) of the time-exact Horner preconditioner, FP64 (dense LU per site)."""
+    v = np.ascontiguousarray(v, np.complex128)
+    p = np.ascontiguousarray(params, np.complex128)
+    out = np.zeros_like(v)
+    load().oracle_horner_apply(dirac.h, layers, _dp(p.view(np.float64)), int(adjoint),
+                               _dp(v.view(np.float64)), _dp(out.view(np.float64)))
+    return out
+
+
+def horner_loss_grad(dirac: "Dirac", layers, params, v, e):
+    """(loss, 2 dloss/dconj(p)) of mean_b |K v_b - e_b|^2/|e_b|^2 for arrays [B, N]."""
+    v = np.ascontiguousarray(v, np.complex128)
+    e = np.ascontiguousarray(e, np.complex128)
+    p = np.ascontiguousarray(params, np.complex128)
+    g = np.zeros_like(p)
+    loss = load().oracle_horner_loss_grad(dirac.h, layers, _dp(p.view(np.float64)), v.shape[0],
+                                          _dp(v.view(np.float64)), _dp(e.view(np.float64)),
+                                          _dp(g.view(np.float64)))
     return loss, g
 
 

@@ -985,6 +985,189 @@ Net make_net(const Dirac* D, int C, int L, int B, const double* params, double s
   return n;
 }
 
+
+// ---- time-exact Horner preconditioner (restatement of qed3_amd's learned method) ----
+// D = T - S split on the operator's own COO entries: T_open = the diagonal section plus the
+// temporal section without the wrap entries (dirac_ext.h:158-187: the "P" entries of slice
+// Lt-1 and the "M" entries of slice 0 are the ones that cross from Lt-1 to 0), S = minus the
+// spatial section (dirac_ext.h:121-156) and minus the wrap entries.  S_th scales each spatial
+// entry (row spin a, column spin b) by A_a when a == b and by B_a otherwise; the wrap entries
+// are not scaled.  T_open is block diagonal over sites; each
+// site's 2Lt x 2Lt block is solved by dense LU with partial pivoting (no use of its
+// bidiagonal structure), so this route is independent of the recurrence the GPU runs.
+//   K v   = sum_{j<L} c_j (T^-1 S_th)^j T^-1 v       (Horner: g <- T^-1 (c_j v + S_th g))
+//   K^+ v = sum_{j<L} conj(c_j) (T^-+ S_th^+)^j T^-+ v
+struct HornerOracle {
+  const Dirac* D;
+  int L;
+  std::vector<Complex> c;
+  Complex A[2], B[2];
+  int V, Nt, M;              // M = 2 Nt
+  std::vector<Complex> blk;  // [V][M][M] T_open blocks, LU factors in place
+  std::vector<int> piv;      // [V][M]
+  std::vector<Complex> blkH; // conjugate transposes, LU factors
+  std::vector<int> pivH;
+  struct SEnt {
+    long i, j;
+    int a, b;
+    Complex v;
+    bool scaled;
+  };  // spatial and wrap entries of D (value v); only the spatial ones carry A, B
+  std::vector<SEnt> sp;
+
+  int loc(long i) const { return int((i / D->Nx) * 2 + (i % 2)); }  // (s, a) within a site
+  int site(long i) const { return int((i % D->Nx) / 2); }
+
+  static void lu(Complex* a, int* p, int m) {
+    for (int k = 0; k < m; k++) {
+      int q = k;
+      for (int r = k + 1; r < m; r++)
+        if (std::abs(a[r * m + k]) > std::abs(a[q * m + k])) q = r;
+      p[k] = q;
+      if (q != k)
+        for (int t = 0; t < m; t++) std::swap(a[k * m + t], a[q * m + t]);
+      for (int r = k + 1; r < m; r++) {
+        a[r * m + k] /= a[k * m + k];
+        for (int t = k + 1; t < m; t++) a[r * m + t] -= a[r * m + k] * a[k * m + t];
+      }
+    }
+  }
+  static void lu_solve(const Complex* a, const int* p, int m, Complex* b) {
+    for (int k = 0; k < m; k++) {
+      std::swap(b[k], b[p[k]]);
+      for (int r = k + 1; r < m; r++) b[r] -= a[r * m + k] * b[k];
+    }
+    for (int k = m - 1; k >= 0; k--) {
+      for (int t = k + 1; t < m; t++) b[k] -= a[k * m + t] * b[t];
+      b[k] /= a[k * m + k];
+    }
+  }
+
+  HornerOracle(const Dirac* d, int layers, const double* params) : D(d), L(layers) {
+    const Complex* pc = reinterpret_cast<const Complex*>(params);
+    c.assign(pc, pc + L);
+    A[0] = pc[L];
+    A[1] = pc[L + 1];
+    B[0] = pc[L + 2];
+    B[1] = pc[L + 3];
+    V = D->lat->n_sites;
+    Nt = D->Nt;
+    M = 2 * Nt;
+    blk.assign(size_t(V) * M * M, 0.0);
+    const long nsp = 4L * D->lat->counter_accum.back();
+    const long len = long(D->is.size());
+    for (long e = 0; e < len; e++) {
+      const long i = D->is[e], j = D->js[e];
+      const Complex v = D->v_coo[e];
+      if (e < nsp * Nt) {
+        sp.push_back({i, j, int(i % 2), int(j % 2), v, true});
+        continue;
+      }
+      if (e < nsp * Nt + 8L * V * Nt) {  // temporal section: 8 entries per (s, x)
+        const long q = e - nsp * Nt;
+        const int s = int(q / (8L * V)), k = int(q % 8);
+        const bool Pent = (k == 0 || k == 1 || k == 4 || k == 5);
+        if ((Pent && s == Nt - 1) || (!Pent && s == 0)) {  // the wrap: part of S
+          sp.push_back({i, j, int(i % 2), int(j % 2), v, false});
+          continue;
+        }
+      }
+      if (site(i) != site(j)) throw std::logic_error("temporal or diagonal entry across sites");
+      blk[size_t(site(i)) * M * M + size_t(loc(i)) * M + loc(j)] += v;
+    }
+    blkH.assign(blk.size(), 0.0);
+    for (int x = 0; x < V; x++)
+      for (int r = 0; r < M; r++)
+        for (int t = 0; t < M; t++)
+          blkH[size_t(x) * M * M + size_t(r) * M + t] = std::conj(blk[size_t(x) * M * M + size_t(t) * M + r]);
+    piv.assign(size_t(V) * M, 0);
+    pivH.assign(size_t(V) * M, 0);
+    for (int x = 0; x < V; x++) {
+      lu(&blk[size_t(x) * M * M], &piv[size_t(x) * M], M);
+      lu(&blkH[size_t(x) * M * M], &pivH[size_t(x) * M], M);
+    }
+  }
+  // out = T_open^-1 in (adj: T_open^-dagger)
+  void tinv(bool adj, const Complex* in, Complex* out) const {
+    std::vector<Complex> b(M);
+    for (int x = 0; x < V; x++) {
+      for (int s = 0; s < Nt; s++)
+        for (int a = 0; a < 2; a++) b[2 * s + a] = in[D->Nx * s + 2 * x + a];
+      if (adj)
+        lu_solve(&blkH[size_t(x) * M * M], &pivH[size_t(x) * M], M, b.data());
+      else
+        lu_solve(&blk[size_t(x) * M * M], &piv[size_t(x) * M], M, b.data());
+      for (int s = 0; s < Nt; s++)
+        for (int a = 0; a < 2; a++) out[D->Nx * s + 2 * x + a] = b[2 * s + a];
+    }
+  }
+  Complex coef(const SEnt& e) const { return !e.scaled ? Complex(1.0) : (e.a == e.b ? A[e.a] : B[e.a]); }
+  // out += S_th in (adj: S_th^dagger)
+  void sth(bool adj, const Complex* in, Complex* out) const {
+    for (const SEnt& e : sp) {
+      const Complex w = -e.v * coef(e);
+      if (adj)
+        out[e.j] += std::conj(w) * in[e.i];
+      else
+        out[e.i] += w * in[e.j];
+    }
+  }
+  void apply(bool adj, const Complex* v, Complex* out) const {
+    const long N = D->N;
+    std::vector<Complex> g(N, 0.0), y(N);
+    for (int t = 0; t < L; t++) {
+      const Complex cj = adj ? std::conj(c[L - 1 - t]) : c[L - 1 - t];
+      for (long i = 0; i < N; i++) y[i] = cj * v[i];
+      if (t > 0) sth(adj, g.data(), y.data());
+      tinv(adj, y.data(), g.data());
+    }
+    std::copy(g.begin(), g.end(), out);
+  }
+  // loss = (1/B) sum_b |K v_b - e_b|^2 / |e_b|^2 and 2 d loss / d conj(param) by the adjoint
+  // of the Horner recursion
+  double loss_grad(int Bn, const Complex* v, const Complex* e, Complex* grad) const {
+    const long N = D->N;
+    for (int k = 0; k < L + 4; k++) grad[k] = 0.0;
+    double loss = 0;
+    std::vector<std::vector<Complex>> G(L, std::vector<Complex>(N));
+    std::vector<Complex> y(N), lam(N), mu(N);
+    for (int b = 0; b < Bn; b++) {
+      const Complex* vb = v + size_t(b) * N;
+      const Complex* eb = e + size_t(b) * N;
+      for (int t = 0; t < L; t++) {
+        for (long i = 0; i < N; i++) y[i] = c[L - 1 - t] * vb[i];
+        if (t > 0) sth(false, G[t - 1].data(), y.data());
+        tinv(false, y.data(), G[t].data());
+      }
+      double en = 0;
+      for (long i = 0; i < N; i++) en += std::norm(eb[i]);
+      const double w = 1.0 / (Bn * en);
+      for (long i = 0; i < N; i++) {
+        lam[i] = w * (G[L - 1][i] - eb[i]);
+        loss += w * std::norm(G[L - 1][i] - eb[i]);
+      }
+      for (int t = L - 1; t >= 0; t--) {
+        tinv(true, lam.data(), mu.data());  // d loss / d conj(y_t)
+        Complex gc = 0.0;
+        for (long i = 0; i < N; i++) gc += std::conj(vb[i]) * mu[i];
+        grad[L - 1 - t] += 2.0 * gc;
+        if (t > 0) {
+          // y_t gains S_th g_{t-1}: the entry (i, j) contributes -v coef g_{t-1}[j]
+          for (const SEnt& en2 : sp) {
+            if (!en2.scaled) continue;
+            const Complex d = -en2.v * G[t - 1][en2.j];
+            const int k = en2.a == en2.b ? L + en2.a : L + 2 + en2.a;
+            grad[k] += 2.0 * std::conj(d) * mu[en2.i];
+          }
+          std::fill(lam.begin(), lam.end(), Complex(0.0));
+          sth(true, mu.data(), lam.data());
+        }
+      }
+    }
+    return loss;
+  }
+};
+
 }  // namespace
 
 // ============================================================================ C ABI
@@ -1100,6 +1283,30 @@ void oracle_apply(void* h, int op, double* out, const double* in) {
 int oracle_cg(void* h, double* x, const double* b, double tol, int maxiter, double* mu) {
   return cg(*static_cast<Dirac*>(h), reinterpret_cast<Complex*>(x),
             reinterpret_cast<const Complex*>(b), tol, maxiter, mu);
+}
+// Time-exact Horner preconditioner (params: [c_0..c_{L-1}, A0, A1, B0, B1] complex)
+void oracle_horner_apply(void* h, int L, const double* params, int adj, const double* in, double* out) {
+  const HornerOracle K(static_cast<Dirac*>(h), L, params);
+  K.apply(adj != 0, reinterpret_cast<const Complex*>(in), reinterpret_cast<Complex*>(out));
+}
+double oracle_horner_loss_grad(void* h, int L, const double* params, int B, const double* v, const double* e,
+                               double* grad) {
+  const HornerOracle K(static_cast<Dirac*>(h), L, params);
+  return K.loss_grad(B, reinterpret_cast<const Complex*>(v), reinterpret_cast<const Complex*>(e),
+                     reinterpret_cast<Complex*>(grad));
+}
+// CSR of D (sparse_dirac.h:52-129 row order): rows[N+1], cols[nnz], vals[2 nnz]
+// (interleaved complex), for scipy-side checks of operator identities at sizes where the
+// dense matrix does not fit.
+void oracle_csr(void* h, long* rows, int* cols, double* vals) {
+  const Dirac& D = *static_cast<Dirac*>(h);
+  for (long i = 0; i <= D.N; i++) rows[i] = D.rows[i];
+  const long nnz = long(D.cols.size());
+  for (long e = 0; e < nnz; e++) {
+    cols[e] = D.cols[e];
+    vals[2 * e] = D.val[e].real();
+    vals[2 * e + 1] = D.val[e].imag();
+  }
 }
 // Dense D (N x N, row major, interleaved complex) for small lattices.
 void oracle_dense(void* h, double* out) {

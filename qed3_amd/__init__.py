@@ -407,6 +407,98 @@ class Net:
         return rep, pa.value, fb.value
 
 
+class Precond:
+    """Time-exact learned preconditioner K (Horner message-passing net over the exact inverse
+    of the operator's time part) on a WilsonDirac with r = 1, and flexible CG with
+    M = K K^dagger: the metric's learned method."""
+
+    def __init__(self, D: WilsonDirac, layers=4):
+        self.D = D
+        self.layers = layers
+        self._h = C.c_void_p()
+        check(load().qed3_precond_create(D._h, layers, C.byref(self._h)))
+        n = C.c_int()
+        check(load().qed3_precond_nparams(self._h, C.byref(n)))
+        self.nparams = n.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().qed3_precond_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        """[c_0 .. c_{L-1}, A0, A1, B0, B1] (complex)"""
+        p = np.zeros(self.nparams, np.complex128)
+        check(load().qed3_precond_get_params(self._h, _dp(p.view(np.float64))))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.complex128)
+        assert p.size == self.nparams
+        check(load().qed3_precond_set_params(self._h, _dp(p.view(np.float64))))
+
+    def apply_device(self, d_out, d_in, adjoint=False):
+        check(load().qed3_precond_apply(self._h, OP_DDAG if adjoint else OP_D, d_out.ptr, d_in.ptr))
+
+    def apply(self, v, adjoint=False):
+        """K v (or K^dagger v) for a host complex field."""
+        v = np.ascontiguousarray(v, np.complex128)
+        d_in = DeviceArray.from_numpy(v)
+        d_out = DeviceArray(v.nbytes)
+        self.apply_device(d_out, d_in, adjoint)
+        return d_out.to_numpy(np.complex128)
+
+    def loss_grad_device(self, d_v, d_e, batch):
+        loss = C.c_double()
+        g = np.zeros(self.nparams, np.complex128)
+        check(load().qed3_precond_loss_grad(self._h, d_v.ptr, d_e.ptr, batch, C.byref(loss),
+                                            _dp(g.view(np.float64))))
+        return loss.value, g
+
+    def loss_grad(self, v, e):
+        """mean_b |K v_b - e_b|^2/|e_b|^2 and its gradient for host arrays [B, N]."""
+        v = np.ascontiguousarray(v, np.complex128)
+        e = np.ascontiguousarray(e, np.complex128)
+        return self.loss_grad_device(DeviceArray.from_numpy(v), DeviceArray.from_numpy(e), v.shape[0])
+
+    def train_device(self, d_v, d_e, npairs, batch=4, epochs=10, lr=1e-2):
+        hist = np.zeros(epochs)
+        check(load().qed3_precond_train(self._h, d_v.ptr, d_e.ptr, npairs, batch, epochs, lr,
+                                        _dp(hist)))
+        return hist
+
+    def solve_device(self, d_x, d_b, tol=1e-10, maxiter=20000, delta=0.01, check_every=0,
+                     raise_on_fail=True):
+        opts = CGOpts(tol, maxiter, CG_HALF, delta, check_every)
+        rep = CGReport()
+        code = load().qed3_pcg(self.D._h, self._h, d_x.ptr, d_b.ptr, C.byref(opts), C.byref(rep))
+        if code != 0 and raise_on_fail:
+            check(code)
+        return rep
+
+    def solve(self, b, **kw):
+        b = np.ascontiguousarray(b, np.complex128)
+        d_b = DeviceArray.from_numpy(b)
+        d_x = DeviceArray(b.nbytes)
+        rep = self.solve_device(d_x, d_b, **kw)
+        return d_x.to_numpy(np.complex128), rep
+
+    def solve_kernel_times(self, d_x, d_b, tol=1e-10, maxiter=20000, delta=0.01):
+        """One solve with HIP-event timing: (report, [K1, K2, K2 with x update, first layer,
+        K^dagger transport layer, K transport layer] ms per launch)."""
+        opts = CGOpts(tol, maxiter, CG_HALF, delta, 0)
+        rep = CGReport()
+        ms = (C.c_double * 6)()
+        check(load().qed3_pcg_kernel_times(self.D._h, self._h, d_x.ptr, d_b.ptr, C.byref(opts),
+                                           C.byref(rep), ms))
+        return rep, list(ms)
+
+
 class HMC:
     """HMC2 + ExplicitLeapfrogML over a WilsonDirac (host FP64 H and accept step)."""
 

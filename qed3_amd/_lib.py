@@ -86,6 +86,20 @@ SIGNATURES = {
                                      c_double_p]),
     "qed3_net_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                  C.c_double, c_double_p]),
+    "qed3_precond_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "qed3_precond_destroy": (None, [C.c_void_p]),
+    "qed3_precond_nparams": (C.c_int, [C.c_void_p, c_int_p]),
+    "qed3_precond_get_params": (C.c_int, [C.c_void_p, c_double_p]),
+    "qed3_precond_set_params": (C.c_int, [C.c_void_p, c_double_p]),
+    "qed3_precond_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "qed3_precond_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p,
+                                         c_double_p]),
+    "qed3_precond_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                     C.c_double, c_double_p]),
+    "qed3_pcg": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
+                           C.POINTER(CGReport)]),
+    "qed3_pcg_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.POINTER(CGOpts), C.POINTER(CGReport), c_double_p]),
     "qed3_fcg": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
                            C.POINTER(CGReport), c_int_p, c_int_p]),
     "qed3_pf_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -15,6 +15,7 @@
 #include "batch.hpp"
 #include "fcg.hpp"
 #include "hmc.hpp"
+#include "horner.hpp"
 #include "mpnn.hpp"
 #include "msolve.hpp"
 #include "rng.hpp"
@@ -46,6 +47,11 @@ struct qed3_rng {
 struct qed3_net {
   qed3_ctx* ctx;
   std::unique_ptr<qed3::Net> net;
+};
+
+struct qed3_precond {
+  qed3_ctx* ctx;
+  std::unique_ptr<qed3::Horner> K;
 };
 
 namespace {
@@ -430,6 +436,132 @@ int qed3_cg_kernel_times(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_c
   qed3::KernelTimes t;
   const int rc = cg_entry(ctx, d_x, d_b, opts, rep, &t);
   for (int k = 0; k < 3; k++) ms[k] = t.ms[k];
+  return rc;
+}
+
+namespace {
+int pcg_entry(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+              qed3_cg_report* rep, qed3::KernelTimes* times, qed3::KernelTimes* ltimes) {
+  if (!ctx || !pc || !d_x || !d_b || !opts) return fail(QED3_ERR_ARG, "null argument");
+  if (pc->ctx != ctx) return fail(QED3_ERR_ARG, "preconditioner belongs to another context");
+  if (d_x == d_b) return fail(QED3_ERR_ARG, "x and b must differ");
+  if (!(opts->tol > 0) || opts->maxiter < 1) return fail(QED3_ERR_ARG, "bad tolerance/maxiter");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    qed3::CGOptions o;
+    o.tol = opts->tol;
+    o.maxiter = opts->maxiter;
+    o.mixed = o.half = true;
+    o.delta = opts->delta > 0 ? opts->delta : 0.01;
+    o.chunk = opts->check_every;
+    o.times = times;
+    const qed3::CGResult r = qed3::pcg_solve(*pc->K, static_cast<qed3::Spin<double>*>(d_x),
+                                             static_cast<const qed3::Spin<double>*>(d_b), o, ltimes);
+    if (rep) {
+      rep->status = r.status == 0 ? QED3_OK : (r.status == 4 ? QED3_ERR_NAN : QED3_ERR_NOCONV);
+      rep->iterations = r.iterations;
+      rep->reliable_updates = r.reliable_updates;
+      rep->true_resid = r.true_resid;
+      rep->seconds = r.seconds;
+    }
+    if (r.status == 4) return fail(QED3_ERR_NAN, "preconditioned CG breakdown (non-finite or <r, M r> <= 0)");
+    if (r.status != 0) return fail(QED3_ERR_NOCONV, "preconditioned CG did not reach the tolerance");
+    return QED3_OK;
+  });
+}
+}  // namespace
+
+int qed3_precond_create(qed3_ctx* ctx, int layers, qed3_precond** out) {
+  if (!ctx || !out) return fail(QED3_ERR_ARG, "null argument");
+  if (layers < 1 || layers > 16) return fail(QED3_ERR_ARG, "layers must be in [1,16]");
+  if (ctx->p.r != 1.0) return fail(QED3_ERR_ARG, "the time-exact preconditioner needs r = 1");
+  return guarded([&]() -> int {
+    auto pc = std::make_unique<qed3_precond>();
+    pc->ctx = ctx;
+    pc->K = std::make_unique<qed3::Horner>(*ctx->D, layers);
+    *out = pc.release();
+    return QED3_OK;
+  });
+}
+
+void qed3_precond_destroy(qed3_precond* pc) { delete pc; }
+
+int qed3_precond_nparams(const qed3_precond* pc, int* n) {
+  if (!pc || !n) return fail(QED3_ERR_ARG, "null argument");
+  *n = pc->K->params().count();
+  return QED3_OK;
+}
+
+int qed3_precond_get_params(const qed3_precond* pc, double* p) {
+  if (!pc || !p) return fail(QED3_ERR_ARG, "null argument");
+  pc->K->params().pack(reinterpret_cast<std::complex<double>*>(p));
+  return QED3_OK;
+}
+
+int qed3_precond_set_params(qed3_precond* pc, const double* p) {
+  if (!pc || !p) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    qed3::HornerParams q = pc->K->params();
+    q.unpack(reinterpret_cast<const std::complex<double>*>(p));
+    for (int j = 0; j < q.count(); j++)
+      if (!std::isfinite(p[2 * j]) || !std::isfinite(p[2 * j + 1])) return fail(QED3_ERR_ARG, "non-finite parameter");
+    pc->K->set_params(q);
+    return QED3_OK;
+  });
+}
+
+int qed3_precond_apply(qed3_precond* pc, int op, void* d_out, const void* d_in) {
+  if (!pc || !d_out || !d_in) return fail(QED3_ERR_ARG, "null argument");
+  if (op != QED3_OP_D && op != QED3_OP_DDAG) return fail(QED3_ERR_ARG, "op must be QED3_OP_D or QED3_OP_DDAG");
+  if (!pc->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    pc->K->apply(op == QED3_OP_DDAG, static_cast<qed3::Spin<double>*>(d_out),
+                 static_cast<const qed3::Spin<double>*>(d_in));
+    return QED3_OK;
+  });
+}
+
+int qed3_precond_loss_grad(qed3_precond* pc, const void* d_v, const void* d_e, int batch, double* loss,
+                           double* grad) {
+  if (!pc || !d_v || !d_e || !loss) return fail(QED3_ERR_ARG, "null argument");
+  if (batch < 1) return fail(QED3_ERR_ARG, "batch must be positive");
+  if (!pc->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    std::vector<std::complex<double>> g;
+    *loss = pc->K->loss_grad(static_cast<const qed3::Spin<double>*>(d_v),
+                             static_cast<const qed3::Spin<double>*>(d_e), batch, grad ? &g : nullptr);
+    if (grad)
+      for (size_t j = 0; j < g.size(); j++) grad[2 * j] = g[j].real(), grad[2 * j + 1] = g[j].imag();
+    return QED3_OK;
+  });
+}
+
+int qed3_precond_train(qed3_precond* pc, const void* d_v, const void* d_e, int npairs, int batch, int epochs,
+                       double lr, double* history) {
+  if (!pc || !d_v || !d_e) return fail(QED3_ERR_ARG, "null argument");
+  if (npairs < 1 || batch < 1 || epochs < 1 || !(lr > 0)) return fail(QED3_ERR_ARG, "bad training sizes");
+  if (!pc->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    const std::vector<double> h = pc->K->train(static_cast<const qed3::Spin<double>*>(d_v),
+                                               static_cast<const qed3::Spin<double>*>(d_e), npairs, batch,
+                                               epochs, lr);
+    if (history)
+      for (int k = 0; k < epochs; k++) history[k] = h[k];
+    return QED3_OK;
+  });
+}
+
+int qed3_pcg(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+             qed3_cg_report* rep) {
+  return pcg_entry(ctx, pc, d_x, d_b, opts, rep, nullptr, nullptr);
+}
+
+int qed3_pcg_kernel_times(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b,
+                          const qed3_cg_opts* opts, qed3_cg_report* rep, double* ms) {
+  if (!ms) return fail(QED3_ERR_ARG, "null argument");
+  qed3::KernelTimes t, lt;
+  const int rc = pcg_entry(ctx, pc, d_x, d_b, opts, rep, &t, &lt);
+  for (int k = 0; k < 3; k++) ms[k] = t.ms[k], ms[3 + k] = lt.ms[k];
   return rc;
 }
 

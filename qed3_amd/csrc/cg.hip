@@ -9,6 +9,7 @@
 
 #include "cg.hpp"
 #include "cg_epi.hpp"
+#include "cg_drive.hpp"
 #include "stencil.hpp"
 
 namespace qed3 {
@@ -16,6 +17,7 @@ namespace qed3 {
 namespace {
 
 using namespace cgepi;
+using namespace cgdrive;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_init(const Spin<double>* __restrict__ b, Spin<double>* x64,
@@ -92,28 +94,6 @@ __global__ __launch_bounds__(256) void k_flush(Spin<double>* x64, Spin<T>* xin, 
   }
 }
 
-// Half-storage start: x = 0, p_old = 0, |b|^2; the scale makes the stored residual
-// unit-rms per component (k_store_b0 then stores r = b / scale).
-__global__ __launch_bounds__(256) void k_init_h(const Spin<double>* __restrict__ b, Spin<double>* x64,
-                                                Spin<float>* xin, SpinH* p, size_t n, GridReduce gr) {
-  double2 acc = {0.0, 0.0};
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    const Spin<double> bb = b[i];
-    Spin<float> zf;
-    zf.a = {0.f, 0.f};
-    zf.b = {0.f, 0.f};
-    xin[i] = zf;
-    p[i] = to_half(zf);
-    Spin<double> z;
-    z.a = {0.0, 0.0};
-    z.b = {0.0, 0.0};
-    x64[i] = z;
-    acc.x += norm2(bb);
-  }
-  store_partial(acc, gr.partials, blockIdx.x);
-}
-
 // finaliser of k_init_h
 struct FinStartH {
   CGScalars* sc;
@@ -122,20 +102,6 @@ struct FinStartH {
   __device__ bool skip() const { return false; }
   __device__ void operator()(double2 t) const { half_start(sc, t.x, ncomp, tol2, delta2, maxiter); }
 };
-
-// Half-storage flush: x64 += scale (x_inner + sum_j alpha_pend[j] ring[j]), x_inner = 0,
-// taking the x terms the current group's iterations left pending (alpha_pend[j] = 0 if none)
-__global__ __launch_bounds__(256) void k_flush_h(Spin<double>* x64, Spin<float>* xin, PendRing ring,
-                                                 size_t n, const CGScalars* sc, int always,
-                                                 const phase16_t* rot) {
-  if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
-  const double s = sc->scale;
-  float ap[kXGroup - 1];
-  for (int j = 0; j < kXGroup - 1; j++) ap[j] = float(sc->alpha_pend[j]);
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x)
-    half_flush_point(x64, xin, ring, ap, s, i, rot);
-}
 
 // r = b / scale in half (FP64 arithmetic, so any finite b is representable)
 __global__ __launch_bounds__(256) void k_store_b0(const Spin<double>* __restrict__ b, SpinH* r, size_t n,
@@ -200,92 +166,6 @@ struct Work {
   Spin<double>* t = nullptr;
 };
 
-size_t stream_blocks(size_t n) { return std::min<size_t>((n + 255) / 256, 8192); }
-// Grid of the per-iteration kernels that only work after a reliable update (flush,
-// rescale) and otherwise exit at once: a smaller grid makes the early exit cheaper and
-// still has eight waves per SIMD for the updates themselves.
-size_t gated_blocks(size_t n) { return std::min<size_t>((n + 255) / 256, 2048); }
-
-// The FP64 true residual of the returned x.  A solve that converged did so in a reliable
-// update, which computed |b - D^dag D x|^2 in FP64 for this same x (every pending inner
-// term was flushed before it), so that value is reported; otherwise it is recomputed.
-double final_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b, const CGScalars& h) {
-  if (h.done == 1 && h.n_ru > 0 && h.bnorm2 > 0) return std::sqrt(h.true_r2 / h.bnorm2);
-  return true_residual(D, x, b);
-}
-
-// Event pairs for KernelTimes: pair(k) hands out a fresh (start, end) pair tagged with
-// kernel class k; finish() averages the pairs of the first `iters` iterations.
-class EventTimer {
- public:
-  explicit EventTimer(KernelTimes* t) : t_(t) {}
-  ~EventTimer() {
-    for (auto& e : ev_) (void)hipEventDestroy(e.first[0]), (void)hipEventDestroy(e.first[1]);
-  }
-  const hipEvent_t* pair(int cls, int iter) {
-    if (!t_) return nullptr;
-    ev_.emplace_back();
-    auto& e = ev_.back();
-    QED3_HIP(hipEventCreate(&e.first[0]));
-    QED3_HIP(hipEventCreate(&e.first[1]));
-    e.second = {cls, iter};
-    return e.first.data();
-  }
-  void finish(int iters) {
-    if (!t_) return;
-    for (auto& e : ev_) {
-      if (e.second.second >= iters) continue;  // launched after convergence: early exit
-      float ms = 0;
-      QED3_HIP(hipEventElapsedTime(&ms, e.first[0], e.first[1]));
-      t_->ms[e.second.first] += ms;
-      t_->count[e.second.first] += 1;
-    }
-    for (int k = 0; k < 3; k++)
-      if (t_->count[k]) t_->ms[k] /= t_->count[k];
-  }
-
- private:
-  KernelTimes* t_;
-  std::vector<std::pair<std::array<hipEvent_t, 2>, std::pair<int, int>>> ev_;
-};
-
-// Runs one_iter(i, k), i = 0 .. chunk-1 (the iteration's position in the chunk), k its
-// index in the solve (-1 while a graph is captured), then tail(), in chunks of iterations, captured once as a HIP graph unless
-// disabled, with one host poll of the device scalars per chunk, until the device marks
-// the solve done.  On return hsc holds the final scalars.
-template <class Iter, class Tail>
-void drive_chunks(const Iter& one_iter, const Tail& tail, const CGOptions& o, size_t n, hipStream_t st,
-                  const CGScalars* sc, CGScalars* hsc) {
-  int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
-  chunk = (chunk + kXGroup - 1) / kXGroup * kXGroup;  // every replay starts a new group
-                                                      // (and an even parity)
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (o.use_graph && !o.times && std::getenv("QED3_NO_GRAPH") == nullptr) {
-    QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; i++) one_iter(i, -1);
-    tail();
-    QED3_HIP(hipStreamEndCapture(st, &graph));
-    QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
-  // every chunk runs at least one iteration (a reliable update ends each chunk), so the
-  // device's iteration cap ends the solve within this many chunks
-  const int max_chunks = o.maxiter + 4;
-  for (int c = 0; c < max_chunks; c++) {
-    if (exec) {
-      QED3_HIP(hipGraphLaunch(exec, st));
-    } else {
-      for (int i = 0; i < chunk; i++) one_iter(i, c * chunk + i);
-      tail();
-    }
-    QED3_HIP(hipGetLastError());
-    QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
-    QED3_HIP(hipStreamSynchronize(st));
-    if (hsc->done != 0) break;
-  }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
-}
 
 template <typename T>
 CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {

@@ -76,11 +76,14 @@ struct CGScalars {
   double rfac;     // half-storage CG: factor from the kept residual to the new stored one
   double alpha_pend[3];  // half-storage CG: alphas of this group's iterations whose x
                          // update is pending (0 = none)
+  double rho;      // preconditioned CG: <r, z> of the last iteration (stored units)
   int done;        // 0 running, 1 converged, 2 breakdown / NaN, 3 maxiter
   int need_ru;     // a reliable update is due in this iteration
   int iter, n_ru, maxiter;
   int rescale;     // half-storage CG: the residual is to be re-stored at the new scale
   int last_pos;    // half-storage CG: group position of the last iteration run
+  int rpar;        // preconditioned CG: which of the two residual buffers is current
+  int fresh;       // preconditioned CG: no previous direction (beta = 0)
 };
 
 class Dirac {

@@ -1,0 +1,389 @@
+// Flexible CG on D^dagger D preconditioned by M = K K^dagger, K the time-exact Horner
+// network (horner.hpp), on the binary16-stored fields of the half-storage CG.
+//
+// FCG(1) (Notay, SIAM J. Sci. Comput. 22 (2000) 1444): z = M r, p = z + beta p_old with
+//   beta = -<A p_old, z> / <p_old, A p_old> = (<r, z> - <r_old, z>) / <r_old, z_old>,
+// the second form because A p_old = (r_old - r) / alpha_old; alpha = <r, z> / |D p|^2.
+// The correction <r_old, z> (zero for an exactly symmetric fixed M) keeps the directions
+// A-conjugate although the binary16 fields make M vary slightly between applications.
+//
+// One iteration is the network's 2L layer launches (K^dagger then K; the last computes
+// <r, z> and <r_old, z> and the scalar update) and the two stencil kernels of the
+// half-storage CG: K1 forms p = z + beta p_old while gathering, stores it and applies D,
+// K2 applies D^dagger, updates r and x.  The residual is double-buffered and spin-planar
+// (the network reads it row by row), the buffer in use chosen on the device, so that a
+// chunk of iterations is one graph replay whatever the reliable updates do.  Reliable
+// updates, scales and the temporal gauge are those of the half-storage CG (cg.hip); after
+// an update beta carries the change of scale (r_old and p_old are kept at the old scale).
+#include <chrono>
+#include <cmath>
+
+#include "cg_drive.hpp"
+#include "cg_epi.hpp"
+#include "horner.hpp"
+#include "horner_kernels.hpp"
+#include "stencil.hpp"
+
+namespace qed3 {
+namespace {
+
+using namespace cgepi;
+using namespace cgdrive;
+
+
+// K1 source: p = z + beta p_old with z spin-planar binary16, p_old a binary16 spinor,
+// rounded to binary16 so D acts on exactly the stored p
+struct SrcPUpdateZ {
+  const half2_t* z;
+  size_t n;
+  const SpinH* p;
+  const CGScalars* sc;
+  struct Bound {
+    const half2_t* __restrict__ z;
+    size_t n;
+    const SpinH* __restrict__ p;
+    float beta;
+    __device__ Spin<float> operator()(size_t i) const {
+      const float2 za = hk::ldc(z + i), zb = hk::ldc(z + n + i);
+      const Spin<float> c = ldh(p + i);
+      Spin<float> a;
+      a.a = {za.x + beta * c.a.x, za.y + beta * c.a.y};
+      a.b = {zb.x + beta * c.b.x, zb.y + beta * c.b.y};
+      return from_half(to_half(a));
+    }
+  };
+  __device__ Bound bind() const { return Bound{z, n, p, float(sc->beta)}; }
+};
+
+__device__ __forceinline__ void st_planar(half2_t* r, size_t n, size_t i, const Spin<float>& v) {
+  hk::stc(r + i, v.a);
+  hk::stc(r + n + i, v.b);
+}
+
+// K2: q = D^dag w; r_new = r - alpha q into the other residual buffer (spin-planar
+// binary16), |r_new|^2 and the reliable-update decision; x grouped as in EpiK2H
+template <int POS>
+struct EpiK2F {
+  static constexpr bool kReduce = true;
+  static constexpr bool kUpdateX = POS == kXGroup - 1;
+  Spin<float>* x;
+  half2_t* rbuf[2];
+  size_t n;
+  const SpinH* ring[kXGroup];
+  CGScalars* sc;
+  float alpha;
+  float ap[kXGroup - 1];
+  const half2_t* rin;
+  half2_t* rout;
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
+  __device__ void begin() {
+    alpha = float(sc->alpha);
+#pragma unroll
+    for (int j = 0; j < kXGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
+    const int par = sc->rpar & 1;
+    rin = rbuf[par];
+    rout = rbuf[par ^ 1];
+  }
+  template <int P = POS, std::enable_if_t<P != kXGroup - 1, int> = 0>
+  __device__ float4 prefetch(size_t i) const {
+    const float2 a = hk::ldc(rin + i), b = hk::ldc(rin + n + i);
+    return {a.x, a.y, b.x, b.y};
+  }
+  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const {
+    const float2 a = hk::ldc(rin + i), b = hk::ldc(rin + n + i);
+    (*this)(i, q, acc, float4{a.x, a.y, b.x, b.y});
+  }
+  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const float4& rpre) const {
+    Spin<float> rr;
+    rr.a = {rpre.x, rpre.y};
+    rr.b = {rpre.z, rpre.w};
+    if constexpr (kUpdateX) {
+      Spin<float> xx = ld(x + i);
+#pragma unroll
+      for (int j = 0; j < kXGroup; j++) {
+        const float a = j == kXGroup - 1 ? alpha : ap[j];
+        if (a != 0.f) {  // uniform
+          const Spin<float> pp = ldh(ring[j] + i);
+          xx.a.x += a * pp.a.x;
+          xx.a.y += a * pp.a.y;
+          xx.b.x += a * pp.b.x;
+          xx.b.y += a * pp.b.y;
+        }
+      }
+      st(x + i, xx);
+    }
+    rr.a.x -= alpha * q.a.x;
+    rr.a.y -= alpha * q.a.y;
+    rr.b.x -= alpha * q.b.x;
+    rr.b.y -= alpha * q.b.y;
+    const Spin<float> rh = from_half(to_half(rr));
+    st_planar(rout, n, i, rh);
+    acc.x += norm2(rh);
+  }
+  __device__ void operator()(double2 t) const {
+    const double m = t.x;
+    if constexpr (kUpdateX) {
+      for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
+    } else {
+      sc->alpha_pend[POS] = sc->alpha;
+    }
+    sc->last_pos = POS;
+    sc->iter += 1;
+    sc->rpar ^= 1;
+    if (!(m >= 0.0 && m < 1e300)) {
+      sc->done = 2;
+      return;
+    }
+    if (m < sc->delta2 * sc->maxr2 || m * sc->scale * sc->scale <= sc->tol2 * sc->bnorm2)
+      sc->need_ru = 1;
+    else
+      sc->maxr2 = fmax(sc->maxr2, m);
+    if (sc->iter >= sc->maxiter && !sc->need_ru) sc->done = 3;
+  }
+};
+
+// the network's layers wait, like K1 and K2, for a pending reliable update
+struct EpiSkip {
+  static constexpr bool kReduce = false;
+  const CGScalars* sc;
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
+  __device__ void operator()(int, size_t, float2, double2&) const {}
+  __device__ void operator()(double2) const {}
+};
+
+// Last layer of K: Re<r, z> and Re<r_old, z> (z as stored) and the FCG scalars.  After a
+// reliable update r is the true residual at the new scale while r_old and p_old are at the
+// old one (rfac = old / new): beta = (<r,z> - rfac <r_old,z>) / (rfac rho_old).
+struct EpiDots {
+  static constexpr bool kReduce = true;
+  const half2_t* r0;
+  const half2_t* r1;
+  size_t n;
+  CGScalars* sc;
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
+  __device__ void operator()(int a, size_t i, float2 z, double2& acc) const {
+    const int par = sc->rpar & 1;
+    const half2_t* rc = par ? r1 : r0;
+    const half2_t* rp = par ? r0 : r1;
+    const float2 u = hk::ldc(rc + size_t(a) * n + i), w = hk::ldc(rp + size_t(a) * n + i);
+    acc.x += double(u.x) * z.x + double(u.y) * z.y;
+    acc.y += double(w.x) * z.x + double(w.y) * z.y;
+  }
+  __device__ void operator()(double2 t) const {
+    const double A = t.x, Bv = t.y;
+    if (!(A > 0.0 && A < 1e300) || !(fabs(Bv) < 1e300)) {
+      sc->done = 2;  // M r must be a descent direction: <r, M r> > 0
+      return;
+    }
+    double beta = 0.0;
+    if (!sc->fresh) {
+      const double f = sc->rescale ? sc->rfac : 1.0;
+      beta = (A - f * Bv) / (f * sc->rho);
+    }
+    sc->beta = beta;
+    sc->rho = A;
+    sc->mu = A;  // K1: alpha = mu / |D p|^2
+    sc->fresh = 0;
+  }
+};
+
+struct FinStartF {
+  CGScalars* sc;
+  double ncomp, tol2, delta2;
+  int maxiter;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(double2 t) const {
+    half_start(sc, t.x, ncomp, tol2, delta2, maxiter);
+    sc->rpar = 0;
+    sc->fresh = 1;
+    sc->rho = 0.0;
+  }
+};
+
+// r = exp(i phi) b / scale (spin-planar binary16, buffer 0); buffer 1 = 0 (no r_old yet)
+__global__ __launch_bounds__(256) void k_store_b0_planar(const Spin<double>* __restrict__ b, half2_t* r0,
+                                                         half2_t* r1, size_t n, const CGScalars* sc,
+                                                         const phase16_t* rot) {
+  if (sc->done != 0) return;
+  const double inv = 1.0 / sc->scale;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<double> bb = b[i];
+    Spin<float> v;
+    v.a = {float(bb.a.x * inv), float(bb.a.y * inv)};
+    v.b = {float(bb.b.x * inv), float(bb.b.y * inv)};
+    rotate(v, rot, i, false);
+    st_planar(r0, n, i, v);
+    Spin<float> z;
+    z.a = {0.f, 0.f};
+    z.b = {0.f, 0.f};
+    st_planar(r1, n, i, z);
+  }
+}
+
+struct Ring4 {
+  SpinH* p[kXGroup];
+};
+
+// after a reliable update: the current residual buffer = exp(i phi) rfac rt; with
+// copy_last the last direction is copied into the p_old slot of the next group
+__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, half2_t* r0, half2_t* r1,
+                                                   Ring4 ring, int copy_last, size_t n, const CGScalars* sc,
+                                                   const phase16_t* rot) {
+  if (sc->done != 0 || sc->rescale == 0) return;
+  const float f = float(sc->rfac);
+  half2_t* r = (sc->rpar & 1) ? r1 : r0;
+  const int last = sc->last_pos;
+  const SpinH* src = (copy_last && last != kXGroup - 1) ? ring.p[last] : nullptr;
+  SpinH* dst = ring.p[kXGroup - 1];
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    Spin<float> v = rt[i];
+    v.a.x *= f;
+    v.a.y *= f;
+    v.b.x *= f;
+    v.b.y *= f;
+    rotate(v, rot, i, false);
+    st_planar(r, n, i, v);
+    if (src) dst[i] = src[i];
+  }
+}
+
+}  // namespace
+
+CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOptions& o,
+                   KernelTimes* layer_times) {
+  Dirac& D = K.dirac();
+  const size_t n = D.points();
+  hipStream_t st = D.stream();
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t hb = n * sizeof(SpinH), fb = n * sizeof(Spin<float>);
+  // the half-storage CG's slots for the stencil-side fields; the network's own fields in K's
+  SpinH* ring[kXGroup] = {static_cast<SpinH*>(D.scratch(1, hb)), static_cast<SpinH*>(D.scratch(2, hb)),
+                          static_cast<SpinH*>(D.scratch(12, hb)), static_cast<SpinH*>(D.scratch(13, hb))};
+  SpinH* w = static_cast<SpinH*>(D.scratch(3, hb));
+  Spin<float>* xin = static_cast<Spin<float>*>(D.scratch(4, fb));
+  Spin<double>* t = static_cast<Spin<double>*>(D.scratch(5, n * sizeof(Spin<double>)));
+  Spin<float>* rt = static_cast<Spin<float>*>(D.scratch(6, fb));
+  CGScalars* sc = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
+  CGScalars* hsc = D.host_scalars();
+  half2_t* R[2] = {static_cast<half2_t*>(K.work(0, hb)), static_cast<half2_t*>(K.work(1, hb))};
+  half2_t* G[2] = {static_cast<half2_t*>(K.work(2, hb)), static_cast<half2_t*>(K.work(3, hb))};
+  half2_t* H = static_cast<half2_t*>(K.work(4, hb));
+  half2_t* Z = static_cast<half2_t*>(K.work(5, hb));
+
+  const GridReduce gr = D.reducer();
+  const int nbk = int(stream_blocks(n));
+  const int ngate = int(gated_blocks(n));
+  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kXGroup - 1], n, gr);
+  hipLaunchKernelGGL(k_finalize<FinStartF>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
+                     FinStartF{sc, 4.0 * double(n), o.tol * o.tol, o.delta * o.delta, o.maxiter});
+  const DiracView<float, phase16_t> vin = D.view_half_temporal_gauge();
+  const phase16_t* rot = D.half_gauge_rotation();
+  hipLaunchKernelGGL(k_store_b0_planar, dim3(nbk), dim3(256), 0, st, b, R[0], R[1], n, sc, rot);
+  QED3_HIP(hipGetLastError());
+
+  const DiracView<double> v64 = D.view<double>();
+  const int nn = D.tables().max_nn;
+  const HaloView hvf = D.halo_for<float>(false), hvt = D.halo_for<float>(true);
+  const HaloView hv64f = D.halo_for<double>(false), hv64t = D.halo_for<double>(true);
+  const HaloView halo = D.halo(256);
+  const HornerView hv = K.view();
+  const int L = K.layers();
+  const double inv_ncomp = 1.0 / (4.0 * double(n));
+  PendRing pend;
+  for (int j = 0; j < kXGroup - 1; j++) pend.p[j] = ring[j];
+  Ring4 ring4;
+  for (int j = 0; j < kXGroup; j++) ring4.p[j] = ring[j];
+  EventTimer timer(o.times), ltimer(layer_times);
+  const hk::SrcPlanarPar src_r{R[0], R[1], &sc->rpar, n};
+  const hk::SrcPlanar<half2_t> src_h{H, n};
+  const EpiSkip eskip{sc};
+  const EpiDots edots{R[0], R[1], n, sc};
+
+  // one network application M r = K (K^dagger r): K^dagger's output in H, K's in Z
+  auto precond = [&](int k) {
+    const half2_t* prev = nullptr;
+    for (int l = 0; l < L; l++) {
+      const std::complex<double> c = K.params().c[L - 1 - l];
+      const float2 cj = {float(c.real()), float(-c.imag())};
+      half2_t* out = l == L - 1 ? H : G[l & 1];
+      const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 1, k);
+      if (l == 0)
+        hk::launch_horner_layer<true, true, true>(hv, vin, halo, src_r, prev, out, cj, eskip, gr, st, ev);
+      else
+        hk::launch_horner_layer<true, false, true>(hv, vin, halo, src_r, prev, out, cj, eskip, gr, st, ev);
+      prev = out;
+    }
+    prev = nullptr;
+    for (int l = 0; l < L; l++) {
+      const std::complex<double> c = K.params().c[L - 1 - l];
+      const float2 cj = {float(c.real()), float(c.imag())};
+      const bool last = l == L - 1;
+      half2_t* out = last ? Z : G[l & 1];
+      const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 2, k);
+      if (l == 0 && last)
+        hk::launch_horner_layer<false, true, true>(hv, vin, halo, src_h, prev, out, cj, edots, gr, st, ev);
+      else if (l == 0)
+        hk::launch_horner_layer<false, true, true>(hv, vin, halo, src_h, prev, out, cj, eskip, gr, st, ev);
+      else if (last)
+        hk::launch_horner_layer<false, false, true>(hv, vin, halo, src_h, prev, out, cj, edots, gr, st, ev);
+      else
+        hk::launch_horner_layer<false, false, true>(hv, vin, halo, src_h, prev, out, cj, eskip, gr, st, ev);
+      prev = out;
+    }
+  };
+  auto k2 = [&](auto tag, int k) {
+    constexpr int POS = decltype(tag)::value;
+    EpiK2F<POS> e{};
+    e.x = xin;
+    e.rbuf[0] = R[0];
+    e.rbuf[1] = R[1];
+    e.n = n;
+    for (int j = 0; j < kXGroup; j++) e.ring[j] = ring[j];
+    e.sc = sc;
+    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
+                                    timer.pair(EpiK2F<POS>::kUpdateX ? 2 : 1, k));
+  };
+  const bool ru_each = o.times != nullptr || layer_times != nullptr;
+  auto reliable_update = [&](int copy_last) {
+    hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0, rot);
+    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
+    launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr, st,
+                                 hv64t);
+    hipLaunchKernelGGL(k_rescale_f, dim3(ngate), dim3(256), 0, st, rt, R[0], R[1], ring4, copy_last, n, sc, rot);
+  };
+  auto one_iter = [&](int i, int k) {
+    const int pos = i % kXGroup;
+    SpinH* pold = ring[(pos + kXGroup - 1) % kXGroup];
+    SpinH* pnew = ring[pos];
+    precond(k);
+    launch_stencil_src<float, false>(vin, nn, SrcPUpdateZ{Z, n, pold, sc}, EpiK1PH{w, pnew, sc},
+                                     D.tile<float>(false), gr, st, hvf, timer.pair(0, k));
+    static_assert(kXGroup == 4, "one K2 instantiation per group position");
+    switch (pos) {
+      case 0: k2(std::integral_constant<int, 0>{}, k); break;
+      case 1: k2(std::integral_constant<int, 1>{}, k); break;
+      case 2: k2(std::integral_constant<int, 2>{}, k); break;
+      default: k2(std::integral_constant<int, 3>{}, k); break;
+    }
+    if (ru_each) reliable_update(0);
+  };
+  if (ru_each)
+    drive_chunks(one_iter, [] {}, o, n, st, sc, hsc);
+  else
+    drive_chunks(one_iter, [&] { reliable_update(1); }, o, n, st, sc, hsc);
+  timer.finish(hsc->iter);
+  ltimer.finish(hsc->iter);
+  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1, rot);
+  QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  QED3_HIP(hipStreamSynchronize(st));
+  CGResult res;
+  res.iterations = hsc->iter;
+  res.reliable_updates = hsc->n_ru;
+  res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
+  res.true_resid = final_residual(D, x, b, *hsc);
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
+  return res;
+}
+
+}  // namespace qed3
