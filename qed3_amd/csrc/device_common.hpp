@@ -106,6 +106,17 @@ __device__ __forceinline__ float2 ldu(const phase16_t* p) {
   return {__builtin_amdgcn_cosf(rev), __builtin_amdgcn_sinf(rev)};
 }
 
+// Link phase as a binary16 (cos, sin) pair (4 B per link): read without a sine or cosine,
+// for kernels whose arithmetic on the phase is itself binary16
+typedef _Float16 hvec2_t __attribute__((ext_vector_type(2)));
+struct phasepair_t {
+  hvec2_t v;
+};
+__device__ __forceinline__ float2 ldu(const phasepair_t* p) {
+  const hvec2_t h = p->v;
+  return {float(h.x), float(h.y)};
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -61,6 +61,17 @@ __global__ void k_tg_links(const double* __restrict__ angS, const double* __rest
   out[i] = angle16(angS[i] + phi64[size_t(s) * V + e.x] - phi64[size_t(s) * V + e.y]);
 }
 
+// (cos, sin) of 16-bit angles as binary16 pairs
+__global__ void k_angle_pairs(const phase16_t* __restrict__ a, phasepair_t* __restrict__ out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 u = ldu(a + i);
+  phasepair_t p;
+  p.v.x = (_Float16)u.x;
+  p.v.y = (_Float16)u.y;
+  out[i] = p;
+}
+
 // <x,y> = sum conj(x) y over complex double fields of n points.
 __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
                                              const Spin<double>* __restrict__ y, size_t n,
@@ -314,6 +325,7 @@ Dirac::~Dirac() {
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
                   (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
                   (void*)uSh_, (void*)uTh_, (void*)uStg_, (void*)uPolyH_, (void*)rotH_, (void*)phi64_,
+                  (void*)uStgP_, (void*)uPolyP_,
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
@@ -405,7 +417,37 @@ void Dirac::fill_temporal_gauge() {
   const size_t nS = size_t(h_.Lt) * h_.E;
   hipLaunchKernelGGL(k_tg_links, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, phi64_,
                      reinterpret_cast<const int4*>(link_ends_), uStg_, h_.E, h_.V, h_.Lt);
+  if (uStgP_) {
+    hipLaunchKernelGGL(k_angle_pairs, dim3((nS + B - 1) / B), dim3(B), 0, stream_, uStg_, uStgP_, nS);
+    hipLaunchKernelGGL(k_angle_pairs, dim3((h_.V + B - 1) / B), dim3(B), 0, stream_, uPolyH_, uPolyP_,
+                       size_t(h_.V));
+  }
   QED3_HIP(hipGetLastError());
+}
+
+DiracView<float, phasepair_t> Dirac::view_half_tg_pairs() {
+  const DiracView<float, phase16_t> tg = view_half_temporal_gauge();
+  if (!uStgP_) {
+    uStgP_ = dalloc<phasepair_t>(size_t(h_.Lt) * h_.E);
+    uPolyP_ = dalloc<phasepair_t>(size_t(h_.V));
+    if (has_gauge_) fill_temporal_gauge();
+  }
+  DiracView<float, phasepair_t> v;
+  v.nbr = tg.nbr;
+  v.lnk = tg.lnk;
+  v.hop = tg.hop;
+  v.hopd = tg.hopd;
+  v.diag = tg.diag;
+  v.kth = tg.kth;
+  v.uS = uStgP_;
+  v.uT = nullptr;
+  v.uPoly = uPolyP_;
+  v.V = tg.V;
+  v.E = tg.E;
+  v.Lt = tg.Lt;
+  v.r = tg.r;
+  v.bc = tg.bc;
+  return v;
 }
 
 DiracView<float, phase16_t> Dirac::view_half_temporal_gauge() {

@@ -116,6 +116,9 @@ class Dirac {
   DiracView<float, phase16_t> view_half_temporal_gauge();
   // phi_s(x) as 16-bit angles, [Lt][V] (valid once view_half_temporal_gauge was called)
   const phase16_t* half_gauge_rotation() const { return rotH_; }
+  // The temporal-gauge spatial links and wrap links of view_half_temporal_gauge as binary16
+  // (cos, sin) pairs (kept up to date with the gauge field from the first call on)
+  DiracView<float, phasepair_t> view_half_tg_pairs();
   template <typename T>
   StencilTile tile(bool dag) const;
 
@@ -185,6 +188,8 @@ class Dirac {
   phase16_t* uStg_ = nullptr;   // [Lt][E] spatial angles in the temporal gauge
   phase16_t* uPolyH_ = nullptr; // [V] the wrap link's angle in the temporal gauge
   phase16_t* rotH_ = nullptr;   // [Lt][V] phi_s(x)
+  phasepair_t* uStgP_ = nullptr;   // [Lt][E] uStg_ as (cos, sin) pairs
+  phasepair_t* uPolyP_ = nullptr;  // [V] uPolyH_ as (cos, sin) pairs
   double* phi64_ = nullptr;     // [Lt][V] phi_s(x) in FP64 while the tables are built
   void fill_temporal_gauge();
   double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)

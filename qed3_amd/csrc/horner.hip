@@ -37,7 +37,7 @@ namespace {
 //   mode 1 (K^dag):   row a = (conj(A_a) Hd_aa, conj(B_abar) Hd_a,abar)
 //   mode 2 (unit):    row a = (H_aa, H_a,abar), for the gradient
 __global__ void k_horner_tables(const float4* __restrict__ hop, const float4* __restrict__ hopd, float r,
-                                float4 AB, float4 ABc, float4* out, size_t ns) {
+                                float4 AB, float4 ABc, float4* out, HopRow16* out16, size_t ns) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const float2 A0 = {AB.x, AB.y}, A1 = {AB.z, AB.w};
@@ -59,34 +59,43 @@ __global__ void k_horner_tables(const float4* __restrict__ hop, const float4* __
     }
     out[(size_t(mode) * 2 + 0) * ns + i] = {s0.x, s0.y, f0.x, f0.y};
     out[(size_t(mode) * 2 + 1) * ns + i] = {s1.x, s1.y, f1.x, f1.y};
+    if (mode < 2) {
+      HopRow16 h0, h1;
+      h0.s = half2_t{(_Float16)s0.x, (_Float16)s0.y};
+      h0.f = half2_t{(_Float16)f0.x, (_Float16)f0.y};
+      h1.s = half2_t{(_Float16)s1.x, (_Float16)s1.y};
+      h1.f = half2_t{(_Float16)f1.x, (_Float16)f1.y};
+      out16[(size_t(mode) * 2 + 0) * ns + i] = h0;
+      out16[(size_t(mode) * 2 + 1) * ns + i] = h1;
+    }
   }
 }
 
-__global__ void k_to_planar(const Spin<double>* __restrict__ in, float2* out, size_t n) {
+__global__ void k_to_f32(const Spin<double>* __restrict__ in, Spin<float>* out, size_t n) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     const Spin<double> v = in[i];
-    out[i] = {float(v.a.x), float(v.a.y)};
-    out[n + i] = {float(v.b.x), float(v.b.y)};
+    Spin<float> f;
+    f.a = {float(v.a.x), float(v.a.y)};
+    f.b = {float(v.b.x), float(v.b.y)};
+    out[i] = f;
   }
 }
 
-__global__ void k_from_planar(const float2* __restrict__ in, Spin<double>* out, size_t n) {
+__global__ void k_to_f64(const Spin<float>* __restrict__ in, Spin<double>* out, size_t n) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    const float2 a = in[i], b = in[n + i];
+    const Spin<float> f = in[i];
     Spin<double> v;
-    v.a = {a.x, a.y};
-    v.b = {b.x, b.y};
+    v.a = {f.a.x, f.a.y};
+    v.b = {f.b.x, f.b.y};
     out[i] = v;
   }
 }
 
-// |e|^2 of a planar FP32 field (2n complex values)
-__global__ __launch_bounds__(256) void k_norm_planar(const float2* __restrict__ e, size_t n2, GridReduce gr) {
+// |e|^2 of an FP32 field (n points)
+__global__ __launch_bounds__(256) void k_norm_f32(const Spin<float>* __restrict__ e, size_t n, GridReduce gr) {
   double2 acc = {0.0, 0.0};
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += size_t(gridDim.x) * blockDim.x) {
-    const float2 v = e[i];
-    acc.x += double(v.x) * v.x + double(v.y) * v.y;
-  }
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    acc.x += norm2(e[i]);
   store_partial(acc, gr.partials, blockIdx.x);
 }
 
@@ -97,16 +106,20 @@ struct FinStore {
 };
 
 // lambda = w (g - e) with w = wscale / |e|^2; accumulates w |g - e|^2
-__global__ __launch_bounds__(256) void k_loss_planar(const float2* __restrict__ g, const float2* __restrict__ e,
-                                                     float2* lam, const double* enorm2, double wscale, size_t n2,
-                                                     GridReduce gr) {
+__global__ __launch_bounds__(256) void k_loss_f32(const Spin<float>* __restrict__ g, const Spin<float>* __restrict__ e,
+                                                  Spin<float>* lam, const double* enorm2, double wscale, size_t n,
+                                                  GridReduce gr) {
   const double w = wscale / *enorm2;
   double2 acc = {0.0, 0.0};
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += size_t(gridDim.x) * blockDim.x) {
-    const float2 a = g[i], b = e[i];
-    const double dx = double(a.x) - b.x, dy = double(a.y) - b.y;
-    acc.x += w * (dx * dx + dy * dy);
-    lam[i] = {float(w * dx), float(w * dy)};
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<float> a = g[i], b = e[i];
+    const double d0 = double(a.a.x) - b.a.x, d1 = double(a.a.y) - b.a.y, d2 = double(a.b.x) - b.b.x,
+                 d3 = double(a.b.y) - b.b.y;
+    acc.x += w * (d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
+    Spin<float> l;
+    l.a = {float(w * d0), float(w * d1)};
+    l.b = {float(w * d2), float(w * d3)};
+    lam[i] = l;
   }
   store_partial(acc, gr.partials, blockIdx.x);
 }
@@ -117,13 +130,13 @@ __global__ __launch_bounds__(256) void k_loss_planar(const float2* __restrict__ 
 //   q[4..5] = sum conj(Q_a) mu_a, Q_a = sum_j u_j H_a,abar g_abar(nb_j)  (-> B_a, times -1)
 // with g the previous step's output (has_g) and mu = dloss / d conj(y_t).
 template <int NS>
-__global__ __launch_bounds__(256) void k_horner_grad(HornerView hv, DiracView<float> d, const float2* __restrict__ v,
-                                                     const float2* __restrict__ mu, const float2* __restrict__ g,
-                                                     int has_g, TileMap tm, double* part) {
+__global__ __launch_bounds__(256) void k_horner_grad(HornerView hv, DiracView<float> d, const Spin<float>* __restrict__ v,
+                                                     const Spin<float>* __restrict__ mu,
+                                                     const Spin<float>* __restrict__ g, int has_g, TileMap tm,
+                                                     double* part) {
   int xr, a;
   tile_of(tm, blockIdx.x, xr, a);
   const int V = hv.V, Lt = hv.Lt;
-  const size_t n = size_t(V) * Lt;
   const int x = xr < tm.nxr ? xr * int(blockDim.x) + int(threadIdx.x) : V;
   double q[6] = {0, 0, 0, 0, 0, 0};
   if (x < V) {
@@ -138,8 +151,8 @@ __global__ __launch_bounds__(256) void k_horner_grad(HornerView hv, DiracView<fl
     }
     for (int s = 0; s < Lt; s++) {
       const size_t i = size_t(s) * V + x;
-      const float2 m = mu[size_t(a) * n + i];
-      const float2 vv = v[size_t(a) * n + i];
+      const float2 m = hk::comp(mu[i], a);
+      const float2 vv = hk::comp(v[i], a);
       q[0] += double(vv.x) * m.x + double(vv.y) * m.y;
       q[1] += double(vv.x) * m.y - double(vv.y) * m.x;
       if (has_g) {
@@ -151,7 +164,8 @@ __global__ __launch_bounds__(256) void k_horner_grad(HornerView hv, DiracView<fl
           float2 u = ldu(uS + (l >= 0 ? l : ~l));
           if (l < 0) u.y = -u.y;
           const size_t iy = size_t(s) * V + nb[j];
-          const float2 ga = g[size_t(a) * n + iy], gb = g[size_t(1 - a) * n + iy];
+          const Spin<float> gy = g[iy];
+          const float2 ga = hk::comp(gy, a), gb = hk::comp(gy, 1 - a);
           P = cadd<float>(P, cmul<float>(u, cmul<float>({hr[j].x, hr[j].y}, ga)));
           Q = cadd<float>(Q, cmul<float>(u, cmul<float>({hr[j].z, hr[j].w}, gb)));
         }
@@ -186,6 +200,7 @@ Horner::Horner(Dirac& D, int layers) : D_(D) {
   p_.c.assign(layers, cd(1.0, 0.0));
   const int V = D.V(), NS = D.tables().max_nn;
   QED3_HIP(hipMalloc((void**)&hrow_, size_t(6) * NS * V * sizeof(float4)));
+  QED3_HIP(hipMalloc((void**)&hrow16_, size_t(4) * NS * V * sizeof(HopRow16)));
   std::vector<float> invc(V), kt(V);
   for (int x = 0; x < V; x++) {
     invc[x] = float(1.0 / D.tables().diag[x]);
@@ -199,7 +214,7 @@ Horner::Horner(Dirac& D, int layers) : D_(D) {
 }
 
 Horner::~Horner() {
-  for (void* p : {(void*)hrow_, (void*)invc_, (void*)kt_, (void*)red_})
+  for (void* p : {(void*)hrow_, (void*)hrow16_, (void*)invc_, (void*)kt_, (void*)red_})
     if (p) (void)hipFree(p);
   for (void* p : work_)
     if (p) (void)hipFree(p);
@@ -217,7 +232,7 @@ void Horner::upload_tables() {
   const float4 AB = {float(p_.A[0].real()), float(p_.A[0].imag()), float(p_.A[1].real()), float(p_.A[1].imag())};
   const float4 Bc = {float(p_.B[0].real()), float(p_.B[0].imag()), float(p_.B[1].real()), float(p_.B[1].imag())};
   hipLaunchKernelGGL(k_horner_tables, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, D_.stream(), f.hop, f.hopd,
-                     float(D_.tables().r), AB, Bc, hrow_, ns);
+                     float(D_.tables().r), AB, Bc, hrow_, hrow16_, ns);
   QED3_HIP(hipGetLastError());
   QED3_HIP(hipStreamSynchronize(D_.stream()));
 }
@@ -225,6 +240,7 @@ void Horner::upload_tables() {
 HornerView Horner::view() const {
   HornerView v;
   v.hrow = hrow_;
+  v.hrow16 = hrow16_;
   v.invc = invc_;
   v.kt = kt_;
   v.V = D_.V();
@@ -247,37 +263,36 @@ void* Horner::work(int slot, size_t bytes) {
 }
 
 namespace {
-// One network application on planar FP32 fields in the original gauge: out = K in (or
-// K^dagger in); g0, g1 ping-pong buffers (out may be either).  Every layer output is kept
-// when keep != nullptr (keep[t] = output of step t).
-void run_layers(Horner& K, bool adj, const float2* in, float2* const* bufs, float2* out,
-                float2* const* keep) {
+// One network application on FP32 fields in the original gauge: out = K in (or
+// K^dagger in); b0, b1 ping-pong buffers.  Every layer output is kept when keep != nullptr
+// (keep[t] = output of step t).
+void run_layers(Horner& K, bool adj, const Spin<float>* in, Spin<float>* const* bufs, Spin<float>* out,
+                Spin<float>* const* keep) {
   Dirac& D = K.dirac();
   const HornerView hv = K.view();
   const DiracView<float> tab = D.view<float>();
   const HaloView halo = D.halo(256);
-  const size_t n = D.points();
   const int L = K.layers();
-  const hk::SrcPlanar<float2> src{in, n};
+  const hk::SrcF<Spin<float>> src{in};
   const GridReduce gr = D.reducer();
-  float2* prev = nullptr;
+  Spin<float>* prev = nullptr;
   for (int t = 0; t < L; t++) {
     const cd c = K.params().c[L - 1 - t];
     const float2 cj = {float(c.real()), float(adj ? -c.imag() : c.imag())};
-    float2* o = keep ? keep[t] : (t == L - 1 ? out : bufs[t & 1]);
+    Spin<float>* o = keep ? keep[t] : (t == L - 1 ? out : bufs[t & 1]);
     if (t == 0) {
       if (adj)
-        hk::launch_horner_layer<true, true, false>(hv, tab, halo, src, (const float2*)nullptr, o, cj, hk::EpiNone{},
-                                                   gr, D.stream());
+        hk::launch_horner_layer<true, true, false>(hv, tab, halo, src, (const Spin<float>*)nullptr, o, cj,
+                                                   hk::EpiNone{}, gr, D.stream());
       else
-        hk::launch_horner_layer<false, true, false>(hv, tab, halo, src, (const float2*)nullptr, o, cj,
+        hk::launch_horner_layer<false, true, false>(hv, tab, halo, src, (const Spin<float>*)nullptr, o, cj,
                                                     hk::EpiNone{}, gr, D.stream());
     } else {
       if (adj)
-        hk::launch_horner_layer<true, false, false>(hv, tab, halo, src, (const float2*)prev, o, cj, hk::EpiNone{},
-                                                    gr, D.stream());
+        hk::launch_horner_layer<true, false, false>(hv, tab, halo, src, (const Spin<float>*)prev, o, cj,
+                                                    hk::EpiNone{}, gr, D.stream());
       else
-        hk::launch_horner_layer<false, false, false>(hv, tab, halo, src, (const float2*)prev, o, cj,
+        hk::launch_horner_layer<false, false, false>(hv, tab, halo, src, (const Spin<float>*)prev, o, cj,
                                                      hk::EpiNone{}, gr, D.stream());
     }
     prev = o;
@@ -290,16 +305,16 @@ unsigned stream_grid(size_t n) { return unsigned(std::min<size_t>((n + 255) / 25
 
 void Horner::apply(bool adj, Spin<double>* out, const Spin<double>* in) {
   const size_t n = D_.points();
-  const size_t fb = 2 * n * sizeof(float2);
-  float2* pin = static_cast<float2*>(work(0, fb));
-  float2* b0 = static_cast<float2*>(work(1, fb));
-  float2* b1 = static_cast<float2*>(work(2, fb));
-  float2* po = static_cast<float2*>(work(3, fb));
+  const size_t fb = n * sizeof(Spin<float>);
+  Spin<float>* pin = static_cast<Spin<float>*>(work(0, fb));
+  Spin<float>* b0 = static_cast<Spin<float>*>(work(1, fb));
+  Spin<float>* b1 = static_cast<Spin<float>*>(work(2, fb));
+  Spin<float>* po = static_cast<Spin<float>*>(work(3, fb));
   hipStream_t st = D_.stream();
-  hipLaunchKernelGGL(k_to_planar, dim3(stream_grid(n)), dim3(256), 0, st, in, pin, n);
-  float2* bufs[2] = {b0, b1};
+  hipLaunchKernelGGL(k_to_f32, dim3(stream_grid(n)), dim3(256), 0, st, in, pin, n);
+  Spin<float>* bufs[2] = {b0, b1};
   run_layers(*this, adj, pin, bufs, po, nullptr);
-  hipLaunchKernelGGL(k_from_planar, dim3(stream_grid(n)), dim3(256), 0, st, po, out, n);
+  hipLaunchKernelGGL(k_to_f64, dim3(stream_grid(n)), dim3(256), 0, st, po, out, n);
   QED3_HIP(hipGetLastError());
   QED3_HIP(hipStreamSynchronize(st));
 }
@@ -308,15 +323,15 @@ double Horner::loss_grad(const Spin<double>* v, const Spin<double>* e, int batch
   if (batch < 1) throw std::invalid_argument("batch must be positive");
   const int L = p_.layers, V = D_.V(), NS = D_.tables().max_nn;
   const size_t n = D_.points();
-  const size_t fb = 2 * n * sizeof(float2);
+  const size_t fb = n * sizeof(Spin<float>);
   hipStream_t st = D_.stream();
-  float2* pv = static_cast<float2*>(work(0, fb));
-  float2* pe = static_cast<float2*>(work(1, fb));
-  float2* lam = static_cast<float2*>(work(2, fb));
-  float2* mu[2] = {static_cast<float2*>(work(3, fb)), static_cast<float2*>(work(4, fb))};
   if (L > kWorkSlots - 6) throw std::invalid_argument("too many layers for the training buffers");
-  std::vector<float2*> G(L);
-  for (int t = 0; t < L; t++) G[t] = static_cast<float2*>(work(6 + t, fb));
+  Spin<float>* pv = static_cast<Spin<float>*>(work(0, fb));
+  Spin<float>* pe = static_cast<Spin<float>*>(work(1, fb));
+  Spin<float>* lam = static_cast<Spin<float>*>(work(2, fb));
+  Spin<float>* mu[2] = {static_cast<Spin<float>*>(work(3, fb)), static_cast<Spin<float>*>(work(4, fb))};
+  std::vector<Spin<float>*> G(L);
+  for (int t = 0; t < L; t++) G[t] = static_cast<Spin<float>*>(work(6 + t, fb));
   double* dscal = static_cast<double*>(work(5, 2 * sizeof(double)));
   const TileMap tm = spin_tiles(V);
   const unsigned nblk = unsigned(tm.nxr_pad) * 2u;
@@ -332,29 +347,29 @@ double Horner::loss_grad(const Spin<double>* v, const Spin<double>* e, int batch
   std::vector<cd> g(p_.count(), cd(0, 0));
   std::vector<double> part(size_t(nblk) * 6);
   double loss = 0;
-  const unsigned nbk = stream_grid(2 * n);
+  const unsigned nbk = stream_grid(n);
   for (int b = 0; b < batch; b++) {
-    hipLaunchKernelGGL(k_to_planar, dim3(stream_grid(n)), dim3(256), 0, st, v + size_t(b) * n, pv, n);
-    hipLaunchKernelGGL(k_to_planar, dim3(stream_grid(n)), dim3(256), 0, st, e + size_t(b) * n, pe, n);
+    hipLaunchKernelGGL(k_to_f32, dim3(nbk), dim3(256), 0, st, v + size_t(b) * n, pv, n);
+    hipLaunchKernelGGL(k_to_f32, dim3(nbk), dim3(256), 0, st, e + size_t(b) * n, pe, n);
     run_layers(*this, false, pv, nullptr, nullptr, G.data());
-    hipLaunchKernelGGL(k_norm_planar, dim3(nbk), dim3(256), 0, st, pe, 2 * n, gr);
+    hipLaunchKernelGGL(k_norm_f32, dim3(nbk), dim3(256), 0, st, pe, n, gr);
     hipLaunchKernelGGL(k_finalize<FinStore>, dim3(1), dim3(256), 0, st, gr.partials, nbk, FinStore{dscal});
-    hipLaunchKernelGGL(k_loss_planar, dim3(nbk), dim3(256), 0, st, G[L - 1], pe, lam, dscal, 1.0 / batch, 2 * n, gr);
+    hipLaunchKernelGGL(k_loss_f32, dim3(nbk), dim3(256), 0, st, G[L - 1], pe, lam, dscal, 1.0 / batch, n, gr);
     hipLaunchKernelGGL(k_finalize<FinStore>, dim3(1), dim3(256), 0, st, gr.partials, nbk, FinStore{dscal + 1});
     QED3_HIP(hipGetLastError());
     double hl[2];
     QED3_HIP(hipMemcpyAsync(hl, dscal, sizeof(hl), hipMemcpyDeviceToHost, st));
     // backward: mu_{L-1} = T^-dag lambda; mu_{t-1} = T^-dag (S_th^dag mu_t)
-    const hk::SrcPlanar<float2> srcl{lam, n};
+    const hk::SrcF<Spin<float>> srcl{lam};
     for (int t = L - 1; t >= 0; t--) {
-      float2* mo = mu[t & 1];
+      Spin<float>* mo = mu[t & 1];
       if (t == L - 1)
-        hk::launch_horner_layer<true, true, false>(hv, tab, halo, srcl, (const float2*)nullptr, mo, float2{1.f, 0.f},
-                                                   hk::EpiNone{}, gr, st);
+        hk::launch_horner_layer<true, true, false>(hv, tab, halo, srcl, (const Spin<float>*)nullptr, mo,
+                                                   float2{1.f, 0.f}, hk::EpiNone{}, gr, st);
       else
-        hk::launch_horner_layer<true, false, false>(hv, tab, halo, srcl, (const float2*)mu[(t + 1) & 1], mo,
+        hk::launch_horner_layer<true, false, false>(hv, tab, halo, srcl, (const Spin<float>*)mu[(t + 1) & 1], mo,
                                                     float2{0.f, 0.f}, hk::EpiNone{}, gr, st);
-      const float2* gp = t > 0 ? G[t - 1] : nullptr;
+      const Spin<float>* gp = t > 0 ? G[t - 1] : nullptr;
       if (NS == 6)
         hipLaunchKernelGGL(k_horner_grad<6>, dim3(nblk), dim3(256), 0, st, hv, tab, pv, mo, gp, int(t > 0), tm, red_);
       else
@@ -364,17 +379,14 @@ double Horner::loss_grad(const Spin<double>* v, const Spin<double>* e, int batch
       QED3_HIP(hipStreamSynchronize(st));
       double q0[6] = {0, 0, 0, 0, 0, 0}, q1[6] = {0, 0, 0, 0, 0, 0};
       for (unsigned k = 0; k < nblk; k++) {
-        int xr, a;
-        // block k of the spin map: same decoding as tile_of
+        // block k's spin row, decoded as tile_of does
+        int a;
         if (tm.remap) {
           const int per = tm.nxr_pad >> 3, j = int(k) >> 3;
           a = j / per;
-          xr = (int(k) & 7) * per + (j - a * per);
         } else {
           a = int(k) / tm.nxr_pad;
-          xr = int(k) - a * tm.nxr_pad;
         }
-        (void)xr;
         double* q = a == 0 ? q0 : q1;
         for (int m = 0; m < 6; m++) q[m] += part[size_t(k) * 6 + m];
       }

@@ -51,9 +51,15 @@ struct HornerParams {
   void unpack(const std::complex<double>* p);
 };
 
+// One spin row's coefficients of one hop (same-spin, spin-flip) in binary16
+struct HopRow16 {
+  half2_t s, f;
+};
+
 // Device tables of one network on one operator.
 struct HornerView {
-  const float4* hrow = nullptr;  // [2 (K, K^dag)][2 (spin row)][nslot][V] (same, flip) coefficients
+  const float4* hrow = nullptr;  // [3 (K, K^dag, unit)][2 (spin row)][nslot][V] (same, flip) coefficients
+  const HopRow16* hrow16 = nullptr;  // [2 (K, K^dag)][2][nslot][V] the same in binary16
   const float* invc = nullptr;   // [V] 1 / c_x
   const float* kt = nullptr;     // [V] kappa_t(x) (the temporal hop weight for r = 1)
   int V = 0, Lt = 1, nslot = 3;
@@ -95,6 +101,7 @@ class Horner {
   Dirac& D_;
   HornerParams p_;
   float4* hrow_ = nullptr;
+  HopRow16* hrow16_ = nullptr;
   float* invc_ = nullptr;
   float* kt_ = nullptr;
   static constexpr int kWorkSlots = 12;

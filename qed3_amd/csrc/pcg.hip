@@ -10,9 +10,9 @@
 // One iteration is the network's 2L layer launches (K^dagger then K; the last computes
 // <r, z> and <r_old, z> and the scalar update) and the two stencil kernels of the
 // half-storage CG: K1 forms p = z + beta p_old while gathering, stores it and applies D,
-// K2 applies D^dagger, updates r and x.  The residual is double-buffered and spin-planar
-// (the network reads it row by row), the buffer in use chosen on the device, so that a
-// chunk of iterations is one graph replay whatever the reliable updates do.  Reliable
+// K2 applies D^dagger, updates r and x.  The residual is double-buffered, the buffer in use
+// chosen on the device, so that a chunk of iterations is one graph replay whatever the
+// reliable updates do.  Reliable
 // updates, scales and the temporal gauge are those of the half-storage CG (cg.hip); after
 // an update beta carries the change of scale (r_old and p_old are kept at the old scale).
 #include <chrono>
@@ -31,50 +31,20 @@ using namespace cgepi;
 using namespace cgdrive;
 
 
-// K1 source: p = z + beta p_old with z spin-planar binary16, p_old a binary16 spinor,
-// rounded to binary16 so D acts on exactly the stored p
-struct SrcPUpdateZ {
-  const half2_t* z;
-  size_t n;
-  const SpinH* p;
-  const CGScalars* sc;
-  struct Bound {
-    const half2_t* __restrict__ z;
-    size_t n;
-    const SpinH* __restrict__ p;
-    float beta;
-    __device__ Spin<float> operator()(size_t i) const {
-      const float2 za = hk::ldc(z + i), zb = hk::ldc(z + n + i);
-      const Spin<float> c = ldh(p + i);
-      Spin<float> a;
-      a.a = {za.x + beta * c.a.x, za.y + beta * c.a.y};
-      a.b = {zb.x + beta * c.b.x, zb.y + beta * c.b.y};
-      return from_half(to_half(a));
-    }
-  };
-  __device__ Bound bind() const { return Bound{z, n, p, float(sc->beta)}; }
-};
-
-__device__ __forceinline__ void st_planar(half2_t* r, size_t n, size_t i, const Spin<float>& v) {
-  hk::stc(r + i, v.a);
-  hk::stc(r + n + i, v.b);
-}
-
-// K2: q = D^dag w; r_new = r - alpha q into the other residual buffer (spin-planar
-// binary16), |r_new|^2 and the reliable-update decision; x grouped as in EpiK2H
+// K2: q = D^dag w; r_new = r - alpha q into the other residual buffer, |r_new|^2 and the
+// reliable-update decision; x grouped as in EpiK2H
 template <int POS>
 struct EpiK2F {
   static constexpr bool kReduce = true;
   static constexpr bool kUpdateX = POS == kXGroup - 1;
   Spin<float>* x;
-  half2_t* rbuf[2];
-  size_t n;
+  SpinH* rbuf[2];
   const SpinH* ring[kXGroup];
   CGScalars* sc;
   float alpha;
   float ap[kXGroup - 1];
-  const half2_t* rin;
-  half2_t* rout;
+  const SpinH* rin;
+  SpinH* rout;
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
@@ -84,19 +54,14 @@ struct EpiK2F {
     rin = rbuf[par];
     rout = rbuf[par ^ 1];
   }
+  // positions 0 .. kXGroup-2 load r with the slice's other loads (as EpiK2H)
   template <int P = POS, std::enable_if_t<P != kXGroup - 1, int> = 0>
-  __device__ float4 prefetch(size_t i) const {
-    const float2 a = hk::ldc(rin + i), b = hk::ldc(rin + n + i);
-    return {a.x, a.y, b.x, b.y};
+  __device__ SpinH prefetch(size_t i) const {
+    return rin[i];
   }
-  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const {
-    const float2 a = hk::ldc(rin + i), b = hk::ldc(rin + n + i);
-    (*this)(i, q, acc, float4{a.x, a.y, b.x, b.y});
-  }
-  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const float4& rpre) const {
-    Spin<float> rr;
-    rr.a = {rpre.x, rpre.y};
-    rr.b = {rpre.z, rpre.w};
+  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const { (*this)(i, q, acc, rin[i]); }
+  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const SpinH& rpre) const {
+    Spin<float> rr = from_half(rpre);
     if constexpr (kUpdateX) {
       Spin<float> xx = ld(x + i);
 #pragma unroll
@@ -116,9 +81,9 @@ struct EpiK2F {
     rr.a.y -= alpha * q.a.y;
     rr.b.x -= alpha * q.b.x;
     rr.b.y -= alpha * q.b.y;
-    const Spin<float> rh = from_half(to_half(rr));
-    st_planar(rout, n, i, rh);
-    acc.x += norm2(rh);
+    const SpinH rh = to_half(rr);
+    sth_nt(rout + i, rh);
+    acc.x += norm2(from_half(rh));
   }
   __device__ void operator()(double2 t) const {
     const double m = t.x;
@@ -147,7 +112,7 @@ struct EpiSkip {
   static constexpr bool kReduce = false;
   const CGScalars* sc;
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
-  __device__ void operator()(int, size_t, float2, double2&) const {}
+  __device__ void operator()(size_t, const Spin<float>&, double2&) const {}
   __device__ void operator()(double2) const {}
 };
 
@@ -156,18 +121,15 @@ struct EpiSkip {
 // old one (rfac = old / new): beta = (<r,z> - rfac <r_old,z>) / (rfac rho_old).
 struct EpiDots {
   static constexpr bool kReduce = true;
-  const half2_t* r0;
-  const half2_t* r1;
-  size_t n;
+  const SpinH* r0;
+  const SpinH* r1;
   CGScalars* sc;
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
-  __device__ void operator()(int a, size_t i, float2 z, double2& acc) const {
+  __device__ void operator()(size_t i, const Spin<float>& z, double2& acc) const {
     const int par = sc->rpar & 1;
-    const half2_t* rc = par ? r1 : r0;
-    const half2_t* rp = par ? r0 : r1;
-    const float2 u = hk::ldc(rc + size_t(a) * n + i), w = hk::ldc(rp + size_t(a) * n + i);
-    acc.x += double(u.x) * z.x + double(u.y) * z.y;
-    acc.y += double(w.x) * z.x + double(w.y) * z.y;
+    const Spin<float> u = ldh((par ? r1 : r0) + i), w = ldh((par ? r0 : r1) + i);
+    acc.x += double(u.a.x) * z.a.x + double(u.a.y) * z.a.y + double(u.b.x) * z.b.x + double(u.b.y) * z.b.y;
+    acc.y += double(w.a.x) * z.a.x + double(w.a.y) * z.a.y + double(w.b.x) * z.b.x + double(w.b.y) * z.b.y;
   }
   __device__ void operator()(double2 t) const {
     const double A = t.x, Bv = t.y;
@@ -200,23 +162,18 @@ struct FinStartF {
   }
 };
 
-// r = exp(i phi) b / scale (spin-planar binary16, buffer 0); buffer 1 = 0 (no r_old yet)
-__global__ __launch_bounds__(256) void k_store_b0_planar(const Spin<double>* __restrict__ b, half2_t* r0,
-                                                         half2_t* r1, size_t n, const CGScalars* sc,
-                                                         const phase16_t* rot) {
+// r = exp(i phi) b / scale in binary16 (buffer 0); buffer 1 = 0 (no r_old yet)
+__global__ __launch_bounds__(256) void k_store_b0_f(const Spin<double>* __restrict__ b, SpinH* r0, SpinH* r1,
+                                                    size_t n, const CGScalars* sc, const phase16_t* rot) {
   if (sc->done != 0) return;
   const double inv = 1.0 / sc->scale;
+  Spin<float> z;
+  z.a = {0.f, 0.f};
+  z.b = {0.f, 0.f};
+  const SpinH zh = to_half(z);
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    const Spin<double> bb = b[i];
-    Spin<float> v;
-    v.a = {float(bb.a.x * inv), float(bb.a.y * inv)};
-    v.b = {float(bb.b.x * inv), float(bb.b.y * inv)};
-    rotate(v, rot, i, false);
-    st_planar(r0, n, i, v);
-    Spin<float> z;
-    z.a = {0.f, 0.f};
-    z.b = {0.f, 0.f};
-    st_planar(r1, n, i, z);
+    half_store_b(b, r0, inv, i, rot);
+    r1[i] = zh;
   }
 }
 
@@ -226,23 +183,17 @@ struct Ring4 {
 
 // after a reliable update: the current residual buffer = exp(i phi) rfac rt; with
 // copy_last the last direction is copied into the p_old slot of the next group
-__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, half2_t* r0, half2_t* r1,
+__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, SpinH* r0, SpinH* r1,
                                                    Ring4 ring, int copy_last, size_t n, const CGScalars* sc,
                                                    const phase16_t* rot) {
   if (sc->done != 0 || sc->rescale == 0) return;
   const float f = float(sc->rfac);
-  half2_t* r = (sc->rpar & 1) ? r1 : r0;
+  SpinH* r = (sc->rpar & 1) ? r1 : r0;
   const int last = sc->last_pos;
   const SpinH* src = (copy_last && last != kXGroup - 1) ? ring.p[last] : nullptr;
   SpinH* dst = ring.p[kXGroup - 1];
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    Spin<float> v = rt[i];
-    v.a.x *= f;
-    v.a.y *= f;
-    v.b.x *= f;
-    v.b.y *= f;
-    rotate(v, rot, i, false);
-    st_planar(r, n, i, v);
+    half_rescale_point(rt, r, f, i, rot);
     if (src) dst[i] = src[i];
   }
 }
@@ -265,10 +216,10 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   Spin<float>* rt = static_cast<Spin<float>*>(D.scratch(6, fb));
   CGScalars* sc = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
   CGScalars* hsc = D.host_scalars();
-  half2_t* R[2] = {static_cast<half2_t*>(K.work(0, hb)), static_cast<half2_t*>(K.work(1, hb))};
-  half2_t* G[2] = {static_cast<half2_t*>(K.work(2, hb)), static_cast<half2_t*>(K.work(3, hb))};
-  half2_t* H = static_cast<half2_t*>(K.work(4, hb));
-  half2_t* Z = static_cast<half2_t*>(K.work(5, hb));
+  SpinH* R[2] = {static_cast<SpinH*>(K.work(0, hb)), static_cast<SpinH*>(K.work(1, hb))};
+  SpinH* G[2] = {static_cast<SpinH*>(K.work(2, hb)), static_cast<SpinH*>(K.work(3, hb))};
+  SpinH* H = static_cast<SpinH*>(K.work(4, hb));
+  SpinH* Z = static_cast<SpinH*>(K.work(5, hb));
 
   const GridReduce gr = D.reducer();
   const int nbk = int(stream_blocks(n));
@@ -277,8 +228,9 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   hipLaunchKernelGGL(k_finalize<FinStartF>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
                      FinStartF{sc, 4.0 * double(n), o.tol * o.tol, o.delta * o.delta, o.maxiter});
   const DiracView<float, phase16_t> vin = D.view_half_temporal_gauge();
+  const DiracView<float, phasepair_t> vpairs = D.view_half_tg_pairs();
   const phase16_t* rot = D.half_gauge_rotation();
-  hipLaunchKernelGGL(k_store_b0_planar, dim3(nbk), dim3(256), 0, st, b, R[0], R[1], n, sc, rot);
+  hipLaunchKernelGGL(k_store_b0_f, dim3(nbk), dim3(256), 0, st, b, R[0], R[1], n, sc, rot);
   QED3_HIP(hipGetLastError());
 
   const DiracView<double> v64 = D.view<double>();
@@ -294,23 +246,23 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   Ring4 ring4;
   for (int j = 0; j < kXGroup; j++) ring4.p[j] = ring[j];
   EventTimer timer(o.times), ltimer(layer_times);
-  const hk::SrcPlanarPar src_r{R[0], R[1], &sc->rpar, n};
-  const hk::SrcPlanar<half2_t> src_h{H, n};
+  const hk::SrcPar src_r{R[0], R[1], &sc->rpar};
+  const hk::SrcF<SpinH> src_h{H};
   const EpiSkip eskip{sc};
-  const EpiDots edots{R[0], R[1], n, sc};
+  const EpiDots edots{R[0], R[1], sc};
 
   // one network application M r = K (K^dagger r): K^dagger's output in H, K's in Z
   auto precond = [&](int k) {
-    const half2_t* prev = nullptr;
+    const SpinH* prev = nullptr;
     for (int l = 0; l < L; l++) {
       const std::complex<double> c = K.params().c[L - 1 - l];
       const float2 cj = {float(c.real()), float(-c.imag())};
-      half2_t* out = l == L - 1 ? H : G[l & 1];
+      SpinH* out = l == L - 1 ? H : G[l & 1];
       const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 1, k);
       if (l == 0)
-        hk::launch_horner_layer<true, true, true>(hv, vin, halo, src_r, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer<true, true, true>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
       else
-        hk::launch_horner_layer<true, false, true>(hv, vin, halo, src_r, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer<true, false, true>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
       prev = out;
     }
     prev = nullptr;
@@ -318,16 +270,16 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
       const std::complex<double> c = K.params().c[L - 1 - l];
       const float2 cj = {float(c.real()), float(c.imag())};
       const bool last = l == L - 1;
-      half2_t* out = last ? Z : G[l & 1];
+      SpinH* out = last ? Z : G[l & 1];
       const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 2, k);
       if (l == 0 && last)
-        hk::launch_horner_layer<false, true, true>(hv, vin, halo, src_h, prev, out, cj, edots, gr, st, ev);
+        hk::launch_horner_layer<false, true, true>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
       else if (l == 0)
-        hk::launch_horner_layer<false, true, true>(hv, vin, halo, src_h, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer<false, true, true>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
       else if (last)
-        hk::launch_horner_layer<false, false, true>(hv, vin, halo, src_h, prev, out, cj, edots, gr, st, ev);
+        hk::launch_horner_layer<false, false, true>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
       else
-        hk::launch_horner_layer<false, false, true>(hv, vin, halo, src_h, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer<false, false, true>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
       prev = out;
     }
   };
@@ -337,7 +289,6 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
     e.x = xin;
     e.rbuf[0] = R[0];
     e.rbuf[1] = R[1];
-    e.n = n;
     for (int j = 0; j < kXGroup; j++) e.ring[j] = ring[j];
     e.sc = sc;
     launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
@@ -356,7 +307,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
     SpinH* pold = ring[(pos + kXGroup - 1) % kXGroup];
     SpinH* pnew = ring[pos];
     precond(k);
-    launch_stencil_src<float, false>(vin, nn, SrcPUpdateZ{Z, n, pold, sc}, EpiK1PH{w, pnew, sc},
+    launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{Z, pold, sc}, EpiK1PH{w, pnew, sc},
                                      D.tile<float>(false), gr, st, hvf, timer.pair(0, k));
     static_assert(kXGroup == 4, "one K2 instantiation per group position");
     switch (pos) {
