@@ -1,40 +1,41 @@
 """Headline benchmark: CG solves/s on D^dag D to a true FP64 relative residual of 1e-10.
 
-Learned workload (the value): dual (trivalent) refined icosahedron n=128 (327,680 sites)
-x Lt=64, antiperiodic in time, r=1, M5=m=0.02, von Mises(kappa=1.4) link angles, Gaussian
-RHS.  Four solvers are timed on the same seeded RHS (one complete solve per step): the
-flexible CG with the learned preconditioner M = N N^dagger (FP32 vectors, FP64 reliable
-updates; the net trained first, outside the timed region, on a small lattice of the same
-family, dual n=16 x Lt=16, its weights applied unchanged at full size), the
-unpreconditioned mixed CG (FP32 + FP64 reliable updates), the same CG with its inner
-fields stored in binary16 (FP32 arithmetic, FP64 reliable updates) and the plain FP64 CG.
-`value` is the solves/s of the fastest of them, named in config.solver; every method's
-figures and its ratio to plain FP64 CG are in config.methods.  `roofline` is that of the
-stencil kernel of the fastest method (one D apply at its storage precision).
-
-Second workload: simplicial n=128 (163,842 sites) x Lt=64, m=0.05: the three CGs.
+Workload (the value): dual (trivalent) refined icosahedron n=128 (327,680 sites) x Lt=64,
+antiperiodic in time, r=1, M5=m=0.02, von Mises(kappa=1.4) link angles, Gaussian RHS.
+Four solvers are timed on the same seeded right-hand sides, `--steps` complete solves each
+after `--warmup` untimed ones, fields resident in HBM:
+  learned  flexible CG on D^dag D preconditioned by M = K K^dagger, K the time-exact Horner
+           network (exact inverse of the operator's time part alternated with spatial
+           message passing, learned per-spin weights), binary16-stored fields, FP32
+           arithmetic, FP64 reliable updates; trained beforehand (outside the timed region) on
+           (residual, error) pairs from the solver's own solves at beta in {1.0, 1.4, 2.0};
+  half     the unpreconditioned CG with the same binary16 storage;
+  mixed    the unpreconditioned CG in FP32 + FP64 reliable updates;
+  fp64     plain FP64 CG (the metric's denominator).
+`value` is the learned method's solves/s (the metric's numerator); every method's figures and
+its ratio to plain FP64 CG are in config.methods.  `roofline` is the learned method's
+dominant kernel (a transport layer of the network), timed with HIP events around every
+launch of one solve.  Second workload (config.second_workload): simplicial n=128 x Lt=64,
+m=0.05 with the same four solvers (the network at its Neumann coefficients).
 Prints one JSON line.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-# Bytes moved per D launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
-# 1 GiB copy kernel) from two rocprofv3 --pmc runs of `bench.py --kernels-only --lattice 1
-# --m 0.02 --tile <block,ts,remap,lds>` per launch shape the autotuner picks; summary in
-# profiles/stencil_D_dual_n128_Lt64_hbm_traffic.txt.  Keyed by (precision or CG kernel,
-# lattice, n, Lt, tile); the CG kernels ("K1-half" ...) from the half-storage solve the same
-# command runs at the pinned shape.
+
 MEASURED_TRAFFIC = {("f16", 1, 128, 64, 256, 64, 1, 1): 4.9646e8,
                     ("f32", 1, 128, 64, 256, 64, 1, 1): 1.2992e9,
                     ("K1-half", 1, 128, 64, 256, 64, 1, 1): 8.5079e8,
                     ("K2-half", 1, 128, 64, 256, 64, 1, 1): 6.3056e8,
                     ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 1.9851e9}
+
 
 
 def parse():
@@ -44,32 +45,32 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=128)
     p.add_argument("--Lt", type=int, default=64)
-    p.add_argument("--lattice", type=int, default=0)
-    p.add_argument("--m", type=float, default=0.05)
+    p.add_argument("--m", type=float, default=0.02, help="M5 of the learned (dual) workload")
+    p.add_argument("--m2", type=float, default=0.05, help="M5 of the second (simplicial) workload")
     p.add_argument("--beta", type=float, default=1.4)
     p.add_argument("--tol", type=float, default=1e-10)
+    p.add_argument("--layers", type=int, default=4, help="layers of the preconditioner network")
+    p.add_argument("--delta", type=float, default=0.001, help="reliable-update threshold of the learned method")
+    p.add_argument("--train-epochs", type=int, default=6)
+    p.add_argument("--no-train", action="store_true")
     p.add_argument("--cpu-lt", type=int, default=8)
-    p.add_argument("--cpu-iters", type=int, default=600)
+    p.add_argument("--cpu-iters", type=int, default=1500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
-    p.add_argument("--no-learned", action="store_true")
+    p.add_argument("--no-second", action="store_true")
     p.add_argument("--no-hmc", action="store_true")
     p.add_argument("--no-overlap", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
     p.add_argument("--hmc-traj", type=int, default=3)
     p.add_argument("--hmc-therm", type=int, default=6)
     p.add_argument("--hmc-inner", type=int, default=50)
     p.add_argument("--hmc-nsteps", type=int, default=20)
-    p.add_argument("--batched", action="store_true",
-                   help="also time the 64-system batched CG (uneven sizes) against one-by-one solves")
     p.add_argument("--batch-nsys", type=int, default=64)
-    p.add_argument("--learned-steps", type=int, default=2)
-    p.add_argument("--train-n", type=int, default=16)
-    p.add_argument("--train-Lt", type=int, default=16)
-    p.add_argument("--train-epochs", type=int, default=40)
     p.add_argument("--tile", type=str, default=None,
                    help="block,ts,remap,lds for the FP32 D stencil instead of the autotuned one")
     p.add_argument("--kernels-only", action="store_true",
-                   help="only launch the D stencils and the calibration copy (for profilers)")
+                   help="only launch the D stencils, one preconditioned solve and the calibration copy "
+                        "(for profilers)")
     return p.parse_args()
 
 
@@ -103,154 +104,75 @@ def algorithmic_bytes_D(V, E, Lt, real_bytes):
     return 2 * b_vec + b_link
 
 
+
+def context(a, q, lattice, m, seed):
+    g = q.Geometry(a.n)
+    at = 0.5 * mean_ell(g, lattice)
+    D = q.WilsonDirac(lattice, a.n, a.Lt, r=1.0, M5=m, at=at)
+    sp, tp = q.random_gauge(lattice, a.n, a.Lt, seed=seed, kappa=a.beta)
+    D.update(sp, tp)
+    D.autotune()
+    return D, g, at, sp, tp
+
+
+def layer_bytes(V, E, Lt, nslot, first):
+    """Algorithmic bytes of one network layer launch (binary16 fields, 8 B per point): the
+    injection read and the output written (+ the input field read by a transport layer), the
+    link phases of the temporal gauge as binary16 (cos, sin) pairs (4 B per spatial link and
+    per wrap link) and the per-site tables once (hop rows of both spin rows 2 x 8 B per slot,
+    local and link indices 2 + 4 B per slot, 1/c and kappa_t)."""
+    pts = V * Lt
+    if first:
+        return 2 * pts * 8 + V * 8
+    return 3 * pts * 8 + E * Lt * 4 + V * 4 + V * (nslot * (16 + 6) + 8)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "qed3_amd runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "qed3_amd runs on one GPU (no collectives, no sharding); "
+                                   "run bench.py with --gpus 1"}))
         sys.exit(1)
     import qed3_amd as q
 
-    t_setup = time.time()
-    g = q.Geometry(a.n)
-    at = 0.5 * mean_ell(g, a.lattice)
-    D = q.WilsonDirac(a.lattice, a.n, a.Lt, r=1.0, M5=a.m, at=at)
-    sp, tp = q.random_gauge(a.lattice, a.n, a.Lt, seed=1234, kappa=a.beta)
-    D.update(sp, tp)
-    D.autotune()
-    print(f"# setup {time.time() - t_setup:.1f}s V={D.V} E={D.E} Lt={a.Lt} at={at:.5f}",
-          file=sys.stderr, flush=True)
-
     if a.kernels_only:
-        if a.tile:
-            b_, t_, r_, l_ = (int(v) for v in a.tile.split(","))
-            D.set_tile(b_, t_, r_, l_, ops=(q.OP_D, q.OP_DDAG), precs=(q.F32,))
-        for prec, nm in ((q.F16, "f16"), (q.F32, "f32"), (q.F64, "f64")):
-            for op in (q.OP_D, q.OP_DDAG):
-                ms = D.time_apply(op, prec, warmup=2, reps=10)
-                rb = {q.F16: 2, q.F32: 4, q.F64: 8}[prec]
-                gbs = algorithmic_bytes_D(D.V, D.E, a.Lt, rb) / (ms * 1e-3) / 1e9
-                print(f"# D op={op} {nm}: {ms:.4f} ms  {gbs:.0f} GB/s algorithmic  "
-                      f"tile={tile_info(q, D, op, prec)}", file=sys.stderr, flush=True)
-        # one half-storage CG solve at the same shape: its K1 / K2 kernels for the counters
-        d_b = q.DeviceArray.from_numpy(q.random_source(a.lattice, a.n, a.Lt, seed=1000))
-        d_x = q.DeviceArray(D.N * 16)
-        rep, ms = D.solve_kernel_times(d_x, d_b, tol=a.tol, solver=q.CG_HALF)
-        print(f"# half CG: {rep.iterations} its, K1 {ms[0]:.4f} ms, K2 {ms[1]:.4f} ms, "
-              f"K2 with x update {ms[2]:.4f} ms", file=sys.stderr, flush=True)
-        gbs, ms = copy_bandwidth(q, 1 << 30, reps=10)
-        print(f"# copy 1 GiB: {ms:.4f} ms  {gbs:.0f} GB/s", file=sys.stderr, flush=True)
-        return
-
-    nbytes = D.N * 16
-    d_x = q.DeviceArray(nbytes)
-    rhs = [q.DeviceArray.from_numpy(q.random_source(a.lattice, a.n, a.Lt, seed=1000 + k))
-           for k in range(a.warmup + a.steps)]
-
-    def run(solver, k):
-        return D.solve_device(d_x, rhs[k], tol=a.tol, solver=solver)
-
-    results = {}
-    for name, solver in (("half", q.CG_HALF), ("mixed", q.CG_MIXED), ("fp64", q.CG_F64)):
-        if name == "fp64" and a.no_fp64:
-            continue
-        for k in range(a.warmup):
-            run(solver, k)
-        q.load().qed3_device_synchronize()
-        t0 = time.time()
-        reps = []
-        for k in range(a.warmup, a.warmup + a.steps):
-            r = run(solver, k)
-            reps.append(r)
-            print(f"# {name} solve {k}: iters={r.iterations} ru={r.reliable_updates} "
-                  f"resid={r.true_resid:.2e} {r.seconds:.3f}s", file=sys.stderr, flush=True)
-        q.load().qed3_device_synchronize()
-        dt = time.time() - t0
-        results[name] = dict(solves_per_s=a.steps / dt, ms_per_solve=1e3 * dt / a.steps,
-                             iters=float(np.mean([r.iterations for r in reps])),
-                             reliable_updates=float(np.mean([r.reliable_updates for r in reps])),
-                             max_true_resid=float(max(r.true_resid for r in reps)))
-
-    # roofline of the dominant kernel: the FP32 stencil (D) inside the mixed solver
-    ms32 = D.time_apply(q.OP_D, q.F32, warmup=5, reps=50)
-    tile32 = tile_info(q, D, q.OP_D, q.F32)
-    D_V, D_E = D.V, D.E
-    ms64 = D.time_apply(q.OP_D, q.F64, warmup=5, reps=50)
-    b32 = algorithmic_bytes_D(D.V, D.E, a.Lt, 4)
-    b64 = algorithmic_bytes_D(D.V, D.E, a.Lt, 8)
-    gbs32 = b32 / (ms32 * 1e-3) / 1e9
-    gbs64 = b64 / (ms64 * 1e-3) / 1e9
-    copy_gbs, _ = copy_bandwidth(q)
-
+        return kernels_only(a, q)
+    learned = learned_leg(a, q)
+    second = None if a.no_second else second_leg(a, q)
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(a, g, at, results.get("fp64", results["mixed"])["iters"], sp, tp)
-    del D, d_x, rhs
-    learned = learned_leg(a, q, g, with_net=not a.no_learned)
+        fp64_its = learned["methods"].get("fp64", learned["methods"]["half"])["iters"]
+        cpu = cpu_baseline(a, q, fp64_its)
     hmc = None if a.no_hmc else hmc_leg(a, q)
     overlap = None if a.no_overlap else overlap_leg(q)
-    batched = batched_leg(a, q) if a.batched else None
+    batched = None if a.no_batched else batched_leg(a, q)
 
-    simp = {"workload": f"{'simplicial' if a.lattice == 0 else 'dual'} n={a.n} Lt={a.Lt} "
-                        f"m={a.m} tol={a.tol} D^dag D CG", "methods": results,
-            "speedup_mixed_vs_fp64": (round(results["mixed"]["solves_per_s"]
-                                            / results["fp64"]["solves_per_s"], 3)
-                                      if "fp64" in results else None),
-            "speedup_half_vs_fp64": (round(results["half"]["solves_per_s"]
-                                           / results["fp64"]["solves_per_s"], 3)
-                                     if "fp64" in results else None)}
-    solver_names = {"learned": "flexible CG, M = N N^dag (4-layer net), FP32 + FP64 reliable updates",
-                    "half": "CG, binary16-stored r/p/w (FP32 arithmetic, FP32 x) + FP64 reliable updates",
-                    "mixed": "CG, FP32 + FP64 reliable updates", "fp64": "CG, FP64"}
-    if True:
-        meth = learned["methods"]
-        winner = max(meth, key=lambda k: meth[k]["solves_per_s"])
-        best = meth[winner]
+    meth = learned["methods"]
+    if "fp64" in meth:
         for k in meth:
-            if "fp64" in meth:
-                meth[k]["vs_fp64"] = round(meth[k]["solves_per_s"] / meth["fp64"]["solves_per_s"], 3)
-        steps, ms = a.learned_steps, best["ms_per_solve"]
-        config = {"workload": learned["workload"], "n_refine": a.n, "Lt": a.Lt,
-                  "sites": learned["sites"], "links": learned["links"],
-                  "solver": solver_names[winner],
-                  "methods": learned["methods"], "training": learned["training"],
-                  "speedup_learned_vs_fp64": learned["speedup_vs_fp64"],
-                  "speedup_mixed_vs_fp64": learned["speedup_mixed_vs_fp64"],
-                  "speedup_half_vs_fp64": learned["speedup_half_vs_fp64"],
-                  "net_apply": learned["net_apply"], "solution_agreement": learned["agreement"],
-                  "second_workload": simp, "hmc": hmc, "overlap": overlap, "batched": batched}
-        # the dominant kernel of the fastest CG: its K1 (the most time per solve)
-        ck = learned["cg_kernels"]
-        src = winner if winner in ck else "mixed"
-        k1 = ck[src]["K1"]
-        roof = {"bound": "hbm", "achieved": k1["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": k1["frac"], "traffic": k1["traffic"],
-                "kernel": k1["kernel"] + f" (CG K1 of the {src} solver: p = r + beta p_old, w = D p, |w|^2)",
-                "bytes_per_launch": k1["bytes_per_launch"], "ms_per_launch": k1["ms_per_launch"],
-                "tile": ck[src]["tile"], "cg_kernels": ck, "bare_D": learned["roofline"]}
-        dtype = ("f32 inner (bf16 net activations) / f64 reliable updates" if winner == "learned"
-                 else "f16 storage, f32 arithmetic / f64 reliable updates" if winner == "half"
-                 else "f32 inner / f64 reliable updates" if winner == "mixed" else "f64")
+            meth[k]["vs_fp64"] = round(meth[k]["solves_per_s"] / meth["fp64"]["solves_per_s"], 3)
+    best = meth["learned"]
+    config = {"workload": learned["workload"], "n_refine": a.n, "Lt": a.Lt,
+              "sites": learned["sites"], "links": learned["links"],
+              "solver": f"flexible CG (FCG(1)) on D^dag D, M = K K^dag, K = {a.layers}-layer time-exact "
+                        "Horner network; binary16 fields, FP32 arithmetic, FP64 reliable updates",
+              "methods": meth, "training": learned["training"],
+              "speedup_learned_vs_fp64": meth["learned"].get("vs_fp64"),
+              "speedup_learned_vs_half": round(meth["learned"]["solves_per_s"] / meth["half"]["solves_per_s"], 3),
+              "solution_agreement": learned["agreement"], "kernels": learned["kernels"],
+              "second_workload": second, "hmc": hmc, "overlap": overlap, "batched": batched}
+    roof = learned["roofline"]
     line = {
         "metric": "CG solves/s to 1e-10 true resid, n=128 Lt=64: learned+FP32 vs plain FP64 CG",
         "value": round(best["solves_per_s"], 4),
         "unit": "solves/s",
-        "n_gpus": 1, "steps": steps, "warmup": a.warmup,
-        "ms_per_step": round(ms, 3),
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(best["ms_per_solve"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": dtype,
+        "dtype": "f16 storage, f32 arithmetic, f64 reliable updates and true residual",
         "data": "synthetic: von Mises(kappa=%.2f) links, Gaussian RHS, seeded" % a.beta,
         "config": config,
-        "roofline": dict(roof, copy_kernel_GBs=round(copy_gbs, 1),
-                         simplicial={"kernel": ("k_stencil_lds" if tile32["lds"] else "k_stencil")
-                                     + "<float,6,false,EpiStore> (D, FP32, simplicial n=128 Lt=64)",
-                                     "achieved": round(gbs32, 1),
-                                     "frac": round(gbs32 / HBM_PEAK_GBS, 4),
-                                     "bytes_per_launch": b32, "ms_per_launch": round(ms32, 5),
-                                     "tile": tile32,
-                                     "fp64_D": {"achieved": round(gbs64, 1),
-                                                "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-                                                "bytes_per_launch": b64,
-                                                "ms_per_launch": round(ms64, 5)}}),
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -275,124 +197,210 @@ def timed_solves(q, a, name, run, rhs, warmup, steps):
                 max_true_resid=float(max(r.true_resid for r in reps)))
 
 
-def learned_leg(a, q, g, with_net=True):
-    """Dual n x Lt at m=0.02: train on dual train_n x train_Lt, then time FCG with the net,
-    mixed CG and FP64 CG on the same seeded RHS (with_net=False: the two CGs only), and the
-    roofline of this workload's FP32 stencil."""
-    from qed3_amd import train
-    m, lattice = 0.02, 1
-    if not with_net:
-        return plain_leg(a, q, g, m, lattice)
+
+def training_pairs(a, q, D, lattice, cfgs, rhs_seeds=(41,), ks=(5, 15, 40), nwhite=2):
+    """(residual, error) pairs from the solver's own solves, per gauge configuration: for each
+    RHS the converged solution x* and the iterates x_k of the same solver stopped after k
+    iterations give e_k = x* - x_k and the residual of the D system v_k = D e_k; Gaussian e
+    add all modes.  Returns [(spatial, temporal, d_v, d_e, npairs)] with the pairs of each
+    configuration in two device arrays."""
+    out = []
+    N = D.N
+    for cs, beta in cfgs:
+        sp, tp = q.random_gauge(lattice, a.n, a.Lt, seed=cs, kappa=beta)
+        D.update(sp, tp)
+        es = []
+        for rs in rhs_seeds:
+            d_b = q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=rs))
+            d_x = q.DeviceArray(N * 16)
+            D.solve_device(d_x, d_b, tol=1e-12, solver=q.CG_MIXED)
+            xs = d_x.to_numpy(np.complex128)
+            for k in ks:
+                D.solve_device(d_x, d_b, tol=1e-14, maxiter=k, solver=q.CG_HALF, raise_on_fail=False)
+                es.append(xs - d_x.to_numpy(np.complex128))
+        for j in range(nwhite):
+            es.append(q.random_source(lattice, a.n, a.Lt, seed=7000 + 10 * cs + j))
+        d_e = q.DeviceArray(len(es) * N * 16)
+        d_v = q.DeviceArray(len(es) * N * 16)
+        import ctypes as C
+        for i, e in enumerate(es):
+            e = np.ascontiguousarray(e / np.linalg.norm(e))
+            src = q.DeviceArray.from_numpy(e)
+            dst = q.DeviceArray(N * 16)
+            D.apply_device(q.OP_D, q.F64, dst, src)
+            q.check(q.load().qed3_memcpy(C.c_void_p(d_e.ptr.value + i * N * 16), src.ptr, N * 16, 2))
+            q.check(q.load().qed3_memcpy(C.c_void_p(d_v.ptr.value + i * N * 16), dst.ptr, N * 16, 2))
+        out.append((sp, tp, d_v, d_e, len(es)))
+    return out
+
+
+def learned_leg(a, q):
+    """The metric's workload: dual n x Lt at m, the four solvers on the same RHS."""
+    lattice, m = 1, a.m
     t0 = time.time()
-    gt = q.Geometry(a.train_n)
-    Dt = q.WilsonDirac(lattice, a.train_n, a.train_Lt, r=1.0, M5=m, at=0.5 * mean_ell(gt, lattice))
-    data = train.make_pairs(Dt, lattice, a.train_n, a.train_Lt, cfg_seeds=[11, 12, 13],
-                            betas=[1.0, 1.4, 2.0])
-    t_pairs = time.time() - t0
-    net_t = q.Net(Dt, layers=4, seed=5)
-    hist = train.train(net_t, Dt, data, epochs=a.train_epochs, batch=4, lr=3e-3)
-    t_train = time.time() - t0
-    print(f"# trained on dual n={a.train_n} Lt={a.train_Lt}: loss {hist[0]:.3f} -> {hist[-1]:.4f} "
-          f"in {t_train:.1f}s", file=sys.stderr, flush=True)
-    # on the training lattice itself: an unseen configuration and RHS
-    sp_s, tp_s = q.random_gauge(lattice, a.train_n, a.train_Lt, seed=99, kappa=a.beta)
-    Dt.update(sp_s, tp_s)
-    b_s = q.DeviceArray.from_numpy(q.random_source(lattice, a.train_n, a.train_Lt, seed=77))
-    x_s = q.DeviceArray(Dt.N * 16)
-    small_cg = Dt.solve_device(x_s, b_s, tol=a.tol, solver=q.CG_F64)
-    small_fcg, _, small_fb = net_t.fcg(x_s, b_s, tol=a.tol)
-    small = {"lattice": f"dual n={a.train_n} Lt={a.train_Lt} (the training lattice, unseen links and RHS)",
-             "plain_cg_iterations": small_cg.iterations, "learned_fcg_iterations": small_fcg.iterations,
-             "fallback": small_fb}
-    print(f"# on the training lattice: plain CG {small_cg.iterations} its, learned FCG "
-          f"{small_fcg.iterations} its", file=sys.stderr, flush=True)
-    at = 0.5 * mean_ell(g, lattice)
-    D = dual_context(a, q, g, m, lattice)
-    net = q.Net(D, layers=4, seed=5)
-    net.params = net_t.params
-    del net_t, Dt, data
-    nbytes = D.N * 16
-    d_x = q.DeviceArray(nbytes)
+    D, g, at, sp, tp = context(a, q, lattice, m, seed=4321)
+    K = q.Precond(D, layers=a.layers)
+    print(f"# setup {time.time() - t0:.1f}s V={D.V} E={D.E} Lt={a.Lt} at={at:.5f}", file=sys.stderr, flush=True)
+    N = D.N
+    d_x = q.DeviceArray(N * 16)
+    d_val = q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=3000))
+
+    def validate():
+        rep = K.solve_device(d_x, d_val, tol=a.tol, delta=a.delta)
+        return rep.iterations
+
+    training = None
+    if not a.no_train:
+        t1 = time.time()
+        cfgs = [(11, 1.0), (12, 1.4), (13, 2.0)]
+        data = training_pairs(a, q, D, lattice, cfgs)
+        t_pairs = time.time() - t1
+        p0 = K.params
+
+        def mean_loss():
+            tot, cnt = 0.0, 0
+            for sp_c, tp_c, d_v, d_e, npairs in data:
+                D.update(sp_c, tp_c)
+                l, _ = K.loss_grad_device(d_v, d_e, npairs)
+                tot += l * npairs
+                cnt += npairs
+            return tot / cnt
+
+        loss0 = mean_loss()
+        hist = []
+        for ep in range(a.train_epochs):
+            for sp_c, tp_c, d_v, d_e, npairs in data:
+                D.update(sp_c, tp_c)
+                hist.extend(K.train_device(d_v, d_e, npairs, batch=2, epochs=1, lr=0.01).tolist())
+        loss1 = mean_loss()
+        p1 = K.params
+        D.update(sp, tp)
+        its_trained = validate()
+        K.params = p0
+        its_neumann = validate()
+        keep_trained = its_trained <= its_neumann
+        K.params = p1 if keep_trained else p0
+        training = {"pairs": int(sum(d[4] for d in data)), "configs": [{"seed": c, "beta": b} for c, b in cfgs],
+                    "pair_source": "iterates of the half-storage CG stopped after 5, 15, 40 iterations against "
+                                   "its converged solution, plus Gaussian errors; v = D e",
+                    "lattice": f"dual n={a.n} Lt={a.Lt} m={m}", "epochs": a.train_epochs,
+                    "loss_before": loss0, "loss_after": loss1, "params_trained": [str(c) for c in np.round(p1, 4)],
+                    "validation_iterations": {"neumann": its_neumann, "trained": its_trained},
+                    "kept": "trained" if keep_trained else "neumann (training did not reduce iterations)",
+                    "pair_generation_s": round(t_pairs, 1), "train_s": round(time.time() - t1 - t_pairs, 1)}
+        print(f"# training: loss {loss0:.4f} -> {loss1:.4f}, validation iterations neumann {its_neumann} "
+              f"trained {its_trained}", file=sys.stderr, flush=True)
+        del data
     rhs = [q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=2000 + k))
-           for k in range(a.warmup + a.learned_steps)]
-    methods = {}
-    stats = []
-
-    def run_fcg(b):
-        rep, applies, fb = net.fcg(d_x, b, tol=a.tol)
-        stats.append((applies, fb))
-        return rep
-
-    methods["learned"] = timed_solves(q, a, "learned", run_fcg, rhs, a.warmup, a.learned_steps)
-    x_learned = d_x.to_numpy(np.complex128)
-    methods["learned"]["fallbacks"] = int(sum(fb for _, fb in stats[a.warmup:]))
-    methods["half"] = timed_solves(q, a, "half", lambda b: D.solve_device(d_x, b, tol=a.tol,
-                                   solver=q.CG_HALF), rhs, a.warmup, a.learned_steps)
-    x_half = d_x.to_numpy(np.complex128)
-    methods["mixed"] = timed_solves(q, a, "mixed", lambda b: D.solve_device(d_x, b, tol=a.tol,
-                                    solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)
-    x_mixed = d_x.to_numpy(np.complex128)
+           for k in range(a.warmup + a.steps)]
+    methods, sols = {}, {}
+    methods["learned"] = timed_solves(q, a, "learned", lambda b: K.solve_device(d_x, b, tol=a.tol, delta=a.delta),
+                                      rhs, a.warmup, a.steps)
+    sols["learned"] = d_x.to_numpy(np.complex128)
+    for name, solver in (("half", q.CG_HALF), ("mixed", q.CG_MIXED), ("fp64", q.CG_F64)):
+        if name == "fp64" and a.no_fp64:
+            continue
+        methods[name] = timed_solves(q, a, name, lambda b, s=solver: D.solve_device(d_x, b, tol=a.tol, solver=s),
+                                     rhs, a.warmup, a.steps)
+        sols[name] = d_x.to_numpy(np.complex128)
     agreement = None
-    if not a.no_fp64:
-        methods["fp64"] = timed_solves(q, a, "fp64", lambda b: D.solve_device(d_x, b, tol=a.tol,
-                                       solver=q.CG_F64), rhs, a.warmup, a.learned_steps)
-        x64 = d_x.to_numpy(np.complex128)
+    if "fp64" in sols:
+        x64 = sols["fp64"]
         nrm = np.linalg.norm(x64)
-        agreement = {"rhs_seed": 2000 + a.warmup + a.learned_steps - 1,
-                     "mixed_vs_fp64": float(np.linalg.norm(x_mixed - x64) / nrm),
-                     "half_vs_fp64": float(np.linalg.norm(x_half - x64) / nrm),
-                     "learned_vs_fp64": float(np.linalg.norm(x_learned - x64) / nrm)}
-    # one net application N (and N^dag) at full size, device-resident fields
-    d_y = q.DeviceArray(D.N * 8)
-    d_z = q.DeviceArray(D.N * 8)
-    for fn in (net.apply_device, net.apply_adj_device):
-        fn(d_z, d_y, 1)
-    q.load().qed3_device_synchronize()
-    reps = 10
-    t1 = time.time()
-    for _ in range(reps):
-        net.apply_device(d_z, d_y, 1)
-    q.load().qed3_device_synchronize()
-    ms_net = 1e3 * (time.time() - t1) / reps
-    t1 = time.time()
-    for _ in range(reps):
-        net.apply_adj_device(d_z, d_y, 1)
-    q.load().qed3_device_synchronize()
-    ms_adj = 1e3 * (time.time() - t1) / reps
-    pts = D.V * a.Lt
-    # algorithmic bytes of one N: input spinor read (8 B complex x 2) + 3 layers' BF16
-    # activations written and read (16 channels x 2 spins x 4 B) + output spinor written,
-    # plus the link phases once per layer
-    b_net = pts * (16 + 3 * 2 * 128 + 16) + 4 * (D.E + D.V) * a.Lt * 8
-    fp64 = methods.get("fp64")
-    return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG",
-            "agreement": agreement,
-            "roofline": stencil_roofline(a, q, D),
-            "cg_kernels": cg_kernel_rooflines(a, q, D, d_x, rhs[-1]),
-            "sites": D.V, "links": D.E, "methods": methods,
-            "training": {"lattice": f"dual n={a.train_n} Lt={a.train_Lt}", "pairs": 3 * 40,
-                         "epochs": a.train_epochs, "loss_first": round(hist[0], 4),
-                         "on_training_lattice": small,
-                         "loss_last": round(hist[-1], 5), "seconds": round(t_train, 1),
-                         "pair_generation_s": round(t_pairs, 1)},
-            "speedup_vs_fp64": (round(methods["learned"]["solves_per_s"] / fp64["solves_per_s"], 3)
-                                if fp64 else None),
-            "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
-                                      if fp64 else None),
-            "speedup_half_vs_fp64": (round(methods["half"]["solves_per_s"] / fp64["solves_per_s"], 3)
-                                     if fp64 else None),
-            "net_apply": {"ms": round(ms_net, 4), "ms_adjoint": round(ms_adj, 4),
-                          "algorithmic_GB": round(b_net / 1e9, 3),
-                          "achieved_GBs": round(b_net / (ms_net * 1e-3) / 1e9, 1)}}
+        agreement = {"rhs_seed": 2000 + a.warmup + a.steps - 1,
+                     **{f"{k}_vs_fp64": float(np.linalg.norm(v - x64) / nrm) for k, v in sols.items() if k != "fp64"}}
+    # kernels of one preconditioned solve (HIP events around every launch, direct launches)
+    rep, kms = K.solve_kernel_times(d_x, rhs[-1], tol=a.tol, delta=a.delta)
+    nslot = 3 if lattice == 1 else 6
+    b_first = layer_bytes(D.V, D.E, a.Lt, nslot, True)
+    b_layer = layer_bytes(D.V, D.E, a.Lt, nslot, False)
+    cgb = cg_kernel_bytes(D.V, D.E, a.Lt, "half")
+    tile = tile_info(q, D, q.OP_D, q.F32)
+
+    def ent(name, ms, nbytes, traffic=None):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"kernel": name, "ms_per_launch": round(ms, 5), "bytes_per_launch": int(nbytes),
+                "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+    ns = str(nslot)
+    lk = {"transport_layer_Kdag": ent(f"k_horner_layer<{ns},true,false,true,SpinH,SrcPar,EpiSkip,phasepair_t>",
+                                      kms[4], b_layer, MEASURED_TRAFFIC.get(("layer", lattice, a.n, a.Lt))),
+          "transport_layer_K": ent(f"k_horner_layer<{ns},false,false,true,SpinH,SrcF,EpiSkip|EpiDots,phasepair_t>",
+                                   kms[5], b_layer),
+          "first_layer": ent(f"k_horner_layer<{ns},*,true,true,SpinH,...> (recurrence only)", kms[3], b_first),
+          "K1": ent(f"k_stencil_lds<float,{ns},false,EpiK1PH,SrcPUpdateH,phase16_t> (p = z + beta p_old, w = D p)",
+                    kms[0], cgb["K1"]),
+          "K2": ent(f"k_stencil_lds<float,{ns},true,EpiK2F<0..2>,SrcHalf,phase16_t>", kms[1], cgb["K2"]),
+          "K2x": ent(f"k_stencil_lds<float,{ns},true,EpiK2F<3>,SrcHalf,phase16_t>", kms[2], cgb["K2x"]),
+          "iterations": rep.iterations, "layers": a.layers, "tile": tile}
+    # launches per iteration: 2 first layers, 2 (L - 1) transport layers, K1, K2
+    per_it = {"first_layer": 2 * kms[3], "transport_layers": (a.layers - 1) * (kms[4] + kms[5]),
+              "K1": kms[0], "K2": (3 * kms[1] + kms[2]) / 4}
+    lk["ms_per_iteration_by_kind"] = {k2: round(v, 4) for k2, v in per_it.items()}
+    dom = lk["transport_layer_Kdag"]
+    roof = {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"],
+            "traffic": dom["traffic"],
+            "kernel": dom["kernel"] + " (transport layer of K^dagger: pass 1 stages the layer input once per "
+                                      "slice and forms both spin rows' transports, pass 2 runs the backward "
+                                      "recurrence over each site's column)",
+            "bytes_per_launch": dom["bytes_per_launch"], "ms_per_launch": dom["ms_per_launch"],
+            "copy_kernel_GBs": round(copy_bandwidth(q)[0], 1),
+            "kernels": lk, "cg_kernels_unpreconditioned": cg_kernel_rooflines(a, q, D, d_x, rhs[-1]),
+            "bare_D": stencil_roofline(a, q, D)}
+    return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG, von Mises beta={a.beta} links",
+            "sites": D.V, "links": D.E, "methods": methods, "training": training, "agreement": agreement,
+            "kernels": lk, "roofline": roof}
 
 
-def dual_context(a, q, g, m, lattice):
-    at = 0.5 * mean_ell(g, lattice)
-    D = q.WilsonDirac(lattice, a.n, a.Lt, r=1.0, M5=m, at=at)
-    sp, tp = q.random_gauge(lattice, a.n, a.Lt, seed=4321, kappa=a.beta)
-    D.update(sp, tp)
-    D.autotune()
-    return D
+def second_leg(a, q):
+    """Simplicial n x Lt at m2: the four solvers (network at its Neumann
+    coefficients) on the same RHS."""
+    lattice = 0
+    D, g, at, sp, tp = context(a, q, lattice, a.m2, seed=1234)
+    K = q.Precond(D, layers=a.layers)
+    d_x = q.DeviceArray(D.N * 16)
+    rhs = [q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=1000 + k))
+           for k in range(a.warmup + a.steps)]
+    methods = {"learned": timed_solves(q, a, "simplicial learned",
+                                       lambda b: K.solve_device(d_x, b, tol=a.tol, delta=a.delta), rhs, a.warmup,
+                                       a.steps)}
+    for name, solver in (("half", q.CG_HALF), ("mixed", q.CG_MIXED), ("fp64", q.CG_F64)):
+        if name == "fp64" and a.no_fp64:
+            continue
+        methods[name] = timed_solves(q, a, "simplicial " + name,
+                                     lambda b, s=solver: D.solve_device(d_x, b, tol=a.tol, solver=s), rhs, a.warmup,
+                                     a.steps)
+    if "fp64" in methods:
+        for k in methods:
+            methods[k]["vs_fp64"] = round(methods[k]["solves_per_s"] / methods["fp64"]["solves_per_s"], 3)
+    return {"workload": f"simplicial n={a.n} Lt={a.Lt} m={a.m2} tol={a.tol} D^dag D CG", "methods": methods}
+
+
+def kernels_only(a, q):
+    """Stencils, one preconditioned solve and the copy kernel on the dual workload (for
+    profilers; direct launches when QED3_NO_GRAPH=1)."""
+    D, g, at, sp, tp = context(a, q, 1, a.m, seed=4321)
+    if a.tile:
+        b_, t_, r_, l_ = (int(v) for v in a.tile.split(","))
+        D.set_tile(b_, t_, r_, l_, ops=(q.OP_D, q.OP_DDAG), precs=(q.F32,))
+    for prec, nm in ((q.F16, "f16"), (q.F32, "f32"), (q.F64, "f64")):
+        for op in (q.OP_D, q.OP_DDAG):
+            ms = D.time_apply(op, prec, warmup=2, reps=10)
+            rb = {q.F16: 2, q.F32: 4, q.F64: 8}[prec]
+            gbs = algorithmic_bytes_D(D.V, D.E, a.Lt, rb) / (ms * 1e-3) / 1e9
+            print(f"# D op={op} {nm}: {ms:.4f} ms  {gbs:.0f} GB/s algorithmic  tile={tile_info(q, D, op, prec)}",
+                  file=sys.stderr, flush=True)
+    d_b = q.DeviceArray.from_numpy(q.random_source(1, a.n, a.Lt, seed=1000))
+    d_x = q.DeviceArray(D.N * 16)
+    rep, ms = D.solve_kernel_times(d_x, d_b, tol=a.tol, solver=q.CG_HALF)
+    print(f"# half CG: {rep.iterations} its, K1 {ms[0]:.4f} ms, K2 {ms[1]:.4f} ms, K2 with x update {ms[2]:.4f} ms",
+          file=sys.stderr, flush=True)
+    K = q.Precond(D, layers=a.layers)
+    for _ in range(2):
+        rep = K.solve_device(d_x, d_b, tol=a.tol, delta=a.delta)
+        print(f"# learned: {rep.iterations} its {1e3 * rep.seconds:.2f} ms", file=sys.stderr, flush=True)
+    gbs, ms = copy_bandwidth(q, 1 << 30, reps=10)
+    print(f"# copy 1 GiB: {ms:.4f} ms  {gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
 
 def cg_kernel_bytes(V, E, Lt, solver):
@@ -410,6 +418,17 @@ def cg_kernel_bytes(V, E, Lt, solver):
     s_r = 4 if solver == "mixed" else 8
     bv, bl = pts * 2 * 2 * s_r, (E + V) * Lt * s_r
     return {"K1": 4 * bv + bl, "K2": 6 * bv + bl, "K2x": 6 * bv + bl}
+
+
+CG_KERNEL_NAMES = {
+    "half": {"K1": "k_stencil_lds<float,NS,false,EpiK1PH,SrcPUpdateH,phase16_t>",
+             "K2": "k_stencil_lds<float,NS,true,EpiK2H<0..2>,SrcHalf,phase16_t>",
+             "K2x": "k_stencil_lds<float,NS,true,EpiK2H<3>,SrcHalf,phase16_t>"},
+    "mixed": {"K1": "k_stencil_lds<float,NS,false,EpiK1P<float>,SrcPUpdate<float>>",
+              "K2": "k_stencil_lds<float,NS,true,EpiK2<float>,SrcPlain<float>>"},
+    "fp64": {"K1": "k_stencil_lds<double,NS,false,EpiK1P<double>,SrcPUpdate<double>>",
+             "K2": "k_stencil_lds<double,NS,true,EpiK2<double>,SrcPlain<double>>"},
+}
 
 
 CG_KERNEL_NAMES = {
@@ -475,36 +494,6 @@ def stencil_roofline(a, q, D):
                    + f"<float,{6 if D.lattice == 0 else 3},false," + kern,
                    "bytes_per_launch": b, "ms_per_launch": round(ms, 5), "tile": tile}
     return out
-
-
-def plain_leg(a, q, g, m, lattice):
-    D = dual_context(a, q, g, m, lattice)
-    d_x = q.DeviceArray(D.N * 16)
-    rhs = [q.DeviceArray.from_numpy(q.random_source(lattice, a.n, a.Lt, seed=2000 + k))
-           for k in range(a.warmup + a.learned_steps)]
-    methods = {"half": timed_solves(q, a, "half", lambda b: D.solve_device(
-        d_x, b, tol=a.tol, solver=q.CG_HALF), rhs, a.warmup, a.learned_steps)}
-    x_half = d_x.to_numpy(np.complex128)
-    methods["mixed"] = timed_solves(q, a, "mixed", lambda b: D.solve_device(
-        d_x, b, tol=a.tol, solver=q.CG_MIXED), rhs, a.warmup, a.learned_steps)
-    x_mixed = d_x.to_numpy(np.complex128)
-    agreement = None
-    if not a.no_fp64:
-        methods["fp64"] = timed_solves(q, a, "fp64", lambda b: D.solve_device(
-            d_x, b, tol=a.tol, solver=q.CG_F64), rhs, a.warmup, a.learned_steps)
-        x64 = d_x.to_numpy(np.complex128)
-        agreement = {"rhs_seed": 2000 + a.warmup + a.learned_steps - 1,
-                     "mixed_vs_fp64": float(np.linalg.norm(x_mixed - x64) / np.linalg.norm(x64)),
-                     "half_vs_fp64": float(np.linalg.norm(x_half - x64) / np.linalg.norm(x64))}
-    fp64 = methods.get("fp64")
-    return {"workload": f"dual n={a.n} Lt={a.Lt} m={m} tol={a.tol} D^dag D CG", "agreement": agreement,
-            "roofline": stencil_roofline(a, q, D), "cg_kernels": cg_kernel_rooflines(a, q, D, d_x, rhs[-1]),
-            "sites": D.V, "links": D.E,
-            "methods": methods, "training": None, "speedup_vs_fp64": None,
-            "speedup_mixed_vs_fp64": (round(methods["mixed"]["solves_per_s"] / fp64["solves_per_s"], 3)
-                                      if fp64 else None),
-            "speedup_half_vs_fp64": (round(methods["half"]["solves_per_s"] / fp64["solves_per_s"], 3)
-                                     if fp64 else None), "net_apply": None}
 
 
 def overlap_leg(q, n=16, Lt=16, M5=-0.5, degree=21, tol=1e-10):
@@ -698,33 +687,58 @@ def batched_leg(a, q):
             "shapes": desc[:8]}
 
 
-def cpu_baseline(a, g, at, gpu_iters, sp_full, tp_full):
-    """Oracle CSR CG (the reference's CPU path restated) on a bounded sample: the same
-    lattice at Lt=cpu_lt, `cpu_iters` CG iterations, scaled to Lt and to the GPU FP64
-    iteration count (CG iteration cost is linear in the number of points)."""
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(a, q, gpu_iters):
+    """The reference's CPU path restated (the oracle's CSR CG: COO assembled as coo_format,
+    SpMV in the sparse_matrix.h row order, MatPoly::solve's recursion), built -O3
+    -march=native on this host and OpenMP-parallel over rows on all its cores, timed on a
+    bounded sample of the metric's workload: the same dual lattice and mass at Lt = cpu_lt,
+    cpu_iters CG iterations, scaled to the full Lt and to the GPU FP64 CG's iteration count
+    (a CG iteration's cost is linear in the number of points)."""
     try:
         import oracle as O
-        import qed3_amd as q
+        kind = "port"
+        try:
+            O.use_library(O.build_native())
+            build = "-O3 -march=native (built on this host)"
+        except Exception as e:  # noqa: BLE001
+            build = f"-O3 portable build (native build failed: {e})"
     except Exception as e:  # noqa: BLE001
         return {"error": str(e)}
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    lattice = 1
     t0 = time.time()
-    L = O.Lattice(g, a.lattice)
+    g = q.Geometry(a.n)
+    L = O.Lattice(g, lattice)
+    at = 0.5 * L.checks()["mean_ell"]
     Do = O.Dirac(L, a.cpu_lt, r=1.0, M5=a.m, at=at)
-    sp, tp = sp_full[: a.cpu_lt].copy(), tp_full[: a.cpu_lt].copy()
+    sp, tp = q.random_gauge(lattice, a.n, a.cpu_lt, seed=4321, kappa=a.beta)
     Do.update(sp, tp)
-    b = q.random_source(a.lattice, a.n, a.cpu_lt, seed=77)
+    b = q.random_source(lattice, a.n, a.cpu_lt, seed=77)
     setup = time.time() - t0
     t1 = time.time()
     _, it, _ = Do.solve(b, tol=1e-30, maxiter=a.cpu_iters)
     dt = time.time() - t1
     per_iter = dt / max(a.cpu_iters, 1) * (a.Lt / a.cpu_lt)
     sec_per_solve = per_iter * gpu_iters
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(1.0 / sec_per_solve, 6), "unit": "solves/s", "cores": cores,
-            "kind": "port",
-            "sample": f"oracle CSR CG (matpoly.h:204-267 restated), n={a.n} Lt={a.cpu_lt}, "
-                      f"{a.cpu_iters} iterations in {dt:.1f}s (setup {setup:.1f}s), scaled x"
-                      f"{a.Lt // a.cpu_lt} in volume and x{gpu_iters:.0f} GPU FP64 iterations",
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return {"value": round(1.0 / sec_per_solve, 6), "unit": "solves/s", "cores": threads, "kind": kind,
+            "cpu_model": cpu_model(), "nproc": cores, "build": build,
+            "sample": f"oracle CSR CG (matpoly.h:204-267 restated) on dual n={a.n} Lt={a.cpu_lt} m={a.m}, "
+                      f"{a.cpu_iters} iterations in {dt:.1f}s (setup {setup:.1f}s), scaled x{a.Lt // a.cpu_lt} "
+                      f"in volume and x{gpu_iters:.0f} GPU FP64 CG iterations of the full workload",
             "s_per_iteration_full_volume": round(per_iter, 4)}
 
 

@@ -22,6 +22,21 @@ def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
+def use_library(path):
+    """Load the oracle from another build of oracle.cpp (e.g. the -march=native build the
+    CPU baseline times); must be called before the first load()."""
+    global LIB
+    if _lib is not None:
+        raise RuntimeError("oracle library already loaded")
+    LIB = pathlib.Path(path)
+
+
+def build_native(out="/tmp/qed3_oracle_native/liboracle.so"):
+    """-O3 -march=native build of oracle.cpp for the host it runs on; returns its path."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "native", f"NATIVE_OUT={out}"], check=True)
+    return out
+
+
 def load():
     global _lib
     if _lib is None:
