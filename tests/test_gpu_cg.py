@@ -48,6 +48,35 @@ def test_cg_matches_oracle(gpu, lattice, n, Lt, M5):
 
 
 @pytest.mark.gpu
+def test_config1_2d_simplicial_uniform_links(gpu):
+    """the 2D operator (Lt = 1, so kappa_t = 0, dirac_ext.h:307-320) on
+    the simplicial n=8 refinement (642 sites, the reference's largest N_REFINE,
+    src/both/prop.cu:22), r=1, m=0.1, link angles uniform in [-pi, pi), Gaussian RHS, FP64 CG
+    to 1e-10: the solution equals the oracle's MatPoly::solve and a dense solve."""
+    lattice, n, Lt, M5 = 0, 8, 1, 0.1
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    assert L.V == 642
+    sp, tp = links(lattice, n, Lt, seed=1, kappa=0.0)  # kappa <= 0: uniform angles
+    assert sp.min() >= -np.pi and sp.max() < np.pi and np.std(sp) > 1.5
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=M5, at=at)
+    Dq.update(sp, tp)
+    b = source(lattice, n, Lt, seed=3)
+    x_ref, it_ref, _ = Do.solve(b, tol=1e-13)
+    A = Do.dense()
+    x_dense = np.linalg.solve(A.conj().T @ A, b)
+    assert rel(x_ref, x_dense) < 1e-10
+    for solver in (q.CG_F64, q.CG_MIXED, q.CG_HALF):
+        x, rep = Dq.solve(b, tol=1e-10, solver=solver)
+        assert rep.status == 0 and rep.true_resid <= 1e-10
+        assert rel(x, x_ref) < 1e-8, (solver, rel(x, x_ref))
+        r = b - Do.apply(2, x)
+        assert abs(np.linalg.norm(r) / np.linalg.norm(b) - rep.true_resid) < 1e-13
+    x, rep = Dq.solve(b, tol=1e-12, solver=q.CG_F64)
+    assert rel(x, x_ref) < 1e-10
+
+
+@pytest.mark.gpu
 def test_cg_iterations_close_to_oracle(gpu):
     """FP64 CG runs the same Krylov recursion: iteration counts agree to a few."""
     Do, Dq = _pair(0, 4, 4, 0.1)

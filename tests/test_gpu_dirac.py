@@ -98,15 +98,55 @@ def test_every_launch_shape_matches_oracle(gpu, lattice, tile):
 
 
 @pytest.mark.gpu
-def test_periodic_bc_matches_oracle(gpu):
-    lattice, n, Lt = 0, 2, 4
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_periodic_bc_matches_oracle(gpu, lattice):
+    """Periodic time boundary at every precision, including the binary16 operator and the
+    half-storage CG (whose temporal-gauge wrap phase carries the boundary sign)."""
+    n, Lt = 2, 4
     L, Do, at = oracle_setup(lattice, n, Lt, M5=0.2, bc=1)
     sp, tp = links(lattice, n, Lt, seed=2)
     Do.update(sp, tp)
     Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.2, at=at, bc=q.PERIODIC)
     Dq.update(sp, tp)
     v = source(lattice, n, Lt, seed=4)
-    assert rel(Dq.mult(v), Do.apply(0, v)) < 1e-14
+    for op in (q.OP_D, q.OP_DDAG, q.OP_DDAGD, q.OP_DDDAG):
+        ref = Do.apply(op, v) if op != q.OP_DDDAG else Do.apply(0, Do.apply(1, v))
+        assert rel(Dq.apply(op, v, q.F64), ref) < 1e-14, op
+        assert rel(Dq.apply(op, v.astype(np.complex64), q.F32), ref) < 2e-6, op
+    v16 = round16(v)
+    for op in (q.OP_D, q.OP_DDAG):
+        assert rel(Dq.apply(op, v16, q.F16), Do.apply(op, v16)) < 1e-3, op
+    b = source(lattice, n, Lt, seed=8)
+    x_ref, _, _ = Do.solve(b, tol=1e-13)
+    for solver in (q.CG_F64, q.CG_MIXED, q.CG_HALF):
+        x, rep = Dq.solve(b, tol=1e-11, solver=solver)
+        assert rep.status == 0 and rep.true_resid <= 1e-11
+        assert rel(x, x_ref) < 1e-10, solver
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice,n,Lt", [(0, 2, 3), (1, 4, 4), (0, 4, 1)])
+def test_dddag_and_dot_match_oracle(gpu, lattice, n, Lt):
+    """D D^dag (DiracPf::DDH, dirac_pf.h:85-92) at every precision and <x, y> (the role of
+    MatPoly::dot, matpoly.h:129-144) against the oracle / numpy."""
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.1)
+    sp, tp = links(lattice, n, Lt, seed=13)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.1, at=at)
+    Dq.update(sp, tp)
+    v = source(lattice, n, Lt, seed=21)
+    ref = Do.apply(0, Do.apply(1, v))
+    assert rel(Dq.apply(q.OP_DDDAG, v, q.F64), ref) < 1e-14
+    assert rel(Dq.apply(q.OP_DDDAG, v.astype(np.complex64), q.F32), ref) < 4e-6
+    v16 = round16(v)
+    assert rel(Dq.apply(q.OP_DDDAG, v16, q.F16), Do.apply(0, Do.apply(1, v16))) < 3e-3
+    x, y = source(lattice, n, Lt, seed=31), source(lattice, n, Lt, seed=32)
+    d_x, d_y = q.DeviceArray.from_numpy(x), q.DeviceArray.from_numpy(y)
+    got = Dq.dot(d_x, d_y)
+    want = np.vdot(x, y)  # sum conj(x) y
+    assert abs(got - want) < 1e-13 * np.linalg.norm(x) * np.linalg.norm(y)
+    assert abs(got.imag - want.imag) < 1e-12 * abs(want) + 1e-12 and abs(want.imag) > 1e-3
+    assert abs(Dq.dot(d_x, d_x) - np.linalg.norm(x) ** 2) < 1e-12 * np.linalg.norm(x) ** 2
 
 
 @pytest.mark.gpu

@@ -161,6 +161,10 @@ def test_training_and_fcg_gain(gpu):
           f"(fallback={fb}, resid={rep.true_resid:.2e})", flush=True)
     assert rep.status == 0 and rep.true_resid <= 1e-10
     assert rep.iterations < rep_cg.iterations
+    # the solution is the oracle's (MatPoly::solve restated, on the same links)
+    Do.update(sp, tp)
+    x_ref, _, _ = Do.solve(b, tol=1e-13)
+    assert rel(d_x.to_numpy(np.complex128), x_ref) < 1e-8
     # the untrained net of the same shape: the gain comes from training
     net0 = q.Net(Dq, layers=4, seed=5)
     rep0, _, fb0 = net0.fcg(d_x, d_b, tol=1e-10, raise_on_fail=False)
