@@ -160,6 +160,17 @@ int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
  * each kernel's reduction finaliser excluded. */
 int qed3_cg_kernel_times(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
                          qed3_cg_report* rep, double* ms);
+/* The FP64 solve of qed3_cg (opts->solver must be QED3_SOLVER_CG_F64) that allocates
+ * nothing: no counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its work fields live in one array of 64 MiB that the library holds on every
+ * GPU it runs on until the process ends, which limits the lattice to V*Lt <= 349,520 points
+ * (six FP64 fields, 192 bytes of work per point); larger lattices return QED3_ERR_ARG with the limit in
+ * qed3_last_error().  Calls share that GPU's array: calls that overlap on one GPU (other
+ * host threads, other contexts) race and must be serialised by the caller.  Same arguments,
+ * results and report as qed3_cg; the search runs with direct kernel launches. */
+int qed3_cg_noalloc(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
+                    qed3_cg_report* rep);
 
 /* ---- learned preconditioner --------------------------------------------------------
  * No counterpart in the reference (it has no ML part).  A gauge-covariant linear

@@ -70,6 +70,8 @@ SIGNATURES = {
                           C.POINTER(CGReport)]),
     "qed3_cg_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
                                        C.POINTER(CGReport), C.POINTER(C.c_double)]),
+    "qed3_cg_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
+                                  C.POINTER(CGReport)]),
     "qed3_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
     "qed3_batch_cg": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p),
                                 C.POINTER(C.c_void_p), C.POINTER(CGOpts), C.POINTER(CGReport)]),

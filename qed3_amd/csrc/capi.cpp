@@ -430,6 +430,30 @@ int qed3_cg(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
   return cg_entry(ctx, d_x, d_b, opts, rep, nullptr);
 }
 
+int qed3_cg_noalloc(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts, qed3_cg_report* rep) {
+  if (!ctx || !d_x || !d_b || !opts) return fail(QED3_ERR_ARG, "null argument");
+  if (d_x == d_b) return fail(QED3_ERR_ARG, "x and b must differ");
+  if (!(opts->tol > 0) || opts->maxiter < 1) return fail(QED3_ERR_ARG, "bad tolerance/maxiter");
+  if (opts->solver != QED3_SOLVER_CG_F64) return fail(QED3_ERR_ARG, "qed3_cg_noalloc runs the FP64 CG only");
+  if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
+  return guarded([&]() -> int {
+    qed3::CGResult r;
+    std::string err;
+    const int rc = qed3::cg_noalloc(*ctx->D, d_x, d_b, opts->tol, opts->maxiter, &r, &err);
+    if (rc != QED3_OK) return fail(rc, err);
+    if (rep) {
+      rep->status = r.status == 0 ? QED3_OK : (r.status == 4 ? QED3_ERR_NAN : QED3_ERR_NOCONV);
+      rep->iterations = r.iterations;
+      rep->reliable_updates = r.reliable_updates;
+      rep->true_resid = r.true_resid;
+      rep->seconds = r.seconds;
+    }
+    if (r.status == 4) return fail(QED3_ERR_NAN, "CG breakdown (non-finite or zero curvature)");
+    if (r.status != 0) return fail(QED3_ERR_NOCONV, "CG did not reach the tolerance");
+    return QED3_OK;
+  });
+}
+
 int qed3_cg_kernel_times(qed3_ctx* ctx, void* d_x, const void* d_b, const qed3_cg_opts* opts,
                          qed3_cg_report* rep, double* ms) {
   if (!ms) return fail(QED3_ERR_ARG, "null argument");

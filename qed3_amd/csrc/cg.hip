@@ -363,6 +363,85 @@ double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b) {
   return h.y > 0 ? std::sqrt(h.x / h.y) : 0.0;
 }
 
+size_t cg_workspace_bytes(size_t points) {
+  // r, p, p2, w, and two fields for a recomputed true residual (FP64), then the scalars and
+  // the residual norms, each region 256-byte aligned
+  const size_t f = (points * sizeof(Spin<double>) + 255) / 256 * 256;
+  return 6 * f + 512 + 256;
+}
+
+CGResult cg_solve_f64_workspace(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o, void* ws,
+                                size_t ws_bytes) {
+  const size_t n = D.points();
+  if (cg_workspace_bytes(n) > ws_bytes) throw std::invalid_argument("work memory too small for this lattice");
+  hipStream_t st = D.stream();
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t f = (n * sizeof(Spin<double>) + 255) / 256 * 256;
+  unsigned char* base = static_cast<unsigned char*>(ws);
+  Spin<double>* r = reinterpret_cast<Spin<double>*>(base);
+  Spin<double>* P[2] = {reinterpret_cast<Spin<double>*>(base + f), reinterpret_cast<Spin<double>*>(base + 2 * f)};
+  Spin<double>* w = reinterpret_cast<Spin<double>*>(base + 3 * f);
+  Spin<double>* t1 = reinterpret_cast<Spin<double>*>(base + 4 * f);
+  Spin<double>* t2 = reinterpret_cast<Spin<double>*>(base + 5 * f);
+  CGScalars* sc = reinterpret_cast<CGScalars*>(base + 6 * f);
+  double2* out = reinterpret_cast<double2*>(base + 6 * f + 512);
+  static_assert(sizeof(CGScalars) <= 512, "scalars region");
+  CGScalars hsc{};
+
+  // launch shapes: the context's, without LDS staging where its halo tables are not built
+  StencilTile tf = D.tile<double>(false), tt = D.tile<double>(true);
+  if (tf.lds && !D.has_halo(tf.block)) tf.lds = 0;
+  if (tt.lds && !D.has_halo(tt.block)) tt.lds = 0;
+  const HaloView hvf = tf.lds ? D.halo(tf.block) : HaloView{}, hvt = tt.lds ? D.halo(tt.block) : HaloView{};
+  const GridReduce gr = D.reducer();
+  const int nbk = int(stream_blocks(n));
+  hipLaunchKernelGGL(k_init<double>, dim3(nbk), dim3(256), 0, st, b, x, x, r, P[0], n, gr);
+  hipLaunchKernelGGL(k_finalize<FinStart>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
+                     FinStart{sc, o.tol * o.tol, 0.0, o.maxiter});
+  QED3_HIP(hipGetLastError());
+  const DiracView<double> v64 = D.view<double>();
+  const int nn = D.tables().max_nn;
+  // the iteration of run_cg<double>
+  auto one_iter = [&](int i, int) {
+    const int par = i & 1;
+    EpiK1P<double> k1;
+    k1.w = w;
+    k1.sc = sc;
+    k1.pnew = P[par ^ 1];
+    launch_stencil_src<double, false>(v64, nn, SrcPUpdate<double>{r, P[par], sc}, k1, tf, gr, st, hvf);
+    launch_stencil_src<double, true>(v64, nn, SrcPlain<double>{w}, EpiK2<double>{x, r, P[par ^ 1], sc, 0.0}, tt, gr,
+                                     st, hvt);
+    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{w, sc}, tf, gr, st, hvf);
+    launch_stencil<double, true>(v64, nn, w, EpiResid<double>{b, r, sc}, tt, gr, st, hvt);
+  };
+  CGOptions od = o;
+  od.use_graph = false;
+  od.times = nullptr;
+  drive_chunks(one_iter, [] {}, od, n, st, sc, &hsc);
+  QED3_HIP(hipMemcpyAsync(&hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  QED3_HIP(hipStreamSynchronize(st));
+  CGResult res;
+  res.iterations = hsc.iter;
+  res.reliable_updates = hsc.n_ru;
+  res.status = hsc.done == 1 ? 0 : (hsc.done == 2 ? 4 : 3);
+  if (hsc.done == 1 && hsc.n_ru > 0 && hsc.bnorm2 > 0) {
+    res.true_resid = std::sqrt(hsc.true_r2 / hsc.bnorm2);
+  } else {  // recompute |b - D^dag D x| / |b| in the work memory
+    launch_stencil<double, false>(v64, nn, x, EpiStore<double>{t1}, tf, gr, st, hvf);
+    launch_stencil<double, true>(v64, nn, t1, EpiStore<double>{t2}, tt, gr, st, hvt);
+    hipLaunchKernelGGL(k_resid_norms, dim3(nbk), dim3(256), 0, st, b, t2, n, gr);
+    hipLaunchKernelGGL(k_finalize<FinOut>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk), FinOut{out});
+    QED3_HIP(hipGetLastError());
+    double2 h;
+    QED3_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, st));
+    QED3_HIP(hipStreamSynchronize(st));
+    res.true_resid = h.y > 0 ? std::sqrt(h.x / h.y) : 0.0;
+  }
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
+  return res;
+}
+
 namespace {
 // Pseudo-random source for the tuning solves (a hash of the component index mapped to
 // [-1, 1)), so their residuals fall as a Gaussian source's do.

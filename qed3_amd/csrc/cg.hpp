@@ -21,6 +21,7 @@
 // component at the start and after every reliable update (one extra pass that only runs
 // then): half the field bytes of the FP32 iteration.
 #pragma once
+#include <string>
 #include "dirac.hpp"
 
 namespace qed3 {
@@ -62,6 +63,17 @@ CGResult cg_solve(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOpti
 
 // ||b - D^dag D x|| / ||b|| in FP64
 double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b);
+
+// The FP64 CG of cg_solve on work memory the caller provides (ws_bytes at ws, at least
+// cg_workspace_bytes(points)), allocating nothing: direct launches instead of a captured
+// graph, scalars read back into host stack memory, and the LDS stencil only where the
+// context already holds its halo tables (the plain stencil otherwise).
+size_t cg_workspace_bytes(size_t points);
+CGResult cg_solve_f64_workspace(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o, void* ws,
+                                size_t ws_bytes);
+// That solve in the library's own per-GPU work array (noalloc.hip); returns a QED3_* status
+// (argument or lookup errors, with their text in err), the solve's outcome in res.
+int cg_noalloc(Dirac& D, void* d_x, const void* d_b, double tol, int maxiter, CGResult* res, std::string* err);
 
 // Refine the FP32 / binary16 launch shape (D and D^dag together) by timing a fixed number
 // of half-storage CG iterations per candidate: the CG kernels carry epilogues the plain

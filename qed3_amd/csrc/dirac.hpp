@@ -158,6 +158,7 @@ class Dirac {
   CGScalars* host_scalars();
   // LDS halo tables for ranges of R sites (built on first use)
   HaloView halo(int R);
+  bool has_halo(int R) const { return halos_.count(R) != 0; }
   template <typename T>
   HaloView halo_for(bool dag) {
     const StencilTile t = tile<T>(dag);
