@@ -68,6 +68,10 @@ def parse():
     p.add_argument("--batch-nsys", type=int, default=64)
     p.add_argument("--tile", type=str, default=None,
                    help="block,ts,remap,lds for the FP32 D stencil instead of the autotuned one")
+    p.add_argument("--relabel", type=int, default=None,
+                   help="kernels-only: build the context on the tables relabelled at random "
+                        "with this seed (sites, triangles, links, orientations), as a caller's "
+                        "own numbering would arrive")
     p.add_argument("--kernels-only", action="store_true",
                    help="only launch the D stencils, one preconditioned solve and the calibration copy "
                         "(for profilers)")
@@ -380,6 +384,16 @@ def kernels_only(a, q):
     """Stencils, one preconditioned solve and the copy kernel on the dual workload (for
     profilers; direct launches when QED3_NO_GRAPH=1)."""
     D, g, at, sp, tp = context(a, q, 1, a.m, seed=4321)
+    if a.relabel is not None:
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+        from renumber import renumber
+        h, relabel = renumber(g, seed=a.relabel)
+        P = relabel(1, a.Lt)
+        del D
+        D = q.WilsonDirac(1, a.n, a.Lt, r=1.0, M5=a.m, at=at, geometry=h)
+        D.update(P.spatial(sp), P.temporal(tp))
+        D.autotune()
+        print(f"# relabelled tables (seed {a.relabel})", file=sys.stderr, flush=True)
     if a.tile:
         b_, t_, r_, l_ = (int(v) for v in a.tile.split(","))
         D.set_tile(b_, t_, r_, l_, ops=(q.OP_D, q.OP_DDAG), precs=(q.F32,))

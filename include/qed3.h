@@ -127,8 +127,20 @@ int qed3_set_gauge(qed3_ctx* ctx, const double* spatial, const double* temporal,
 
 /* out = op(in) (replaces DiracPf::{mult,adj,DHD,DDH}_deviceAsyncLaunch,
  * src/both_3d/dirac_pf.h:66-92).  prec selects complex double, complex float or binary16
- * fields (QED3_PREC_*). */
+ * fields (QED3_PREC_*).  Returns when out is written. */
 int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in);
+/* The same, queued on the context's stream without waiting (the role of the stream argument
+ * of LinOp::Async / *_deviceAsyncLaunch, src/both_3d/sparse_matrix.h:4-9,
+ * dirac_pf.h:66-92): out is written when the work queued before it on that stream and the
+ * apply itself have run.  The first call on a lattice may still wait (it builds tables). */
+int qed3_apply_async(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in);
+/* Every later call on the context queues its device work on `stream` (a hipStream_t; NULL
+ * selects the context's own stream again), ordered after what the caller queued there
+ * before; the call first waits for the work already queued on the previous stream.  The
+ * blocking entry points still return only when their results are complete.  The stream
+ * must not be the null stream (the solvers capture hipGraphs on it) and must outlive its
+ * use by the context. */
+int qed3_ctx_set_stream(qed3_ctx* ctx, struct ihipStream_t* stream);
 
 /* ---- CG on D^dagger D (replaces MatPoly::solve, src/both_3d/matpoly.h:204-267) --- */
 typedef struct qed3_cg_opts {

@@ -144,6 +144,9 @@ class Dirac:
         load().oracle_dirac_update(self.h, _dp(self._sp), _dp(self._tp))
 
     def apply(self, op, v):
+        """op 0 = D, 1 = D^dagger, 2 = D^dagger D, 3 = D D^dagger."""
+        if op not in (0, 1, 2, 3):
+            raise ValueError(f"unknown operator {op}")
         v = np.ascontiguousarray(v, np.complex128)
         out = np.zeros_like(v)
         load().oracle_apply(self.h, op, _dp(out.view(np.float64)), _dp(v.view(np.float64)))

@@ -1274,10 +1274,15 @@ void oracle_apply(void* h, int op, double* out, const double* in) {
   const auto* i = reinterpret_cast<const Complex*>(in);
   if (op == 0) D.matmul(o, i, false);
   if (op == 1) D.matmul(o, i, true);
-  if (op == 2) {
+  if (op == 2) {  // D^dag D: DiracPf::DHD_deviceAsyncLaunch (dirac_pf.h:66-71)
     std::vector<Complex> t(D.N);
     D.matmul(t.data(), i, false);
     D.matmul(o, t.data(), true);
+  }
+  if (op == 3) {  // D D^dag: DiracPf::DDH_deviceAsyncLaunch (dirac_pf.h:73-78)
+    std::vector<Complex> t(D.N);
+    D.matmul(t.data(), i, true);
+    D.matmul(o, t.data(), false);
   }
 }
 int oracle_cg(void* h, double* x, const double* b, double tol, int maxiter, double* mu) {

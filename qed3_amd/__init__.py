@@ -197,6 +197,17 @@ class WilsonDirac:
     def apply_device(self, op, prec, d_out, d_in):
         check(load().qed3_apply(self._h, op, prec, d_out.ptr, d_in.ptr))
 
+    def apply_async(self, op, prec, out_ptr, in_ptr):
+        """op(in) queued on the context's stream without waiting; raw device addresses
+        (ints, e.g. torch's data_ptr()) or DeviceArrays."""
+        ptr = lambda a: a.ptr if isinstance(a, DeviceArray) else C.c_void_p(int(a))
+        check(load().qed3_apply_async(self._h, op, prec, ptr(out_ptr), ptr(in_ptr)))
+
+    def set_stream(self, stream):
+        """Queue this context's work on `stream` (a hipStream_t as an int, e.g. torch's
+        Stream.cuda_stream; None or 0 selects the context's own stream)."""
+        check(load().qed3_ctx_set_stream(self._h, C.c_void_p(int(stream or 0))))
+
     def apply(self, op, v: np.ndarray, prec=F64):
         """op(v) for a host complex vector (round trip through the device; F16 packs the
         vector into binary16 (re, im) pairs and returns complex128)."""

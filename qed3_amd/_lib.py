@@ -66,6 +66,8 @@ SIGNATURES = {
                                     c_double_p]),
     "qed3_set_gauge": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "qed3_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "qed3_apply_async": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "qed3_ctx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "qed3_cg": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
                           C.POINTER(CGReport)]),
     "qed3_cg_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),

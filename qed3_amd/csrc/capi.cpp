@@ -245,6 +245,13 @@ int qed3_geom_import(int n_refine, int V, int E, int F, const double* sites, con
       G.dual_alpha_bwd.push_back(dual_alpha[2 * l + 1]);
       G.dual_omega.push_back(dual_omega[l]);
     }
+    for (int x = 0; x < V; x++)
+      if (nn_offsets[x + 1] < nn_offsets[x] || dual_face_offsets[x + 1] < dual_face_offsets[x])
+        throw std::invalid_argument("offsets must not decrease");
+    if (nn_offsets[0] != 0 || nn_offsets[V] != 2 * E)
+      throw std::invalid_argument("nn_offsets must run from 0 to 2E");
+    if (dual_face_offsets[0] != 0 || dual_face_offsets[V] != 3 * F)
+      throw std::invalid_argument("dual_face_offsets must run from 0 to 3F");
     G.nns.resize(V);
     G.dual_faces.resize(V);
     for (int x = 0; x < V; x++) {
@@ -343,7 +350,12 @@ int qed3_set_gauge(qed3_ctx* ctx, const double* spatial, const double* temporal,
   });
 }
 
-int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
+namespace {
+// qed3_apply and qed3_apply_async; the two-pass products keep their intermediate field in
+// the context's scratch slot kApplySlot, so the asynchronous form needs no allocation that
+// would outlive the call.
+constexpr int kApplySlot = 11;
+int apply_entry(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in, bool sync) {
   if (!ctx || !d_out || !d_in) return fail(QED3_ERR_ARG, "null argument");
   if (op < 0 || op > 3) return fail(QED3_ERR_ARG, "bad op");
   if (prec != QED3_PREC_F64 && prec != QED3_PREC_F32 && prec != QED3_PREC_F16)
@@ -357,35 +369,48 @@ int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
       auto* o = static_cast<qed3::Spin<T>*>(d_out);
       auto* i = static_cast<const qed3::Spin<T>*>(d_in);
       if (op == QED3_OP_D || op == QED3_OP_DDAG) {
-        D.apply<T>(op == QED3_OP_DDAG, o, i, true);
+        D.apply<T>(op == QED3_OP_DDAG, o, i, sync);
         return;
       }
-      qed3::Spin<T>* tmp = nullptr;
-      QED3_HIP(hipMalloc(&tmp, D.points() * sizeof(qed3::Spin<T>)));
+      auto* tmp = static_cast<qed3::Spin<T>*>(D.scratch(kApplySlot, D.points() * sizeof(qed3::Spin<T>)));
       if (op == QED3_OP_DDAGD)
-        D.apply_dagd<T>(o, i, tmp, true);
+        D.apply_dagd<T>(o, i, tmp, sync);
       else
-        D.apply_ddag<T>(o, i, tmp, true);
-      QED3_HIP(hipFree(tmp));
+        D.apply_ddag<T>(o, i, tmp, sync);
     };
     if (prec == QED3_PREC_F16) {
       auto* o = static_cast<qed3::SpinH*>(d_out);
       auto* i = static_cast<const qed3::SpinH*>(d_in);
       if (op == QED3_OP_D || op == QED3_OP_DDAG) {
-        D.apply_half(op == QED3_OP_DDAG, o, i, true);
+        D.apply_half(op == QED3_OP_DDAG, o, i, sync);
         return QED3_OK;
       }
-      qed3::SpinH* tmp = nullptr;
-      QED3_HIP(hipMalloc(&tmp, D.points() * sizeof(qed3::SpinH)));
+      auto* tmp = static_cast<qed3::SpinH*>(D.scratch(kApplySlot, D.points() * sizeof(qed3::SpinH)));
       D.apply_half(op == QED3_OP_DDAGD ? false : true, tmp, i, false);
-      D.apply_half(op == QED3_OP_DDAGD ? true : false, o, tmp, true);
-      QED3_HIP(hipFree(tmp));
+      D.apply_half(op == QED3_OP_DDAGD ? true : false, o, tmp, sync);
       return QED3_OK;
     }
     if (prec == QED3_PREC_F64)
       go(double{});
     else
       go(float{});
+    return QED3_OK;
+  });
+}
+}  // namespace
+
+int qed3_apply(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
+  return apply_entry(ctx, op, prec, d_out, d_in, true);
+}
+
+int qed3_apply_async(qed3_ctx* ctx, int op, int prec, void* d_out, const void* d_in) {
+  return apply_entry(ctx, op, prec, d_out, d_in, false);
+}
+
+int qed3_ctx_set_stream(qed3_ctx* ctx, struct ihipStream_t* stream) {
+  if (!ctx) return fail(QED3_ERR_ARG, "null context");
+  return guarded([&]() -> int {
+    ctx->D->set_stream(stream);
     return QED3_OK;
   });
 }

@@ -238,7 +238,8 @@ double time_copy(size_t bytes, int reps) {
 
 Dirac::Dirac(const Geometry& g, int lattice, int Lt, int bc, double r, double M5, double at)
     : h_(make_hop_tables(g, lattice, Lt, r, M5, at)), bc_(bc) {
-  QED3_HIP(hipStreamCreate(&stream_));
+  QED3_HIP(hipStreamCreate(&own_stream_));
+  stream_ = own_stream_;
   upload_tables(d64_);
   upload_tables(d32_);
   // link ends for the force kernel
@@ -287,6 +288,13 @@ void Dirac::upload_tables(DeviceTables<T>& d) {
   d.uT = dalloc<cplx_t<T>>(size_t(h_.Lt) * h_.V);
 }
 
+void Dirac::set_stream(hipStream_t s) {
+  if (!s) s = own_stream_;
+  if (s == stream_) return;
+  QED3_HIP(hipStreamSynchronize(stream_));
+  stream_ = s;
+}
+
 void* Dirac::scratch(int slot, size_t bytes) {
   if (slot < 0 || slot >= kScratchSlots) throw std::logic_error("bad scratch slot");
   if (scratch_bytes_[slot] < bytes) {
@@ -315,6 +323,7 @@ CGScalars* Dirac::host_scalars() {
 }
 
 Dirac::~Dirac() {
+  if (stream_) (void)hipStreamSynchronize(stream_);  // work queued on a caller's stream
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -334,7 +343,7 @@ Dirac::~Dirac() {
     f(kv.second.off);
     f(kv.second.sites);
   }
-  if (stream_) (void)hipStreamDestroy(stream_);
+  if (own_stream_) (void)hipStreamDestroy(own_stream_);
 }
 
 HaloView Dirac::halo(int R) {

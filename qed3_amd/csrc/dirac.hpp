@@ -99,6 +99,10 @@ class Dirac {
   int Lt() const { return h_.Lt; }
   size_t points() const { return size_t(h_.V) * h_.Lt; }
   hipStream_t stream() const { return stream_; }
+  // Queue this operator's work on `s` from now on (nullptr: its own stream again), after
+  // the work already queued on the previous stream has finished.  `s` must not be the null
+  // stream (the solvers capture graphs on it) and must outlive its use here.
+  void set_stream(hipStream_t s);
   bool has_gauge() const { return has_gauge_; }
 
   // angles in the reference's GaugeExt layout; on host or device
@@ -180,7 +184,8 @@ class Dirac {
 
   HopTables h_;
   int bc_ = 0;
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr;      // where the work goes
+  hipStream_t own_stream_ = nullptr;  // created and destroyed here
   DeviceTables<double> d64_;
   DeviceTables<float> d32_;
   phase16_t* uSh_ = nullptr;  // [Lt][E] 16-bit fixed-point angles (allocated by view_half_phases)
