@@ -282,6 +282,11 @@ int qed3_batch_cg(qed3_ctx* const* ctxs, int nsys, void* const* d_x, const void*
  * <x,y> = sum conj(x_i) y_i (the role of cublasZdotc in MatPoly::dot, matpoly.h:130-144) */
 int qed3_dot(qed3_ctx* ctx, const void* d_x, const void* d_y, double* re, double* im);
 
+/* ---- pseudofermion action ---------------------------------------------------------
+ * S_f = Re <phi, eta> with eta = (D^dag D)^{-1} phi from qed3_cg (replaces
+ * PseudoFermion::S, pseudofermion.h:53-58); complex double device fields. */
+int qed3_pf_action(qed3_ctx* ctx, const void* d_phi, const void* d_eta, double* S);
+
 /* ---- pseudofermion force -----------------------------------------------------------
  * F_l = -2 Re <D eta, dD/dtheta_l eta> for every spatial link (s,l) and temporal link
  * (s,x) at once (replaces GaugeExt::compute's per-link loop, gauge_ext.h:204-222, over

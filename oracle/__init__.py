@@ -5,13 +5,15 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 and only as the checker / CPU baseline.
 """
 import ctypes as C
+import os
 import pathlib
 import subprocess
 
 import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
-LIB = HERE / "liboracle.so"
+# QED3_ORACLE_LIB selects another build of oracle.cpp (the sanitizer build of the CPU tests)
+LIB = pathlib.Path(os.environ.get("QED3_ORACLE_LIB") or HERE / "liboracle.so")
 
 _dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
 _ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731

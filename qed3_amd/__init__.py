@@ -270,6 +270,12 @@ class WilsonDirac:
         check(load().qed3_dot(self._h, d_x.ptr, d_y.ptr, C.byref(re), C.byref(im)))
         return complex(re.value, im.value)
 
+    def pf_action(self, d_phi, d_eta):
+        """S_f = Re <phi, eta> of device fields (PseudoFermion::S)."""
+        S = C.c_double()
+        check(load().qed3_pf_action(self._h, d_phi.ptr, d_eta.ptr, C.byref(S)))
+        return S.value
+
     def force(self, eta: np.ndarray):
         """Pseudofermion force (spatial[Lt][E], temporal[Lt][V]) from eta (host)."""
         d_eta = DeviceArray.from_numpy(np.ascontiguousarray(eta, np.complex128))

@@ -4,7 +4,8 @@ import os
 import pathlib
 
 _HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = _HERE / "libqed3.so"
+# QED3_LIB selects another build of the same sources (the sanitizer build of the CPU tests)
+LIB_PATH = pathlib.Path(os.environ.get("QED3_LIB") or _HERE / "libqed3.so")
 
 c_int_p = C.POINTER(C.c_int)
 c_double_p = C.POINTER(C.c_double)
@@ -75,6 +76,7 @@ SIGNATURES = {
     "qed3_cg_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(CGOpts),
                                   C.POINTER(CGReport)]),
     "qed3_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
+    "qed3_pf_action": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, c_double_p]),
     "qed3_batch_cg": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p),
                                 C.POINTER(C.c_void_p), C.POINTER(CGOpts), C.POINTER(CGReport)]),
     "qed3_net_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),

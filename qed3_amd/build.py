@@ -34,9 +34,17 @@ def _common_flags():
     ]
 
 
-def _compile(src: pathlib.Path) -> pathlib.Path:
-    obj = OBJ / (src.name + ".o")
-    cmd = [_hipcc()] + _common_flags() + ["-c", str(src), "-o", str(obj)]
+# AddressSanitizer + UndefinedBehaviorSanitizer on the host code only (the device code is
+# built as usual); the library then needs the sanitizer runtime preloaded into the process.
+SAN_OBJ = HERE / "_build_asan"
+SAN_LIB = HERE / "_asan" / "libqed3.so"
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g"]
+
+
+def _compile(src: pathlib.Path, sanitize: bool = False) -> pathlib.Path:
+    obj = (SAN_OBJ if sanitize else OBJ) / (src.name + ".o")
+    cmd = [_hipcc()] + _common_flags() + (SAN_FLAGS if sanitize else []) + ["-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -45,23 +53,27 @@ def _compile(src: pathlib.Path) -> pathlib.Path:
     return obj
 
 
-def build(verbose: bool = False) -> pathlib.Path:
-    OBJ.mkdir(exist_ok=True)
+def build(verbose: bool = False, sanitize: bool = False) -> pathlib.Path:
+    """Compile every source and link libqed3.so; sanitize=True builds the host-instrumented
+    copy into qed3_amd/_asan/ instead (for the CPU tests, tests/run_sanitized.py)."""
+    obj_dir, lib = (SAN_OBJ, SAN_LIB) if sanitize else (OBJ, LIB)
+    obj_dir.mkdir(exist_ok=True)
+    lib.parent.mkdir(exist_ok=True)
     srcs = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp)] + [
-        str(o) for o in objs] + ["-lpthread"]
+        objs = list(ex.map(lambda s: _compile(s, sanitize), srcs))
+    tmp = lib.with_suffix(".so.tmp")
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp)] + (
+        SAN_FLAGS if sanitize else []) + [str(o) for o in objs] + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {lib}")
+    return lib
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, sanitize="--sanitize" in sys.argv[1:])
     sys.exit(0)

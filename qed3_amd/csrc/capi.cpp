@@ -786,6 +786,16 @@ int qed3_dot(qed3_ctx* ctx, const void* d_x, const void* d_y, double* re, double
   });
 }
 
+int qed3_pf_action(qed3_ctx* ctx, const void* d_phi, const void* d_eta, double* S) {
+  if (!ctx || !d_phi || !d_eta || !S) return fail(QED3_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    double im = 0.0;
+    ctx->D->dot(static_cast<const qed3::Spin<double>*>(d_phi), static_cast<const qed3::Spin<double>*>(d_eta), S,
+                &im);
+    return QED3_OK;
+  });
+}
+
 int qed3_pf_force(qed3_ctx* ctx, const void* d_eta, double* d_fs, double* d_ft) {
   if (!ctx || !d_eta || !d_fs || !d_ft) return fail(QED3_ERR_ARG, "null argument");
   if (!ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");

@@ -57,3 +57,27 @@ def test_force_is_gradient_of_action(gpu):
         b[s, x] -= eps
         fd = (S(sp, a) - S(sp, b)) / (2 * eps)
         assert abs(fd - ft[s, x]) < 1e-6 * max(1, abs(fd))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice,n,Lt", [(0, 2, 4), (1, 4, 8)])
+def test_pf_action(gpu, lattice, n, Lt):
+    """qed3_pf_action (PseudoFermion::S, pseudofermion.h:53-58) on phi = D^dag xi and
+    eta = (D^dag D)^{-1} phi: S_f = Re <phi, eta> = |xi|^2 exactly, and = the oracle's."""
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=0.1)
+    sp, tp = links(lattice, n, Lt, seed=3)
+    Do.update(sp, tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=0.1, at=at)
+    Dq.update(sp, tp)
+    xi = source(lattice, n, Lt, seed=44)
+    d_xi = q.DeviceArray.from_numpy(xi)
+    d_phi = q.DeviceArray(xi.nbytes)
+    Dq.apply_device(q.OP_DDAG, q.F64, d_phi, d_xi)
+    d_eta = q.DeviceArray(xi.nbytes)
+    rep = Dq.solve_device(d_eta, d_phi, tol=1e-12)
+    assert rep.status == 0
+    S = Dq.pf_action(d_phi, d_eta)
+    assert abs(S - np.vdot(xi, xi).real) < 1e-10 * S
+    phi = Do.apply(1, xi)
+    eta, _, _ = Do.solve(phi, tol=1e-13)
+    assert abs(S - np.vdot(phi, eta).real) < 1e-10 * S
