@@ -61,10 +61,14 @@ def parse():
     p.add_argument("--no-hmc", action="store_true")
     p.add_argument("--no-overlap", action="store_true")
     p.add_argument("--no-batched", action="store_true")
-    p.add_argument("--hmc-traj", type=int, default=3)
-    p.add_argument("--hmc-therm", type=int, default=6)
+    p.add_argument("--hmc-traj", type=int, default=12, help="HMC trajectories with the accept step")
+    p.add_argument("--hmc-gauge-therm", type=int, default=50, help="pure-gauge thermalisation trajectories")
+    p.add_argument("--hmc-therm", type=int, default=10, help="Nf=2 thermalisation trajectories")
     p.add_argument("--hmc-inner", type=int, default=50)
-    p.add_argument("--hmc-nsteps", type=int, default=20)
+    p.add_argument("--hmc-nsteps", type=int, default=80)
+    p.add_argument("--hmc-solver", choices=("learned", "half"), default="learned")
+    p.add_argument("--no-hmc-oracle", action="store_true", help="skip the dH check against the oracle")
+    p.add_argument("--hmc-only", action="store_true", help="run the HMC leg alone")
     p.add_argument("--batch-nsys", type=int, default=64)
     p.add_argument("--tile", type=str, default=None,
                    help="block,ts,remap,lds for the FP32 D stencil instead of the autotuned one")
@@ -141,6 +145,9 @@ def main():
 
     if a.kernels_only:
         return kernels_only(a, q)
+    if a.hmc_only:
+        print(json.dumps({"hmc": hmc_leg(a, q)}), flush=True)
+        return
     learned = learned_leg(a, q)
     second = None if a.no_second else second_leg(a, q)
     cpu = None
@@ -566,37 +573,97 @@ def overlap_leg(q, n=16, Lt=16, M5=-0.5, degree=21, tol=1e-10):
 
 
 def hmc_leg(a, q):
-    """Full HMC (simplicial n=64, Lt=32, m=0.1, compact U(1) at beta=1.4, tau=1, 20 outer
-    (fermion) steps of hmc_inner gauge steps each, Nf=2 via one pseudofermion, half-storage
-    CG):
-    hmc_therm forced-accept trajectories from the von Mises start, then hmc_traj timed
-    trajectories with the accept step (seconds per trajectory, dH, acceptance, CG work);
-    the pseudofermion force at the same size against the FP64 CPU path (the oracle's local
-    route) on the same eta, and the force launch against its algorithmic bytes.  The gauge
-    coupling per face, beta_s = a_t/(vol_f g^2), is ~220 at n=64 with a_t = <l>/2, which
-    sets the small gauge step."""
+    """Full HMC: simplicial n=64, Lt=32, m=0.1, compact U(1) at beta=1.4, Nf=2 (one
+    pseudofermion), trajectory length 1, pseudofermion solves by the learned method (flexible
+    CG with the time-exact preconditioner; FP64 true residual 1e-10), FP64 Hamiltonian and
+    accept step.  a_t = <l>/2 makes the couplings per plaquette ~90 (beta_s = a_t/(vol_f g^2),
+    beta_t = 2 vol_l/(a_t l^2 g^2)), so the dynamics are stiff: 20 outer steps give
+    <dH> ~ 30 on the thermalised field, 40 x 50 steps <dH> ~ 3 (acceptance 0.3), and
+    hmc_nsteps x hmc_inner (80 x 50) are used.
+    Start from the ordered field (all angles 0); thermalise with hmc_gauge_therm pure-gauge
+    and hmc_therm Nf=2 trajectories, forced-accept, lengths drawn uniformly from [0.5, 1.5]
+    (a fixed length leaves the modes whose period divides it unthermalised); then hmc_traj
+    trajectories with the accept step: acceptance, dH and <exp(-dH)> (Creutz: 1 at
+    equilibrium).  Then one short trajectory (tau = 0.05, one outer step of two gauge steps)
+    from the thermalised field against the FP64 CPU path on the same seeds (the oracle's
+    HMC2 restatement), and the force launch against the oracle and its algorithmic bytes."""
     import oracle as O
-    n, Lt, m, lattice = 64, 32, 0.1, 0
+    n, Lt, m, lattice, gsq = 64, 32, 0.1, 0, 1 / 1.4
     g = q.Geometry(n)
     at = 0.5 * mean_ell(g, lattice)
     D = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=m, at=at)
-    sp, tp = q.random_gauge(lattice, n, Lt, seed=55, kappa=1.4)
-    sp, tp = np.ascontiguousarray(sp), np.ascontiguousarray(tp)
+    sp = np.zeros((Lt, D.E))
+    tp = np.zeros((Lt, D.V))
     D.update(sp, tp)
     D.autotune()
-    H = q.HMC(D, 1 / 1.4, compact=True, npf=1, tau=1.0, nsteps=a.hmc_nsteps, nsteps_inner=a.hmc_inner,
-              tol=1e-10, solver=q.CG_HALF, seed=7)
+    solver = q.PCG if a.hmc_solver == "learned" else q.CG_HALF
+    rng = np.random.default_rng(17)
+    t0 = time.time()
+    H = q.HMC(D, gsq, compact=True, npf=0, tau=1.0, nsteps=1, nsteps_inner=1000, tol=1e-10, solver=solver, seed=5)
+    for k in range(a.hmc_gauge_therm):
+        tau = rng.uniform(0.5, 1.5)
+        H.set_steps(tau, 1, int(round(1000 * tau)))
+        H.trajectory(sp, tp, no_reject=True)
+    del H
+    S_gauge = D.gauge_action(sp, tp, gsq, compact=True)
+    print(f"# hmc pure-gauge thermalisation: {a.hmc_gauge_therm} trajectories, S_g={S_gauge:.6e}, "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    H = q.HMC(D, gsq, compact=True, npf=1, tau=1.0, nsteps=a.hmc_nsteps, nsteps_inner=a.hmc_inner, tol=1e-10,
+              solver=solver, seed=7)
     for k in range(a.hmc_therm):
+        tau = rng.uniform(0.5, 1.5)
+        H.set_steps(tau, max(1, int(round(a.hmc_nsteps * tau))), a.hmc_inner)
         r = H.trajectory(sp, tp, no_reject=True)
-        print(f"# hmc thermalisation {k}: dH={r['dH']:.3e}", file=sys.stderr, flush=True)
+        print(f"# hmc thermalisation {k}: tau={tau:.2f} dH={r['dH']:.3e} h0={r['h0']:.6e}", file=sys.stderr, flush=True)
+    H.set_steps(1.0, a.hmc_nsteps, a.hmc_inner)
     trajs = []
     for k in range(a.hmc_traj):
         t0 = time.time()
         r = H.trajectory(sp, tp)
         r["wall_s"] = time.time() - t0
         trajs.append(r)
-        print(f"# hmc traj {k}: dH={r['dH']:.3e} accept={r['accept']} cg={r['cg_solves']}/"
+        print(f"# hmc traj {k}: dH={r['dH']:.4f} accept={r['accept']} cg={r['cg_solves']}/"
               f"{r['cg_iterations']} {r['wall_s']:.2f}s", file=sys.stderr, flush=True)
+    del H
+    dh = np.array([t["dH"] for t in trajs])
+    w = np.exp(-dh)
+    out = {"workload": f"simplicial n={n} Lt={Lt} m={m} compact beta=1.4 a_t=<l>/2 tau=1 "
+                       f"nsteps={a.hmc_nsteps}x{a.hmc_inner} npf=1 (Nf=2), solver={a.hmc_solver}",
+           "start": f"ordered; {a.hmc_gauge_therm} pure-gauge + {a.hmc_therm} Nf=2 forced-accept trajectories "
+                    "of lengths U[0.5, 1.5]",
+           "acceptance": float(np.mean([t["accept"] for t in trajs])),
+           "mean_dH": float(dh.mean()), "mean_abs_dH": float(np.abs(dh).mean()),
+           "creutz_exp_minus_dH": float(w.mean()),
+           "creutz_err": float(w.std(ddof=1) / np.sqrt(len(w))) if len(w) > 1 else None,
+           "seconds_per_trajectory": round(float(np.mean([t["wall_s"] for t in trajs])), 3),
+           "cg_iterations_per_solve": round(float(np.sum([t["cg_iterations"] for t in trajs])
+                                                  / max(1, np.sum([t["cg_solves"] for t in trajs]))), 1),
+           "trajectories": [{k: t[k] for k in ("dH", "accept", "cg_solves", "cg_iterations", "max_resid")}
+                            for t in trajs]}
+    L = O.Lattice(g, lattice)
+    if not a.no_hmc_oracle:
+        # dH of a short trajectory from the thermalised field: GPU (FP64 CG) vs the oracle
+        Sg = O.GaugeAction(L, Lt, gsq=gsq, at=at, compact=True)
+        Do_h = O.Dirac(L, Lt, r=1.0, M5=m, at=at)
+        H = q.HMC(D, gsq, compact=True, npf=1, tau=0.05, nsteps=1, nsteps_inner=2, tol=1e-12, solver=q.CG_F64,
+                  seed=12)
+        uu, tt = sp.copy(), tp.copy()
+        rg = H.trajectory(uu, tt, no_reject=True)
+        del H
+        t0 = time.time()
+        ro = O.hmc_trajectory(Do_h, Sg, seed=12, npf=1, tau=0.05, nsteps=1, nsteps_inner=2, tol=1e-12,
+                              sp=sp, tp=tp)
+        t_or = time.time() - t0
+        del Do_h
+        out["dH_vs_cpu_fp64"] = {
+            "trajectory": "tau=0.05, 1 outer step x 2 gauge steps, same seeds, CG tol 1e-12 (GPU FP64 CG, oracle CG)",
+            "h0_gpu": rg["h0"], "h0_cpu": ro["h0"], "dH_gpu": rg["dH"], "dH_cpu": ro["dH"],
+            "abs_diff_dH": abs(rg["dH"] - ro["dH"]),
+            "rel_diff_dH": abs(rg["dH"] - ro["dH"]) / max(1.0, abs(ro["dH"])),
+            "max_abs_diff_field": float(max(np.abs(uu - ro["u"]).max(), np.abs(tt - ro["t"]).max())),
+            "cpu_seconds": round(t_or, 1)}
+        print(f"# hmc dH vs oracle: gpu {rg['dH']:.10e} cpu {ro['dH']:.10e} ({t_or:.1f}s on the CPU)",
+              file=sys.stderr, flush=True)
     # force at this size: GPU vs the FP64 CPU path on the same eta
     D.update(sp, tp)
     phi = q.random_source(lattice, n, Lt, seed=3)
@@ -615,7 +682,6 @@ def hmc_leg(a, q):
     fs = d_fs.to_numpy(np.float64).reshape(Lt, D.E)
     ft = d_ft.to_numpy(np.float64).reshape(Lt, D.V)
     t0 = time.time()
-    L = O.Lattice(g, lattice)
     Do = O.Dirac(L, Lt, r=1.0, M5=m, at=at)
     Do.update(sp, tp)
     t_setup = time.time() - t0
@@ -628,17 +694,11 @@ def hmc_leg(a, q):
     link = (D.E + D.V) * Lt * 16
     # one FP64 D apply (D eta), then the link kernel: eta and D eta in, phases, one double out
     b_force = (2 * vec + link) + (2 * vec + link + (D.E + D.V) * Lt * 8)
-    return {"workload": f"simplicial n={n} Lt={Lt} m={m} compact beta=1.4 tau=1 "
-                        f"nsteps={a.hmc_nsteps}x{a.hmc_inner} npf=1, after {a.hmc_therm} forced-accept "
-                        "trajectories",
-            "acceptance": float(np.mean([t["accept"] for t in trajs])),
-            "seconds_per_trajectory": round(float(np.mean([t["wall_s"] for t in trajs])), 3),
-            "trajectories": [{k: t[k] for k in ("dH", "accept", "cg_solves", "cg_iterations",
-                                                "max_resid")} for t in trajs],
-            "force": {"ms_gpu": round(ms_force, 4), "algorithmic_GB": round(b_force / 1e9, 4),
-                      "achieved_GBs": round(b_force / (ms_force * 1e-3) / 1e9, 1),
-                      "max_rel_diff_vs_cpu_fp64": float(err), "cpu_fp64_s": round(t_cpu, 3),
-                      "cpu_setup_s": round(t_setup, 1)}}
+    out["force"] = {"ms_gpu": round(ms_force, 4), "algorithmic_GB": round(b_force / 1e9, 4),
+                    "achieved_GBs": round(b_force / (ms_force * 1e-3) / 1e9, 1),
+                    "max_rel_diff_vs_cpu_fp64": float(err), "cpu_fp64_s": round(t_cpu, 3),
+                    "cpu_setup_s": round(t_setup, 1)}
+    return out
 
 
 def batched_leg(a, q):

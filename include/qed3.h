@@ -39,6 +39,9 @@ enum { QED3_PREC_F64 = 0, QED3_PREC_F32 = 1, QED3_PREC_F16 = 2 };
 /* FP64 CG; FP32 inner CG with FP64 reliable updates; the same with the inner r, p and w
  * stored in IEEE binary16 (scaled to unit rms at every reliable update) */
 enum { QED3_SOLVER_CG_F64 = 0, QED3_SOLVER_CG_MIXED = 1, QED3_SOLVER_CG_HALF = 2 };
+/* The learned method: flexible CG with the time-exact preconditioner (qed3_pcg, four layers,
+ * Neumann coefficients); a solver of the HMC's pseudofermion solves (qed3_hmc_params). */
+enum { QED3_SOLVER_PCG = 3 };
 enum { QED3_HOST = 0, QED3_DEVICE = 1 };
 
 const char* qed3_last_error(void);
@@ -317,7 +320,7 @@ typedef struct qed3_hmc_params {
   int nsteps;       /* outer (fermion) leapfrog steps */
   int nsteps_inner; /* gauge steps per outer step (integrator.h:58-73) */
   double tol;       /* true-residual tolerance of every pseudofermion solve */
-  int solver;       /* QED3_SOLVER_* */
+  int solver;       /* QED3_SOLVER_* (QED3_SOLVER_PCG: needs r = 1) */
   int seed;         /* ParallelRngExt master seed (rng.h:91-120) */
 } qed3_hmc_params;
 
@@ -339,6 +342,11 @@ void qed3_hmc_destroy(qed3_hmc* h);
  * in place by the new field when accepted (or always when no_reject != 0). */
 int qed3_hmc_trajectory(qed3_hmc* h, double* spatial, double* temporal, int no_reject,
                         qed3_hmc_report* rep);
+/* Trajectory length and leapfrog steps of the following trajectories (no counterpart in the
+ * reference, whose HMC2 keeps them fixed): thermalisation from an ordered start converges
+ * geometrically only when the length varies from trajectory to trajectory (a fixed length
+ * leaves the modes whose period divides it nearly untouched). */
+int qed3_hmc_set_steps(qed3_hmc* h, double tau, int nsteps, int nsteps_inner);
 /* Overlap fermions in the HMC (hmc.cu:155-186 with IS_OVERLAP, Fermion =
  * Overlap<WilsonDirac>): phi = D_ov^dag xi, eta = (D_ov + D_ov^dag)^{-1} phi, the overlap
  * force, all through ov (created on the HMC's context; it must outlive its use here).

@@ -19,6 +19,7 @@ ANTIPERIODIC, PERIODIC = 0, 1
 OP_D, OP_DDAG, OP_DDAGD, OP_DDDAG = 0, 1, 2, 3
 F64, F32, F16 = 0, 1, 2
 CG_F64, CG_MIXED, CG_HALF = 0, 1, 2
+PCG = 3  # the learned method (time-exact preconditioner in flexible CG), as an HMC solver
 
 
 def _dp(a):
@@ -548,6 +549,10 @@ class HMC:
 
     def rng_load(self, path):
         check(load().qed3_hmc_rng_load(self._h, str(path).encode()))
+
+    def set_steps(self, tau, nsteps, nsteps_inner):
+        """Trajectory length and leapfrog steps of the following trajectories."""
+        check(load().qed3_hmc_set_steps(self._h, float(tau), int(nsteps), int(nsteps_inner)))
 
     def trajectory(self, spatial, temporal, no_reject=False):
         """Runs one trajectory; spatial/temporal (float64 arrays) are updated in place."""

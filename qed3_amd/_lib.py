@@ -115,6 +115,7 @@ SIGNATURES = {
     "qed3_hmc_destroy": (None, [C.c_void_p]),
     "qed3_hmc_set_overlap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "qed3_hmc_trajectory": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_int, C.c_void_p]),
+    "qed3_hmc_set_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int]),
     "qed3_hmc_rng_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "qed3_hmc_rng_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "qed3_plaquette_correlator": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p]),

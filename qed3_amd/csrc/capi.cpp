@@ -831,7 +831,7 @@ int qed3_plaquette_correlator(const qed3_geom* g, const qed3_params* p, const do
 int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
   if (!ctx || !p || !out) return fail(QED3_ERR_ARG, "null argument");
   if (p->solver != QED3_SOLVER_CG_F64 && p->solver != QED3_SOLVER_CG_MIXED &&
-      p->solver != QED3_SOLVER_CG_HALF)
+      p->solver != QED3_SOLVER_CG_HALF && p->solver != QED3_SOLVER_PCG)
     return fail(QED3_ERR_ARG, "bad solver");
   return guarded([&]() -> int {
     qed3::HMCParams hp;
@@ -844,6 +844,7 @@ int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
     hp.tol = p->tol;
     hp.mixed = p->solver != QED3_SOLVER_CG_F64;
     hp.half = p->solver == QED3_SOLVER_CG_HALF;
+    hp.pcg = p->solver == QED3_SOLVER_PCG;
     hp.seed = p->seed;
     auto h = std::make_unique<qed3_hmc>();
     h->ctx = ctx;
@@ -854,6 +855,15 @@ int qed3_hmc_create(qed3_ctx* ctx, const qed3_hmc_params* p, qed3_hmc** out) {
 }
 
 void qed3_hmc_destroy(qed3_hmc* h) { delete h; }
+
+int qed3_hmc_set_steps(qed3_hmc* h, double tau, int nsteps, int nsteps_inner) {
+  if (!h) return fail(QED3_ERR_ARG, "null handle");
+  if (!(tau > 0) || nsteps < 1 || nsteps_inner < 1) return fail(QED3_ERR_ARG, "bad trajectory length or steps");
+  return guarded([&]() -> int {
+    h->hmc->set_steps(tau, nsteps, nsteps_inner);
+    return QED3_OK;
+  });
+}
 
 int qed3_hmc_set_overlap(qed3_hmc* h, qed3_overlap* ov, int refresh) {
   if (!h) return fail(QED3_ERR_ARG, "null argument");

@@ -127,6 +127,7 @@ HMC::HMC(Dirac& D, const HMCParams& p)
   if (p.npf < 0 || p.nsteps < 1 || p.nsteps_inner < 1) throw std::invalid_argument("bad HMC parameters");
   const HopTables& h = D.tables();
   rng_ = std::make_unique<ParallelRng>(h.Lt, h.E, h.V, p.seed);
+  if (p.pcg) K_ = std::make_unique<Horner>(D, 4);
   const size_t n = D.points();
   phi_.resize(p.npf);
   eta_.resize(p.npf);
@@ -159,7 +160,10 @@ void HMC::update_eta(int k) {
   o.mixed = p_.mixed;
   o.half = p_.half;
   // Wilson: CG on D^dag D; overlap: FP64 CG on D_ov + D_ov^dag (PseudoFermion::update_eta)
-  const CGResult r = ov_ ? ov_->solve_dhd(eta_[k], phi_[k], p_.tol, o.maxiter) : cg_solve(D_, eta_[k], phi_[k], o);
+  if (K_) o.delta = 0.001;  // the learned method's reliable-update threshold
+  const CGResult r = ov_   ? ov_->solve_dhd(eta_[k], phi_[k], p_.tol, o.maxiter)
+                     : K_ ? pcg_solve(*K_, eta_[k], phi_[k], o)
+                          : cg_solve(D_, eta_[k], phi_[k], o);
   rep_.cg_solves++;
   rep_.cg_iterations += r.iterations;
   rep_.max_resid = std::max(rep_.max_resid, r.true_resid);

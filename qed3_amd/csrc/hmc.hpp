@@ -11,6 +11,7 @@
 
 #include "cg.hpp"
 #include "dirac.hpp"
+#include "horner.hpp"
 #include "gauge_dev.hpp"
 #include "msolve.hpp"
 #include "rng.hpp"
@@ -52,6 +53,7 @@ struct HMCParams {
   double tol = 1e-10;
   bool mixed = true;
   bool half = false;  // binary16-stored inner CG fields
+  bool pcg = false;   // flexible CG with the time-exact preconditioner (r = 1 only)
   int seed = 1;
 };
 
@@ -79,6 +81,11 @@ class HMC {
     ov_ = ov;
     ov_refresh_ = refresh;
   }
+  void set_steps(double tau, int nsteps, int nsteps_inner) {
+    p_.tau = tau;
+    p_.nsteps = nsteps;
+    p_.nsteps_inner = nsteps_inner;
+  }
   ParallelRng& rng() { return *rng_; }
   const ParallelRng& rng() const { return *rng_; }
 
@@ -95,6 +102,7 @@ class HMC {
   HMCParams p_;
   GaugeAction Sg_;
   std::unique_ptr<ParallelRng> rng_;
+  std::unique_ptr<Horner> K_;  // the preconditioner of the pcg solver
   std::vector<Spin<double>*> phi_, eta_;
   Spin<double>* xi_ = nullptr;
   double* dfs_ = nullptr;

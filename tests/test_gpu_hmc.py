@@ -105,3 +105,46 @@ def test_hmc_with_reduced_precision_solvers_matches_oracle(gpu, solver):
     assert abs(rep["h0"] - ref["h0"]) < 1e-8 * abs(ref["h0"])
     assert abs(rep["dH"] - ref["dH"]) < 1e-7 * max(1.0, abs(ref["h0"]))
     assert rel(u, ref["u"]) < 1e-8 and rel(t, ref["t"]) < 1e-8
+
+
+@pytest.mark.gpu
+def test_hmc_with_learned_solver_matches_oracle(gpu):
+    """The pseudofermion solves by the learned method (flexible CG with the time-exact
+    preconditioner): the trajectory still equals the oracle's (h0, dH, field)."""
+    lattice, n, Lt, M5, gsq = 0, 2, 8, 0.1, 1 / 1.4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    sp, tp = links(lattice, n, Lt, seed=6, kappa=1.4)
+    Sg = O.GaugeAction(L, Lt, gsq=gsq, at=at, compact=True)
+    ref = O.hmc_trajectory(Do, Sg, seed=31, npf=1, tau=0.5, nsteps=4, nsteps_inner=3,
+                           tol=1e-13, sp=sp, tp=tp)
+    Dq = q.WilsonDirac(lattice, n, Lt, M5=M5, at=at)
+    H = q.HMC(Dq, gsq, compact=True, npf=1, tau=0.5, nsteps=4, nsteps_inner=3, tol=1e-12,
+              solver=q.PCG, seed=31)
+    u, t = sp.copy(), tp.copy()
+    rep = H.trajectory(u, t, no_reject=True)
+    assert rep["max_resid"] <= 1e-12
+    assert abs(rep["h0"] - ref["h0"]) < 1e-9 * abs(ref["h0"])
+    assert abs(rep["dH"] - ref["dH"]) < 1e-8 * max(1.0, abs(ref["dH"]))
+    assert rel(u, ref["u"]) < 1e-10 and rel(t, ref["t"]) < 1e-10
+
+
+@pytest.mark.gpu
+def test_hmc_set_steps(gpu):
+    """set_steps before a trajectory gives the trajectory of a handle created with those
+    steps (same seed), and is rejected for a non-positive length."""
+    lattice, n, Lt, M5, gsq = 1, 1, 4, 0.1, 1 / 1.4
+    L, Do, at = oracle_setup(lattice, n, Lt, M5=M5)
+    sp, tp = links(lattice, n, Lt, seed=2, kappa=1.4)
+    Dq = q.WilsonDirac(lattice, n, Lt, M5=M5, at=at)
+    H1 = q.HMC(Dq, gsq, compact=True, npf=1, tau=1.0, nsteps=2, nsteps_inner=2, tol=1e-12,
+               solver=q.CG_F64, seed=5)
+    H1.set_steps(0.3, 3, 4)
+    u1, t1 = sp.copy(), tp.copy()
+    r1 = H1.trajectory(u1, t1, no_reject=True)
+    H2 = q.HMC(Dq, gsq, compact=True, npf=1, tau=0.3, nsteps=3, nsteps_inner=4, tol=1e-12,
+               solver=q.CG_F64, seed=5)
+    u2, t2 = sp.copy(), tp.copy()
+    r2 = H2.trajectory(u2, t2, no_reject=True)
+    assert r1["dH"] == r2["dH"] and np.array_equal(u1, u2) and np.array_equal(t1, t2)
+    with pytest.raises(q.Qed3Error):
+        H1.set_steps(0.0, 3, 4)
