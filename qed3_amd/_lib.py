@@ -98,6 +98,7 @@ SIGNATURES = {
     "qed3_precond_get_params": (C.c_int, [C.c_void_p, c_double_p]),
     "qed3_precond_set_params": (C.c_int, [C.c_void_p, c_double_p]),
     "qed3_precond_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "qed3_precond_apply_solver": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "qed3_precond_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p,
                                          c_double_p]),
     "qed3_precond_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,

@@ -328,5 +328,269 @@ inline void launch_horner_layer(const HornerView& hv, const DiracView<float, P>&
     hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, nblk, epi);
 }
 
+// ---- parallel-in-time layer (the solver's path: binary16 fields, temporal gauge) ------
+// In the temporal gauge every temporal link inside a column is 1, so both recurrences have
+// the constant real coefficient a = kappa_t / c_x:
+//   forward  st(s) = a st(s-1) + y(s) / c_x,   backward  st(s) = a st(s+1) + y(s) / c_x,
+// with the wrap term (bc kappa_t w g_in at the column's far end) added to y at the first
+// (forward) or last (backward) slice.  A linear recurrence splits over time chunks: lane k
+// of a site owns slices [kT, kT + T) and runs both recurrences over its chunk from a zero
+// start, holding its chunk in registers; the chunks' end values are exchanged in LDS and
+// every lane adds carry a^m to its values (m the distance from its chunk's start, for the
+// backward row from its end), with carry_k = E_{k-1} + a^T carry_{k-1} (forward) and
+// carry_k = E_{k+1} + a^{len_{k+1}} carry_{k+1} (backward), E the lanes' end values.  So a
+// layer reads its input and injection once and writes its output once, and a site's time
+// column runs on C = ceil(Lt / T) threads (C * S = 256, S sites per block; the threads of a
+// lane are consecutive, so a wave reads and writes consecutive sites of one slice).  Per step the block stages the C slices its lanes are at, for its S sites and
+// their halo (halo tables of ranges of S sites).
+template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi>
+__global__ __launch_bounds__(256, (NS == 3 ? 4 : 2)) void k_horner_layer_tc(HornerView hv, DiracView<float, phasepair_t> d,
+                                                         HaloView halo, Src src, const SpinH* __restrict__ gin,
+                                                         SpinH* __restrict__ gout, float2 cj, TileMap tm, Epi epi,
+                                                         GridReduce gr, int C) {
+  if (epi.skip()) return;  // uniform
+  extern __shared__ float4 hk_tile_raw[];
+  const int S = int(blockDim.x) / C;
+  const int lane = int(threadIdx.x) / S, sl = int(threadIdx.x) % S;  // a wave: consecutive sites, one chunk
+  SpinH* tile = reinterpret_cast<SpinH*>(hk_tile_raw);         // [2][C][S + hmax]
+  const int row = S + halo.hmax;
+  float2* carry = reinterpret_cast<float2*>(tile + (FIRST ? 0 : 2 * C * row));  // [2][256]
+  HopRow16* hqs = reinterpret_cast<HopRow16*>(carry + 2 * 256);  // [S][2 rows][NS], the sites' hop rows
+  int xr, st_;
+  tile_of(tm, blockIdx.x, xr, st_);
+  (void)st_;
+  const int V = hv.V, Lt = hv.Lt;
+  const bool live = xr < tm.nxr;
+  const int x0 = xr * S;
+  const int x = live ? x0 + sl : V;
+  const bool act = x < V;
+  const int s0 = lane * T;
+  const int len = act ? max(0, min(T, Lt - s0)) : 0;
+  constexpr int rF = ADJ ? 1 : 0, rB = 1 - rF;
+  // halo of this block's range: lane c stages slice cT + j of it, its S threads share it
+  const int h0 = (live && !FIRST) ? halo.halo_off[xr] : 0;
+  const int nh = (live && !FIRST) ? halo.halo_off[xr + 1] - h0 : 0;
+  const int hid0 = (sl < nh) ? halo.halo[h0 + sl] : -1;
+  const int hid1 = (sl + S < nh) ? halo.halo[h0 + sl + S] : -1;
+  int nb[NS], lk[NS];
+  float invc = 0.f, kt = 0.f;
+  if (act) {
+    if constexpr (!FIRST) {
+#pragma unroll
+      for (int j = 0; j < NS; j++) {
+        nb[j] = halo.nbl[j * V + x];
+        lk[j] = d.lnk[j * V + x];
+      }
+      // the site's hop rows go to LDS once (lane 0), read per step by all its lanes
+      if (lane == 0) {
+        const size_t mF = size_t((ADJ ? 2 : 0) + rF) * NS * V, mB = size_t((ADJ ? 2 : 0) + rB) * NS * V;
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+          hqs[(sl * 2 + 0) * NS + j] = hv.hrow16[mF + size_t(j) * V + x];
+          hqs[(sl * 2 + 1) * NS + j] = hv.hrow16[mB + size_t(j) * V + x];
+        }
+      }
+    }
+    invc = hv.invc[x];
+    kt = hv.kt[x];
+  }
+  const HopRow16* hqF = hqs + (sl * 2 + 0) * NS;
+  const HopRow16* hqB = hqs + (sl * 2 + 1) * NS;
+  const float a = invc * kt;
+  const auto in = src.bind();
+  // the wrap terms, weighted: the forward row's on the first slice of lane 0, the backward
+  // row's on the last slice of the last lane
+  float2 wF = {0.f, 0.f}, wB = {0.f, 0.f};
+  if constexpr (!FIRST) {
+    if (act && s0 == 0) {
+      const float2 w = ldu(d.uPoly + x);
+      const float2 g = comp(gin[size_t(Lt - 1) * V + x], rF);
+      const float2 t = cmul<float>({w.x, -w.y}, g);
+      wF = {d.bc * kt * t.x, d.bc * kt * t.y};
+    }
+    if (act && len > 0 && s0 + len == Lt) {
+      const float2 w = ldu(d.uPoly + x);
+      const float2 g = comp(gin[x], rB);
+      const float2 t = cmul<float>(w, g);
+      wB = {d.bc * kt * t.x, d.bc * kt * t.y};
+    }
+  }
+  struct Loads {
+    SpinH v, gc, gh0, gh1;
+    phasepair_t u[NS];
+  };
+  // slice s0 + j of this lane (own point, links) and slice c T + j of the halo share
+  auto load = [&](int j, Loads& Ld) {
+    const int s = s0 + j;
+    if (j < len) {
+      const size_t sV = size_t(s) * V;
+      Ld.v = in.raw(sV + x);
+      if constexpr (!FIRST) {
+        const phasepair_t* uS = d.uS + size_t(s) * d.E;
+#pragma unroll
+        for (int q = 0; q < NS; q++) Ld.u[q] = uS[lk[q] >= 0 ? lk[q] : ~lk[q]];
+        Ld.gc = gin[sV + x];
+      }
+    }
+    if constexpr (!FIRST) {
+      if (live && s < Lt) {
+        if (hid0 >= 0) Ld.gh0 = gin[size_t(s) * V + hid0];
+        if (hid1 >= 0) Ld.gh1 = gin[size_t(s) * V + hid1];
+      }
+    }
+  };
+  auto transport = [&](const Loads& Ld, const SpinH* tb, const HopRow16* hq, int r) -> float2 {
+    half2_t t16 = {(_Float16)0.f, (_Float16)0.f};
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      half2_t u = Ld.u[q].v;
+      if (lk[q] < 0) u.y = -u.y;
+      const SpinH g = tb[nb[q]];
+      t16 += cmul16(u, cmul16(hq[q].s, comp16(g, r)) + cmul16(hq[q].f, comp16(g, 1 - r)));
+    }
+    return {float(t16.x), float(t16.y)};
+  };
+  float2 zF[T];   // forward row: the chunk's states (FP32)
+  half2_t zB[T];  // backward row: its recurrence inputs, then its states (binary16)
+  float2 prevF = {0.f, 0.f};
+  Loads cur, nxt;
+  load(0, cur);
+#pragma unroll
+  for (int j = 0; j < T; j++) {
+    if (j + 1 < T) load(j + 1, nxt);
+    SpinH* tb = tile + ((j & 1) * C + lane) * row;
+    if constexpr (!FIRST) {
+      if (j < len) tb[sl] = cur.gc;
+      if (live && s0 + j < Lt) {
+        if (hid0 >= 0) tb[S + sl] = cur.gh0;
+        if (hid1 >= 0) tb[2 * S + sl] = cur.gh1;
+        for (int q = sl + 2 * S; q < nh; q += S) tb[S + q] = gin[size_t(s0 + j) * V + halo.halo[h0 + q]];
+      }
+      __syncthreads();
+    }
+    if (j < len) {
+      float2 yF = cmul<float>(cj, comp(cur.v, rF)), yB = cmul<float>(cj, comp(cur.v, rB));
+      if constexpr (!FIRST) {
+        const float2 tF = transport(cur, tb, hqF, rF), tB = transport(cur, tb, hqB, rB);
+        yF.x -= tF.x;
+        yF.y -= tF.y;
+        yB.x -= tB.x;
+        yB.y -= tB.y;
+      }
+      if (j == 0) {
+        yF.x += wF.x;
+        yF.y += wF.y;
+      }
+      if (j == len - 1) {
+        yB.x += wB.x;
+        yB.y += wB.y;
+      }
+      prevF = {a * prevF.x + invc * yF.x, a * prevF.y + invc * yF.y};
+      zF[j] = prevF;
+      zB[j] = half2_t{(_Float16)yB.x, (_Float16)yB.y};
+    }
+    if (j + 1 < T) cur = nxt;
+  }
+  // local backward recurrence over the chunk held in registers
+  float2 nextB = {0.f, 0.f};
+#pragma unroll
+  for (int j = T - 1; j >= 0; j--) {
+    if (j < len) {
+      nextB = {a * nextB.x + invc * float(zB[j].x), a * nextB.y + invc * float(zB[j].y)};
+      zB[j] = half2_t{(_Float16)nextB.x, (_Float16)nextB.y};
+    }
+  }
+  // carries across the lanes of each site
+  float aT = 1.f;
+#pragma unroll
+  for (int j = 0; j < T; j++) aT *= a;
+  carry[threadIdx.x] = prevF;                       // forward end value (0 for an empty lane)
+  carry[blockDim.x + threadIdx.x] = nextB;          // backward end value
+  __syncthreads();
+  float2 cF = {0.f, 0.f}, cB = {0.f, 0.f};
+  for (int c = 0; c < lane; c++) {
+    const float2 e = carry[c * S + sl];
+    cF = {aT * cF.x + e.x, aT * cF.y + e.y};
+  }
+  for (int c = C - 1; c > lane; c--) {
+    const int lc = max(0, min(T, Lt - c * T));
+    float ac = 1.f;
+    for (int q = 0; q < lc; q++) ac *= a;
+    const float2 e = carry[blockDim.x + c * S + sl];
+    cB = {ac * cB.x + e.x, ac * cB.y + e.y};
+  }
+  // add the carries and store
+  double2 acc = {0.0, 0.0};
+  float pF = a;
+#pragma unroll
+  for (int j = 0; j < T; j++) {
+    if (j < len) {
+      zF[j].x += pF * cF.x;
+      zF[j].y += pF * cF.y;
+      pF *= a;
+    }
+  }
+  // backward row: add a^(len - j) carry while storing, last slice first
+  float pB = a;
+#pragma unroll
+  for (int j = T - 1; j >= 0; j--) {
+    if (j < len) {
+      const float2 b = {float(zB[j].x) + pB * cB.x, float(zB[j].y) + pB * cB.y};
+      pB *= a;
+      const size_t i = size_t(s0 + j) * V + x;
+      if constexpr (rF == 0) {
+        put(gout + i, zF[j], b);
+        epi(i, stored(gout, zF[j], b), acc);
+      } else {
+        put(gout + i, b, zF[j]);
+        epi(i, stored(gout, b, zF[j]), acc);
+      }
+    }
+  }
+  if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
+}
+
+// Lanes per site of the parallel-in-time layer: C = ceil(Lt / T) rounded up to a power of
+// two (C * S = 256 threads); at most 64.
+constexpr int kTcChunk = 16;
+inline int tc_lanes(int Lt) {
+  const int c = (Lt + kTcChunk - 1) / kTcChunk;
+  int p = 1;
+  while (p < c) p <<= 1;
+  if (p > 64) throw std::invalid_argument("time extent too long for the parallel-in-time layer");
+  return p;
+}
+inline int tc_sites(int Lt) { return 256 / tc_lanes(Lt); }
+
+template <bool ADJ, bool FIRST, class Src, class Epi>
+inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, phasepair_t>& tables,
+                                   const HaloView& halo, const Src& src, const SpinH* gin, SpinH* gout, float2 cj,
+                                   const Epi& epi, GridReduce gr, hipStream_t st, const hipEvent_t* bracket = nullptr) {
+  const int C = tc_lanes(hv.Lt), S = 256 / C;
+  if (!tables.uPoly) throw std::logic_error("parallel-in-time layer needs the temporal gauge");
+  if (!FIRST && !halo.nbl) throw std::logic_error("parallel-in-time layer launched without halo tables");
+  TileMap tm;
+  tm.nxr = (hv.V + S - 1) / S;
+  tm.nxr_pad = (tm.nxr + 7) / 8 * 8;
+  tm.nst = 1;
+  tm.remap = 1;
+  const unsigned nblk = unsigned(tm.nxr_pad);
+  if (Epi::kReduce && nblk > kReduceCapacity) throw std::invalid_argument("layer grid exceeds the reduction scratch");
+  const size_t smem =
+      (FIRST ? 0 : size_t(2) * C * (S + halo.hmax) * sizeof(SpinH)) + size_t(2) * 256 * sizeof(float2) +
+      (FIRST ? 0 : size_t(S) * 2 * hv.nslot * sizeof(HopRow16));
+  if (smem > 64 * 1024) throw std::invalid_argument("parallel-in-time layer: halo too large for LDS");
+  if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
+  if (hv.nslot == 6)
+    hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi>), dim3(nblk), dim3(256), smem, st,
+                       hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C);
+  else
+    hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi>), dim3(nblk), dim3(256), smem, st,
+                       hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C);
+  if (bracket) QED3_HIP(hipEventRecord(bracket[1], st));
+  if constexpr (Epi::kReduce)
+    hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, nblk, epi);
+}
+
 }  // namespace hk
 }  // namespace qed3

@@ -200,6 +200,72 @@ __global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict
 
 }  // namespace
 
+namespace {
+// FP64 field (original gauge) -> binary16 in the temporal gauge, and back
+__global__ __launch_bounds__(256) void k_to_tg16(const Spin<double>* __restrict__ in, const phase16_t* rot,
+                                                 SpinH* out, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const Spin<double> d = in[i];
+    Spin<float> v;
+    v.a = {float(d.a.x), float(d.a.y)};
+    v.b = {float(d.b.x), float(d.b.y)};
+    cgepi::rotate(v, rot, i, false);
+    out[i] = to_half(v);
+  }
+}
+__global__ __launch_bounds__(256) void k_from_tg16(const SpinH* __restrict__ in, const phase16_t* rot,
+                                                   Spin<double>* out, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    Spin<float> v = from_half(in[i]);
+    cgepi::rotate(v, rot, i, true);
+    Spin<double> d;
+    d.a = {double(v.a.x), double(v.a.y)};
+    d.b = {double(v.b.x), double(v.b.y)};
+    out[i] = d;
+  }
+}
+}  // namespace
+
+void horner_apply_solver(Horner& K, bool adj, Spin<double>* out, const Spin<double>* in) {
+  Dirac& D = K.dirac();
+  const size_t n = D.points();
+  hipStream_t st = D.stream();
+  const size_t hb = n * sizeof(SpinH);
+  (void)D.view_half_temporal_gauge();
+  const DiracView<float, phasepair_t> vpairs = D.view_half_tg_pairs();
+  const phase16_t* rot = D.half_gauge_rotation();
+  const HornerView hv = K.view();
+  const HaloView halo = D.halo(hk::tc_sites(hv.Lt));
+  SpinH* V16 = static_cast<SpinH*>(K.work(6, hb));
+  SpinH* G[2] = {static_cast<SpinH*>(K.work(7, hb)), static_cast<SpinH*>(K.work(8, hb))};
+  const int nbk = int(stream_blocks(n));
+  hipLaunchKernelGGL(k_to_tg16, dim3(nbk), dim3(256), 0, st, in, rot, V16, n);
+  const hk::SrcF<SpinH> src{V16};
+  const hk::EpiNone none{};
+  const int L = K.layers();
+  const SpinH* prev = nullptr;
+  for (int l = 0; l < L; l++) {
+    const std::complex<double> c = K.params().c[L - 1 - l];
+    const float2 cj = {float(c.real()), float(adj ? -c.imag() : c.imag())};
+    SpinH* dst = G[l & 1];
+    if (adj) {
+      if (l == 0)
+        hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+      else
+        hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+    } else {
+      if (l == 0)
+        hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+      else
+        hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+    }
+    prev = dst;
+  }
+  hipLaunchKernelGGL(k_from_tg16, dim3(nbk), dim3(256), 0, st, prev, rot, out, n);
+  QED3_HIP(hipGetLastError());
+  QED3_HIP(hipStreamSynchronize(st));
+}
+
 CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOptions& o,
                    KernelTimes* layer_times) {
   Dirac& D = K.dirac();
@@ -237,8 +303,8 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   const int nn = D.tables().max_nn;
   const HaloView hvf = D.halo_for<float>(false), hvt = D.halo_for<float>(true);
   const HaloView hv64f = D.halo_for<double>(false), hv64t = D.halo_for<double>(true);
-  const HaloView halo = D.halo(256);
   const HornerView hv = K.view();
+  const HaloView halo = D.halo(hk::tc_sites(hv.Lt));
   const int L = K.layers();
   const double inv_ncomp = 1.0 / (4.0 * double(n));
   PendRing pend;
@@ -260,9 +326,9 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
       SpinH* out = l == L - 1 ? H : G[l & 1];
       const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 1, k);
       if (l == 0)
-        hk::launch_horner_layer<true, true, true>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
       else
-        hk::launch_horner_layer<true, false, true>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
       prev = out;
     }
     prev = nullptr;
@@ -273,13 +339,13 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
       SpinH* out = last ? Z : G[l & 1];
       const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 2, k);
       if (l == 0 && last)
-        hk::launch_horner_layer<false, true, true>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
+        hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
       else if (l == 0)
-        hk::launch_horner_layer<false, true, true>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
       else if (last)
-        hk::launch_horner_layer<false, false, true>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
+        hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
       else
-        hk::launch_horner_layer<false, false, true>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
+        hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
       prev = out;
     }
   };

@@ -43,6 +43,27 @@ def test_precond_apply_matches_oracle(gpu, lattice, n, Lt, bc, layers):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lattice,n,Lt,bc,layers", [(1, 2, 4, 0, 3), (0, 2, 16, 0, 4), (1, 2, 20, 1, 4),
+                                                    (0, 2, 48, 0, 2), (1, 2, 64, 0, 4), (1, 3, 1, 0, 3),
+                                                    (0, 4, 33, 0, 3)])
+def test_solver_kernels_match_oracle(gpu, lattice, n, Lt, bc, layers):
+    """The solver's parallel-in-time layers (binary16 fields in the temporal gauge, time
+    chunks of 16 slices per lane with carries between lanes) apply K and K^dagger as the
+    oracle does, to binary16 accuracy: every chunking case (one lane, a partial last lane,
+    an empty padding lane, four lanes) and the 2D operator."""
+    print(f"case lattice={lattice} n={n} Lt={Lt} bc={bc} layers={layers}", flush=True)
+    Do, Dq = _pair(lattice, n, Lt, 0.05, bc=bc)
+    K = q.Precond(Dq, layers=layers)
+    v = source(lattice, n, Lt, seed=9)
+    for p in (np.ones(layers + 4, complex), _learned(layers, Lt)):
+        K.params = p
+        for adj in (False, True):
+            ref = O.horner_apply(Do, layers, p, v, adjoint=adj)
+            got = K.apply_solver(v, adjoint=adj)
+            assert rel(got, ref) < 5e-3, (adj, rel(got, ref))
+
+
+@pytest.mark.gpu
 def test_precond_rejects_r_not_one(gpu):
     L, Do, at = oracle_setup(0, 2, 4, M5=0.1, r=0.5)
     Dq = q.WilsonDirac(0, 2, 4, r=0.5, M5=0.1, at=at)
