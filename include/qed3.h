@@ -251,7 +251,8 @@ int qed3_precond_apply(qed3_precond* pc, int op, void* d_out, const void* d_in);
 /* The same with the solver's own kernels (no counterpart in the reference; for checking
  * them): the input rounded to binary16 in the solver's temporal gauge, the
  * parallel-in-time layers (binary16 fields and transports, FP32 recurrences), the output
- * rotated back to the caller's gauge as complex double. */
+ * rotated back to the caller's gauge as complex double.  op = QED3_OP_DDDAG applies the
+ * solver's M = K K^dagger (K's first layer fused into K^dagger's last). */
 int qed3_precond_apply_solver(qed3_precond* pc, int op, void* d_out, const void* d_in);
 /* loss = mean_b |K v_b - e_b|^2 / |e_b|^2 over `batch` consecutive device field pairs and
  * its gradient d loss / d Re p + i d loss / d Im p (2 * nparams doubles), forward and

@@ -463,12 +463,14 @@ class Precond:
     def apply_device(self, d_out, d_in, adjoint=False):
         check(load().qed3_precond_apply(self._h, OP_DDAG if adjoint else OP_D, d_out.ptr, d_in.ptr))
 
-    def apply_solver(self, v, adjoint=False):
-        """K v (or K^dagger v) by the solver's kernels (binary16, temporal gauge)."""
+    def apply_solver(self, v, adjoint=False, both=False):
+        """K v (K^dagger v; with both, K K^dagger v) by the solver's kernels (binary16,
+        temporal gauge)."""
         v = np.ascontiguousarray(v, np.complex128)
         d_in = DeviceArray.from_numpy(v)
         d_out = DeviceArray(v.nbytes)
-        check(load().qed3_precond_apply_solver(self._h, OP_DDAG if adjoint else OP_D, d_out.ptr, d_in.ptr))
+        op = OP_DDDAG if both else (OP_DDAG if adjoint else OP_D)
+        check(load().qed3_precond_apply_solver(self._h, op, d_out.ptr, d_in.ptr))
         return d_out.to_numpy(np.complex128)
 
     def apply(self, v, adjoint=False):

@@ -573,12 +573,13 @@ int qed3_precond_apply(qed3_precond* pc, int op, void* d_out, const void* d_in) 
 
 int qed3_precond_apply_solver(qed3_precond* pc, int op, void* d_out, const void* d_in) {
   if (!pc || !d_out || !d_in) return fail(QED3_ERR_ARG, "null argument");
-  if (op != QED3_OP_D && op != QED3_OP_DDAG) return fail(QED3_ERR_ARG, "op must be QED3_OP_D or QED3_OP_DDAG");
+  if (op != QED3_OP_D && op != QED3_OP_DDAG && op != QED3_OP_DDDAG)
+    return fail(QED3_ERR_ARG, "op must be QED3_OP_D, QED3_OP_DDAG or QED3_OP_DDDAG");
   if (d_out == d_in) return fail(QED3_ERR_ARG, "in-place apply is not supported");
   if (!pc->ctx->D->has_gauge()) return fail(QED3_ERR_STATE, "no gauge field set");
   return guarded([&]() -> int {
-    qed3::horner_apply_solver(*pc->K, op == QED3_OP_DDAG, static_cast<qed3::Spin<double>*>(d_out),
-                              static_cast<const qed3::Spin<double>*>(d_in));
+    qed3::horner_apply_solver(*pc->K, op == QED3_OP_DDAG ? 1 : (op == QED3_OP_DDDAG ? 2 : 0),
+                              static_cast<qed3::Spin<double>*>(d_out), static_cast<const qed3::Spin<double>*>(d_in));
     return QED3_OK;
   });
 }

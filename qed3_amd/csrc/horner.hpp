@@ -117,9 +117,10 @@ class Horner {
 CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOptions& o,
                    KernelTimes* layer_times = nullptr);
 
-// out = K in (adj = false) or K^dagger in with the solver's kernels: the input rounded to
+// out = K in (which = 0), K^dagger in (1) or K K^dagger in (2, the solver's sequence with K's
+// first layer fused into K^dagger's last) with the solver's kernels: the input rounded to
 // binary16 in the temporal gauge, the parallel-in-time layers, the output rotated back to
 // the original gauge in FP64 (for checking those kernels).
-void horner_apply_solver(Horner& K, bool adj, Spin<double>* out, const Spin<double>* in);
+void horner_apply_solver(Horner& K, int which, Spin<double>* out, const Spin<double>* in);
 
 }  // namespace qed3

@@ -340,22 +340,30 @@ inline void launch_horner_layer(const HornerView& hv, const DiracView<float, P>&
 // backward row from its end), with carry_k = E_{k-1} + a^T carry_{k-1} (forward) and
 // carry_k = E_{k+1} + a^{len_{k+1}} carry_{k+1} (backward), E the lanes' end values.  So a
 // layer reads its input and injection once and writes its output once, and a site's time
-// column runs on C = ceil(Lt / T) threads (C * S = 256, S sites per block; the threads of a
+// column runs on C = ceil(Lt / T) threads (C * S = kTcBlock, S sites per block; the threads of a
 // lane are consecutive, so a wave reads and writes consecutive sites of one slice).  Per step the block stages the C slices its lanes are at, for its S sites and
 // their halo (halo tables of ranges of S sites).
-template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi>
-__global__ __launch_bounds__(256, (NS == 3 ? 4 : 2)) void k_horner_layer_tc(HornerView hv, DiracView<float, phasepair_t> d,
+constexpr int kTcChunk = 16;   // slices per lane
+constexpr int kTcBlock = 256;  // threads per block (S sites x C lanes)
+
+// FUSE (K^dagger's last layer only): the kernel then also applies the first layer of K to
+// the output it has just stored, from the same registers: gout2 = T^{-1}(cj2 gout) with
+// gout as stored (binary16), which saves K's first launch and its read of gout.
+template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi, bool FUSE = false>
+__global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc(HornerView hv, DiracView<float, phasepair_t> d,
                                                          HaloView halo, Src src, const SpinH* __restrict__ gin,
                                                          SpinH* __restrict__ gout, float2 cj, TileMap tm, Epi epi,
-                                                         GridReduce gr, int C) {
+                                                         GridReduce gr, int C, SpinH* __restrict__ gout2,
+                                                         float2 cj2) {
+  static_assert(!FUSE || ADJ, "K's first layer fuses into a layer of K^dagger");
   if (epi.skip()) return;  // uniform
   extern __shared__ float4 hk_tile_raw[];
   const int S = int(blockDim.x) / C;
   const int lane = int(threadIdx.x) / S, sl = int(threadIdx.x) % S;  // a wave: consecutive sites, one chunk
   SpinH* tile = reinterpret_cast<SpinH*>(hk_tile_raw);         // [2][C][S + hmax]
   const int row = S + halo.hmax;
-  float2* carry = reinterpret_cast<float2*>(tile + (FIRST ? 0 : 2 * C * row));  // [2][256]
-  HopRow16* hqs = reinterpret_cast<HopRow16*>(carry + 2 * 256);  // [S][2 rows][NS], the sites' hop rows
+  float2* carry = reinterpret_cast<float2*>(tile + (FIRST ? 0 : 2 * C * row));  // [2][kTcBlock]
+  HopRow16* hqs = reinterpret_cast<HopRow16*>(carry + 2 * kTcBlock);  // [S][2 rows][NS], the sites' hop rows
   int xr, st_;
   tile_of(tm, blockIdx.x, xr, st_);
   (void)st_;
@@ -545,14 +553,72 @@ __global__ __launch_bounds__(256, (NS == 3 ? 4 : 2)) void k_horner_layer_tc(Horn
         put(gout + i, b, zF[j]);
         epi(i, stored(gout, b, zF[j]), acc);
       }
+      if constexpr (FUSE) zB[j] = half2_t{(_Float16)b.x, (_Float16)b.y};  // row rB as stored
     }
   }
   if constexpr (Epi::kReduce) store_partial(acc, gr.partials, blockIdx.x);
+  if constexpr (FUSE) {
+    // K's first layer on the stored output (rows: ADJ's forward row rF = 1 is K's backward
+    // row, ADJ's rB = 0 K's forward row): forward over cj2 out_0, backward over cj2 out_1
+    float2 p0 = {0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < T; j++) {
+      if (j < len) {
+        const float2 h0 = {float(zB[j].x), float(zB[j].y)};
+        const float2 h1 = {float((_Float16)zF[j].x), float((_Float16)zF[j].y)};
+        const float2 y0 = cmul<float>(cj2, h0), y1 = cmul<float>(cj2, h1);
+        p0 = {a * p0.x + invc * y0.x, a * p0.y + invc * y0.y};
+        zF[j] = p0;
+        zB[j] = half2_t{(_Float16)y1.x, (_Float16)y1.y};
+      }
+    }
+    float2 p1 = {0.f, 0.f};
+#pragma unroll
+    for (int j = T - 1; j >= 0; j--) {
+      if (j < len) {
+        p1 = {a * p1.x + invc * float(zB[j].x), a * p1.y + invc * float(zB[j].y)};
+        zB[j] = half2_t{(_Float16)p1.x, (_Float16)p1.y};
+      }
+    }
+    __syncthreads();  // every lane has read the first exchange
+    carry[threadIdx.x] = p0;
+    carry[blockDim.x + threadIdx.x] = p1;
+    __syncthreads();
+    float2 c0 = {0.f, 0.f}, c1 = {0.f, 0.f};
+    for (int c = 0; c < lane; c++) {
+      const float2 e = carry[c * S + sl];
+      c0 = {aT * c0.x + e.x, aT * c0.y + e.y};
+    }
+    for (int c = C - 1; c > lane; c--) {
+      const int lc = max(0, min(T, Lt - c * T));
+      float ac = 1.f;
+      for (int q = 0; q < lc; q++) ac *= a;
+      const float2 e = carry[blockDim.x + c * S + sl];
+      c1 = {ac * c1.x + e.x, ac * c1.y + e.y};
+    }
+    float q0 = a;
+#pragma unroll
+    for (int j = 0; j < T; j++) {
+      if (j < len) {
+        zF[j].x += q0 * c0.x;
+        zF[j].y += q0 * c0.y;
+        q0 *= a;
+      }
+    }
+    float q1 = a;
+#pragma unroll
+    for (int j = T - 1; j >= 0; j--) {
+      if (j < len) {
+        const float2 b = {float(zB[j].x) + q1 * c1.x, float(zB[j].y) + q1 * c1.y};
+        q1 *= a;
+        put(gout2 + size_t(s0 + j) * V + x, zF[j], b);
+      }
+    }
+  }
 }
 
 // Lanes per site of the parallel-in-time layer: C = ceil(Lt / T) rounded up to a power of
-// two (C * S = 256 threads); at most 64.
-constexpr int kTcChunk = 16;
+// two (C * S = kTcBlock threads); at most 64.
 inline int tc_lanes(int Lt) {
   const int c = (Lt + kTcChunk - 1) / kTcChunk;
   int p = 1;
@@ -560,13 +626,17 @@ inline int tc_lanes(int Lt) {
   if (p > 64) throw std::invalid_argument("time extent too long for the parallel-in-time layer");
   return p;
 }
-inline int tc_sites(int Lt) { return 256 / tc_lanes(Lt); }
+inline int tc_sites(int Lt) { return kTcBlock / tc_lanes(Lt); }
 
+// gout2 != nullptr (ADJ only): also K's first layer with coefficient cj2 into gout2 (FUSE).
 template <bool ADJ, bool FIRST, class Src, class Epi>
 inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, phasepair_t>& tables,
                                    const HaloView& halo, const Src& src, const SpinH* gin, SpinH* gout, float2 cj,
-                                   const Epi& epi, GridReduce gr, hipStream_t st, const hipEvent_t* bracket = nullptr) {
-  const int C = tc_lanes(hv.Lt), S = 256 / C;
+                                   const Epi& epi, GridReduce gr, hipStream_t st, const hipEvent_t* bracket = nullptr,
+                                   SpinH* gout2 = nullptr, float2 cj2 = {0.f, 0.f}) {
+  if (gout2 && !ADJ) throw std::logic_error("only a K^dagger layer carries K's first layer");
+  if (gout2 && (gout2 == gout || gout2 == gin)) throw std::logic_error("fused output aliases an operand");
+  const int C = tc_lanes(hv.Lt), S = kTcBlock / C;
   if (!tables.uPoly) throw std::logic_error("parallel-in-time layer needs the temporal gauge");
   if (!FIRST && !halo.nbl) throw std::logic_error("parallel-in-time layer launched without halo tables");
   TileMap tm;
@@ -577,16 +647,23 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
   const unsigned nblk = unsigned(tm.nxr_pad);
   if (Epi::kReduce && nblk > kReduceCapacity) throw std::invalid_argument("layer grid exceeds the reduction scratch");
   const size_t smem =
-      (FIRST ? 0 : size_t(2) * C * (S + halo.hmax) * sizeof(SpinH)) + size_t(2) * 256 * sizeof(float2) +
+      (FIRST ? 0 : size_t(2) * C * (S + halo.hmax) * sizeof(SpinH)) + size_t(2) * kTcBlock * sizeof(float2) +
       (FIRST ? 0 : size_t(S) * 2 * hv.nslot * sizeof(HopRow16));
   if (smem > 64 * 1024) throw std::invalid_argument("parallel-in-time layer: halo too large for LDS");
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
-  if (hv.nslot == 6)
-    hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi>), dim3(nblk), dim3(256), smem, st,
-                       hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C);
+  auto go = [&](auto fuse) {
+    constexpr bool F = decltype(fuse)::value && ADJ;
+    if (hv.nslot == 6)
+      hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F>), dim3(nblk), dim3(kTcBlock), smem,
+                         st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+    else
+      hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F>), dim3(nblk), dim3(kTcBlock), smem,
+                         st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+  };
+  if (gout2)
+    go(std::true_type{});
   else
-    hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi>), dim3(nblk), dim3(256), smem, st,
-                       hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C);
+    go(std::false_type{});
   if (bracket) QED3_HIP(hipEventRecord(bracket[1], st));
   if constexpr (Epi::kReduce)
     hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, nblk, epi);

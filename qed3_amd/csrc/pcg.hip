@@ -226,7 +226,54 @@ __global__ __launch_bounds__(256) void k_from_tg16(const SpinH* __restrict__ in,
 }
 }  // namespace
 
-void horner_apply_solver(Horner& K, bool adj, Spin<double>* out, const Spin<double>* in) {
+namespace {
+// M v = K (K^dagger v) by the parallel-in-time layers: K^dagger's layers inject src and
+// end in H; its last layer also runs K's first layer (L >= 2) into a G slot; K's layers
+// inject H and end in Z, the last one with epi_last.  G[2] alternate as the layers' outputs.
+template <class Src, class Epi, class EpiLast>
+void precond_layers(Horner& K, const HornerView& hv, const DiracView<float, phasepair_t>& vpairs, const HaloView& halo,
+                    const Src& src, SpinH* H, SpinH* const* G, SpinH* Z, const Epi& epi, const EpiLast& epi_last,
+                    GridReduce gr, hipStream_t st, EventTimer* timer, int k) {
+  const int L = K.layers();
+  const hk::SrcF<SpinH> src_h{H};
+  auto ev = [&](int cls) { return timer ? timer->pair(cls, k) : nullptr; };
+  const bool fuse = L >= 2;
+  const std::complex<double> c0 = K.params().c[L - 1];
+  const float2 ck0 = {float(c0.real()), float(c0.imag())};  // K's first layer
+  const SpinH* prev = nullptr;
+  for (int l = 0; l < L; l++) {
+    const std::complex<double> c = K.params().c[L - 1 - l];
+    const float2 cj = {float(c.real()), float(-c.imag())};
+    const bool last = l == L - 1;
+    SpinH* out = last ? H : G[l & 1];
+    SpinH* out2 = (last && fuse) ? G[(L - 1) & 1] : nullptr;
+    if (l == 0)
+      hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src, prev, out, cj, epi, gr, st, ev(0), out2, ck0);
+    else
+      hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src, prev, out, cj, epi, gr, st, ev(1), out2, ck0);
+    prev = out;
+  }
+  prev = fuse ? G[(L - 1) & 1] : nullptr;
+  for (int l = fuse ? 1 : 0; l < L; l++) {
+    const std::complex<double> c = K.params().c[L - 1 - l];
+    const float2 cj = {float(c.real()), float(c.imag())};
+    const bool last = l == L - 1;
+    SpinH* out = last ? Z : G[(l + L - 1) & 1];
+    if (l == 0 && last)
+      hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(0));
+    else if (l == 0)
+      hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, epi, gr, st, ev(0));
+    else if (last)
+      hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(2));
+    else
+      hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, epi, gr, st, ev(2));
+    prev = out;
+  }
+}
+}  // namespace
+
+void horner_apply_solver(Horner& K, int which, Spin<double>* out, const Spin<double>* in) {
+  if (which < 0 || which > 2) throw std::invalid_argument("which: 0 = K, 1 = K^dagger, 2 = K K^dagger");
   Dirac& D = K.dirac();
   const size_t n = D.points();
   hipStream_t st = D.stream();
@@ -244,22 +291,30 @@ void horner_apply_solver(Horner& K, bool adj, Spin<double>* out, const Spin<doub
   const hk::EpiNone none{};
   const int L = K.layers();
   const SpinH* prev = nullptr;
-  for (int l = 0; l < L; l++) {
-    const std::complex<double> c = K.params().c[L - 1 - l];
-    const float2 cj = {float(c.real()), float(adj ? -c.imag() : c.imag())};
-    SpinH* dst = G[l & 1];
-    if (adj) {
-      if (l == 0)
-        hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
-      else
-        hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
-    } else {
-      if (l == 0)
-        hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
-      else
-        hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+  if (which == 2) {  // the solver's own sequence, K's first layer fused into K^dagger's last
+    SpinH* H = static_cast<SpinH*>(K.work(9, hb));
+    SpinH* Z = static_cast<SpinH*>(K.work(10, hb));
+    precond_layers(K, hv, vpairs, halo, src, H, G, Z, none, none, D.reducer(), st, nullptr, 0);
+    prev = Z;
+  } else {
+    const bool adj = which == 1;
+    for (int l = 0; l < L; l++) {
+      const std::complex<double> c = K.params().c[L - 1 - l];
+      const float2 cj = {float(c.real()), float(adj ? -c.imag() : c.imag())};
+      SpinH* dst = G[l & 1];
+      if (adj) {
+        if (l == 0)
+          hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+        else
+          hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+      } else {
+        if (l == 0)
+          hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+        else
+          hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src, prev, dst, cj, none, D.reducer(), st);
+      }
+      prev = dst;
     }
-    prev = dst;
   }
   hipLaunchKernelGGL(k_from_tg16, dim3(nbk), dim3(256), 0, st, prev, rot, out, n);
   QED3_HIP(hipGetLastError());
@@ -319,35 +374,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
 
   // one network application M r = K (K^dagger r): K^dagger's output in H, K's in Z
   auto precond = [&](int k) {
-    const SpinH* prev = nullptr;
-    for (int l = 0; l < L; l++) {
-      const std::complex<double> c = K.params().c[L - 1 - l];
-      const float2 cj = {float(c.real()), float(-c.imag())};
-      SpinH* out = l == L - 1 ? H : G[l & 1];
-      const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 1, k);
-      if (l == 0)
-        hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
-      else
-        hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src_r, prev, out, cj, eskip, gr, st, ev);
-      prev = out;
-    }
-    prev = nullptr;
-    for (int l = 0; l < L; l++) {
-      const std::complex<double> c = K.params().c[L - 1 - l];
-      const float2 cj = {float(c.real()), float(c.imag())};
-      const bool last = l == L - 1;
-      SpinH* out = last ? Z : G[l & 1];
-      const hipEvent_t* ev = ltimer.pair(l == 0 ? 0 : 2, k);
-      if (l == 0 && last)
-        hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
-      else if (l == 0)
-        hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
-      else if (last)
-        hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, edots, gr, st, ev);
-      else
-        hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, eskip, gr, st, ev);
-      prev = out;
-    }
+    precond_layers(K, hv, vpairs, halo, src_r, H, G, Z, eskip, edots, gr, st, &ltimer, k);
   };
   auto k2 = [&](auto tag, int k) {
     constexpr int POS = decltype(tag)::value;
@@ -360,7 +387,9 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
     launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
                                     timer.pair(EpiK2F<POS>::kUpdateX ? 2 : 1, k));
   };
-  const bool ru_each = o.times != nullptr || layer_times != nullptr;
+  // The reliable update's kernels follow every iteration and exit at once unless it flagged
+  // one: with an update every three to four iterations (delta = 0.001), waiting for a
+  // chunk's end would leave the rest of the chunk's iterations idle.
   auto reliable_update = [&](int copy_last) {
     hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0, rot);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
@@ -382,12 +411,9 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
       case 2: k2(std::integral_constant<int, 2>{}, k); break;
       default: k2(std::integral_constant<int, 3>{}, k); break;
     }
-    if (ru_each) reliable_update(0);
+    reliable_update(0);
   };
-  if (ru_each)
-    drive_chunks(one_iter, [] {}, o, n, st, sc, hsc);
-  else
-    drive_chunks(one_iter, [&] { reliable_update(1); }, o, n, st, sc, hsc);
+  drive_chunks(one_iter, [] {}, o, n, st, sc, hsc);
   timer.finish(hsc->iter);
   ltimer.finish(hsc->iter);
   hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1, rot);

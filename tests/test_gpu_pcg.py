@@ -61,6 +61,10 @@ def test_solver_kernels_match_oracle(gpu, lattice, n, Lt, bc, layers):
             ref = O.horner_apply(Do, layers, p, v, adjoint=adj)
             got = K.apply_solver(v, adjoint=adj)
             assert rel(got, ref) < 5e-3, (adj, rel(got, ref))
+        # the solver's M = K K^dagger, K's first layer fused into K^dagger's last
+        ref = O.horner_apply(Do, layers, p, O.horner_apply(Do, layers, p, v, adjoint=True))
+        got = K.apply_solver(v, both=True)
+        assert rel(got, ref) < 8e-3, rel(got, ref)
 
 
 @pytest.mark.gpu
