@@ -269,9 +269,10 @@ int qed3_precond_train(qed3_precond* pc, const void* d_v, const void* d_e, int n
 int qed3_pcg(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b, const qed3_cg_opts* opts,
              qed3_cg_report* rep);
 /* The same solve with direct launches and HIP events around its kernels:
- * ms[0] K1, ms[1] K2 with the x term pending, ms[2] K2 updating x, ms[3] a first layer of
- * the network (recurrence only), ms[4] a transport layer of K^dagger, ms[5] a transport
- * layer of K (means per launch over the iterations run) */
+ * ms[0] K1, ms[1] K2 with the x term pending, ms[2] K2 updating x (0 if no group of eight
+ * iterations ran without a reliable update), ms[3] a first layer of the network
+ * (recurrence only), ms[4] a layer of K^dagger (its last also runs K's first layer),
+ * ms[5] a transport layer of K (means per launch over the iterations run) */
 int qed3_pcg_kernel_times(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b,
                           const qed3_cg_opts* opts, qed3_cg_report* rep, double* ms);
 

@@ -171,6 +171,7 @@ struct EpiK1PH {
 // alpha p in one read-modify-write of x.  A reliable update or the end of the solve
 // flushes the pending terms (k_flush_h).
 constexpr int kXGroup = 4;
+constexpr int kMaxPend = 7;  // entries of CGScalars::alpha_pend
 template <int POS>
 struct EpiK2H {
   static constexpr bool kReduce = true;
@@ -276,7 +277,7 @@ struct EpiResidH {
     sc->need_ru = 0;
     sc->n_ru += 1;
     sc->true_r2 = m;
-    for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;  // flushed before this update
+    for (int j = 0; j < kMaxPend; j++) sc->alpha_pend[j] = 0.0;  // flushed before this update
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
       return;
@@ -310,7 +311,7 @@ __device__ __forceinline__ void half_start(CGScalars* sc, double b2, double ncom
   sc->maxr2 = sc->mu;
   sc->true_r2 = b2;
   sc->rfac = 1.0;
-  for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
+  for (int j = 0; j < kMaxPend; j++) sc->alpha_pend[j] = 0.0;
   sc->rescale = 0;
   sc->last_pos = kXGroup - 1;
   sc->gam = 0;

@@ -31,31 +31,38 @@ using namespace cgepi;
 using namespace cgdrive;
 
 
+// The directions of the last kFGroup iterations stay in a ring of buffers and their x terms
+// pending (alpha_pend) until a reliable update flushes them, or until the ring is full, when
+// the group's last K2 adds them: with an update every three or four iterations K2 hardly
+// ever touches x.
+constexpr int kFGroup = 8;
+static_assert(kFGroup - 1 <= kMaxPend, "pending alphas of a group");
+
 // K2: q = D^dag w; r_new = r - alpha q into the other residual buffer, |r_new|^2 and the
-// reliable-update decision; x grouped as in EpiK2H
+// reliable-update decision; at the group's last position x += the group's alpha p terms
 template <int POS>
 struct EpiK2F {
   static constexpr bool kReduce = true;
-  static constexpr bool kUpdateX = POS == kXGroup - 1;
+  static constexpr bool kUpdateX = POS == kFGroup - 1;
   Spin<float>* x;
   SpinH* rbuf[2];
-  const SpinH* ring[kXGroup];
+  const SpinH* ring[kFGroup];
   CGScalars* sc;
   float alpha;
-  float ap[kXGroup - 1];
+  float ap[kFGroup - 1];
   const SpinH* rin;
   SpinH* rout;
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
 #pragma unroll
-    for (int j = 0; j < kXGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
+    for (int j = 0; j < kFGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
     const int par = sc->rpar & 1;
     rin = rbuf[par];
     rout = rbuf[par ^ 1];
   }
-  // positions 0 .. kXGroup-2 load r with the slice's other loads (as EpiK2H)
-  template <int P = POS, std::enable_if_t<P != kXGroup - 1, int> = 0>
+  // r is loaded with the slice's other loads (as EpiK2H) except where x is updated
+  template <int P = POS, std::enable_if_t<P != kFGroup - 1, int> = 0>
   __device__ SpinH prefetch(size_t i) const {
     return rin[i];
   }
@@ -65,8 +72,8 @@ struct EpiK2F {
     if constexpr (kUpdateX) {
       Spin<float> xx = ld(x + i);
 #pragma unroll
-      for (int j = 0; j < kXGroup; j++) {
-        const float a = j == kXGroup - 1 ? alpha : ap[j];
+      for (int j = 0; j < kFGroup; j++) {
+        const float a = j == kFGroup - 1 ? alpha : ap[j];
         if (a != 0.f) {  // uniform
           const Spin<float> pp = ldh(ring[j] + i);
           xx.a.x += a * pp.a.x;
@@ -88,7 +95,7 @@ struct EpiK2F {
   __device__ void operator()(double2 t) const {
     const double m = t.x;
     if constexpr (kUpdateX) {
-      for (int j = 0; j < kXGroup - 1; j++) sc->alpha_pend[j] = 0.0;
+      for (int j = 0; j < kFGroup - 1; j++) sc->alpha_pend[j] = 0.0;
     } else {
       sc->alpha_pend[POS] = sc->alpha;
     }
@@ -106,6 +113,45 @@ struct EpiK2F {
     if (sc->iter >= sc->maxiter && !sc->need_ru) sc->done = 3;
   }
 };
+
+struct PendRing8 {
+  const SpinH* p[kFGroup - 1];
+};
+
+// x64 += scale exp(-i phi) (x_inner + sum_j alpha_pend[j] ring[j]), x_inner = 0 (before a
+// reliable update, or always at the end)
+__global__ __launch_bounds__(256) void k_flush_f(Spin<double>* x64, Spin<float>* xin, PendRing8 ring, size_t n,
+                                                 const CGScalars* sc, int always, const phase16_t* rot) {
+  if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
+  float ap[kFGroup - 1];
+#pragma unroll
+  for (int j = 0; j < kFGroup - 1; j++) ap[j] = float(sc->alpha_pend[j]);
+  const double s = sc->scale;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    Spin<float> a = xin[i];
+#pragma unroll
+    for (int j = 0; j < kFGroup - 1; j++) {
+      if (ap[j] != 0.f) {  // uniform
+        const Spin<float> pp = ldh(ring.p[j] + i);
+        a.a.x += ap[j] * pp.a.x;
+        a.a.y += ap[j] * pp.a.y;
+        a.b.x += ap[j] * pp.b.x;
+        a.b.y += ap[j] * pp.b.y;
+      }
+    }
+    rotate(a, rot, i, true);
+    Spin<double> xx = x64[i];
+    xx.a.x += s * a.a.x;
+    xx.a.y += s * a.a.y;
+    xx.b.x += s * a.b.x;
+    xx.b.y += s * a.b.y;
+    x64[i] = xx;
+    Spin<float> z;
+    z.a = {0.f, 0.f};
+    z.b = {0.f, 0.f};
+    xin[i] = z;
+  }
+}
 
 // the network's layers wait, like K1 and K2, for a pending reliable update
 struct EpiSkip {
@@ -177,25 +223,14 @@ __global__ __launch_bounds__(256) void k_store_b0_f(const Spin<double>* __restri
   }
 }
 
-struct Ring4 {
-  SpinH* p[kXGroup];
-};
-
-// after a reliable update: the current residual buffer = exp(i phi) rfac rt; with
-// copy_last the last direction is copied into the p_old slot of the next group
-__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, SpinH* r0, SpinH* r1,
-                                                   Ring4 ring, int copy_last, size_t n, const CGScalars* sc,
-                                                   const phase16_t* rot) {
+// after a reliable update: the current residual buffer = exp(i phi) rfac rt
+__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, SpinH* r0, SpinH* r1, size_t n,
+                                                   const CGScalars* sc, const phase16_t* rot) {
   if (sc->done != 0 || sc->rescale == 0) return;
   const float f = float(sc->rfac);
   SpinH* r = (sc->rpar & 1) ? r1 : r0;
-  const int last = sc->last_pos;
-  const SpinH* src = (copy_last && last != kXGroup - 1) ? ring.p[last] : nullptr;
-  SpinH* dst = ring.p[kXGroup - 1];
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     half_rescale_point(rt, r, f, i, rot);
-    if (src) dst[i] = src[i];
-  }
 }
 
 }  // namespace
@@ -328,9 +363,12 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   hipStream_t st = D.stream();
   const auto t0 = std::chrono::steady_clock::now();
   const size_t hb = n * sizeof(SpinH), fb = n * sizeof(Spin<float>);
-  // the half-storage CG's slots for the stencil-side fields; the network's own fields in K's
-  SpinH* ring[kXGroup] = {static_cast<SpinH*>(D.scratch(1, hb)), static_cast<SpinH*>(D.scratch(2, hb)),
-                          static_cast<SpinH*>(D.scratch(12, hb)), static_cast<SpinH*>(D.scratch(13, hb))};
+  // the half-storage CG's slots for the stencil-side fields (and half of the ring); the
+  // network's own fields and the rest of the ring in K's
+  SpinH* ring[kFGroup] = {static_cast<SpinH*>(D.scratch(1, hb)),  static_cast<SpinH*>(D.scratch(2, hb)),
+                          static_cast<SpinH*>(D.scratch(12, hb)), static_cast<SpinH*>(D.scratch(13, hb)),
+                          static_cast<SpinH*>(K.work(6, hb)),     static_cast<SpinH*>(K.work(7, hb)),
+                          static_cast<SpinH*>(K.work(8, hb)),     static_cast<SpinH*>(K.work(9, hb))};
   SpinH* w = static_cast<SpinH*>(D.scratch(3, hb));
   Spin<float>* xin = static_cast<Spin<float>*>(D.scratch(4, fb));
   Spin<double>* t = static_cast<Spin<double>*>(D.scratch(5, n * sizeof(Spin<double>)));
@@ -345,7 +383,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   const GridReduce gr = D.reducer();
   const int nbk = int(stream_blocks(n));
   const int ngate = int(gated_blocks(n));
-  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kXGroup - 1], n, gr);
+  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kFGroup - 1], n, gr);
   hipLaunchKernelGGL(k_finalize<FinStartF>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
                      FinStartF{sc, 4.0 * double(n), o.tol * o.tol, o.delta * o.delta, o.maxiter});
   const DiracView<float, phase16_t> vin = D.view_half_temporal_gauge();
@@ -362,10 +400,8 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   const HaloView halo = D.halo(hk::tc_sites(hv.Lt));
   const int L = K.layers();
   const double inv_ncomp = 1.0 / (4.0 * double(n));
-  PendRing pend;
-  for (int j = 0; j < kXGroup - 1; j++) pend.p[j] = ring[j];
-  Ring4 ring4;
-  for (int j = 0; j < kXGroup; j++) ring4.p[j] = ring[j];
+  PendRing8 pend;
+  for (int j = 0; j < kFGroup - 1; j++) pend.p[j] = ring[j];
   EventTimer timer(o.times), ltimer(layer_times);
   const hk::SrcPar src_r{R[0], R[1], &sc->rpar};
   const hk::SrcF<SpinH> src_h{H};
@@ -382,7 +418,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
     e.x = xin;
     e.rbuf[0] = R[0];
     e.rbuf[1] = R[1];
-    for (int j = 0; j < kXGroup; j++) e.ring[j] = ring[j];
+    for (int j = 0; j < kFGroup; j++) e.ring[j] = ring[j];
     e.sc = sc;
     launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
                                     timer.pair(EpiK2F<POS>::kUpdateX ? 2 : 1, k));
@@ -390,33 +426,40 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   // The reliable update's kernels follow every iteration and exit at once unless it flagged
   // one: with an update every three to four iterations (delta = 0.001), waiting for a
   // chunk's end would leave the rest of the chunk's iterations idle.
-  auto reliable_update = [&](int copy_last) {
-    hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0, rot);
+  auto reliable_update = [&] {
+    hipLaunchKernelGGL(k_flush_f, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0, rot);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
     launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr, st,
                                  hv64t);
-    hipLaunchKernelGGL(k_rescale_f, dim3(ngate), dim3(256), 0, st, rt, R[0], R[1], ring4, copy_last, n, sc, rot);
+    hipLaunchKernelGGL(k_rescale_f, dim3(ngate), dim3(256), 0, st, rt, R[0], R[1], n, sc, rot);
   };
   auto one_iter = [&](int i, int k) {
-    const int pos = i % kXGroup;
-    SpinH* pold = ring[(pos + kXGroup - 1) % kXGroup];
+    const int pos = i % kFGroup;
+    SpinH* pold = ring[(pos + kFGroup - 1) % kFGroup];
     SpinH* pnew = ring[pos];
     precond(k);
     launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{Z, pold, sc}, EpiK1PH{w, pnew, sc},
                                      D.tile<float>(false), gr, st, hvf, timer.pair(0, k));
-    static_assert(kXGroup == 4, "one K2 instantiation per group position");
+    static_assert(kFGroup == 8, "one K2 instantiation per group position");
     switch (pos) {
       case 0: k2(std::integral_constant<int, 0>{}, k); break;
       case 1: k2(std::integral_constant<int, 1>{}, k); break;
       case 2: k2(std::integral_constant<int, 2>{}, k); break;
-      default: k2(std::integral_constant<int, 3>{}, k); break;
+      case 3: k2(std::integral_constant<int, 3>{}, k); break;
+      case 4: k2(std::integral_constant<int, 4>{}, k); break;
+      case 5: k2(std::integral_constant<int, 5>{}, k); break;
+      case 6: k2(std::integral_constant<int, 6>{}, k); break;
+      default: k2(std::integral_constant<int, 7>{}, k); break;
     }
-    reliable_update(0);
+    reliable_update();
   };
-  drive_chunks(one_iter, [] {}, o, n, st, sc, hsc);
+  // ring positions run 0 .. kFGroup-1 within every chunk
+  CGOptions oc = o;
+  oc.chunk = ((o.chunk > 0 ? o.chunk : kFGroup) + kFGroup - 1) / kFGroup * kFGroup;
+  drive_chunks(one_iter, [] {}, oc, n, st, sc, hsc);
   timer.finish(hsc->iter);
   ltimer.finish(hsc->iter);
-  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1, rot);
+  hipLaunchKernelGGL(k_flush_f, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1, rot);
   QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   QED3_HIP(hipStreamSynchronize(st));
   CGResult res;
