@@ -30,7 +30,10 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 
-MEASURED_TRAFFIC = {("f16", 1, 128, 64, 256, 64, 1, 1): 4.9646e8,
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), keyed by
+# kernel, lattice, n, Lt (and launch shape for the stencils)
+MEASURED_TRAFFIC = {("layer-tc", 1, 128, 64): 7.873e8,
+                    ("f16", 1, 128, 64, 256, 64, 1, 1): 4.9646e8,
                     ("f32", 1, 128, 64, 256, 64, 1, 1): 1.2992e9,
                     ("K1-half", 1, 128, 64, 256, 64, 1, 1): 8.5079e8,
                     ("K2-half", 1, 128, 64, 256, 64, 1, 1): 6.3056e8,
@@ -61,9 +64,9 @@ def parse():
     p.add_argument("--no-hmc", action="store_true")
     p.add_argument("--no-overlap", action="store_true")
     p.add_argument("--no-batched", action="store_true")
-    p.add_argument("--hmc-traj", type=int, default=12, help="HMC trajectories with the accept step")
+    p.add_argument("--hmc-traj", type=int, default=16, help="HMC trajectories with the accept step")
     p.add_argument("--hmc-gauge-therm", type=int, default=50, help="pure-gauge thermalisation trajectories")
-    p.add_argument("--hmc-therm", type=int, default=10, help="Nf=2 thermalisation trajectories")
+    p.add_argument("--hmc-therm", type=int, default=30, help="Nf=2 thermalisation trajectories")
     p.add_argument("--hmc-inner", type=int, default=50)
     p.add_argument("--hmc-nsteps", type=int, default=80)
     p.add_argument("--hmc-solver", choices=("learned", "half"), default="learned")
@@ -334,26 +337,39 @@ def learned_leg(a, q):
                 "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
 
     ns = str(nslot)
-    lk = {"transport_layer_Kdag": ent(f"k_horner_layer<{ns},true,false,true,SpinH,SrcPar,EpiSkip,phasepair_t>",
-                                      kms[4], b_layer, MEASURED_TRAFFIC.get(("layer", lattice, a.n, a.Lt))),
-          "transport_layer_K": ent(f"k_horner_layer<{ns},false,false,true,SpinH,SrcF,EpiSkip|EpiDots,phasepair_t>",
-                                   kms[5], b_layer),
-          "first_layer": ent(f"k_horner_layer<{ns},*,true,true,SpinH,...> (recurrence only)", kms[3], b_first),
+    pts = D.V * a.Lt
+    tl = "k_horner_layer_tc<" + ns + ",{adj},{first},16,{src},{epi},{fuse}>"
+    lk = {"transport_layer_Kdag": ent(tl.format(adj="true", first="false", src="SrcPar", epi="EpiSkip", fuse="false"),
+                                      kms[4], b_layer, MEASURED_TRAFFIC.get(("layer-tc", lattice, a.n, a.Lt))),
+          "transport_layer_K": ent(tl.format(adj="false", first="false", src="SrcF<SpinH>", epi="EpiSkip",
+                                             fuse="false"), kms[5], b_layer),
+          "first_layer": ent(tl.format(adj="true", first="true", src="SrcPar", epi="EpiSkip", fuse="false") +
+                             " (recurrence only)", kms[3], b_first),
+          # + K's first layer written from the same registers
+          "last_layer_Kdag_with_K_first": ent(tl.format(adj="true", first="false", src="SrcPar", epi="EpiSkip",
+                                                        fuse="true"), kms[6], b_layer + pts * 8),
+          # + r and r_old read for <r,z>, <r_old,z>
+          "last_layer_K_with_dots": ent(tl.format(adj="false", first="false", src="SrcF<SpinH>", epi="EpiDots",
+                                                  fuse="false"), kms[7], b_layer + 2 * pts * 8),
           "K1": ent(f"k_stencil_lds<float,{ns},false,EpiK1PH,SrcPUpdateH,phase16_t> (p = z + beta p_old, w = D p)",
                     kms[0], cgb["K1"]),
-          "K2": ent(f"k_stencil_lds<float,{ns},true,EpiK2F<0..2>,SrcHalf,phase16_t>", kms[1], cgb["K2"]),
-          "K2x": ent(f"k_stencil_lds<float,{ns},true,EpiK2F<3>,SrcHalf,phase16_t>", kms[2], cgb["K2x"]),
+          "K2": ent(f"k_stencil_lds<float,{ns},true,EpiK2F<0..6>,SrcHalf,phase16_t>", kms[1], cgb["K2"]),
           "iterations": rep.iterations, "layers": a.layers, "tile": tile}
-    # launches per iteration: 2 first layers, 2 (L - 1) transport layers, K1, K2
-    per_it = {"first_layer": 2 * kms[3], "transport_layers": (a.layers - 1) * (kms[4] + kms[5]),
-              "K1": kms[0], "K2": (3 * kms[1] + kms[2]) / 4}
+    if kms[2] > 0:
+        lk["K2x"] = ent(f"k_stencil_lds<float,{ns},true,EpiK2F<7>,SrcHalf,phase16_t> (x += the ring of 8 "
+                        "directions)", kms[2], cgb["K2x"])
+    # launches per iteration: K^dagger's first layer, L - 2 transport layers of K^dagger and of
+    # K each, K^dagger's last layer with K's first, K's last layer with the reductions, K1, K2
+    per_it = {"first_layer": kms[3], "transport_layers": (a.layers - 2) * (kms[4] + kms[5]),
+              "last_layers": kms[6] + kms[7], "K1": kms[0], "K2": kms[1]}
     lk["ms_per_iteration_by_kind"] = {k2: round(v, 4) for k2, v in per_it.items()}
     dom = lk["transport_layer_Kdag"]
     roof = {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"],
             "traffic": dom["traffic"],
-            "kernel": dom["kernel"] + " (transport layer of K^dagger: pass 1 stages the layer input once per "
-                                      "slice and forms both spin rows' transports, pass 2 runs the backward "
-                                      "recurrence over each site's column)",
+            "kernel": dom["kernel"] + " (transport layer of K^dagger: lanes of 16 time slices per site hold "
+                                      "their chunk in registers, stage the layer input of their slice and halo "
+                                      "in LDS per step, form both spin rows' transports and recurrences, and "
+                                      "exchange the chunk carries in LDS)",
             "bytes_per_launch": dom["bytes_per_launch"], "ms_per_launch": dom["ms_per_launch"],
             "copy_kernel_GBs": round(copy_bandwidth(q)[0], 1),
             "kernels": lk, "cg_kernels_unpreconditioned": cg_kernel_rooflines(a, q, D, d_x, rhs[-1]),

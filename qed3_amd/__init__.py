@@ -518,10 +518,11 @@ class Precond:
 
     def solve_kernel_times(self, d_x, d_b, tol=1e-10, maxiter=20000, delta=0.01):
         """One solve with HIP-event timing: (report, [K1, K2, K2 with x update, first layer,
-        K^dagger transport layer, K transport layer] ms per launch)."""
+        K^dagger transport layer, K transport layer, K^dagger's last layer fused with K's
+        first, K's last layer with the reductions] ms per launch)."""
         opts = CGOpts(tol, maxiter, CG_HALF, delta, 0)
         rep = CGReport()
-        ms = (C.c_double * 6)()
+        ms = (C.c_double * 8)()
         check(load().qed3_pcg_kernel_times(self.D._h, self._h, d_x.ptr, d_b.ptr, C.byref(opts),
                                            C.byref(rep), ms))
         return rep, list(ms)

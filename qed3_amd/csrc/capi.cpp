@@ -624,7 +624,8 @@ int qed3_pcg_kernel_times(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void
   if (!ms) return fail(QED3_ERR_ARG, "null argument");
   qed3::KernelTimes t, lt;
   const int rc = pcg_entry(ctx, pc, d_x, d_b, opts, rep, &t, &lt);
-  for (int k = 0; k < 3; k++) ms[k] = t.ms[k], ms[3 + k] = lt.ms[k];
+  for (int k = 0; k < 3; k++) ms[k] = t.ms[k];
+  for (int k = 0; k < qed3::KernelTimes::kClasses; k++) ms[3 + k] = lt.ms[k];
   return rc;
 }
 

@@ -33,8 +33,9 @@ namespace qed3 {
 // iterations that leave their x term pending), 2 = K2 with the x update (half-storage
 // solver; = class 1 for the others).
 struct KernelTimes {
-  double ms[3] = {0, 0, 0};
-  int count[3] = {0, 0, 0};
+  static constexpr int kClasses = 5;
+  double ms[kClasses] = {};
+  int count[kClasses] = {};
 };
 
 struct CGOptions {

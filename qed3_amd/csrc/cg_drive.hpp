@@ -8,6 +8,7 @@
 #include <array>
 #include <cmath>
 #include <cstdlib>
+#include <stdexcept>
 #include <utility>
 #include <vector>
 
@@ -41,6 +42,7 @@ class EventTimer {
   }
   const hipEvent_t* pair(int cls, int iter) {
     if (!t_) return nullptr;
+    if (cls < 0 || cls >= KernelTimes::kClasses) throw std::logic_error("bad kernel class");
     ev_.emplace_back();
     auto& e = ev_.back();
     QED3_HIP(hipEventCreate(&e.first[0]));
@@ -57,7 +59,7 @@ class EventTimer {
       t_->ms[e.second.first] += ms;
       t_->count[e.second.first] += 1;
     }
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < KernelTimes::kClasses; k++)
       if (t_->count[k]) t_->ms[k] /= t_->count[k];
   }
 

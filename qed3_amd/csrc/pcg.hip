@@ -275,6 +275,8 @@ void precond_layers(Horner& K, const HornerView& hv, const DiracView<float, phas
   const bool fuse = L >= 2;
   const std::complex<double> c0 = K.params().c[L - 1];
   const float2 ck0 = {float(c0.real()), float(c0.imag())};  // K's first layer
+  // timing classes: 0 a first layer, 1 a transport layer of K^dagger, 2 one of K, 3 K^dagger's
+  // last layer carrying K's first, 4 K's last layer with epi_last
   const SpinH* prev = nullptr;
   for (int l = 0; l < L; l++) {
     const std::complex<double> c = K.params().c[L - 1 - l];
@@ -283,9 +285,11 @@ void precond_layers(Horner& K, const HornerView& hv, const DiracView<float, phas
     SpinH* out = last ? H : G[l & 1];
     SpinH* out2 = (last && fuse) ? G[(L - 1) & 1] : nullptr;
     if (l == 0)
-      hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src, prev, out, cj, epi, gr, st, ev(0), out2, ck0);
+      hk::launch_horner_layer_tc<true, true>(hv, vpairs, halo, src, prev, out, cj, epi, gr, st, ev(out2 ? 3 : 0), out2,
+                                             ck0);
     else
-      hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src, prev, out, cj, epi, gr, st, ev(1), out2, ck0);
+      hk::launch_horner_layer_tc<true, false>(hv, vpairs, halo, src, prev, out, cj, epi, gr, st, ev(out2 ? 3 : 1),
+                                              out2, ck0);
     prev = out;
   }
   prev = fuse ? G[(L - 1) & 1] : nullptr;
@@ -295,11 +299,11 @@ void precond_layers(Horner& K, const HornerView& hv, const DiracView<float, phas
     const bool last = l == L - 1;
     SpinH* out = last ? Z : G[(l + L - 1) & 1];
     if (l == 0 && last)
-      hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(0));
+      hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(4));
     else if (l == 0)
       hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, epi, gr, st, ev(0));
     else if (last)
-      hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(2));
+      hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(4));
     else
       hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, epi, gr, st, ev(2));
     prev = out;
