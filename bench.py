@@ -722,7 +722,10 @@ def batched_leg(a, q):
     {32,64,96,128}, Lt in {16,32,64}, both lattices, m ~ U[0.02, 0.2] (seeded), von
     Mises(1.4) links, one Gaussian RHS each; one batched call against the same systems
     solved one by one with the single-system half-storage CG.  The launch shapes are
-    autotuned once per (lattice, n, Lt) and shared by the systems of that shape."""
+    autotuned once per (lattice, n, Lt) and shared by the systems of that shape.  Links and
+    sources come from numpy's generator (von Mises, complex Gaussian): the per-link stream
+    scheme of the reference's RNG, which the parity tests use, takes minutes to seed for
+    this many points, and nothing here is compared with the oracle."""
     rng = np.random.default_rng(31)
     geos, tuned = {}, {}
     ops, xs, bs, desc = [], [], [], []
@@ -735,7 +738,9 @@ def batched_leg(a, q):
         if n not in geos:
             geos[n] = q.Geometry(n)
         D = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=m, at=0.5 * mean_ell(geos[n], lattice))
-        sp, tp = q.random_gauge(lattice, n, Lt, seed=500 + k, kappa=a.beta)
+        lr = np.random.default_rng(500 + k)
+        sp = lr.vonmises(0.0, a.beta, (Lt, D.E))
+        tp = lr.vonmises(0.0, a.beta, (Lt, D.V))
         D.update(sp, tp)
         key = (lattice, n, Lt)
         if key not in tuned:
@@ -745,7 +750,8 @@ def batched_leg(a, q):
             for (op, prec), t in tuned[key].items():
                 D.set_tile(t["block"], t["ts"], t["remap"], t["lds"], ops=(op,), precs=(prec,))
         ops.append(D)
-        b = q.random_source(lattice, n, Lt, seed=900 + k)
+        sr = np.random.default_rng(900 + k)
+        b = (sr.standard_normal(2 * D.V * Lt) + 1j * sr.standard_normal(2 * D.V * Lt)) / np.sqrt(2.0)
         bs.append(q.DeviceArray.from_numpy(b))
         xs.append(q.DeviceArray(b.nbytes))
         desc.append((lattice, n, Lt, round(m, 4)))

@@ -279,12 +279,14 @@ int qed3_pcg_kernel_times(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void
 
 /* ---- batched CG ----------------------------------------------------------------------
  * Solves D_k^dag D_k x_k = b_k for nsys contexts (any mix of lattices, refinements, Lt and
- * masses, all on the current device) together: every CG phase is one launch over all
- * systems, per-system scalars and convergence masks, iterations captured in a hipGraph;
- * binary16-storage inner iteration (as QED3_SOLVER_CG_HALF) with FP64 reliable updates and
- * the true-residual stop (opts->solver is ignored); time tiles from each context's
- * autotuned FP32 / FP64 shapes, 256-thread LDS-staged blocks.  The
- * reference runs one CG per right-hand side (PseudoFermion::update_eta,
+ * masses, all on the current device) together, binary16-storage inner iteration (as
+ * QED3_SOLVER_CG_HALF) with FP64 reliable updates and the true-residual stop (opts->solver
+ * is ignored), time tiles from each context's autotuned shapes.  Systems large enough to
+ * cover the GPU with one launch (>= 256 blocks of 256) run in rounds: one hipGraph chunk of
+ * iterations of every unfinished system queued on the system's own stream, then one host
+ * poll per system; systems too small for that share launches in which every CG phase runs
+ * over all of them (per-system scalars and convergence masks, iterations captured in a
+ * hipGraph).  The reference runs one CG per right-hand side (PseudoFermion::update_eta,
  * pseudofermion.h:51, looped in integrator.h:156-166).  reps[nsys] may be NULL. */
 int qed3_batch_cg(qed3_ctx* const* ctxs, int nsys, void* const* d_x, const void* const* d_b,
                   const qed3_cg_opts* opts, qed3_cg_report* reps);

@@ -14,7 +14,7 @@ namespace {
 using namespace cgepi;
 
 constexpr int kBlock = 256;
-constexpr long kLargeBlocks = 1024;  // 4 blocks per CU of a 256-CU part
+constexpr long kLargeBlocks = 256;  // one block per CU of a 256-CU part
 
 // One system of the batch: its operator views, LDS halo tables for ranges of kBlock
 // sites, its blocks in the stencil grids and the streaming grids, and its fields.
@@ -218,10 +218,14 @@ std::vector<CGResult> batch_launches(const std::vector<Dirac*>& Ds, const std::v
                                      const std::vector<const Spin<double>*>& bs, const CGOptions& o);
 }  // namespace
 
-// Systems whose own K1 grid fills the GPU (at least kLargeBlocks blocks of 256 at their
-// time tile) are solved one after another by the single-system kernels, which already
-// keep every CU busy and run at lower register pressure than the batched kernels; the
-// others share the batched launches, where several small systems fill the GPU together.
+// Systems whose own K1 grid covers the GPU (at least kLargeBlocks blocks of 256 at their
+// time tile) run the single-system kernels, which run at lower register pressure than the
+// batched kernels and in the temporal gauge, in rounds of one chunk per system on the
+// systems' own streams (cg_solve_half_many: no GPU idle time on host polls, and systems
+// that do not fill the GPU overlap); the others, too small for a launch of their own to
+// occupy the GPU, share the batched launches.  On the 64 systems of n in {32..128},
+// Lt in {16..64} the rounds were 12% faster than solving one system after another, the
+// batched launches 2% slower.
 std::vector<CGResult> batch_cg(const std::vector<Dirac*>& Ds, const std::vector<Spin<double>*>& xs,
                                const std::vector<const Spin<double>*>& bs, const CGOptions& o) {
   const int ns = int(Ds.size());
@@ -240,7 +244,18 @@ std::vector<CGResult> batch_cg(const std::vector<Dirac*>& Ds, const std::vector<
   std::vector<CGResult> out(ns);
   CGOptions os = o;
   os.mixed = os.half = true;
-  for (int k : big) out[k] = cg_solve(*Ds[k], xs[k], bs[k], os);
+  if (!big.empty()) {
+    std::vector<Dirac*> d;
+    std::vector<Spin<double>*> x;
+    std::vector<const Spin<double>*> b;
+    for (int k : big) {
+      d.push_back(Ds[k]);
+      x.push_back(xs[k]);
+      b.push_back(bs[k]);
+    }
+    const std::vector<CGResult> r = cg_solve_half_many(d, x, b, os);
+    for (size_t j = 0; j < big.size(); j++) out[big[j]] = r[j];
+  }
   if (!small.empty()) {
     std::vector<Dirac*> d;
     std::vector<Spin<double>*> x;

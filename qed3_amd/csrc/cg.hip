@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <functional>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -247,101 +249,148 @@ CGResult run_cg(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOption
 // FP32 arithmetic on binary16-stored r, p, w (scaled to unit rms at the start and at
 // every reliable update), FP32 x_inner, FP64 reliable updates.  Same kernel sequence as
 // the mixed solver plus the rescale pass, which only runs after an update.
-CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
-  const size_t n = D.points();
-  hipStream_t st = D.stream();
-  auto t0 = std::chrono::steady_clock::now();
-  const size_t hb = n * sizeof(SpinH), fb = n * sizeof(Spin<float>);
-  SpinH* r = static_cast<SpinH*>(D.scratch(0, hb));
-  // ring of search directions: iteration k writes ring[k % kXGroup] and reads its
-  // predecessor's slot; ring[kXGroup-1] is the zero p_old of the first iteration
-  SpinH* ring[kXGroup] = {static_cast<SpinH*>(D.scratch(1, hb)), static_cast<SpinH*>(D.scratch(2, hb)),
-                          static_cast<SpinH*>(D.scratch(12, hb)), static_cast<SpinH*>(D.scratch(13, hb))};
-  SpinH* w = static_cast<SpinH*>(D.scratch(3, hb));
-  Spin<float>* xin = static_cast<Spin<float>*>(D.scratch(4, fb));
-  Spin<double>* t = static_cast<Spin<double>*>(D.scratch(5, n * sizeof(Spin<double>)));
-  Spin<float>* rt = static_cast<Spin<float>*>(D.scratch(6, fb));
-  CGScalars* sc = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
-  CGScalars* hsc = D.host_scalars();
+// The half-storage solve as an object: the constructor queues the start kernels and prepares
+// the chunk loop (graph capture), loop() runs it, finish() flushes x and reports; so several
+// systems' chunks can be queued on their streams before any host wait.
+class HalfCG {
+ public:
+  HalfCG(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o)
+      : D_(D), x_(x), b_(b), o_(o), n_(D.points()), st_(D.stream()), t0_(std::chrono::steady_clock::now()),
+        timer_(o.times) {
+    const size_t hb = n_ * sizeof(SpinH), fb = n_ * sizeof(Spin<float>);
+    r_ = static_cast<SpinH*>(D.scratch(0, hb));
+    // ring of search directions: iteration k writes ring[k % kXGroup] and reads its
+    // predecessor's slot; ring[kXGroup-1] is the zero p_old of the first iteration
+    ring_[0] = static_cast<SpinH*>(D.scratch(1, hb));
+    ring_[1] = static_cast<SpinH*>(D.scratch(2, hb));
+    ring_[2] = static_cast<SpinH*>(D.scratch(12, hb));
+    ring_[3] = static_cast<SpinH*>(D.scratch(13, hb));
+    w_ = static_cast<SpinH*>(D.scratch(3, hb));
+    xin_ = static_cast<Spin<float>*>(D.scratch(4, fb));
+    t_ = static_cast<Spin<double>*>(D.scratch(5, n_ * sizeof(Spin<double>)));
+    rt_ = static_cast<Spin<float>*>(D.scratch(6, fb));
+    sc_ = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
+    hsc_ = D.host_scalars();
+    gr_ = D.reducer();
+    nbk_ = int(stream_blocks(n_));
+    ngate_ = int(gated_blocks(n_));
+    hipLaunchKernelGGL(k_init_h, dim3(nbk_), dim3(256), 0, st_, b, x, xin_, ring_[kXGroup - 1], n_, gr_);
+    hipLaunchKernelGGL(k_finalize<FinStartH>, dim3(1), dim3(256), 0, st_, gr_.partials, unsigned(nbk_),
+                       FinStartH{sc_, 4.0 * double(n_), o.tol * o.tol, o.delta * o.delta, o.maxiter});
+    // the inner fields live in the temporal gauge (exp(i phi) at every point): b is rotated
+    // into it here, the true residual at every reliable update, and x out of it at each flush
+    vin_ = D.view_half_temporal_gauge();
+    rot_ = D.half_gauge_rotation();
+    hipLaunchKernelGGL(k_store_b0, dim3(nbk_), dim3(256), 0, st_, b, r_, n_, sc_, rot_);
+    QED3_HIP(hipGetLastError());
+    v64_ = D.view<double>();
+    nn_ = D.tables().max_nn;
+    hvf_ = D.halo_for<float>(false);
+    hvt_ = D.halo_for<float>(true);
+    hv64f_ = D.halo_for<double>(false);
+    hv64t_ = D.halo_for<double>(true);
+    inv_ncomp_ = 1.0 / (4.0 * double(n_));
+    for (int j = 0; j < kXGroup - 1; j++) pend_.p[j] = ring_[j];
+    for (int j = 0; j < kXGroup; j++) ring_all_.p[j] = ring_[j];
+    // The reliable update (flush the pending x terms, FP64 r = b - D^dag D x, re-store r at
+    // the new scale); its kernels exit at once unless an update is due.  It runs once per
+    // chunk of iterations: an iteration that asks for it makes the rest of the chunk's
+    // iterations exit at once (EpiK1PH/EpiK2H::skip), and the next chunk starts a group with
+    // the last direction copied into the group's p_old slot.  With kernel timing (direct
+    // launches whose events must each bracket a real iteration) it runs after every
+    // iteration instead.
+    ru_each_ = o.times != nullptr;
+    std::function<void()> tail = [] {};
+    if (!ru_each_) tail = [this] { reliable_update(1); };
+    loop_ = std::make_unique<ChunkLoop>([this](int i, int k) { one_iter(i, k); }, tail, o, n_, st_, sc_, hsc_);
+  }
+  ChunkLoop& loop() { return *loop_; }
+  CGResult finish() {
+    timer_.finish(hsc_->iter);
+    hipLaunchKernelGGL(k_flush_h, dim3(nbk_), dim3(256), 0, st_, x_, xin_, pend_, n_, sc_, 1, rot_);
+    QED3_HIP(hipMemcpyAsync(hsc_, sc_, sizeof(CGScalars), hipMemcpyDeviceToHost, st_));
+    QED3_HIP(hipStreamSynchronize(st_));
+    CGResult res;
+    res.iterations = hsc_->iter;
+    res.reliable_updates = hsc_->n_ru;
+    res.status = hsc_->done == 1 ? 0 : (hsc_->done == 2 ? 4 : 3);
+    res.true_resid = final_residual(D_, x_, b_, *hsc_);
+    res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+    if (res.status == 0 && !(res.true_resid <= o_.tol * 1.0000001)) res.status = 3;
+    return res;
+  }
 
-  const GridReduce gr = D.reducer();
-  const int nbk = int(stream_blocks(n));
-  const int ngate = int(gated_blocks(n));
-  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kXGroup - 1], n, gr);
-  hipLaunchKernelGGL(k_finalize<FinStartH>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
-                     FinStartH{sc, 4.0 * double(n), o.tol * o.tol, o.delta * o.delta, o.maxiter});
-  // the inner fields live in the temporal gauge (exp(i phi) at every point): b is rotated
-  // into it here, the true residual at every reliable update, and x out of it at each flush
-  const DiracView<float, phase16_t> vin = D.view_half_temporal_gauge();
-  const phase16_t* rot = D.half_gauge_rotation();
-  hipLaunchKernelGGL(k_store_b0, dim3(nbk), dim3(256), 0, st, b, r, n, sc, rot);
-  QED3_HIP(hipGetLastError());
-
-  const DiracView<double> v64 = D.view<double>();
-  const int nn = D.tables().max_nn;
-  const HaloView hvf = D.halo_for<float>(false), hvt = D.halo_for<float>(true);
-  const HaloView hv64f = D.halo_for<double>(false), hv64t = D.halo_for<double>(true);
-  const double inv_ncomp = 1.0 / (4.0 * double(n));
-  PendRing pend;
-  for (int j = 0; j < kXGroup - 1; j++) pend.p[j] = ring[j];
-  EventTimer timer(o.times);
-  auto k2 = [&](auto tag, int k) {
-    constexpr int POS = decltype(tag)::value;
+ private:
+  template <int POS>
+  void k2(int k) {
     EpiK2H<POS> e{};
-    e.x = xin;
-    e.r = r;
-    for (int j = 0; j < kXGroup; j++) e.ring[j] = ring[j];
-    e.sc = sc;
-    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
-                                    timer.pair(EpiK2H<POS>::kUpdateX ? 2 : 1, k));
-  };
-  // The reliable update (flush the pending x terms, FP64 r = b - D^dag D x, re-store r at the
-  // new scale); its kernels exit at once unless an update is due.  It runs once per chunk of
-  // iterations: an iteration that asks for it makes the rest of the chunk's iterations exit
-  // at once (EpiK1PH/EpiK2H::skip), and the next chunk starts a group with the last
-  // direction copied into the group's p_old slot.  With kernel timing (direct launches whose
-  // events must each bracket a real iteration) it runs after every iteration instead.
-  const bool ru_each = o.times != nullptr;
-  RingAll ring_all;
-  for (int j = 0; j < kXGroup; j++) ring_all.p[j] = ring[j];
-  auto reliable_update = [&](int copy_last) {
-    hipLaunchKernelGGL(k_flush_h, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0, rot);
-    launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
-    launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr,
-                                 st, hv64t);
-    hipLaunchKernelGGL(k_rescale, dim3(ngate), dim3(256), 0, st, rt, r, ring_all, copy_last, n, sc, rot);
-  };
-  auto one_iter = [&](int i, int k) {
+    e.x = xin_;
+    e.r = r_;
+    for (int j = 0; j < kXGroup; j++) e.ring[j] = ring_[j];
+    e.sc = sc_;
+    launch_stencil_src<float, true>(vin_, nn_, SrcHalf{w_}, e, D_.tile<float>(true), gr_, st_, hvt_,
+                                    timer_.pair(EpiK2H<POS>::kUpdateX ? 2 : 1, k));
+  }
+  void reliable_update(int copy_last) {
+    hipLaunchKernelGGL(k_flush_h, dim3(ngate_), dim3(256), 0, st_, x_, xin_, pend_, n_, sc_, 0, rot_);
+    launch_stencil<double, false>(v64_, nn_, x_, EpiStoreIfRU{t_, sc_}, D_.tile<double>(false), gr_, st_, hv64f_);
+    launch_stencil<double, true>(v64_, nn_, t_, EpiResidH{b_, rt_, sc_, inv_ncomp_, 1.0}, D_.tile<double>(true), gr_,
+                                 st_, hv64t_);
+    hipLaunchKernelGGL(k_rescale, dim3(ngate_), dim3(256), 0, st_, rt_, r_, ring_all_, copy_last, n_, sc_, rot_);
+  }
+  void one_iter(int i, int k) {
     const int pos = i % kXGroup;
-    SpinH* pold = ring[(pos + kXGroup - 1) % kXGroup];
-    SpinH* pnew = ring[pos];
-    launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{r, pold, sc}, EpiK1PH{w, pnew, sc},
-                                     D.tile<float>(false), gr, st, hvf, timer.pair(0, k));
+    SpinH* pold = ring_[(pos + kXGroup - 1) % kXGroup];
+    SpinH* pnew = ring_[pos];
+    launch_stencil_src<float, false>(vin_, nn_, SrcPUpdateH{r_, pold, sc_}, EpiK1PH{w_, pnew, sc_},
+                                     D_.tile<float>(false), gr_, st_, hvf_, timer_.pair(0, k));
     static_assert(kXGroup == 4, "one K2 instantiation per group position");
     switch (pos) {
-      case 0: k2(std::integral_constant<int, 0>{}, k); break;
-      case 1: k2(std::integral_constant<int, 1>{}, k); break;
-      case 2: k2(std::integral_constant<int, 2>{}, k); break;
-      default: k2(std::integral_constant<int, 3>{}, k); break;
+      case 0: k2<0>(k); break;
+      case 1: k2<1>(k); break;
+      case 2: k2<2>(k); break;
+      default: k2<3>(k); break;
     }
-    if (ru_each) reliable_update(0);
-  };
-  if (ru_each)
-    drive_chunks(one_iter, [] {}, o, n, st, sc, hsc);
-  else
-    drive_chunks(one_iter, [&] { reliable_update(1); }, o, n, st, sc, hsc);
-  timer.finish(hsc->iter);
-  hipLaunchKernelGGL(k_flush_h, dim3(nbk), dim3(256), 0, st, x, xin, pend, n, sc, 1, rot);
-  QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
-  QED3_HIP(hipStreamSynchronize(st));
-  CGResult res;
-  res.iterations = hsc->iter;
-  res.reliable_updates = hsc->n_ru;
-  res.status = hsc->done == 1 ? 0 : (hsc->done == 2 ? 4 : 3);
-  res.true_resid = final_residual(D, x, b, *hsc);
-  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (res.status == 0 && !(res.true_resid <= o.tol * 1.0000001)) res.status = 3;
-  return res;
+    if (ru_each_) reliable_update(0);
+  }
+
+  Dirac& D_;
+  Spin<double>* x_;
+  const Spin<double>* b_;
+  CGOptions o_;
+  size_t n_;
+  hipStream_t st_;
+  std::chrono::steady_clock::time_point t0_;
+  EventTimer timer_;
+  SpinH* r_ = nullptr;
+  SpinH* ring_[kXGroup] = {};
+  SpinH* w_ = nullptr;
+  Spin<float>* xin_ = nullptr;
+  Spin<double>* t_ = nullptr;
+  Spin<float>* rt_ = nullptr;
+  CGScalars* sc_ = nullptr;
+  CGScalars* hsc_ = nullptr;
+  GridReduce gr_{};
+  int nbk_ = 0, ngate_ = 0, nn_ = 0;
+  DiracView<float, phase16_t> vin_{};
+  const phase16_t* rot_ = nullptr;
+  DiracView<double> v64_{};
+  HaloView hvf_{}, hvt_{}, hv64f_{}, hv64t_{};
+  double inv_ncomp_ = 0;
+  PendRing pend_{};
+  RingAll ring_all_{};
+  bool ru_each_ = false;
+  std::unique_ptr<ChunkLoop> loop_;
+};
+
+CGResult run_cg_half(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o) {
+  HalfCG s(D, x, b, o);
+  ChunkLoop& loop = s.loop();
+  do {
+    loop.launch();
+    QED3_HIP(hipStreamSynchronize(loop.stream()));
+  } while (!loop.finished());
+  return s.finish();
 }
 
 }  // namespace
@@ -536,6 +585,30 @@ CGResult cg_solve(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOpti
   if (o.half) return run_cg_half(D, x, b, o);
   if (o.mixed) return run_cg<float>(D, x, b, o);
   return run_cg<double>(D, x, b, o);
+}
+
+std::vector<CGResult> cg_solve_half_many(const std::vector<Dirac*>& Ds, const std::vector<Spin<double>*>& xs,
+                                         const std::vector<const Spin<double>*>& bs, const CGOptions& o) {
+  const size_t ns = Ds.size();
+  if (xs.size() != ns || bs.size() != ns) throw std::invalid_argument("systems: sizes differ");
+  std::vector<std::unique_ptr<HalfCG>> solves;
+  solves.reserve(ns);
+  for (size_t k = 0; k < ns; k++) solves.push_back(std::make_unique<HalfCG>(*Ds[k], xs[k], bs[k], o));
+  // rounds: one chunk of every unfinished system queued on its own stream, then the polls
+  std::vector<size_t> active(ns);
+  for (size_t k = 0; k < ns; k++) active[k] = k;
+  while (!active.empty()) {
+    for (size_t k : active) solves[k]->loop().launch();
+    std::vector<size_t> next;
+    for (size_t k : active) {
+      QED3_HIP(hipStreamSynchronize(solves[k]->loop().stream()));
+      if (!solves[k]->loop().finished()) next.push_back(k);
+    }
+    active.swap(next);
+  }
+  std::vector<CGResult> out(ns);
+  for (size_t k = 0; k < ns; k++) out[k] = solves[k]->finish();
+  return out;
 }
 
 }  // namespace qed3

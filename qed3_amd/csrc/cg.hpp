@@ -22,6 +22,7 @@
 // then): half the field bytes of the FP32 iteration.
 #pragma once
 #include <string>
+#include <vector>
 #include "dirac.hpp"
 
 namespace qed3 {
@@ -70,6 +71,12 @@ double true_residual(Dirac& D, const Spin<double>* x, const Spin<double>* b);
 // graph, scalars read back into host stack memory, and the LDS stencil only where the
 // context already holds its halo tables (the plain stencil otherwise).
 size_t cg_workspace_bytes(size_t points);
+// Several systems by the half-storage solver (o.half implied), each on its own context's
+// stream: every round queues one chunk of iterations of every unfinished system before the
+// host waits for any, so the GPU runs the systems' chunks back to back (and concurrently
+// where their streams overlap) without idling on host polls.
+std::vector<CGResult> cg_solve_half_many(const std::vector<Dirac*>& Ds, const std::vector<Spin<double>*>& xs,
+                                         const std::vector<const Spin<double>*>& bs, const CGOptions& o);
 CGResult cg_solve_f64_workspace(Dirac& D, Spin<double>* x, const Spin<double>* b, const CGOptions& o, void* ws,
                                 size_t ws_bytes);
 // That solve in the library's own per-GPU work array (noalloc.hip); returns a QED3_* status

@@ -8,6 +8,7 @@
 #include <array>
 #include <cmath>
 #include <cstdlib>
+#include <functional>
 #include <stdexcept>
 #include <utility>
 #include <vector>
@@ -68,42 +69,71 @@ class EventTimer {
   std::vector<std::pair<std::array<hipEvent_t, 2>, std::pair<int, int>>> ev_;
 };
 
-// Runs one_iter(i, k), i = 0 .. chunk-1 (the iteration's position in the chunk), k its
-// index in the solve (-1 while a graph is captured), then tail(), in chunks of iterations, captured once as a HIP graph unless
-// disabled, with one host poll of the device scalars per chunk, until the device marks
-// the solve done.  On return hsc holds the final scalars.
+// One solve's iterations in chunks: one_iter(i, k), i = 0 .. chunk-1 (the iteration's
+// position in the chunk), k its index in the solve (-1 while a graph is captured), then
+// tail(); the chunk is captured once as a HIP graph unless disabled.  launch() queues one
+// chunk and a copy of the device scalars into hsc; after the stream has drained them,
+// finished() tells whether the device marked the solve done (or the chunk budget ran out).
+// Several loops on different streams can be launched before any host wait.
+class ChunkLoop {
+ public:
+  ChunkLoop(std::function<void(int, int)> one_iter, std::function<void()> tail, const CGOptions& o, size_t n,
+            hipStream_t st, const CGScalars* sc, CGScalars* hsc)
+      : one_iter_(std::move(one_iter)), tail_(std::move(tail)), st_(st), sc_(sc), hsc_(hsc) {
+    chunk_ = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
+    chunk_ = (chunk_ + cgepi::kXGroup - 1) / cgepi::kXGroup * cgepi::kXGroup;  // every replay starts a new group
+    // every chunk runs at least one iteration (a reliable update ends each chunk), so the
+    // device's iteration cap ends the solve within this many chunks
+    max_chunks_ = o.maxiter + 4;
+    if (o.use_graph && !o.times && std::getenv("QED3_NO_GRAPH") == nullptr) {
+      QED3_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < chunk_; i++) one_iter_(i, -1);
+      tail_();
+      QED3_HIP(hipStreamEndCapture(st_, &graph_));
+      QED3_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    }
+  }
+  ~ChunkLoop() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+  }
+  ChunkLoop(const ChunkLoop&) = delete;
+  ChunkLoop& operator=(const ChunkLoop&) = delete;
+  hipStream_t stream() const { return st_; }
+  void launch() {
+    if (exec_) {
+      QED3_HIP(hipGraphLaunch(exec_, st_));
+    } else {
+      for (int i = 0; i < chunk_; i++) one_iter_(i, launched_ * chunk_ + i);
+      tail_();
+    }
+    QED3_HIP(hipGetLastError());
+    QED3_HIP(hipMemcpyAsync(hsc_, sc_, sizeof(CGScalars), hipMemcpyDeviceToHost, st_));
+    launched_++;
+  }
+  bool finished() const { return launched_ > 0 && (hsc_->done != 0 || launched_ >= max_chunks_); }
+
+ private:
+  std::function<void(int, int)> one_iter_;
+  std::function<void()> tail_;
+  hipStream_t st_;
+  const CGScalars* sc_;
+  CGScalars* hsc_;
+  int chunk_ = 8, max_chunks_ = 1, launched_ = 0;
+  hipGraphExec_t exec_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+};
+
+// One solve's chunks with one host poll per chunk, until the device marks the solve done.
+// On return hsc holds the final scalars.
 template <class Iter, class Tail>
 void drive_chunks(const Iter& one_iter, const Tail& tail, const CGOptions& o, size_t n, hipStream_t st,
                   const CGScalars* sc, CGScalars* hsc) {
-  int chunk = o.chunk > 0 ? o.chunk : (n > (1u << 20) ? 8 : 16);
-  chunk = (chunk + cgepi::kXGroup - 1) / cgepi::kXGroup * cgepi::kXGroup;  // every replay starts a new group
-                                                      // (and an even parity)
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (o.use_graph && !o.times && std::getenv("QED3_NO_GRAPH") == nullptr) {
-    QED3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; i++) one_iter(i, -1);
-    tail();
-    QED3_HIP(hipStreamEndCapture(st, &graph));
-    QED3_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
-  // every chunk runs at least one iteration (a reliable update ends each chunk), so the
-  // device's iteration cap ends the solve within this many chunks
-  const int max_chunks = o.maxiter + 4;
-  for (int c = 0; c < max_chunks; c++) {
-    if (exec) {
-      QED3_HIP(hipGraphLaunch(exec, st));
-    } else {
-      for (int i = 0; i < chunk; i++) one_iter(i, c * chunk + i);
-      tail();
-    }
-    QED3_HIP(hipGetLastError());
-    QED3_HIP(hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  ChunkLoop loop(one_iter, tail, o, n, st, sc, hsc);
+  do {
+    loop.launch();
     QED3_HIP(hipStreamSynchronize(st));
-    if (hsc->done != 0) break;
-  }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  } while (!loop.finished());
 }
 
 // Kernels of the half-storage start and flush, shared by the half-storage and the

@@ -95,3 +95,30 @@ def test_batch_mixes_large_and_small_systems(gpu):
         assert rep.status == 0 and rep.true_resid <= 1e-11
         xs, rs = D.solve(b, tol=1e-11, solver=q.CG_HALF)
         assert rel(x.to_numpy(np.complex128), xs) < 1e-9
+
+
+@pytest.mark.gpu
+def test_batch_rounds_of_large_and_small_systems(gpu):
+    """Systems large enough to cover the GPU (>= 256 blocks of 256 at their time tile) run
+    in rounds of one chunk per system on their own streams, the small ones in the shared
+    launches; every solution equals the oracle's, a NaN source in a large system is reported
+    while the others converge."""
+    systems = [(1, 8, 64, 0.05), (0, 8, 96, 0.1), (1, 2, 4, 0.1), (1, 4, 64, 0.02)]
+    ops, refs, bs = _build(systems, seed0=70)
+    for D in ops:  # one slice per block: ceil(V / 256) * Lt blocks
+        D.set_tile(256, 1, 1, 0, ops=(q.OP_D, q.OP_DDAG), precs=(q.F32, q.F64))
+    large = [-(-D.V // 256) * D.Lt >= 256 for D in ops]
+    assert large == [True, True, False, False], large
+    d_b = [q.DeviceArray.from_numpy(b) for b in bs]
+    d_x = [q.DeviceArray(b.nbytes) for b in bs]
+    reps = q.batch_solve(ops, d_x, d_b, tol=1e-11)
+    for Do, b, x, rep in zip(refs, bs, d_x, reps):
+        assert rep.status == 0 and rep.true_resid <= 1e-11
+        x_ref, _, _ = Do.solve(b, tol=1e-13)
+        assert rel(x.to_numpy(np.complex128), x_ref) < 1e-9
+    bn = bs[0].copy()
+    bn[7] = np.nan
+    d_b[0] = q.DeviceArray.from_numpy(bn)
+    reps = q.batch_solve(ops, d_x, d_b, tol=1e-11, raise_on_fail=False)
+    assert reps[0].status == 4
+    assert all(r.status == 0 and r.true_resid <= 1e-11 for r in reps[1:])
