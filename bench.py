@@ -339,18 +339,19 @@ def learned_leg(a, q):
     ns = str(nslot)
     pts = D.V * a.Lt
     tl = "k_horner_layer_tc<" + ns + ",{adj},{first},16,{src},{epi},{fuse}>"
-    lk = {"transport_layer_Kdag": ent(tl.format(adj="true", first="false", src="SrcPar", epi="EpiSkip", fuse="false"),
+    lk = {"transport_layer_Kdag": ent(tl.format(adj="true", first="false", src="SrcF<SpinH>", epi="EpiSkip",
+                                                fuse="false"),
                                       kms[4], b_layer, MEASURED_TRAFFIC.get(("layer-tc", lattice, a.n, a.Lt))),
           "transport_layer_K": ent(tl.format(adj="false", first="false", src="SrcF<SpinH>", epi="EpiSkip",
                                              fuse="false"), kms[5], b_layer),
-          "first_layer": ent(tl.format(adj="true", first="true", src="SrcPar", epi="EpiSkip", fuse="false") +
+          "first_layer": ent(tl.format(adj="true", first="true", src="SrcF<SpinH>", epi="EpiSkip", fuse="false") +
                              " (recurrence only)", kms[3], b_first),
           # + K's first layer written from the same registers
-          "last_layer_Kdag_with_K_first": ent(tl.format(adj="true", first="false", src="SrcPar", epi="EpiSkip",
-                                                        fuse="true"), kms[6], b_layer + pts * 8),
-          # + r and r_old read for <r,z>, <r_old,z>
-          "last_layer_K_with_dots": ent(tl.format(adj="false", first="false", src="SrcF<SpinH>", epi="EpiDots",
-                                                  fuse="false"), kms[7], b_layer + 2 * pts * 8),
+          "last_layer_Kdag_with_K_first": ent(tl.format(adj="true", first="false", src="SrcF<SpinH>",
+                                                        epi="EpiSkip", fuse="true"), kms[6], b_layer + pts * 8),
+          # + the previous H read beside the injection H for |H|^2 = <r,z>, <H_old,H> = <r_old,z>
+          "last_layer_K_with_dots": ent(tl.format(adj="false", first="false", src="SrcDots", epi="EpiFcg",
+                                                  fuse="false"), kms[7], b_layer + pts * 8),
           "K1": ent(f"k_stencil_lds<float,{ns},false,EpiK1PH,SrcPUpdateH,phase16_t> (p = z + beta p_old, w = D p)",
                     kms[0], cgb["K1"]),
           "K2": ent(f"k_stencil_lds<float,{ns},true,EpiK2F<0..6>,SrcHalf,phase16_t>", kms[1], cgb["K2"]),
@@ -359,7 +360,7 @@ def learned_leg(a, q):
         lk["K2x"] = ent(f"k_stencil_lds<float,{ns},true,EpiK2F<7>,SrcHalf,phase16_t> (x += the ring of 8 "
                         "directions)", kms[2], cgb["K2x"])
     # launches per iteration: K^dagger's first layer, L - 2 transport layers of K^dagger and of
-    # K each, K^dagger's last layer with K's first, K's last layer with the reductions, K1, K2
+    # K each, K^dagger's last layer with K's first and the reductions, K's last layer, K1, K2
     per_it = {"first_layer": kms[3], "transport_layers": (a.layers - 2) * (kms[4] + kms[5]),
               "last_layers": kms[6] + kms[7], "K1": kms[0], "K2": kms[1]}
     lk["ms_per_iteration_by_kind"] = {k2: round(v, 4) for k2, v in per_it.items()}

@@ -85,6 +85,22 @@ struct SrcPar {
   __device__ SrcF<SpinH> bind() const { return {(*par & 1) ? p1 : p0}; }
 };
 
+// Injection with a side field (parallel-in-time layer only): the layer also accumulates
+// |v|^2 and Re<o, v> over its points (v the injection as stored, o the side field) into the
+// reduction of its epilogue, which must have kReduce.
+struct SrcDots {
+  static constexpr bool kDots = true;
+  const SpinH* p;
+  const SpinH* o;
+  __device__ SrcDots bind() const { return *this; }
+  __device__ SpinH raw(size_t i) const { return p[i]; }
+  __device__ SpinH side(size_t i) const { return o[i]; }
+};
+template <class S, class = void>
+struct SrcHasDots : std::false_type {};
+template <class S>
+struct SrcHasDots<S, std::void_t<decltype(S::kDots)>> : std::bool_constant<S::kDots> {};
+
 struct EpiNone {
   static constexpr bool kReduce = false;
   __device__ bool skip() const { return false; }
@@ -356,6 +372,8 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
                                                          GridReduce gr, int C, SpinH* __restrict__ gout2,
                                                          float2 cj2) {
   static_assert(!FUSE || ADJ, "K's first layer fuses into a layer of K^dagger");
+  constexpr bool kDots = SrcHasDots<Src>::value;
+  static_assert(!kDots || Epi::kReduce, "the injection's products need the epilogue's reduction");
   if (epi.skip()) return;  // uniform
   extern __shared__ float4 hk_tile_raw[];
   const int S = int(blockDim.x) / C;
@@ -461,6 +479,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   float2 zF[T];   // forward row: the chunk's states (FP32)
   half2_t zB[T];  // backward row: its recurrence inputs, then its states (binary16)
   float2 prevF = {0.f, 0.f};
+  float2 dots = {0.f, 0.f};  // kDots: this thread's |v|^2 and Re<o, v> over its chunk
   Loads cur, nxt;
   load(0, cur);
 #pragma unroll
@@ -477,6 +496,11 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
       __syncthreads();
     }
     if (j < len) {
+      if constexpr (kDots) {
+        const Spin<float> v = from_half(cur.v), o = from_half(in.side(size_t(s0 + j) * V + x));
+        dots.x += v.a.x * v.a.x + v.a.y * v.a.y + v.b.x * v.b.x + v.b.y * v.b.y;
+        dots.y += o.a.x * v.a.x + o.a.y * v.a.y + o.b.x * v.b.x + o.b.y * v.b.y;
+      }
       float2 yF = cmul<float>(cj, comp(cur.v, rF)), yB = cmul<float>(cj, comp(cur.v, rB));
       if constexpr (!FIRST) {
         const float2 tF = transport(cur, tb, hqF, rF), tB = transport(cur, tb, hqB, rB);
@@ -528,7 +552,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     cB = {ac * cB.x + e.x, ac * cB.y + e.y};
   }
   // add the carries and store
-  double2 acc = {0.0, 0.0};
+  double2 acc = {double(dots.x), double(dots.y)};
   float pF = a;
 #pragma unroll
   for (int j = 0; j < T; j++) {

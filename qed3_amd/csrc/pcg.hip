@@ -7,14 +7,16 @@
 // The correction <r_old, z> (zero for an exactly symmetric fixed M) keeps the directions
 // A-conjugate although the binary16 fields make M vary slightly between applications.
 //
-// One iteration is the network's 2L layer launches (K^dagger then K; the last computes
-// <r, z> and <r_old, z> and the scalar update) and the two stencil kernels of the
-// half-storage CG: K1 forms p = z + beta p_old while gathering, stores it and applies D,
-// K2 applies D^dagger, updates r and x.  The residual is double-buffered, the buffer in use
-// chosen on the device, so that a chunk of iterations is one graph replay whatever the
-// reliable updates do.  Reliable
+// One iteration is the network's 2L - 1 layer launches (K^dagger, whose last layer also
+// runs K's first, then K, whose last layer computes the two products below and the scalar
+// update) and the two stencil kernels of the half-storage CG: K1 forms p = z + beta p_old
+// while gathering, stores it and applies D, K2 applies D^dagger, updates r in place and x.
+// With H = K^dagger r, <r, z> = |H|^2 and <r_old, z> = <K^dagger r_old, H> = <H_old, H>, so
+// the products need neither residual again: H is double-buffered (by the iteration's
+// parity, fixed within a chunk of an even number of iterations) and K's layers, which read
+// H as their injection anyway, take the previous one beside it in the last layer.  Reliable
 // updates, scales and the temporal gauge are those of the half-storage CG (cg.hip); after
-// an update beta carries the change of scale (r_old and p_old are kept at the old scale).
+// an update beta carries the change of scale (H_old and p_old are kept at the old scale).
 #include <chrono>
 #include <cmath>
 
@@ -38,35 +40,30 @@ using namespace cgdrive;
 constexpr int kFGroup = 8;
 static_assert(kFGroup - 1 <= kMaxPend, "pending alphas of a group");
 
-// K2: q = D^dag w; r_new = r - alpha q into the other residual buffer, |r_new|^2 and the
-// reliable-update decision; at the group's last position x += the group's alpha p terms
+// K2: q = D^dag w; r -= alpha q in place, |r|^2 and the reliable-update decision; at the
+// group's last position x += the group's alpha p terms
 template <int POS>
 struct EpiK2F {
   static constexpr bool kReduce = true;
   static constexpr bool kUpdateX = POS == kFGroup - 1;
   Spin<float>* x;
-  SpinH* rbuf[2];
+  SpinH* r;
   const SpinH* ring[kFGroup];
   CGScalars* sc;
   float alpha;
   float ap[kFGroup - 1];
-  const SpinH* rin;
-  SpinH* rout;
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
 #pragma unroll
     for (int j = 0; j < kFGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
-    const int par = sc->rpar & 1;
-    rin = rbuf[par];
-    rout = rbuf[par ^ 1];
   }
   // r is loaded with the slice's other loads (as EpiK2H) except where x is updated
   template <int P = POS, std::enable_if_t<P != kFGroup - 1, int> = 0>
   __device__ SpinH prefetch(size_t i) const {
-    return rin[i];
+    return r[i];
   }
-  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const { (*this)(i, q, acc, rin[i]); }
+  __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const { (*this)(i, q, acc, r[i]); }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const SpinH& rpre) const {
     Spin<float> rr = from_half(rpre);
     if constexpr (kUpdateX) {
@@ -89,7 +86,7 @@ struct EpiK2F {
     rr.b.x -= alpha * q.b.x;
     rr.b.y -= alpha * q.b.y;
     const SpinH rh = to_half(rr);
-    sth_nt(rout + i, rh);
+    sth_nt(r + i, rh);
     acc.x += norm2(from_half(rh));
   }
   __device__ void operator()(double2 t) const {
@@ -101,7 +98,6 @@ struct EpiK2F {
     }
     sc->last_pos = POS;
     sc->iter += 1;
-    sc->rpar ^= 1;
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
       return;
@@ -162,24 +158,18 @@ struct EpiSkip {
   __device__ void operator()(double2) const {}
 };
 
-// Last layer of K: Re<r, z> and Re<r_old, z> (z as stored) and the FCG scalars.  After a
-// reliable update r is the true residual at the new scale while r_old and p_old are at the
-// old one (rfac = old / new): beta = (<r,z> - rfac <r_old,z>) / (rfac rho_old).
-struct EpiDots {
+// Last layer of K, injecting H = K^dagger r as stored beside H_old (hk::SrcDots), which
+// reduces t = (|H|^2, <H_old, H>) = (<r, z>, <r_old, z>): the FCG scalars.  After a reliable
+// update r is the true residual at the new scale while H_old and p_old are at the old one
+// (rfac = old / new): beta = (|H|^2 - rfac <H_old, H>) / (rfac rho_old).
+struct EpiFcg {
   static constexpr bool kReduce = true;
-  const SpinH* r0;
-  const SpinH* r1;
   CGScalars* sc;
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
-  __device__ void operator()(size_t i, const Spin<float>& z, double2& acc) const {
-    const int par = sc->rpar & 1;
-    const Spin<float> u = ldh((par ? r1 : r0) + i), w = ldh((par ? r0 : r1) + i);
-    acc.x += double(u.a.x) * z.a.x + double(u.a.y) * z.a.y + double(u.b.x) * z.b.x + double(u.b.y) * z.b.y;
-    acc.y += double(w.a.x) * z.a.x + double(w.a.y) * z.a.y + double(w.b.x) * z.b.x + double(w.b.y) * z.b.y;
-  }
+  __device__ void operator()(size_t, const Spin<float>&, double2&) const {}
   __device__ void operator()(double2 t) const {
     const double A = t.x, Bv = t.y;
-    if (!(A > 0.0 && A < 1e300) || !(fabs(Bv) < 1e300)) {
+    if (!(A > 0.0 && A < 1e300) || (!sc->fresh && !(fabs(Bv) < 1e300))) {
       sc->done = 2;  // M r must be a descent direction: <r, M r> > 0
       return;
     }
@@ -202,14 +192,13 @@ struct FinStartF {
   __device__ bool skip() const { return false; }
   __device__ void operator()(double2 t) const {
     half_start(sc, t.x, ncomp, tol2, delta2, maxiter);
-    sc->rpar = 0;
     sc->fresh = 1;
     sc->rho = 0.0;
   }
 };
 
-// r = exp(i phi) b / scale in binary16 (buffer 0); buffer 1 = 0 (no r_old yet)
-__global__ __launch_bounds__(256) void k_store_b0_f(const Spin<double>* __restrict__ b, SpinH* r0, SpinH* r1,
+// r = exp(i phi) b / scale in binary16; H = 0 (no H_old yet)
+__global__ __launch_bounds__(256) void k_store_b0_f(const Spin<double>* __restrict__ b, SpinH* r, SpinH* h,
                                                     size_t n, const CGScalars* sc, const phase16_t* rot) {
   if (sc->done != 0) return;
   const double inv = 1.0 / sc->scale;
@@ -218,17 +207,16 @@ __global__ __launch_bounds__(256) void k_store_b0_f(const Spin<double>* __restri
   z.b = {0.f, 0.f};
   const SpinH zh = to_half(z);
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    half_store_b(b, r0, inv, i, rot);
-    r1[i] = zh;
+    half_store_b(b, r, inv, i, rot);
+    h[i] = zh;
   }
 }
 
-// after a reliable update: the current residual buffer = exp(i phi) rfac rt
-__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, SpinH* r0, SpinH* r1, size_t n,
+// after a reliable update: r = exp(i phi) rfac rt
+__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, SpinH* r, size_t n,
                                                    const CGScalars* sc, const phase16_t* rot) {
   if (sc->done != 0 || sc->rescale == 0) return;
   const float f = float(sc->rfac);
-  SpinH* r = (sc->rpar & 1) ? r1 : r0;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     half_rescale_point(rt, r, f, i, rot);
 }
@@ -264,11 +252,12 @@ __global__ __launch_bounds__(256) void k_from_tg16(const SpinH* __restrict__ in,
 namespace {
 // M v = K (K^dagger v) by the parallel-in-time layers: K^dagger's layers inject src and
 // end in H; its last layer also runs K's first layer (L >= 2) into a G slot; K's layers
-// inject H and end in Z, the last one with epi_last.  G[2] alternate as the layers' outputs.
-template <class Src, class Epi, class EpiLast>
+// inject H and end in Z, the last one injecting src_last (H, possibly with a side field)
+// with epi_last.  G[2] alternate as the layers' outputs; the other layers carry epi.
+template <class Src, class Epi, class SrcLast, class EpiLast>
 void precond_layers(Horner& K, const HornerView& hv, const DiracView<float, phasepair_t>& vpairs, const HaloView& halo,
-                    const Src& src, SpinH* H, SpinH* const* G, SpinH* Z, const Epi& epi, const EpiLast& epi_last,
-                    GridReduce gr, hipStream_t st, EventTimer* timer, int k) {
+                    const Src& src, SpinH* H, SpinH* const* G, SpinH* Z, const Epi& epi, const SrcLast& src_last,
+                    const EpiLast& epi_last, GridReduce gr, hipStream_t st, EventTimer* timer, int k) {
   const int L = K.layers();
   const hk::SrcF<SpinH> src_h{H};
   auto ev = [&](int cls) { return timer ? timer->pair(cls, k) : nullptr; };
@@ -299,11 +288,11 @@ void precond_layers(Horner& K, const HornerView& hv, const DiracView<float, phas
     const bool last = l == L - 1;
     SpinH* out = last ? Z : G[(l + L - 1) & 1];
     if (l == 0 && last)
-      hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(4));
+      hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_last, prev, out, cj, epi_last, gr, st, ev(4));
     else if (l == 0)
       hk::launch_horner_layer_tc<false, true>(hv, vpairs, halo, src_h, prev, out, cj, epi, gr, st, ev(0));
     else if (last)
-      hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, epi_last, gr, st, ev(4));
+      hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_last, prev, out, cj, epi_last, gr, st, ev(4));
     else
       hk::launch_horner_layer_tc<false, false>(hv, vpairs, halo, src_h, prev, out, cj, epi, gr, st, ev(2));
     prev = out;
@@ -333,7 +322,7 @@ void horner_apply_solver(Horner& K, int which, Spin<double>* out, const Spin<dou
   if (which == 2) {  // the solver's own sequence, K's first layer fused into K^dagger's last
     SpinH* H = static_cast<SpinH*>(K.work(9, hb));
     SpinH* Z = static_cast<SpinH*>(K.work(10, hb));
-    precond_layers(K, hv, vpairs, halo, src, H, G, Z, none, none, D.reducer(), st, nullptr, 0);
+    precond_layers(K, hv, vpairs, halo, src, H, G, Z, none, hk::SrcF<SpinH>{H}, none, D.reducer(), st, nullptr, 0);
     prev = Z;
   } else {
     const bool adj = which == 1;
@@ -379,9 +368,10 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   Spin<float>* rt = static_cast<Spin<float>*>(D.scratch(6, fb));
   CGScalars* sc = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
   CGScalars* hsc = D.host_scalars();
-  SpinH* R[2] = {static_cast<SpinH*>(K.work(0, hb)), static_cast<SpinH*>(K.work(1, hb))};
+  SpinH* R = static_cast<SpinH*>(K.work(0, hb));
   SpinH* G[2] = {static_cast<SpinH*>(K.work(2, hb)), static_cast<SpinH*>(K.work(3, hb))};
-  SpinH* H = static_cast<SpinH*>(K.work(4, hb));
+  // H of the even and the odd iterations
+  SpinH* H[2] = {static_cast<SpinH*>(K.work(4, hb)), static_cast<SpinH*>(K.work(1, hb))};
   SpinH* Z = static_cast<SpinH*>(K.work(5, hb));
 
   const GridReduce gr = D.reducer();
@@ -393,7 +383,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   const DiracView<float, phase16_t> vin = D.view_half_temporal_gauge();
   const DiracView<float, phasepair_t> vpairs = D.view_half_tg_pairs();
   const phase16_t* rot = D.half_gauge_rotation();
-  hipLaunchKernelGGL(k_store_b0_f, dim3(nbk), dim3(256), 0, st, b, R[0], R[1], n, sc, rot);
+  hipLaunchKernelGGL(k_store_b0_f, dim3(nbk), dim3(256), 0, st, b, R, H[1], n, sc, rot);
   QED3_HIP(hipGetLastError());
 
   const DiracView<double> v64 = D.view<double>();
@@ -407,21 +397,21 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   PendRing8 pend;
   for (int j = 0; j < kFGroup - 1; j++) pend.p[j] = ring[j];
   EventTimer timer(o.times), ltimer(layer_times);
-  const hk::SrcPar src_r{R[0], R[1], &sc->rpar};
-  const hk::SrcF<SpinH> src_h{H};
+  const hk::SrcF<SpinH> src_r{R};
   const EpiSkip eskip{sc};
-  const EpiDots edots{R[0], R[1], sc};
+  const EpiFcg efcg{sc};
 
-  // one network application M r = K (K^dagger r): K^dagger's output in H, K's in Z
-  auto precond = [&](int k) {
-    precond_layers(K, hv, vpairs, halo, src_r, H, G, Z, eskip, edots, gr, st, &ltimer, k);
+  // one network application M r = K (K^dagger r): K^dagger's output in H[i & 1], K's in Z
+  auto precond = [&](int i, int k) {
+    SpinH* h = H[i & 1];
+    precond_layers(K, hv, vpairs, halo, src_r, h, G, Z, eskip, hk::SrcDots{h, H[(i & 1) ^ 1]}, efcg, gr, st,
+                   &ltimer, k);
   };
   auto k2 = [&](auto tag, int k) {
     constexpr int POS = decltype(tag)::value;
     EpiK2F<POS> e{};
     e.x = xin;
-    e.rbuf[0] = R[0];
-    e.rbuf[1] = R[1];
+    e.r = R;
     for (int j = 0; j < kFGroup; j++) e.ring[j] = ring[j];
     e.sc = sc;
     launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
@@ -435,13 +425,13 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
     launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr, st,
                                  hv64t);
-    hipLaunchKernelGGL(k_rescale_f, dim3(ngate), dim3(256), 0, st, rt, R[0], R[1], n, sc, rot);
+    hipLaunchKernelGGL(k_rescale_f, dim3(ngate), dim3(256), 0, st, rt, R, n, sc, rot);
   };
   auto one_iter = [&](int i, int k) {
     const int pos = i % kFGroup;
     SpinH* pold = ring[(pos + kFGroup - 1) % kFGroup];
     SpinH* pnew = ring[pos];
-    precond(k);
+    precond(i, k);
     launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{Z, pold, sc}, EpiK1PH{w, pnew, sc},
                                      D.tile<float>(false), gr, st, hvf, timer.pair(0, k));
     static_assert(kFGroup == 8, "one K2 instantiation per group position");
