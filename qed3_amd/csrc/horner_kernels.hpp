@@ -31,6 +31,33 @@ __device__ __forceinline__ half2_t cmul16(half2_t x, half2_t y) {
   return x.xx * y + x.yy * ys;
 }
 
+// element idx of base through a 32-bit byte offset (idx * sizeof(T) < 2^32), so that the
+// access takes a uniform base from scalar registers and a 32-bit vector offset
+template <class T>
+__device__ __forceinline__ const T& at32(const T* base, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * unsigned(sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ T* ptr32(T* base, unsigned idx) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ const T* cptr32(const T* base, unsigned idx) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * unsigned(sizeof(T)));
+}
+
+// Loads through a buffer resource (base, nbytes): byte offset voff (per thread) + soff (a
+// scalar register: uniform across the wave); loads beyond nbytes return zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, unsigned nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), short(0), int(nbytes), 0x00020000);
+}
+__device__ __forceinline__ SpinH bload_h(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(SpinH, __builtin_amdgcn_raw_buffer_load_b64(buf_rsrc(base, nbytes), voff, soff, 0));
+}
+__device__ __forceinline__ phasepair_t bload_u(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(phasepair_t, __builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(base, nbytes), voff, soff, 0));
+}
+
 // component a of a point as FP32 complex
 __device__ __forceinline__ float2 comp(const Spin<float>& v, int a) { return a == 0 ? v.a : v.b; }
 __device__ __forceinline__ float2 comp(const SpinH& v, int a) {
@@ -75,6 +102,7 @@ struct SrcF {
   const F* p;
   __device__ SrcF bind() const { return *this; }
   __device__ F raw(size_t i) const { return p[i]; }
+  __device__ F bat(unsigned nbytes, unsigned voff, unsigned soff) const { return bload_h(p, nbytes, voff, soff); }
 };
 // binary16 field chosen on the device: buffer par[0] & 1 of two (the solver's
 // double-buffered residual)
@@ -94,7 +122,10 @@ struct SrcDots {
   const SpinH* o;
   __device__ SrcDots bind() const { return *this; }
   __device__ SpinH raw(size_t i) const { return p[i]; }
-  __device__ SpinH side(size_t i) const { return o[i]; }
+  __device__ SpinH bat(unsigned nbytes, unsigned voff, unsigned soff) const { return bload_h(p, nbytes, voff, soff); }
+  __device__ SpinH bside(unsigned nbytes, unsigned voff, unsigned soff) const {
+    return bload_h(o, nbytes, voff, soff);
+  }
 };
 template <class S, class = void>
 struct SrcHasDots : std::false_type {};
@@ -360,12 +391,13 @@ inline void launch_horner_layer(const HornerView& hv, const DiracView<float, P>&
 // lane are consecutive, so a wave reads and writes consecutive sites of one slice).  Per step the block stages the C slices its lanes are at, for its S sites and
 // their halo (halo tables of ranges of S sites).
 constexpr int kTcChunk = 16;   // slices per lane
+constexpr int kTcDepth = 2;    // steps whose loads are in flight ahead of the one computed
 constexpr int kTcBlock = 256;  // threads per block (S sites x C lanes)
 
 // FUSE (K^dagger's last layer only): the kernel then also applies the first layer of K to
 // the output it has just stored, from the same registers: gout2 = T^{-1}(cj2 gout) with
 // gout as stored (binary16), which saves K's first launch and its read of gout.
-template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi, bool FUSE = false>
+template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi, bool FUSE = false, bool UL = false>
 __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc(HornerView hv, DiracView<float, phasepair_t> d,
                                                          HaloView halo, Src src, const SpinH* __restrict__ gin,
                                                          SpinH* __restrict__ gout, float2 cj, TileMap tm, Epi epi,
@@ -377,7 +409,11 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   if (epi.skip()) return;  // uniform
   extern __shared__ float4 hk_tile_raw[];
   const int S = int(blockDim.x) / C;
-  const int lane = int(threadIdx.x) / S, sl = int(threadIdx.x) % S;  // a wave: consecutive sites, one chunk
+  // a wave: consecutive sites of one lane (UL: S a multiple of the wave size, so the lane, its
+  // slices and their offsets are wave-uniform and live in scalar registers)
+  int lane = int(threadIdx.x) / S;
+  if constexpr (UL) lane = __builtin_amdgcn_readfirstlane(lane);
+  const int sl = int(threadIdx.x) - lane * S;
   SpinH* tile = reinterpret_cast<SpinH*>(hk_tile_raw);         // [2][C][S + hmax]
   const int row = S + halo.hmax;
   float2* carry = reinterpret_cast<float2*>(tile + (FIRST ? 0 : 2 * C * row));  // [2][kTcBlock]
@@ -390,22 +426,35 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   const int x0 = xr * S;
   const int x = live ? x0 + sl : V;
   const bool act = x < V;
+  const unsigned xc = act ? unsigned(x) : 0u;  // idle threads load site 0 and discard it
   const int s0 = lane * T;
-  const int len = act ? max(0, min(T, Lt - s0)) : 0;
+  const int lenU = max(0, min(T, Lt - s0));  // slices of this lane
+  const int len = act ? lenU : 0;            // ... that this thread stores
+  // slices this thread computes: UL computes them on idle threads too (discarded), so that the
+  // step loop branches on the wave-uniform count
+  const int lenC = UL ? lenU : len;
   constexpr int rF = ADJ ? 1 : 0, rB = 1 - rF;
   // halo of this block's range: lane c stages slice cT + j of it, its S threads share it
   const int h0 = (live && !FIRST) ? halo.halo_off[xr] : 0;
   const int nh = (live && !FIRST) ? halo.halo_off[xr + 1] - h0 : 0;
   const int hid0 = (sl < nh) ? halo.halo[h0 + sl] : -1;
   const int hid1 = (sl + S < nh) ? halo.halo[h0 + sl + S] : -1;
-  int nb[NS], lk[NS];
+  const unsigned hc0 = hid0 >= 0 ? unsigned(hid0) : xc, hc1 = hid1 >= 0 ? unsigned(hid1) : xc;
+  // neighbour slots in the tile, link indices, and the sign bit that conjugates the phase of
+  // a link stored in the opposite orientation
+  int nb[NS];
+  unsigned li[NS], usg[NS];
+#pragma unroll
+  for (int j = 0; j < NS; j++) nb[j] = 0, li[j] = 0u, usg[j] = 0u;
   float invc = 0.f, kt = 0.f;
   if (act) {
     if constexpr (!FIRST) {
 #pragma unroll
       for (int j = 0; j < NS; j++) {
         nb[j] = halo.nbl[j * V + x];
-        lk[j] = d.lnk[j * V + x];
+        const int l = d.lnk[j * V + x];
+        li[j] = l >= 0 ? unsigned(l) : unsigned(~l);
+        usg[j] = l < 0 ? 0x80000000u : 0u;
       }
       // the site's hop rows go to LDS once (lane 0), read per step by all its lanes
       if (lane == 0) {
@@ -445,32 +494,38 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     SpinH v, gc, gh0, gh1;
     phasepair_t u[NS];
   };
-  // slice s0 + j of this lane (own point, links) and slice c T + j of the halo share
-  auto load = [&](int j, Loads& Ld) {
-    const int s = s0 + j;
-    if (j < len) {
-      const size_t sV = size_t(s) * V;
-      Ld.v = in.raw(sV + x);
-      if constexpr (!FIRST) {
-        const phasepair_t* uS = d.uS + size_t(s) * d.E;
+  // slice s0 + j of this lane (own point, links) and slice c T + j of the halo share.  Every
+  // thread issues the same loads at every step (lanes past the column's end re-read its last
+  // slice, idle threads site 0; the values are not used), so that the loads of the next
+  // kTcDepth steps stay in flight while this one computes; 32-bit offsets (checked at launch).
+  // buffer loads: the slice's byte offset in the scalar offset when it is wave-uniform (UL),
+  // the thread's site or link in the vector offset
+  const unsigned fbytes = unsigned(V) * unsigned(Lt) * unsigned(sizeof(SpinH));
+  const unsigned ubytes = unsigned(d.E) * unsigned(Lt) * unsigned(sizeof(phasepair_t));
+  const unsigned vx = xc * unsigned(sizeof(SpinH)), vh0 = hc0 * unsigned(sizeof(SpinH)),
+                 vh1 = hc1 * unsigned(sizeof(SpinH));
+  unsigned vl[NS];
 #pragma unroll
-        for (int q = 0; q < NS; q++) Ld.u[q] = uS[lk[q] >= 0 ? lk[q] : ~lk[q]];
-        Ld.gc = gin[sV + x];
-      }
-    }
+  for (int q = 0; q < NS; q++) vl[q] = li[q] * unsigned(sizeof(phasepair_t));
+  auto load = [&](int j, Loads& Ld) {
+    const unsigned s = unsigned(min(s0 + j, Lt - 1));
+    const unsigned oG = s * unsigned(V) * unsigned(sizeof(SpinH));       // the slice's field offset
+    const unsigned oU = s * unsigned(d.E) * unsigned(sizeof(phasepair_t));  // ... and link offset
+    const unsigned sG = UL ? oG : 0u, sU = UL ? oU : 0u, vG = UL ? 0u : oG, vU = UL ? 0u : oU;
+    Ld.v = in.bat(fbytes, vG + vx, sG);
     if constexpr (!FIRST) {
-      if (live && s < Lt) {
-        if (hid0 >= 0) Ld.gh0 = gin[size_t(s) * V + hid0];
-        if (hid1 >= 0) Ld.gh1 = gin[size_t(s) * V + hid1];
-      }
+#pragma unroll
+      for (int q = 0; q < NS; q++) Ld.u[q] = bload_u(d.uS, ubytes, vU + vl[q], sU);
+      Ld.gc = bload_h(gin, fbytes, vG + vx, sG);
+      Ld.gh0 = bload_h(gin, fbytes, vG + vh0, sG);
+      Ld.gh1 = bload_h(gin, fbytes, vG + vh1, sG);
     }
   };
   auto transport = [&](const Loads& Ld, const SpinH* tb, const HopRow16* hq, int r) -> float2 {
     half2_t t16 = {(_Float16)0.f, (_Float16)0.f};
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      half2_t u = Ld.u[q].v;
-      if (lk[q] < 0) u.y = -u.y;
+      const half2_t u = __builtin_bit_cast(half2_t, __builtin_bit_cast(unsigned, Ld.u[q].v) ^ usg[q]);
       const SpinH g = tb[nb[q]];
       t16 += cmul16(u, cmul16(hq[q].s, comp16(g, r)) + cmul16(hq[q].f, comp16(g, 1 - r)));
     }
@@ -480,14 +535,17 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   half2_t zB[T];  // backward row: its recurrence inputs, then its states (binary16)
   float2 prevF = {0.f, 0.f};
   float2 dots = {0.f, 0.f};  // kDots: this thread's |v|^2 and Re<o, v> over its chunk
-  Loads cur, nxt;
-  load(0, cur);
+  constexpr int NB = kTcDepth + 1;  // load buffers: this step's and the next kTcDepth steps'
+  Loads lb[NB];
+#pragma unroll
+  for (int j = 0; j < kTcDepth && j < T; j++) load(j, lb[j]);
 #pragma unroll
   for (int j = 0; j < T; j++) {
-    if (j + 1 < T) load(j + 1, nxt);
+    if (j + kTcDepth < T) load(j + kTcDepth, lb[(j + kTcDepth) % NB]);
+    const Loads& cur = lb[j % NB];
     SpinH* tb = tile + ((j & 1) * C + lane) * row;
     if constexpr (!FIRST) {
-      if (j < len) tb[sl] = cur.gc;
+      if (j < lenC) tb[sl] = cur.gc;
       if (live && s0 + j < Lt) {
         if (hid0 >= 0) tb[S + sl] = cur.gh0;
         if (hid1 >= 0) tb[2 * S + sl] = cur.gh1;
@@ -495,11 +553,15 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
       }
       __syncthreads();
     }
-    if (j < len) {
+    if (j < lenC) {
       if constexpr (kDots) {
-        const Spin<float> v = from_half(cur.v), o = from_half(in.side(size_t(s0 + j) * V + x));
-        dots.x += v.a.x * v.a.x + v.a.y * v.a.y + v.b.x * v.b.x + v.b.y * v.b.y;
-        dots.y += o.a.x * v.a.x + o.a.y * v.a.y + o.b.x * v.b.x + o.b.y * v.b.y;
+        if (act) {
+          const unsigned oG = unsigned(s0 + j) * unsigned(V) * unsigned(sizeof(SpinH));
+          const Spin<float> v = from_half(cur.v),
+                            o = from_half(UL ? in.bside(fbytes, vx, oG) : in.bside(fbytes, oG + vx, 0u));
+          dots.x += v.a.x * v.a.x + v.a.y * v.a.y + v.b.x * v.b.x + v.b.y * v.b.y;
+          dots.y += o.a.x * v.a.x + o.a.y * v.a.y + o.b.x * v.b.x + o.b.y * v.b.y;
+        }
       }
       float2 yF = cmul<float>(cj, comp(cur.v, rF)), yB = cmul<float>(cj, comp(cur.v, rB));
       if constexpr (!FIRST) {
@@ -513,7 +575,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
         yF.x += wF.x;
         yF.y += wF.y;
       }
-      if (j == len - 1) {
+      if (j == lenC - 1) {
         yB.x += wB.x;
         yB.y += wB.y;
       }
@@ -521,13 +583,12 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
       zF[j] = prevF;
       zB[j] = half2_t{(_Float16)yB.x, (_Float16)yB.y};
     }
-    if (j + 1 < T) cur = nxt;
   }
   // local backward recurrence over the chunk held in registers
   float2 nextB = {0.f, 0.f};
 #pragma unroll
   for (int j = T - 1; j >= 0; j--) {
-    if (j < len) {
+    if (j < lenC) {
       nextB = {a * nextB.x + invc * float(zB[j].x), a * nextB.y + invc * float(zB[j].y)};
       zB[j] = half2_t{(_Float16)nextB.x, (_Float16)nextB.y};
     }
@@ -556,7 +617,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   float pF = a;
 #pragma unroll
   for (int j = 0; j < T; j++) {
-    if (j < len) {
+    if (j < lenC) {
       zF[j].x += pF * cF.x;
       zF[j].y += pF * cF.y;
       pF *= a;
@@ -566,16 +627,18 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   float pB = a;
 #pragma unroll
   for (int j = T - 1; j >= 0; j--) {
-    if (j < len) {
+    if (j < lenC) {
       const float2 b = {float(zB[j].x) + pB * cB.x, float(zB[j].y) + pB * cB.y};
       pB *= a;
-      const size_t i = size_t(s0 + j) * V + x;
-      if constexpr (rF == 0) {
-        put(gout + i, zF[j], b);
-        epi(i, stored(gout, zF[j], b), acc);
-      } else {
-        put(gout + i, b, zF[j]);
-        epi(i, stored(gout, b, zF[j]), acc);
+      if (act) {
+        const unsigned i = unsigned(s0 + j) * unsigned(V) + unsigned(x);
+        if constexpr (rF == 0) {
+          put(ptr32(gout, i), zF[j], b);
+          epi(size_t(i), stored(gout, zF[j], b), acc);
+        } else {
+          put(ptr32(gout, i), b, zF[j]);
+          epi(size_t(i), stored(gout, b, zF[j]), acc);
+        }
       }
       if constexpr (FUSE) zB[j] = half2_t{(_Float16)b.x, (_Float16)b.y};  // row rB as stored
     }
@@ -587,7 +650,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     float2 p0 = {0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < T; j++) {
-      if (j < len) {
+      if (j < lenC) {
         const float2 h0 = {float(zB[j].x), float(zB[j].y)};
         const float2 h1 = {float((_Float16)zF[j].x), float((_Float16)zF[j].y)};
         const float2 y0 = cmul<float>(cj2, h0), y1 = cmul<float>(cj2, h1);
@@ -599,7 +662,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     float2 p1 = {0.f, 0.f};
 #pragma unroll
     for (int j = T - 1; j >= 0; j--) {
-      if (j < len) {
+      if (j < lenC) {
         p1 = {a * p1.x + invc * float(zB[j].x), a * p1.y + invc * float(zB[j].y)};
         zB[j] = half2_t{(_Float16)p1.x, (_Float16)p1.y};
       }
@@ -623,7 +686,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     float q0 = a;
 #pragma unroll
     for (int j = 0; j < T; j++) {
-      if (j < len) {
+      if (j < lenC) {
         zF[j].x += q0 * c0.x;
         zF[j].y += q0 * c0.y;
         q0 *= a;
@@ -632,10 +695,10 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     float q1 = a;
 #pragma unroll
     for (int j = T - 1; j >= 0; j--) {
-      if (j < len) {
+      if (j < lenC) {
         const float2 b = {float(zB[j].x) + q1 * c1.x, float(zB[j].y) + q1 * c1.y};
         q1 *= a;
-        put(gout2 + size_t(s0 + j) * V + x, zF[j], b);
+        if (act) put(ptr32(gout2, unsigned(s0 + j) * unsigned(V) + unsigned(x)), zF[j], b);
       }
     }
   }
@@ -674,20 +737,30 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
       (FIRST ? 0 : size_t(2) * C * (S + halo.hmax) * sizeof(SpinH)) + size_t(2) * kTcBlock * sizeof(float2) +
       (FIRST ? 0 : size_t(S) * 2 * hv.nslot * sizeof(HopRow16));
   if (smem > 64 * 1024) throw std::invalid_argument("parallel-in-time layer: halo too large for LDS");
+  if (size_t(hv.V) * size_t(hv.Lt) * sizeof(SpinH) > 0xffffffffull ||
+      size_t(tables.E) * size_t(hv.Lt) * sizeof(phasepair_t) > 0xffffffffull)
+    throw std::invalid_argument("parallel-in-time layer: fields beyond 32-bit offsets");
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
-  auto go = [&](auto fuse) {
+  auto go = [&](auto fuse, auto ul) {
     constexpr bool F = decltype(fuse)::value && ADJ;
+    constexpr bool U = decltype(ul)::value;
     if (hv.nslot == 6)
-      hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F>), dim3(nblk), dim3(kTcBlock), smem,
-                         st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+      hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F, U>), dim3(nblk), dim3(kTcBlock),
+                         smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
     else
-      hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F>), dim3(nblk), dim3(kTcBlock), smem,
-                         st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+      hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F, U>), dim3(nblk), dim3(kTcBlock),
+                         smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
   };
-  if (gout2)
-    go(std::true_type{});
+  // the wave-uniform form where a wave's 64 threads belong to one lane (S a multiple of 64)
+  const bool ul = S % 64 == 0;
+  if (gout2 && ul)
+    go(std::true_type{}, std::true_type{});
+  else if (gout2)
+    go(std::true_type{}, std::false_type{});
+  else if (ul)
+    go(std::false_type{}, std::true_type{});
   else
-    go(std::false_type{});
+    go(std::false_type{}, std::false_type{});
   if (bracket) QED3_HIP(hipEventRecord(bracket[1], st));
   if constexpr (Epi::kReduce)
     hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, nblk, epi);
