@@ -72,6 +72,21 @@ __global__ void k_angle_pairs(const phase16_t* __restrict__ a, phasepair_t* __re
   out[i] = p;
 }
 
+// Per site and slot, the phase of the slot's link as the site sees it ([Lt][M][V], M the
+// slots per site): conjugated where the link table gives the link reversed (~id).
+__global__ void k_site_pairs(const phasepair_t* __restrict__ uS, const int* __restrict__ lnk,
+                             phasepair_t* __restrict__ out, int V, int E, int M, int Lt) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(Lt) * M * V) return;
+  const int x = int(i % V);
+  const size_t sq = i / V;
+  const int q = int(sq % M), s = int(sq / M);
+  const int l = lnk[size_t(q) * V + x];
+  phasepair_t p = uS[size_t(s) * E + (l >= 0 ? l : ~l)];
+  if (l < 0) p.v.y = -p.v.y;
+  out[i] = p;
+}
+
 // <x,y> = sum conj(x) y over complex double fields of n points.
 __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
                                              const Spin<double>* __restrict__ y, size_t n,
@@ -334,7 +349,7 @@ Dirac::~Dirac() {
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
                   (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
                   (void*)uSh_, (void*)uTh_, (void*)uStg_, (void*)uPolyH_, (void*)rotH_, (void*)phi64_,
-                  (void*)uStgP_, (void*)uPolyP_,
+                  (void*)uStgP_, (void*)uPolyP_, (void*)uSiteP_,
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
@@ -430,6 +445,9 @@ void Dirac::fill_temporal_gauge() {
     hipLaunchKernelGGL(k_angle_pairs, dim3((nS + B - 1) / B), dim3(B), 0, stream_, uStg_, uStgP_, nS);
     hipLaunchKernelGGL(k_angle_pairs, dim3((h_.V + B - 1) / B), dim3(B), 0, stream_, uPolyH_, uPolyP_,
                        size_t(h_.V));
+    const size_t nq = size_t(h_.Lt) * h_.max_nn * h_.V;
+    hipLaunchKernelGGL(k_site_pairs, dim3((nq + B - 1) / B), dim3(B), 0, stream_, uStgP_, d32_.lnk, uSiteP_, h_.V,
+                       h_.E, h_.max_nn, h_.Lt);
   }
   QED3_HIP(hipGetLastError());
 }
@@ -439,6 +457,7 @@ DiracView<float, phasepair_t> Dirac::view_half_tg_pairs() {
   if (!uStgP_) {
     uStgP_ = dalloc<phasepair_t>(size_t(h_.Lt) * h_.E);
     uPolyP_ = dalloc<phasepair_t>(size_t(h_.V));
+    uSiteP_ = dalloc<phasepair_t>(size_t(h_.Lt) * h_.max_nn * h_.V);
     if (has_gauge_) fill_temporal_gauge();
   }
   DiracView<float, phasepair_t> v;
@@ -451,6 +470,7 @@ DiracView<float, phasepair_t> Dirac::view_half_tg_pairs() {
   v.uS = uStgP_;
   v.uT = nullptr;
   v.uPoly = uPolyP_;
+  v.uSite = uSiteP_;
   v.V = tg.V;
   v.E = tg.E;
   v.Lt = tg.Lt;

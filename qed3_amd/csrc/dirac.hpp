@@ -29,6 +29,10 @@ struct DiracView {
   // Temporal gauge (set instead of uT): every temporal link is 1 except those from the last
   // slice to the first, whose phase is the product of the site's temporal phases ([V]).
   const P* uPoly = nullptr;
+  // Per site and slot (set by the temporal-gauge pair view): the phase of the slot's link as
+  // the site sees it, conjugated where the link is stored in the opposite orientation, so
+  // that a wave reads consecutive entries ([Lt][nslot][V]).
+  const P* uSite = nullptr;
   int V = 0, E = 0, Lt = 1;
   T r = 1;
   T bc = -1;  // sign on the time wrap: -1 antiperiodic, +1 periodic
@@ -196,6 +200,7 @@ class Dirac {
   phase16_t* rotH_ = nullptr;   // [Lt][V] phi_s(x)
   phasepair_t* uStgP_ = nullptr;   // [Lt][E] uStg_ as (cos, sin) pairs
   phasepair_t* uPolyP_ = nullptr;  // [V] uPolyH_ as (cos, sin) pairs
+  phasepair_t* uSiteP_ = nullptr;  // [Lt][nslot][V] uStgP_ per site and slot (DiracView::uSite)
   double* phi64_ = nullptr;     // [Lt][V] phi_s(x) in FP64 while the tables are built
   void fill_temporal_gauge();
   double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)

@@ -397,7 +397,8 @@ constexpr int kTcBlock = 256;  // threads per block (S sites x C lanes)
 // FUSE (K^dagger's last layer only): the kernel then also applies the first layer of K to
 // the output it has just stored, from the same registers: gout2 = T^{-1}(cj2 gout) with
 // gout as stored (binary16), which saves K's first launch and its read of gout.
-template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi, bool FUSE = false, bool UL = false>
+template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi, bool FUSE = false, bool UL = false,
+          bool H2 = false>
 __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc(HornerView hv, DiracView<float, phasepair_t> d,
                                                          HaloView halo, Src src, const SpinH* __restrict__ gin,
                                                          SpinH* __restrict__ gout, float2 cj, TileMap tm, Epi epi,
@@ -440,22 +441,15 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   const int hid0 = (sl < nh) ? halo.halo[h0 + sl] : -1;
   const int hid1 = (sl + S < nh) ? halo.halo[h0 + sl + S] : -1;
   const unsigned hc0 = hid0 >= 0 ? unsigned(hid0) : xc, hc1 = hid1 >= 0 ? unsigned(hid1) : xc;
-  // neighbour slots in the tile, link indices, and the sign bit that conjugates the phase of
-  // a link stored in the opposite orientation
+  // neighbour slots in the tile
   int nb[NS];
-  unsigned li[NS], usg[NS];
 #pragma unroll
-  for (int j = 0; j < NS; j++) nb[j] = 0, li[j] = 0u, usg[j] = 0u;
+  for (int j = 0; j < NS; j++) nb[j] = 0;
   float invc = 0.f, kt = 0.f;
   if (act) {
     if constexpr (!FIRST) {
 #pragma unroll
-      for (int j = 0; j < NS; j++) {
-        nb[j] = halo.nbl[j * V + x];
-        const int l = d.lnk[j * V + x];
-        li[j] = l >= 0 ? unsigned(l) : unsigned(~l);
-        usg[j] = l < 0 ? 0x80000000u : 0u;
-      }
+      for (int j = 0; j < NS; j++) nb[j] = halo.nbl[j * V + x];
       // the site's hop rows go to LDS once (lane 0), read per step by all its lanes
       if (lane == 0) {
         const size_t mF = size_t((ADJ ? 2 : 0) + rF) * NS * V, mB = size_t((ADJ ? 2 : 0) + rB) * NS * V;
@@ -501,31 +495,29 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   // buffer loads: the slice's byte offset in the scalar offset when it is wave-uniform (UL),
   // the thread's site or link in the vector offset
   const unsigned fbytes = unsigned(V) * unsigned(Lt) * unsigned(sizeof(SpinH));
-  const unsigned ubytes = unsigned(d.E) * unsigned(Lt) * unsigned(sizeof(phasepair_t));
+  const unsigned ubytes = unsigned(V) * unsigned(NS) * unsigned(Lt) * unsigned(sizeof(phasepair_t));
   const unsigned vx = xc * unsigned(sizeof(SpinH)), vh0 = hc0 * unsigned(sizeof(SpinH)),
-                 vh1 = hc1 * unsigned(sizeof(SpinH));
-  unsigned vl[NS];
-#pragma unroll
-  for (int q = 0; q < NS; q++) vl[q] = li[q] * unsigned(sizeof(phasepair_t));
+                 vh1 = hc1 * unsigned(sizeof(SpinH)), vxu = xc * unsigned(sizeof(phasepair_t));
   auto load = [&](int j, Loads& Ld) {
     const unsigned s = unsigned(min(s0 + j, Lt - 1));
-    const unsigned oG = s * unsigned(V) * unsigned(sizeof(SpinH));       // the slice's field offset
-    const unsigned oU = s * unsigned(d.E) * unsigned(sizeof(phasepair_t));  // ... and link offset
+    const unsigned oG = s * unsigned(V) * unsigned(sizeof(SpinH));  // the slice's field offset
+    const unsigned oU = s * unsigned(NS * V) * unsigned(sizeof(phasepair_t));  // ... and phases
     const unsigned sG = UL ? oG : 0u, sU = UL ? oU : 0u, vG = UL ? 0u : oG, vU = UL ? 0u : oU;
     Ld.v = in.bat(fbytes, vG + vx, sG);
     if constexpr (!FIRST) {
 #pragma unroll
-      for (int q = 0; q < NS; q++) Ld.u[q] = bload_u(d.uS, ubytes, vU + vl[q], sU);
+      for (int q = 0; q < NS; q++)
+        Ld.u[q] = bload_u(d.uSite, ubytes, vU + vxu, sU + unsigned(q * V) * unsigned(sizeof(phasepair_t)));
       Ld.gc = bload_h(gin, fbytes, vG + vx, sG);
       Ld.gh0 = bload_h(gin, fbytes, vG + vh0, sG);
-      Ld.gh1 = bload_h(gin, fbytes, vG + vh1, sG);
+      if constexpr (H2) Ld.gh1 = bload_h(gin, fbytes, vG + vh1, sG);
     }
   };
   auto transport = [&](const Loads& Ld, const SpinH* tb, const HopRow16* hq, int r) -> float2 {
     half2_t t16 = {(_Float16)0.f, (_Float16)0.f};
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      const half2_t u = __builtin_bit_cast(half2_t, __builtin_bit_cast(unsigned, Ld.u[q].v) ^ usg[q]);
+      const half2_t u = Ld.u[q].v;
       const SpinH g = tb[nb[q]];
       t16 += cmul16(u, cmul16(hq[q].s, comp16(g, r)) + cmul16(hq[q].f, comp16(g, 1 - r)));
     }
@@ -548,8 +540,10 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
       if (j < lenC) tb[sl] = cur.gc;
       if (live && s0 + j < Lt) {
         if (hid0 >= 0) tb[S + sl] = cur.gh0;
-        if (hid1 >= 0) tb[2 * S + sl] = cur.gh1;
-        for (int q = sl + 2 * S; q < nh; q += S) tb[S + q] = gin[size_t(s0 + j) * V + halo.halo[h0 + q]];
+        if constexpr (H2) {  // a halo longer than the block's range
+          if (hid1 >= 0) tb[2 * S + sl] = cur.gh1;
+          for (int q = sl + 2 * S; q < nh; q += S) tb[S + q] = gin[size_t(s0 + j) * V + halo.halo[h0 + q]];
+        }
       }
       __syncthreads();
     }
@@ -724,7 +718,7 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
   if (gout2 && !ADJ) throw std::logic_error("only a K^dagger layer carries K's first layer");
   if (gout2 && (gout2 == gout || gout2 == gin)) throw std::logic_error("fused output aliases an operand");
   const int C = tc_lanes(hv.Lt), S = kTcBlock / C;
-  if (!tables.uPoly) throw std::logic_error("parallel-in-time layer needs the temporal gauge");
+  if (!tables.uPoly || !tables.uSite) throw std::logic_error("parallel-in-time layer needs the temporal gauge");
   if (!FIRST && !halo.nbl) throw std::logic_error("parallel-in-time layer launched without halo tables");
   TileMap tm;
   tm.nxr = (hv.V + S - 1) / S;
@@ -738,29 +732,37 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
       (FIRST ? 0 : size_t(S) * 2 * hv.nslot * sizeof(HopRow16));
   if (smem > 64 * 1024) throw std::invalid_argument("parallel-in-time layer: halo too large for LDS");
   if (size_t(hv.V) * size_t(hv.Lt) * sizeof(SpinH) > 0xffffffffull ||
-      size_t(tables.E) * size_t(hv.Lt) * sizeof(phasepair_t) > 0xffffffffull)
+      size_t(hv.V) * size_t(hv.nslot) * size_t(hv.Lt) * sizeof(phasepair_t) > 0xffffffffull)
     throw std::invalid_argument("parallel-in-time layer: fields beyond 32-bit offsets");
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
-  auto go = [&](auto fuse, auto ul) {
+  auto go = [&](auto fuse, auto ul, auto h2) {
     constexpr bool F = decltype(fuse)::value && ADJ;
     constexpr bool U = decltype(ul)::value;
+    constexpr bool H = decltype(h2)::value && !FIRST;
     if (hv.nslot == 6)
-      hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F, U>), dim3(nblk), dim3(kTcBlock),
-                         smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+      hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H>), dim3(nblk),
+                         dim3(kTcBlock), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
     else
-      hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F, U>), dim3(nblk), dim3(kTcBlock),
-                         smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+      hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H>), dim3(nblk),
+                         dim3(kTcBlock), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
   };
-  // the wave-uniform form where a wave's 64 threads belong to one lane (S a multiple of 64)
-  const bool ul = S % 64 == 0;
-  if (gout2 && ul)
-    go(std::true_type{}, std::true_type{});
-  else if (gout2)
-    go(std::true_type{}, std::false_type{});
-  else if (ul)
-    go(std::false_type{}, std::true_type{});
+  // the wave-uniform form where a wave's 64 threads belong to one lane (S a multiple of 64);
+  // the second halo load only where some block's halo is longer than its range
+  const bool ul = S % 64 == 0, h2 = !FIRST && halo.hmax > S;
+  auto go2 = [&](auto fuse) {
+    if (ul && h2)
+      go(fuse, std::true_type{}, std::true_type{});
+    else if (ul)
+      go(fuse, std::true_type{}, std::false_type{});
+    else if (h2)
+      go(fuse, std::false_type{}, std::true_type{});
+    else
+      go(fuse, std::false_type{}, std::false_type{});
+  };
+  if (gout2)
+    go2(std::true_type{});
   else
-    go(std::false_type{}, std::false_type{});
+    go2(std::false_type{});
   if (bracket) QED3_HIP(hipEventRecord(bracket[1], st));
   if constexpr (Epi::kReduce)
     hipLaunchKernelGGL((k_finalize<Epi>), dim3(1), dim3(256), 0, st, gr.partials, nblk, epi);
