@@ -87,6 +87,20 @@ __global__ void k_site_pairs(const phasepair_t* __restrict__ uS, const int* __re
   out[i] = p;
 }
 
+// The same for 16-bit angles: a reversed link's angle negated
+__global__ void k_site_angles(const phase16_t* __restrict__ uS, const int* __restrict__ lnk,
+                              phase16_t* __restrict__ out, int V, int E, int M, int Lt) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(Lt) * M * V) return;
+  const int x = int(i % V);
+  const size_t sq = i / V;
+  const int q = int(sq % M), s = int(sq / M);
+  const int l = lnk[size_t(q) * V + x];
+  phase16_t a = uS[size_t(s) * E + (l >= 0 ? l : ~l)];
+  if (l < 0) a.v = short(-a.v);
+  out[i] = a;
+}
+
 // <x,y> = sum conj(x) y over complex double fields of n points.
 __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
                                              const Spin<double>* __restrict__ y, size_t n,
@@ -349,7 +363,7 @@ Dirac::~Dirac() {
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
                   (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
                   (void*)uSh_, (void*)uTh_, (void*)uStg_, (void*)uPolyH_, (void*)rotH_, (void*)phi64_,
-                  (void*)uStgP_, (void*)uPolyP_, (void*)uSiteP_,
+                  (void*)uStgP_, (void*)uPolyP_, (void*)uSiteP_, (void*)uSiteH_,
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
@@ -441,11 +455,13 @@ void Dirac::fill_temporal_gauge() {
   const size_t nS = size_t(h_.Lt) * h_.E;
   hipLaunchKernelGGL(k_tg_links, dim3((nS + B - 1) / B), dim3(B), 0, stream_, angS_, phi64_,
                      reinterpret_cast<const int4*>(link_ends_), uStg_, h_.E, h_.V, h_.Lt);
+  const size_t nq = size_t(h_.Lt) * h_.max_nn * h_.V;
+  hipLaunchKernelGGL(k_site_angles, dim3((nq + B - 1) / B), dim3(B), 0, stream_, uStg_, d32_.lnk, uSiteH_, h_.V,
+                     h_.E, h_.max_nn, h_.Lt);
   if (uStgP_) {
     hipLaunchKernelGGL(k_angle_pairs, dim3((nS + B - 1) / B), dim3(B), 0, stream_, uStg_, uStgP_, nS);
     hipLaunchKernelGGL(k_angle_pairs, dim3((h_.V + B - 1) / B), dim3(B), 0, stream_, uPolyH_, uPolyP_,
                        size_t(h_.V));
-    const size_t nq = size_t(h_.Lt) * h_.max_nn * h_.V;
     hipLaunchKernelGGL(k_site_pairs, dim3((nq + B - 1) / B), dim3(B), 0, stream_, uStgP_, d32_.lnk, uSiteP_, h_.V,
                        h_.E, h_.max_nn, h_.Lt);
   }
@@ -483,6 +499,7 @@ DiracView<float, phase16_t> Dirac::view_half_temporal_gauge() {
   DiracView<float, phase16_t> v = view_half_phases();
   if (!uStg_) {
     uStg_ = dalloc<phase16_t>(size_t(h_.Lt) * h_.E);
+    uSiteH_ = dalloc<phase16_t>(size_t(h_.Lt) * h_.max_nn * h_.V);
     uPolyH_ = dalloc<phase16_t>(size_t(h_.V));
     rotH_ = dalloc<phase16_t>(size_t(h_.Lt) * h_.V);
     phi64_ = dalloc<double>(size_t(h_.Lt) * h_.V);
@@ -491,6 +508,7 @@ DiracView<float, phase16_t> Dirac::view_half_temporal_gauge() {
   v.uS = uStg_;
   v.uT = nullptr;
   v.uPoly = uPolyH_;
+  v.uSite = uSiteH_;
   return v;
 }
 

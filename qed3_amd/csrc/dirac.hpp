@@ -201,6 +201,7 @@ class Dirac {
   phasepair_t* uStgP_ = nullptr;   // [Lt][E] uStg_ as (cos, sin) pairs
   phasepair_t* uPolyP_ = nullptr;  // [V] uPolyH_ as (cos, sin) pairs
   phasepair_t* uSiteP_ = nullptr;  // [Lt][nslot][V] uStgP_ per site and slot (DiracView::uSite)
+  phase16_t* uSiteH_ = nullptr;    // [Lt][nslot][V] uStg_ per site and slot (DiracView::uSite)
   double* phi64_ = nullptr;     // [Lt][V] phi_s(x) in FP64 while the tables are built
   void fill_temporal_gauge();
   double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)

@@ -124,6 +124,20 @@ __device__ __forceinline__ cplx_t<T> temporal_phase(const DiracView<T, P>& d, in
     return ldu(d.uT + size_t(s) * d.V + x);
 }
 
+// Phase of the spatial link in slot j of site x at slice s (l = the slot's link id or ~id):
+// in the temporal gauge from the view's per-site table (consecutive sites, consecutive
+// entries; orientation applied), otherwise through the link table.
+template <bool TG, int NS, typename T, typename P>
+__device__ __forceinline__ cplx_t<T> slot_phase(const DiracView<T, P>& d, int s, int x, int j, int l) {
+  if constexpr (TG) {
+    return ldu(d.uSite + (size_t(s) * NS + j) * d.V + x);
+  } else {
+    cplx_t<T> u = ldu(d.uS + size_t(s) * d.E + (l >= 0 ? l : ~l));
+    if (l < 0) u.y = -u.y;
+    return u;
+  }
+}
+
 // Calls body(std::bool_constant<TG>) with TG = whether the view is in the temporal gauge,
 // decided once per block (a uniform branch on a kernel argument).  Only the 16-bit-angle
 // views of the half-storage solver have a temporal-gauge form.
@@ -206,13 +220,10 @@ __device__ __forceinline__ void stencil_site_src(const DiracView<T, P>& d, const
       typename EpiPre<Epi>::type pre{};
       if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(size_t(s) * V + x);
       C o0 = cscale<T>(c, cur.a), o1 = cscale<T>(c, cur.b);
-      const P* uS = d.uS + size_t(s) * d.E;
       const size_t sV = size_t(s) * V;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
-        const int l = lk[j];
-        C u = ldu(uS + (l >= 0 ? l : ~l));
-        if (l < 0) u.y = -u.y;
+        const C u = slot_phase<TG, NS>(d, s, x, j, lk[j]);
         const Spin<T> py = in(sV + nb[j]);
         const C p0 = cmul<T>(u, py.a), p1 = cmul<T>(u, py.b);
         const T Cc = hp[j].x, Ss = hp[j].y, A = hp[j].z, B = hp[j].w;
@@ -342,13 +353,8 @@ __device__ __forceinline__ void stencil_lds_block(const DiracView<T, P>& d, cons
       next = in(size_t(sp) * V + x);
       if constexpr (EpiPre<Epi>::value) pre = epi.prefetch(sV + x);
       ucur = temporal_phase<TG>(d, s, x);
-      const P* uS = d.uS + size_t(s) * d.E;
 #pragma unroll
-      for (int j = 0; j < NS; j++) {
-        const int l = lk[j];
-        uu[j] = ldu(uS + (l >= 0 ? l : ~l));
-        if (l < 0) uu[j].y = -uu[j].y;
-      }
+      for (int j = 0; j < NS; j++) uu[j] = slot_phase<TG, NS>(d, s, x, j, lk[j]);
     }
     Spin<T>* tb = tileS + (s & 1) * (R + hv.hmax);
     if (act) tb[threadIdx.x] = cur;
