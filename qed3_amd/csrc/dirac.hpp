@@ -86,8 +86,8 @@ struct CGScalars {
   int iter, n_ru, maxiter;
   int rescale;     // half-storage CG: the residual is to be re-stored at the new scale
   int last_pos;    // half-storage CG: group position of the last iteration run
-  int rpar;        // preconditioned CG: which of the two residual buffers is current
   int fresh;       // preconditioned CG: no previous direction (beta = 0)
+  double rpend;    // preconditioned CG: factor the stored residual still owes (1 = none)
 };
 
 class Dirac {

@@ -103,6 +103,20 @@ struct SrcF {
   __device__ SrcF bind() const { return *this; }
   __device__ F raw(size_t i) const { return p[i]; }
   __device__ F bat(unsigned nbytes, unsigned voff, unsigned soff) const { return bload_h(p, nbytes, voff, soff); }
+  __device__ float2 scale(float2 c) const { return c; }
+};
+// binary16 field whose stored values still owe the factor *f (parallel-in-time layer only):
+// the layer folds it into its injection coefficient
+struct SrcFS {
+  const SpinH* p;
+  const double* f;
+  struct Bound {
+    const SpinH* p;
+    float f;
+    __device__ SpinH bat(unsigned nbytes, unsigned voff, unsigned soff) const { return bload_h(p, nbytes, voff, soff); }
+    __device__ float2 scale(float2 c) const { return {c.x * f, c.y * f}; }
+  };
+  __device__ Bound bind() const { return {p, float(*f)}; }
 };
 // binary16 field chosen on the device: buffer par[0] & 1 of two (the solver's
 // double-buffered residual)
@@ -126,6 +140,7 @@ struct SrcDots {
   __device__ SpinH bside(unsigned nbytes, unsigned voff, unsigned soff) const {
     return bload_h(o, nbytes, voff, soff);
   }
+  __device__ float2 scale(float2 c) const { return c; }
 };
 template <class S, class = void>
 struct SrcHasDots : std::false_type {};
@@ -467,6 +482,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   const HopRow16* hqB = hqs + (sl * 2 + 1) * NS;
   const float a = invc * kt;
   const auto in = src.bind();
+  const float2 cjs = in.scale(cj);  // the injection's coefficient with what its values owe
   // the wrap terms, weighted: the forward row's on the first slice of lane 0, the backward
   // row's on the last slice of the last lane
   float2 wF = {0.f, 0.f}, wB = {0.f, 0.f};
@@ -557,7 +573,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
           dots.y += o.a.x * v.a.x + o.a.y * v.a.y + o.b.x * v.b.x + o.b.y * v.b.y;
         }
       }
-      float2 yF = cmul<float>(cj, comp(cur.v, rF)), yB = cmul<float>(cj, comp(cur.v, rB));
+      float2 yF = cmul<float>(cjs, comp(cur.v, rF)), yB = cmul<float>(cjs, comp(cur.v, rB));
       if constexpr (!FIRST) {
         const float2 tF = transport(cur, tb, hqF, rF), tB = transport(cur, tb, hqB, rB);
         yF.x -= tF.x;

@@ -51,10 +51,12 @@ struct EpiK2F {
   const SpinH* ring[kFGroup];
   CGScalars* sc;
   float alpha;
+  float f;  // the factor the stored r still owes after a reliable update (else 1)
   float ap[kFGroup - 1];
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
+    f = float(sc->rpend);
 #pragma unroll
     for (int j = 0; j < kFGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
   }
@@ -81,10 +83,10 @@ struct EpiK2F {
       }
       st(x + i, xx);
     }
-    rr.a.x -= alpha * q.a.x;
-    rr.a.y -= alpha * q.a.y;
-    rr.b.x -= alpha * q.b.x;
-    rr.b.y -= alpha * q.b.y;
+    rr.a.x = f * rr.a.x - alpha * q.a.x;
+    rr.a.y = f * rr.a.y - alpha * q.a.y;
+    rr.b.x = f * rr.b.x - alpha * q.b.x;
+    rr.b.y = f * rr.b.y - alpha * q.b.y;
     const SpinH rh = to_half(rr);
     sth_nt(r + i, rh);
     acc.x += norm2(from_half(rh));
@@ -98,6 +100,7 @@ struct EpiK2F {
     }
     sc->last_pos = POS;
     sc->iter += 1;
+    sc->rpend = 1.0;  // r is re-stored at the current scale
     if (!(m >= 0.0 && m < 1e300)) {
       sc->done = 2;
       return;
@@ -193,6 +196,7 @@ struct FinStartF {
   __device__ void operator()(double2 t) const {
     half_start(sc, t.x, ncomp, tol2, delta2, maxiter);
     sc->fresh = 1;
+    sc->rpend = 1.0;
     sc->rho = 0.0;
   }
 };
@@ -212,14 +216,37 @@ __global__ __launch_bounds__(256) void k_store_b0_f(const Spin<double>* __restri
   }
 }
 
-// after a reliable update: r = exp(i phi) rfac rt
-__global__ __launch_bounds__(256) void k_rescale_f(const Spin<float>* __restrict__ rt, SpinH* r, size_t n,
-                                                   const CGScalars* sc, const phase16_t* rot) {
-  if (sc->done != 0 || sc->rescale == 0) return;
-  const float f = float(sc->rfac);
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
-    half_rescale_point(rt, r, f, i, rot);
-}
+// U3 of the preconditioned solve: r = b - D^dag t in FP64, stored in binary16 in the temporal
+// gauge at the outgoing scale (its values near delta, so a pass that re-stores it at the new
+// scale is not needed: its readers, K^dagger's layers and K2, apply rfac until K2 has
+// re-stored it); |r|^2 decides convergence, the new scale and beta as for the half-storage CG
+struct EpiResidF {
+  static constexpr bool kReduce = true;
+  const Spin<double>* b;
+  SpinH* r;
+  CGScalars* sc;
+  const phase16_t* rot;
+  double inv_ncomp;
+  double inv_so;
+  __device__ bool skip() const { return sc->done != 0 || sc->need_ru == 0; }
+  __device__ void begin() { inv_so = 1.0 / sc->scale; }
+  __device__ void operator()(size_t i, const Spin<double>& v, double2& acc) const {
+    const Spin<double> bb = ld(b + i);
+    Spin<double> rr;
+    rr.a = {bb.a.x - v.a.x, bb.a.y - v.a.y};
+    rr.b = {bb.b.x - v.b.x, bb.b.y - v.b.y};
+    acc.x += norm2(rr);
+    Spin<float> rf;
+    rf.a = {float(rr.a.x * inv_so), float(rr.a.y * inv_so)};
+    rf.b = {float(rr.b.x * inv_so), float(rr.b.y * inv_so)};
+    rotate(rf, rot, i, false);
+    sth_nt(r + i, to_half(rf));
+  }
+  __device__ void operator()(double2 t) const {
+    EpiResidH{b, nullptr, sc, inv_ncomp, 1.0}(t);
+    if (sc->done == 0) sc->rpend = sc->rfac;
+  }
+};
 
 }  // namespace
 
@@ -365,7 +392,6 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   SpinH* w = static_cast<SpinH*>(D.scratch(3, hb));
   Spin<float>* xin = static_cast<Spin<float>*>(D.scratch(4, fb));
   Spin<double>* t = static_cast<Spin<double>*>(D.scratch(5, n * sizeof(Spin<double>)));
-  Spin<float>* rt = static_cast<Spin<float>*>(D.scratch(6, fb));
   CGScalars* sc = static_cast<CGScalars*>(D.scratch(7, sizeof(CGScalars)));
   CGScalars* hsc = D.host_scalars();
   SpinH* R = static_cast<SpinH*>(K.work(0, hb));
@@ -397,7 +423,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   PendRing8 pend;
   for (int j = 0; j < kFGroup - 1; j++) pend.p[j] = ring[j];
   EventTimer timer(o.times), ltimer(layer_times);
-  const hk::SrcF<SpinH> src_r{R};
+  const hk::SrcFS src_r{R, &sc->rpend};
   const EpiSkip eskip{sc};
   const EpiFcg efcg{sc};
 
@@ -423,9 +449,8 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   auto reliable_update = [&] {
     hipLaunchKernelGGL(k_flush_f, dim3(ngate), dim3(256), 0, st, x, xin, pend, n, sc, 0, rot);
     launch_stencil<double, false>(v64, nn, x, EpiStoreIfRU{t, sc}, D.tile<double>(false), gr, st, hv64f);
-    launch_stencil<double, true>(v64, nn, t, EpiResidH{b, rt, sc, inv_ncomp, 1.0}, D.tile<double>(true), gr, st,
+    launch_stencil<double, true>(v64, nn, t, EpiResidF{b, R, sc, rot, inv_ncomp, 1.0}, D.tile<double>(true), gr, st,
                                  hv64t);
-    hipLaunchKernelGGL(k_rescale_f, dim3(ngate), dim3(256), 0, st, rt, R, n, sc, rot);
   };
   auto one_iter = [&](int i, int k) {
     const int pos = i % kFGroup;
