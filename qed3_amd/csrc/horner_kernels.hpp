@@ -407,7 +407,8 @@ inline void launch_horner_layer(const HornerView& hv, const DiracView<float, P>&
 // their halo (halo tables of ranges of S sites).
 constexpr int kTcChunk = 16;   // slices per lane
 constexpr int kTcDepth = 2;    // steps whose loads are in flight ahead of the one computed
-constexpr int kTcBlock = 256;  // threads per block (S sites x C lanes)
+constexpr int kTcBlock = 256;  // most threads per block (S sites x C lanes)
+constexpr int kTcSites = 256;  // most sites per block
 
 // FUSE (K^dagger's last layer only): the kernel then also applies the first layer of K to
 // the output it has just stored, from the same registers: gout2 = T^{-1}(cj2 gout) with
@@ -715,7 +716,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
 }
 
 // Lanes per site of the parallel-in-time layer: C = ceil(Lt / T) rounded up to a power of
-// two (C * S = kTcBlock threads); at most 64.
+// two (C * S <= kTcBlock threads, S <= kTcSites sites); at most 64.
 inline int tc_lanes(int Lt) {
   const int c = (Lt + kTcChunk - 1) / kTcChunk;
   int p = 1;
@@ -723,7 +724,7 @@ inline int tc_lanes(int Lt) {
   if (p > 64) throw std::invalid_argument("time extent too long for the parallel-in-time layer");
   return p;
 }
-inline int tc_sites(int Lt) { return kTcBlock / tc_lanes(Lt); }
+inline int tc_sites(int Lt) { return std::min(kTcSites, kTcBlock / tc_lanes(Lt)); }
 
 // gout2 != nullptr (ADJ only): also K's first layer with coefficient cj2 into gout2 (FUSE).
 template <bool ADJ, bool FIRST, class Src, class Epi>
@@ -733,7 +734,7 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
                                    SpinH* gout2 = nullptr, float2 cj2 = {0.f, 0.f}) {
   if (gout2 && !ADJ) throw std::logic_error("only a K^dagger layer carries K's first layer");
   if (gout2 && (gout2 == gout || gout2 == gin)) throw std::logic_error("fused output aliases an operand");
-  const int C = tc_lanes(hv.Lt), S = kTcBlock / C;
+  const int C = tc_lanes(hv.Lt), S = tc_sites(hv.Lt);
   if (!tables.uPoly || !tables.uSite) throw std::logic_error("parallel-in-time layer needs the temporal gauge");
   if (!FIRST && !halo.nbl) throw std::logic_error("parallel-in-time layer launched without halo tables");
   TileMap tm;
@@ -757,10 +758,10 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
     constexpr bool H = decltype(h2)::value && !FIRST;
     if (hv.nslot == 6)
       hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H>), dim3(nblk),
-                         dim3(kTcBlock), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+                         dim3(C * S), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
     else
       hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H>), dim3(nblk),
-                         dim3(kTcBlock), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
+                         dim3(C * S), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
   };
   // the wave-uniform form where a wave's 64 threads belong to one lane (S a multiple of 64);
   // the second halo load only where some block's halo is longer than its range
