@@ -87,6 +87,8 @@ struct CGScalars {
   int rescale;     // half-storage CG: the residual is to be re-stored at the new scale
   int last_pos;    // half-storage CG: group position of the last iteration run
   int fresh;       // preconditioned CG: no previous direction (beta = 0)
+  int x_set;       // preconditioned CG: x holds a flushed value (else it is taken as 0)
+  int xin_set;     // preconditioned CG: x_inner holds folded terms (else it is taken as 0)
   double rpend;    // preconditioned CG: factor the stored residual still owes (1 = none)
 };
 

@@ -52,11 +52,13 @@ struct EpiK2F {
   CGScalars* sc;
   float alpha;
   float f;  // the factor the stored r still owes after a reliable update (else 1)
+  bool xset;  // x_inner holds earlier folds (else it is taken as 0)
   float ap[kFGroup - 1];
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
     f = float(sc->rpend);
+    xset = kUpdateX && sc->xin_set != 0;
 #pragma unroll
     for (int j = 0; j < kFGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
   }
@@ -69,7 +71,13 @@ struct EpiK2F {
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const SpinH& rpre) const {
     Spin<float> rr = from_half(rpre);
     if constexpr (kUpdateX) {
-      Spin<float> xx = ld(x + i);
+      Spin<float> xx;
+      if (xset) {
+        xx = ld(x + i);
+      } else {
+        xx.a = {0.f, 0.f};
+        xx.b = {0.f, 0.f};
+      }
 #pragma unroll
       for (int j = 0; j < kFGroup; j++) {
         const float a = j == kFGroup - 1 ? alpha : ap[j];
@@ -95,6 +103,7 @@ struct EpiK2F {
     const double m = t.x;
     if constexpr (kUpdateX) {
       for (int j = 0; j < kFGroup - 1; j++) sc->alpha_pend[j] = 0.0;
+      sc->xin_set = 1;
     } else {
       sc->alpha_pend[POS] = sc->alpha;
     }
@@ -117,8 +126,10 @@ struct PendRing8 {
   const SpinH* p[kFGroup - 1];
 };
 
-// x64 += scale exp(-i phi) (x_inner + sum_j alpha_pend[j] ring[j]), x_inner = 0 (before a
-// reliable update, or always at the end)
+// x64 += scale exp(-i phi) (x_inner + sum_j alpha_pend[j] ring[j]) (before a reliable
+// update, or always at the end).  Neither x64 nor x_inner is cleared at the start: each is
+// read only once it holds a value (sc->x_set, sc->xin_set), and x_inner counts as cleared
+// again after the update (its flag is reset by the update's last kernel).
 __global__ __launch_bounds__(256) void k_flush_f(Spin<double>* x64, Spin<float>* xin, PendRing8 ring, size_t n,
                                                  const CGScalars* sc, int always, const phase16_t* rot) {
   if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
@@ -126,8 +137,15 @@ __global__ __launch_bounds__(256) void k_flush_f(Spin<double>* x64, Spin<float>*
 #pragma unroll
   for (int j = 0; j < kFGroup - 1; j++) ap[j] = float(sc->alpha_pend[j]);
   const double s = sc->scale;
+  const bool xset = sc->x_set != 0, iset = sc->xin_set != 0;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    Spin<float> a = xin[i];
+    Spin<float> a;
+    if (iset) {
+      a = xin[i];
+    } else {
+      a.a = {0.f, 0.f};
+      a.b = {0.f, 0.f};
+    }
 #pragma unroll
     for (int j = 0; j < kFGroup - 1; j++) {
       if (ap[j] != 0.f) {  // uniform
@@ -139,17 +157,34 @@ __global__ __launch_bounds__(256) void k_flush_f(Spin<double>* x64, Spin<float>*
       }
     }
     rotate(a, rot, i, true);
-    Spin<double> xx = x64[i];
+    Spin<double> xx;
+    if (xset) {
+      xx = x64[i];
+    } else {
+      xx.a = {0.0, 0.0};
+      xx.b = {0.0, 0.0};
+    }
     xx.a.x += s * a.a.x;
     xx.a.y += s * a.a.y;
     xx.b.x += s * a.b.x;
     xx.b.y += s * a.b.y;
     x64[i] = xx;
-    Spin<float> z;
-    z.a = {0.f, 0.f};
-    z.b = {0.f, 0.f};
-    xin[i] = z;
   }
+}
+
+// Start: |b|^2 and p_old = 0 (x and x_inner are not cleared: see k_flush_f)
+__global__ __launch_bounds__(256) void k_init_f(const Spin<double>* __restrict__ b, SpinH* p, size_t n,
+                                                GridReduce gr) {
+  double2 acc = {0.0, 0.0};
+  Spin<float> z;
+  z.a = {0.f, 0.f};
+  z.b = {0.f, 0.f};
+  const SpinH zh = to_half(z);
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    acc.x += norm2(ld(b + i));
+    p[i] = zh;
+  }
+  store_partial(acc, gr.partials, blockIdx.x);
 }
 
 // the network's layers wait, like K1 and K2, for a pending reliable update
@@ -197,6 +232,8 @@ struct FinStartF {
     half_start(sc, t.x, ncomp, tol2, delta2, maxiter);
     sc->fresh = 1;
     sc->rpend = 1.0;
+    sc->x_set = 0;
+    sc->xin_set = 0;
     sc->rho = 0.0;
   }
 };
@@ -243,6 +280,8 @@ struct EpiResidF {
     sth_nt(r + i, to_half(rf));
   }
   __device__ void operator()(double2 t) const {
+    sc->x_set = 1;    // the update's flush has written x
+    sc->xin_set = 0;  // ... and taken x_inner
     EpiResidH{b, nullptr, sc, inv_ncomp, 1.0}(t);
     if (sc->done == 0) sc->rpend = sc->rfac;
   }
@@ -403,7 +442,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   const GridReduce gr = D.reducer();
   const int nbk = int(stream_blocks(n));
   const int ngate = int(gated_blocks(n));
-  hipLaunchKernelGGL(k_init_h, dim3(nbk), dim3(256), 0, st, b, x, xin, ring[kFGroup - 1], n, gr);
+  hipLaunchKernelGGL(k_init_f, dim3(nbk), dim3(256), 0, st, b, ring[kFGroup - 1], n, gr);
   hipLaunchKernelGGL(k_finalize<FinStartF>, dim3(1), dim3(256), 0, st, gr.partials, unsigned(nbk),
                      FinStartF{sc, 4.0 * double(n), o.tol * o.tol, o.delta * o.delta, o.maxiter});
   const DiracView<float, phase16_t> vin = D.view_half_temporal_gauge();
