@@ -32,12 +32,12 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), keyed by
 # kernel, lattice, n, Lt (and launch shape for the stencils)
-MEASURED_TRAFFIC = {("layer-tc", 1, 128, 64): 7.873e8,
-                    ("f16", 1, 128, 64, 256, 64, 1, 1): 4.9646e8,
-                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.2992e9,
-                    ("K1-half", 1, 128, 64, 256, 64, 1, 1): 8.5079e8,
-                    ("K2-half", 1, 128, 64, 256, 64, 1, 1): 6.3056e8,
-                    ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 1.9851e9}
+MEASURED_TRAFFIC = {("layer-tc", 1, 128, 64): 8.418e8,
+                    ("f16", 1, 128, 64, 256, 64, 1, 1): 5.017e8,
+                    ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3096e9,
+                    ("K1-half", 1, 128, 64, 256, 64, 1, 1): 8.836e8,
+                    ("K2-half", 1, 128, 64, 256, 64, 1, 1): 6.817e8,
+                    ("K2x-half", 1, 128, 64, 256, 64, 1, 1): 2.0910e9}
 
 
 
@@ -339,15 +339,15 @@ def learned_leg(a, q):
     ns = str(nslot)
     pts = D.V * a.Lt
     tl = "k_horner_layer_tc<" + ns + ",{adj},{first},16,{src},{epi},{fuse}>"
-    lk = {"transport_layer_Kdag": ent(tl.format(adj="true", first="false", src="SrcF<SpinH>", epi="EpiSkip",
+    lk = {"transport_layer_Kdag": ent(tl.format(adj="true", first="false", src="SrcFS", epi="EpiSkip",
                                                 fuse="false"),
                                       kms[4], b_layer, MEASURED_TRAFFIC.get(("layer-tc", lattice, a.n, a.Lt))),
           "transport_layer_K": ent(tl.format(adj="false", first="false", src="SrcF<SpinH>", epi="EpiSkip",
                                              fuse="false"), kms[5], b_layer),
-          "first_layer": ent(tl.format(adj="true", first="true", src="SrcF<SpinH>", epi="EpiSkip", fuse="false") +
+          "first_layer": ent(tl.format(adj="true", first="true", src="SrcFS", epi="EpiSkip", fuse="false") +
                              " (recurrence only)", kms[3], b_first),
           # + K's first layer written from the same registers
-          "last_layer_Kdag_with_K_first": ent(tl.format(adj="true", first="false", src="SrcF<SpinH>",
+          "last_layer_Kdag_with_K_first": ent(tl.format(adj="true", first="false", src="SrcFS",
                                                         epi="EpiSkip", fuse="true"), kms[6], b_layer + pts * 8),
           # + the previous H read beside the injection H for |H|^2 = <r,z>, <H_old,H> = <r_old,z>
           "last_layer_K_with_dots": ent(tl.format(adj="false", first="false", src="SrcDots", epi="EpiFcg",
