@@ -72,22 +72,8 @@ __global__ void k_angle_pairs(const phase16_t* __restrict__ a, phasepair_t* __re
   out[i] = p;
 }
 
-// Per site and slot, the phase of the slot's link as the site sees it ([Lt][M][V], M the
-// slots per site): conjugated where the link table gives the link reversed (~id).
-__global__ void k_site_pairs(const phasepair_t* __restrict__ uS, const int* __restrict__ lnk,
-                             phasepair_t* __restrict__ out, int V, int E, int M, int Lt) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= size_t(Lt) * M * V) return;
-  const int x = int(i % V);
-  const size_t sq = i / V;
-  const int q = int(sq % M), s = int(sq / M);
-  const int l = lnk[size_t(q) * V + x];
-  phasepair_t p = uS[size_t(s) * E + (l >= 0 ? l : ~l)];
-  if (l < 0) p.v.y = -p.v.y;
-  out[i] = p;
-}
-
-// The same for 16-bit angles: a reversed link's angle negated
+// Per site and slot, the 16-bit angle of the slot's link as the site sees it ([Lt][M][V], M
+// the slots per site): negated where the link table gives the link reversed (~id).
 __global__ void k_site_angles(const phase16_t* __restrict__ uS, const int* __restrict__ lnk,
                               phase16_t* __restrict__ out, int V, int E, int M, int Lt) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -363,7 +349,7 @@ Dirac::~Dirac() {
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
                   (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
                   (void*)uSh_, (void*)uTh_, (void*)uStg_, (void*)uPolyH_, (void*)rotH_, (void*)phi64_,
-                  (void*)uStgP_, (void*)uPolyP_, (void*)uSiteP_, (void*)uSiteH_,
+                  (void*)uPolyP_, (void*)uSiteH_,
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
@@ -458,22 +444,16 @@ void Dirac::fill_temporal_gauge() {
   const size_t nq = size_t(h_.Lt) * h_.max_nn * h_.V;
   hipLaunchKernelGGL(k_site_angles, dim3((nq + B - 1) / B), dim3(B), 0, stream_, uStg_, d32_.lnk, uSiteH_, h_.V,
                      h_.E, h_.max_nn, h_.Lt);
-  if (uStgP_) {
-    hipLaunchKernelGGL(k_angle_pairs, dim3((nS + B - 1) / B), dim3(B), 0, stream_, uStg_, uStgP_, nS);
+  if (uPolyP_)
     hipLaunchKernelGGL(k_angle_pairs, dim3((h_.V + B - 1) / B), dim3(B), 0, stream_, uPolyH_, uPolyP_,
                        size_t(h_.V));
-    hipLaunchKernelGGL(k_site_pairs, dim3((nq + B - 1) / B), dim3(B), 0, stream_, uStgP_, d32_.lnk, uSiteP_, h_.V,
-                       h_.E, h_.max_nn, h_.Lt);
-  }
   QED3_HIP(hipGetLastError());
 }
 
 DiracView<float, phasepair_t> Dirac::view_half_tg_pairs() {
   const DiracView<float, phase16_t> tg = view_half_temporal_gauge();
-  if (!uStgP_) {
-    uStgP_ = dalloc<phasepair_t>(size_t(h_.Lt) * h_.E);
+  if (!uPolyP_) {
     uPolyP_ = dalloc<phasepair_t>(size_t(h_.V));
-    uSiteP_ = dalloc<phasepair_t>(size_t(h_.Lt) * h_.max_nn * h_.V);
     if (has_gauge_) fill_temporal_gauge();
   }
   DiracView<float, phasepair_t> v;
@@ -483,10 +463,10 @@ DiracView<float, phasepair_t> Dirac::view_half_tg_pairs() {
   v.hopd = tg.hopd;
   v.diag = tg.diag;
   v.kth = tg.kth;
-  v.uS = uStgP_;
+  v.uS = nullptr;  // the spatial phases only per site and slot, as angles (uSiteA)
   v.uT = nullptr;
   v.uPoly = uPolyP_;
-  v.uSite = uSiteP_;
+  v.uSiteA = uSiteH_;
   v.V = tg.V;
   v.E = tg.E;
   v.Lt = tg.Lt;

@@ -33,6 +33,7 @@ struct DiracView {
   // the site sees it, conjugated where the link is stored in the opposite orientation, so
   // that a wave reads consecutive entries ([Lt][nslot][V]).
   const P* uSite = nullptr;
+  const phase16_t* uSiteA = nullptr;  // the same as 16-bit angles (the pair view's only spatial phases)
   int V = 0, E = 0, Lt = 1;
   T r = 1;
   T bc = -1;  // sign on the time wrap: -1 antiperiodic, +1 periodic
@@ -200,9 +201,7 @@ class Dirac {
   phase16_t* uStg_ = nullptr;   // [Lt][E] spatial angles in the temporal gauge
   phase16_t* uPolyH_ = nullptr; // [V] the wrap link's angle in the temporal gauge
   phase16_t* rotH_ = nullptr;   // [Lt][V] phi_s(x)
-  phasepair_t* uStgP_ = nullptr;   // [Lt][E] uStg_ as (cos, sin) pairs
   phasepair_t* uPolyP_ = nullptr;  // [V] uPolyH_ as (cos, sin) pairs
-  phasepair_t* uSiteP_ = nullptr;  // [Lt][nslot][V] uStgP_ per site and slot (DiracView::uSite)
   phase16_t* uSiteH_ = nullptr;    // [Lt][nslot][V] uStg_ per site and slot (DiracView::uSite)
   double* phi64_ = nullptr;     // [Lt][V] phi_s(x) in FP64 while the tables are built
   void fill_temporal_gauge();

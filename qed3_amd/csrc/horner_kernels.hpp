@@ -54,8 +54,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uns
 __device__ __forceinline__ SpinH bload_h(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(SpinH, __builtin_amdgcn_raw_buffer_load_b64(buf_rsrc(base, nbytes), voff, soff, 0));
 }
-__device__ __forceinline__ phasepair_t bload_u(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(phasepair_t, __builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(base, nbytes), voff, soff, 0));
+__device__ __forceinline__ short bload_a(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(short, __builtin_amdgcn_raw_buffer_load_b16(buf_rsrc(base, nbytes), voff, soff, 0));
+}
+// e^{i theta} of a 16-bit angle (theta = 2 pi a / 65536) as a binary16 pair; the hardware sine
+// and cosine take revolutions
+__device__ __forceinline__ half2_t angle_pair(short a) {
+  const float rev = float(a) * (1.0f / 65536.0f);
+  return half2_t{(_Float16)__builtin_amdgcn_cosf(rev), (_Float16)__builtin_amdgcn_sinf(rev)};
 }
 
 // component a of a point as FP32 complex
@@ -402,9 +408,12 @@ inline void launch_horner_layer(const HornerView& hv, const DiracView<float, P>&
 // backward row from its end), with carry_k = E_{k-1} + a^T carry_{k-1} (forward) and
 // carry_k = E_{k+1} + a^{len_{k+1}} carry_{k+1} (backward), E the lanes' end values.  So a
 // layer reads its input and injection once and writes its output once, and a site's time
-// column runs on C = ceil(Lt / T) threads (C * S = kTcBlock, S sites per block; the threads of a
-// lane are consecutive, so a wave reads and writes consecutive sites of one slice).  Per step the block stages the C slices its lanes are at, for its S sites and
-// their halo (halo tables of ranges of S sites).
+// column runs on C = ceil(Lt / T) threads (C * S <= kTcBlock, S sites per block; the threads
+// of a lane are consecutive, so a wave reads and writes consecutive sites of one slice).  Per
+// step the block stages the C slices its lanes are at, for its S sites and their halo (halo
+// tables of ranges of S sites).  The spatial link phases come as 16-bit angles per site and
+// slot (the view's uSiteA, orientation applied: a wave's loads are consecutive, 2 B each),
+// turned into binary16 (cos, sin) pairs with the hardware sine and cosine.
 constexpr int kTcChunk = 16;   // slices per lane
 constexpr int kTcDepth = 2;    // steps whose loads are in flight ahead of the one computed
 constexpr int kTcBlock = 256;  // most threads per block (S sites x C lanes)
@@ -503,7 +512,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   }
   struct Loads {
     SpinH v, gc, gh0, gh1;
-    phasepair_t u[NS];
+    short u[NS];  // the slots' link angles
   };
   // slice s0 + j of this lane (own point, links) and slice c T + j of the halo share.  Every
   // thread issues the same loads at every step (lanes past the column's end re-read its last
@@ -512,19 +521,19 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   // buffer loads: the slice's byte offset in the scalar offset when it is wave-uniform (UL),
   // the thread's site or link in the vector offset
   const unsigned fbytes = unsigned(V) * unsigned(Lt) * unsigned(sizeof(SpinH));
-  const unsigned ubytes = unsigned(V) * unsigned(NS) * unsigned(Lt) * unsigned(sizeof(phasepair_t));
+  constexpr unsigned usz = sizeof(phase16_t);
+  const unsigned ubytes = unsigned(V) * unsigned(NS) * unsigned(Lt) * usz;
   const unsigned vx = xc * unsigned(sizeof(SpinH)), vh0 = hc0 * unsigned(sizeof(SpinH)),
-                 vh1 = hc1 * unsigned(sizeof(SpinH)), vxu = xc * unsigned(sizeof(phasepair_t));
+                 vh1 = hc1 * unsigned(sizeof(SpinH)), vxu = xc * usz;
   auto load = [&](int j, Loads& Ld) {
     const unsigned s = unsigned(min(s0 + j, Lt - 1));
     const unsigned oG = s * unsigned(V) * unsigned(sizeof(SpinH));  // the slice's field offset
-    const unsigned oU = s * unsigned(NS * V) * unsigned(sizeof(phasepair_t));  // ... and phases
+    const unsigned oU = s * unsigned(NS * V) * usz;  // ... and phases
     const unsigned sG = UL ? oG : 0u, sU = UL ? oU : 0u, vG = UL ? 0u : oG, vU = UL ? 0u : oU;
     Ld.v = in.bat(fbytes, vG + vx, sG);
     if constexpr (!FIRST) {
 #pragma unroll
-      for (int q = 0; q < NS; q++)
-        Ld.u[q] = bload_u(d.uSite, ubytes, vU + vxu, sU + unsigned(q * V) * unsigned(sizeof(phasepair_t)));
+      for (int q = 0; q < NS; q++) Ld.u[q] = bload_a(d.uSiteA, ubytes, vU + vxu, sU + unsigned(q * V) * usz);
       Ld.gc = bload_h(gin, fbytes, vG + vx, sG);
       Ld.gh0 = bload_h(gin, fbytes, vG + vh0, sG);
       if constexpr (H2) Ld.gh1 = bload_h(gin, fbytes, vG + vh1, sG);
@@ -534,7 +543,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     half2_t t16 = {(_Float16)0.f, (_Float16)0.f};
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      const half2_t u = Ld.u[q].v;
+      const half2_t u = angle_pair(Ld.u[q]);
       const SpinH g = tb[nb[q]];
       t16 += cmul16(u, cmul16(hq[q].s, comp16(g, r)) + cmul16(hq[q].f, comp16(g, 1 - r)));
     }
@@ -735,7 +744,8 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
   if (gout2 && !ADJ) throw std::logic_error("only a K^dagger layer carries K's first layer");
   if (gout2 && (gout2 == gout || gout2 == gin)) throw std::logic_error("fused output aliases an operand");
   const int C = tc_lanes(hv.Lt), S = tc_sites(hv.Lt);
-  if (!tables.uPoly || !tables.uSite) throw std::logic_error("parallel-in-time layer needs the temporal gauge");
+  if (!tables.uPoly || !tables.uSiteA)
+    throw std::logic_error("parallel-in-time layer needs the temporal gauge");
   if (!FIRST && !halo.nbl) throw std::logic_error("parallel-in-time layer launched without halo tables");
   TileMap tm;
   tm.nxr = (hv.V + S - 1) / S;
@@ -749,7 +759,7 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
       (FIRST ? 0 : size_t(S) * 2 * hv.nslot * sizeof(HopRow16));
   if (smem > 64 * 1024) throw std::invalid_argument("parallel-in-time layer: halo too large for LDS");
   if (size_t(hv.V) * size_t(hv.Lt) * sizeof(SpinH) > 0xffffffffull ||
-      size_t(hv.V) * size_t(hv.nslot) * size_t(hv.Lt) * sizeof(phasepair_t) > 0xffffffffull)
+      size_t(hv.V) * size_t(hv.nslot) * size_t(hv.Lt) * sizeof(phase16_t) > 0xffffffffull)
     throw std::invalid_argument("parallel-in-time layer: fields beyond 32-bit offsets");
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
   auto go = [&](auto fuse, auto ul, auto h2) {
