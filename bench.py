@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), keyed by
 # kernel, lattice, n, Lt (and launch shape for the stencils)
-MEASURED_TRAFFIC = {("layer-tc", 1, 128, 64): 8.418e8,
+MEASURED_TRAFFIC = {("layer-tc", 1, 128, 64): 7.074e8,
                     ("f16", 1, 128, 64, 256, 64, 1, 1): 5.017e8,
                     ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3096e9,
                     ("K1-half", 1, 128, 64, 256, 64, 1, 1): 8.836e8,
@@ -129,13 +129,14 @@ def context(a, q, lattice, m, seed):
 def layer_bytes(V, E, Lt, nslot, first):
     """Algorithmic bytes of one network layer launch (binary16 fields, 8 B per point): the
     injection read and the output written (+ the input field read by a transport layer), the
-    link phases of the temporal gauge as binary16 (cos, sin) pairs (4 B per spatial link and
-    per wrap link) and the per-site tables once (hop rows of both spin rows 2 x 8 B per slot,
-    local and link indices 2 + 4 B per slot, 1/c and kappa_t)."""
+    spatial link phases of the temporal gauge as 16-bit angles (2 B per link and slice, each
+    link once), the wrap phase per site as a binary16 (cos, sin) pair (4 B) and the per-site
+    tables once (hop rows of both spin rows 2 x 8 B per slot, a 2 B local index per slot,
+    1/c and kappa_t)."""
     pts = V * Lt
     if first:
         return 2 * pts * 8 + V * 8
-    return 3 * pts * 8 + E * Lt * 4 + V * 4 + V * (nslot * (16 + 6) + 8)
+    return 3 * pts * 8 + E * Lt * 2 + V * 4 + V * (nslot * (16 + 2) + 8)
 
 
 def main():
