@@ -732,6 +732,7 @@ def batched_leg(a, q):
     geos, tuned = {}, {}
     ops, xs, bs, desc = [], [], [], []
     t0 = time.time()
+    shapes = []
     for k in range(a.batch_nsys):
         n = int(rng.choice([32, 64, 96, 128]))
         Lt = int(rng.choice([16, 32, 64]))
@@ -739,10 +740,28 @@ def batched_leg(a, q):
         m = float(rng.uniform(0.02, 0.2))
         if n not in geos:
             geos[n] = q.Geometry(n)
-        D = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=m, at=0.5 * mean_ell(geos[n], lattice))
+        g = geos[n]
+        V, E = (g.F, g.E) if lattice == 1 else (g.V, g.E)
+        shapes.append((n, Lt, lattice, m, V, E))
+
+    def draw(k):  # each system's own seeded streams, drawn on host threads
+        n, Lt, lattice, m, V, E = shapes[k]
         lr = np.random.default_rng(500 + k)
-        sp = lr.vonmises(0.0, a.beta, (Lt, D.E))
-        tp = lr.vonmises(0.0, a.beta, (Lt, D.V))
+        sp = lr.vonmises(0.0, a.beta, (Lt, E))
+        tp = lr.vonmises(0.0, a.beta, (Lt, V))
+        sr = np.random.default_rng(900 + k)
+        b = (sr.standard_normal(2 * V * Lt) + 1j * sr.standard_normal(2 * V * Lt)) / np.sqrt(2.0)
+        return sp, tp, b
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        drawn = list(ex.map(draw, range(a.batch_nsys)))
+    for k in range(a.batch_nsys):
+        n, Lt, lattice, m, V, E = shapes[k]
+        sp, tp, b = drawn[k]
+        drawn[k] = None
+        D = q.WilsonDirac(lattice, n, Lt, r=1.0, M5=m, at=0.5 * mean_ell(geos[n], lattice))
+        assert (D.V, D.E) == (V, E)
         D.update(sp, tp)
         key = (lattice, n, Lt)
         if key not in tuned:
@@ -752,8 +771,6 @@ def batched_leg(a, q):
             for (op, prec), t in tuned[key].items():
                 D.set_tile(t["block"], t["ts"], t["remap"], t["lds"], ops=(op,), precs=(prec,))
         ops.append(D)
-        sr = np.random.default_rng(900 + k)
-        b = (sr.standard_normal(2 * D.V * Lt) + 1j * sr.standard_normal(2 * D.V * Lt)) / np.sqrt(2.0)
         bs.append(q.DeviceArray.from_numpy(b))
         xs.append(q.DeviceArray(b.nbytes))
         desc.append((lattice, n, Lt, round(m, 4)))
