@@ -124,13 +124,7 @@ struct SrcFS {
   };
   __device__ Bound bind() const { return {p, float(*f)}; }
 };
-// binary16 field chosen on the device: buffer par[0] & 1 of two (the solver's
-// double-buffered residual)
-struct SrcPar {
-  const SpinH* p0;
-  const SpinH* p1;
-  const int* par;
-  __device__ SrcF<SpinH> bind() const { return {(*par & 1) ? p1 : p0}; }
+}
 };
 
 // Injection with a side field (parallel-in-time layer only): the layer also accumulates
