@@ -124,8 +124,6 @@ struct SrcFS {
   };
   __device__ Bound bind() const { return {p, float(*f)}; }
 };
-}
-};
 
 // Injection with a side field (parallel-in-time layer only): the layer also accumulates
 // |v|^2 and Re<o, v> over its points (v the injection as stored, o the side field) into the
