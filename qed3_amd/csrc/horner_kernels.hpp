@@ -11,10 +11,9 @@
 //     output point, that row's result beside the other row's y (its recurrence input);
 //   pass 2, s = Lt-1 .. 0: every thread reads its own column back, runs the backward row's
 //     recurrence and stores both rows' results (no neighbours, no barriers).
-// So every neighbour, link phase and input load serves both rows.  Fields are
-// spin-interleaved points: binary16 (SpinH, the solver's path: transports in packed
-// binary16 arithmetic from binary16 (cos, sin) phases and hop coefficients, recurrences in
-// FP32) or FP32 (Spin<float>, the checked apply: all FP32).
+// So every neighbour, link phase and input load serves both rows.  Fields are FP32
+// spin-interleaved points (the checked apply and the training gradient, in any gauge); the
+// solver's binary16 layers are the parallel-in-time form further down.
 // The first layer of a network (FIRST) has no input field: no staging, no barrier.
 #pragma once
 #include <type_traits>
@@ -152,17 +151,8 @@ struct EpiNone {
   __device__ void operator()(double2) const {}
 };
 
-// LDS tile entry: a point as stored (binary16) or as FP32
-template <typename F>
-struct TileT {
-  using type = float4;
-};
-template <>
-struct TileT<SpinH> {
-  using type = SpinH;
-};
+// LDS tile entry of the two-pass layer: an FP32 point
 __device__ __forceinline__ float4 to_tile(const Spin<float>& v) { return {v.a.x, v.a.y, v.b.x, v.b.y}; }
-__device__ __forceinline__ SpinH to_tile(const SpinH& v) { return v; }
 
 template <int NS, typename F, typename P>
 struct SliceLoads {
@@ -176,8 +166,8 @@ template <int NS, bool ADJ, bool FIRST, bool TG, typename F, class Src, class Ep
 __global__ __launch_bounds__(256) void k_horner_layer(HornerView hv, DiracView<float, P> d, HaloView halo, Src src,
                                                       const F* __restrict__ gin, F* __restrict__ gout, float2 cj,
                                                       TileMap tm, Epi epi, GridReduce gr) {
-  constexpr bool kHalf = std::is_same<F, SpinH>::value;
-  using Tile = typename TileT<F>::type;
+  static_assert(std::is_same<F, Spin<float>>::value, "the two-pass layer runs on FP32 fields");
+  using Tile = float4;
   if (epi.skip()) return;  // uniform
   extern __shared__ float4 hk_tile_raw[];
   Tile* hk_tile = reinterpret_cast<Tile*>(hk_tile_raw);
@@ -196,7 +186,6 @@ __global__ __launch_bounds__(256) void k_horner_layer(HornerView hv, DiracView<f
   const int hid1 = (int(threadIdx.x) + R < nh) ? halo.halo[h0 + threadIdx.x + R] : -1;
   int nb[NS], lk[NS];
   float4 hrF[NS], hrB[NS];
-  HopRow16 hqF[NS], hqB[NS];
   float invc = 0.f, kt = 0.f;
   if (act) {
     if constexpr (!FIRST) {
@@ -205,13 +194,8 @@ __global__ __launch_bounds__(256) void k_horner_layer(HornerView hv, DiracView<f
       for (int j = 0; j < NS; j++) {
         nb[j] = halo.nbl[j * V + x];
         lk[j] = d.lnk[j * V + x];
-        if constexpr (kHalf) {
-          hqF[j] = hv.hrow16[mF + size_t(j) * V + x];
-          hqB[j] = hv.hrow16[mB + size_t(j) * V + x];
-        } else {
-          hrF[j] = hv.hrow[mF + size_t(j) * V + x];
-          hrB[j] = hv.hrow[mB + size_t(j) * V + x];
-        }
+        hrF[j] = hv.hrow[mF + size_t(j) * V + x];
+        hrB[j] = hv.hrow[mB + size_t(j) * V + x];
       }
     }
     invc = hv.invc[x];
@@ -241,32 +225,19 @@ __global__ __launch_bounds__(256) void k_horner_layer(HornerView hv, DiracView<f
     }
   };
   // sum_j u_j (m_same g_a + m_flip g_abar)(nb_j) for row a from the staged tile
-  auto transport = [&](const SliceLoads<NS, F, P>& L, const Tile* tb, const float4* hr, const HopRow16* hq,
-                       int a) -> float2 {
-    if constexpr (kHalf) {
-      half2_t t16 = {(_Float16)0.f, (_Float16)0.f};
+  auto transport = [&](const SliceLoads<NS, F, P>& L, const Tile* tb, const float4* hr, int a) -> float2 {
+    float2 t = {0.f, 0.f};
 #pragma unroll
-      for (int j = 0; j < NS; j++) {
-        half2_t u = L.u[j].v;
-        if (lk[j] < 0) u.y = -u.y;
-        const SpinH g = tb[nb[j]];
-        t16 += cmul16(u, cmul16(hq[j].s, comp16(g, a)) + cmul16(hq[j].f, comp16(g, 1 - a)));
-      }
-      return {float(t16.x), float(t16.y)};
-    } else {
-      float2 t = {0.f, 0.f};
-#pragma unroll
-      for (int j = 0; j < NS; j++) {
-        float2 u = ldu(&L.u[j]);
-        if (lk[j] < 0) u.y = -u.y;
-        const float4 g = tb[nb[j]];
-        const float2 ga = a == 0 ? float2{g.x, g.y} : float2{g.z, g.w};
-        const float2 gb = a == 0 ? float2{g.z, g.w} : float2{g.x, g.y};
-        const float2 cm = cadd<float>(cmul<float>({hr[j].x, hr[j].y}, ga), cmul<float>({hr[j].z, hr[j].w}, gb));
-        t = cadd<float>(t, cmul<float>(u, cm));
-      }
-      return t;
+    for (int j = 0; j < NS; j++) {
+      float2 u = ldu(&L.u[j]);
+      if (lk[j] < 0) u.y = -u.y;
+      const float4 g = tb[nb[j]];
+      const float2 ga = a == 0 ? float2{g.x, g.y} : float2{g.z, g.w};
+      const float2 gb = a == 0 ? float2{g.z, g.w} : float2{g.x, g.y};
+      const float2 cm = cadd<float>(cmul<float>({hr[j].x, hr[j].y}, ga), cmul<float>({hr[j].z, hr[j].w}, gb));
+      t = cadd<float>(t, cmul<float>(u, cm));
     }
+    return t;
   };
   float2 stF = {0.f, 0.f}, stB = {0.f, 0.f};
   // S holds the time wrap (the link from Lt-1 to 0, which T leaves out): each recurrence
@@ -298,7 +269,7 @@ __global__ __launch_bounds__(256) void k_horner_layer(HornerView hv, DiracView<f
     if (act) {
       float2 yF = cmul<float>(cj, comp(cur.v, rF)), yB = cmul<float>(cj, comp(cur.v, rB));
       if constexpr (!FIRST) {
-        const float2 tF = transport(cur, tb, hrF, hqF, rF), tB = transport(cur, tb, hrB, hqB, rB);
+        const float2 tF = transport(cur, tb, hrF, rF), tB = transport(cur, tb, hrB, rB);
         yF.x -= tF.x;
         yF.y -= tF.y;
         yB.x -= tB.x;
@@ -375,7 +346,7 @@ inline void launch_horner_layer(const HornerView& hv, const DiracView<float, P>&
   tm.remap = 1;
   const unsigned nblk = unsigned(tm.nxr_pad);
   if (Epi::kReduce && nblk > kReduceCapacity) throw std::invalid_argument("layer grid exceeds the reduction scratch");
-  const size_t smem = FIRST ? 0 : size_t(2) * (R + halo.hmax) * sizeof(typename TileT<F>::type);
+  const size_t smem = FIRST ? 0 : size_t(2) * (R + halo.hmax) * sizeof(float4);
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
   if (hv.nslot == 6)
     hipLaunchKernelGGL((k_horner_layer<6, ADJ, FIRST, TG, F, Src, Epi, P>), dim3(nblk), dim3(R), smem, st, hv,
