@@ -87,6 +87,24 @@ __global__ void k_site_angles(const phase16_t* __restrict__ uS, const int* __res
   out[i] = a;
 }
 
+// The per-site angles rounded to 10 bits (to nearest, 2 pi wrapping to 0) and packed three
+// slots to a word: word w of site x at slice s holds slot 3w + k in bits 10k..10k+9 ([Lt][W][V],
+// W = ceil(M / 3), slots past M zero).
+__global__ void k_site_angles10(const phase16_t* __restrict__ a, unsigned* __restrict__ out, int V, int M, int Lt) {
+  const int W = (M + 2) / 3;
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(Lt) * W * V) return;
+  const int x = int(i % V);
+  const size_t sw = i / V;
+  const int w = int(sw % W), s = int(sw / W);
+  unsigned word = 0;
+  for (int k = 0; k < 3 && 3 * w + k < M; k++) {
+    const unsigned u16 = unsigned(static_cast<unsigned short>(a[(size_t(s) * M + 3 * w + k) * V + x].v));
+    word |= (((u16 + 32u) >> 6) & 1023u) << (10 * k);
+  }
+  out[i] = word;
+}
+
 // <x,y> = sum conj(x) y over complex double fields of n points.
 __global__ __launch_bounds__(256) void k_dot(const Spin<double>* __restrict__ x,
                                              const Spin<double>* __restrict__ y, size_t n,
@@ -349,7 +367,7 @@ Dirac::~Dirac() {
                   (void*)d32_.nbr, (void*)d32_.lnk, (void*)d32_.hop, (void*)d32_.hopd,
                   (void*)d32_.diag, (void*)d32_.kth, (void*)d32_.uS, (void*)d32_.uT,
                   (void*)uSh_, (void*)uTh_, (void*)uStg_, (void*)uPolyH_, (void*)rotH_, (void*)phi64_,
-                  (void*)uPolyP_, (void*)uSiteH_,
+                  (void*)uPolyP_, (void*)uSiteH_, (void*)uSite10_,
                   (void*)angS_, (void*)angT_, (void*)link_ends_, (void*)partials_,
                   (void*)counter_, (void*)dot_out_, (void*)tmp64_})
     f(p);
@@ -447,6 +465,11 @@ void Dirac::fill_temporal_gauge() {
   if (uPolyP_)
     hipLaunchKernelGGL(k_angle_pairs, dim3((h_.V + B - 1) / B), dim3(B), 0, stream_, uPolyH_, uPolyP_,
                        size_t(h_.V));
+  if (uSite10_) {
+    const size_t nw = size_t(h_.Lt) * ((h_.max_nn + 2) / 3) * h_.V;
+    hipLaunchKernelGGL(k_site_angles10, dim3((nw + B - 1) / B), dim3(B), 0, stream_, uSiteH_, uSite10_, h_.V,
+                       h_.max_nn, h_.Lt);
+  }
   QED3_HIP(hipGetLastError());
 }
 
@@ -454,6 +477,7 @@ DiracView<float, phasepair_t> Dirac::view_half_tg_pairs() {
   const DiracView<float, phase16_t> tg = view_half_temporal_gauge();
   if (!uPolyP_) {
     uPolyP_ = dalloc<phasepair_t>(size_t(h_.V));
+    uSite10_ = dalloc<unsigned>(size_t(h_.Lt) * ((h_.max_nn + 2) / 3) * h_.V);
     if (has_gauge_) fill_temporal_gauge();
   }
   DiracView<float, phasepair_t> v;
@@ -463,10 +487,10 @@ DiracView<float, phasepair_t> Dirac::view_half_tg_pairs() {
   v.hopd = tg.hopd;
   v.diag = tg.diag;
   v.kth = tg.kth;
-  v.uS = nullptr;  // the spatial phases only per site and slot, as angles (uSiteA)
+  v.uS = nullptr;  // the spatial phases only per site and slot, as packed angles (uSite10)
   v.uT = nullptr;
   v.uPoly = uPolyP_;
-  v.uSiteA = uSiteH_;
+  v.uSite10 = uSite10_;
   v.V = tg.V;
   v.E = tg.E;
   v.Lt = tg.Lt;

@@ -33,7 +33,10 @@ struct DiracView {
   // the site sees it, conjugated where the link is stored in the opposite orientation, so
   // that a wave reads consecutive entries ([Lt][nslot][V]).
   const P* uSite = nullptr;
-  const phase16_t* uSiteA = nullptr;  // the same as 16-bit angles (the pair view's only spatial phases)
+  // The pair view's only spatial phases: the per-site angles rounded to 10 bits and packed
+  // three slots to a 32-bit word (slot 3w + k in bits 10k..10k+9 of word w; [Lt][W][V],
+  // W = ceil(nslot / 3)), for the learned preconditioner's layers.
+  const unsigned* uSite10 = nullptr;
   int V = 0, E = 0, Lt = 1;
   T r = 1;
   T bc = -1;  // sign on the time wrap: -1 antiperiodic, +1 periodic
@@ -203,6 +206,7 @@ class Dirac {
   phase16_t* rotH_ = nullptr;   // [Lt][V] phi_s(x)
   phasepair_t* uPolyP_ = nullptr;  // [V] uPolyH_ as (cos, sin) pairs
   phase16_t* uSiteH_ = nullptr;    // [Lt][nslot][V] uStg_ per site and slot (DiracView::uSite)
+  unsigned* uSite10_ = nullptr;    // [Lt][W][V] uSiteH_ packed at 10 bits (DiracView::uSite10)
   double* phi64_ = nullptr;     // [Lt][V] phi_s(x) in FP64 while the tables are built
   void fill_temporal_gauge();
   double* angS_ = nullptr;  // [Lt][E] FP64 angles (device copy)

@@ -53,13 +53,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uns
 __device__ __forceinline__ SpinH bload_h(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(SpinH, __builtin_amdgcn_raw_buffer_load_b64(buf_rsrc(base, nbytes), voff, soff, 0));
 }
-__device__ __forceinline__ short bload_a(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(short, __builtin_amdgcn_raw_buffer_load_b16(buf_rsrc(base, nbytes), voff, soff, 0));
+__device__ __forceinline__ unsigned bload_w(const void* base, unsigned nbytes, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(unsigned, __builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(base, nbytes), voff, soff, 0));
 }
-// e^{i theta} of a 16-bit angle (theta = 2 pi a / 65536) as a binary16 pair; the hardware sine
-// and cosine take revolutions
-__device__ __forceinline__ half2_t angle_pair(short a) {
-  const float rev = float(a) * (1.0f / 65536.0f);
+// e^{i theta} of the 10-bit angle in bits 10k..10k+9 of a packed word (theta = 2 pi a / 1024)
+// as a binary16 pair; the hardware sine and cosine take revolutions
+__device__ __forceinline__ half2_t angle_pair10(unsigned w, int k) {
+  const float rev = float((w >> (10 * k)) & 1023u) * (1.0f / 1024.0f);
   return half2_t{(_Float16)__builtin_amdgcn_cosf(rev), (_Float16)__builtin_amdgcn_sinf(rev)};
 }
 
@@ -374,9 +374,12 @@ inline void launch_horner_layer(const HornerView& hv, const DiracView<float, P>&
 // column runs on C = ceil(Lt / T) threads (C * S <= kTcBlock, S sites per block; the threads
 // of a lane are consecutive, so a wave reads and writes consecutive sites of one slice).  Per
 // step the block stages the C slices its lanes are at, for its S sites and their halo (halo
-// tables of ranges of S sites).  The spatial link phases come as 16-bit angles per site and
-// slot (the view's uSiteA, orientation applied: a wave's loads are consecutive, 2 B each),
-// turned into binary16 (cos, sin) pairs with the hardware sine and cosine.
+// tables of ranges of S sites).  The spatial link phases come per site as the view's uSite10:
+// the slots' angles (orientation applied) at 10 bits, three to a 32-bit word, so a thread
+// reads one word per slice on the trivalent lattice (a wave's loads consecutive), turned into
+// binary16 (cos, sin) pairs with the hardware sine and cosine.  The rounding (at most
+// pi/1024 rad) is of the order of the binary16 arithmetic's and only enters the
+// preconditioner.
 constexpr int kTcChunk = 16;   // slices per lane
 constexpr int kTcDepth = 2;    // steps whose loads are in flight ahead of the one computed
 constexpr int kTcBlock = 256;  // most threads per block (S sites x C lanes)
@@ -473,9 +476,10 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
       wB = {d.bc * kt * t.x, d.bc * kt * t.y};
     }
   }
+  constexpr int NW = (NS + 2) / 3;  // packed angle words per site and slice
   struct Loads {
     SpinH v, gc, gh0, gh1;
-    short u[NS];  // the slots' link angles
+    unsigned u[NW];  // the slots' link angles, packed
   };
   // slice s0 + j of this lane (own point, links) and slice c T + j of the halo share.  Every
   // thread issues the same loads at every step (lanes past the column's end re-read its last
@@ -484,19 +488,19 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   // buffer loads: the slice's byte offset in the scalar offset when it is wave-uniform (UL),
   // the thread's site or link in the vector offset
   const unsigned fbytes = unsigned(V) * unsigned(Lt) * unsigned(sizeof(SpinH));
-  constexpr unsigned usz = sizeof(phase16_t);
-  const unsigned ubytes = unsigned(V) * unsigned(NS) * unsigned(Lt) * usz;
+  constexpr unsigned usz = sizeof(unsigned);
+  const unsigned ubytes = unsigned(V) * unsigned(NW) * unsigned(Lt) * usz;
   const unsigned vx = xc * unsigned(sizeof(SpinH)), vh0 = hc0 * unsigned(sizeof(SpinH)),
                  vh1 = hc1 * unsigned(sizeof(SpinH)), vxu = xc * usz;
   auto load = [&](int j, Loads& Ld) {
     const unsigned s = unsigned(min(s0 + j, Lt - 1));
     const unsigned oG = s * unsigned(V) * unsigned(sizeof(SpinH));  // the slice's field offset
-    const unsigned oU = s * unsigned(NS * V) * usz;  // ... and phases
+    const unsigned oU = s * unsigned(NW * V) * usz;  // ... and phases
     const unsigned sG = UL ? oG : 0u, sU = UL ? oU : 0u, vG = UL ? 0u : oG, vU = UL ? 0u : oU;
     Ld.v = in.bat(fbytes, vG + vx, sG);
     if constexpr (!FIRST) {
 #pragma unroll
-      for (int q = 0; q < NS; q++) Ld.u[q] = bload_a(d.uSiteA, ubytes, vU + vxu, sU + unsigned(q * V) * usz);
+      for (int w = 0; w < NW; w++) Ld.u[w] = bload_w(d.uSite10, ubytes, vU + vxu, sU + unsigned(w * V) * usz);
       Ld.gc = bload_h(gin, fbytes, vG + vx, sG);
       Ld.gh0 = bload_h(gin, fbytes, vG + vh0, sG);
       if constexpr (H2) Ld.gh1 = bload_h(gin, fbytes, vG + vh1, sG);
@@ -506,7 +510,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     half2_t t16 = {(_Float16)0.f, (_Float16)0.f};
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      const half2_t u = angle_pair(Ld.u[q]);
+      const half2_t u = angle_pair10(Ld.u[q / 3], q % 3);
       const SpinH g = tb[nb[q]];
       t16 += cmul16(u, cmul16(hq[q].s, comp16(g, r)) + cmul16(hq[q].f, comp16(g, 1 - r)));
     }
@@ -707,7 +711,7 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
   if (gout2 && !ADJ) throw std::logic_error("only a K^dagger layer carries K's first layer");
   if (gout2 && (gout2 == gout || gout2 == gin)) throw std::logic_error("fused output aliases an operand");
   const int C = tc_lanes(hv.Lt), S = tc_sites(hv.Lt);
-  if (!tables.uPoly || !tables.uSiteA)
+  if (!tables.uPoly || !tables.uSite10)
     throw std::logic_error("parallel-in-time layer needs the temporal gauge");
   if (!FIRST && !halo.nbl) throw std::logic_error("parallel-in-time layer launched without halo tables");
   TileMap tm;
@@ -722,7 +726,7 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
       (FIRST ? 0 : size_t(S) * 2 * hv.nslot * sizeof(HopRow16));
   if (smem > 64 * 1024) throw std::invalid_argument("parallel-in-time layer: halo too large for LDS");
   if (size_t(hv.V) * size_t(hv.Lt) * sizeof(SpinH) > 0xffffffffull ||
-      size_t(hv.V) * size_t(hv.nslot) * size_t(hv.Lt) * sizeof(phase16_t) > 0xffffffffull)
+      size_t(hv.V) * size_t((hv.nslot + 2) / 3) * size_t(hv.Lt) * sizeof(unsigned) > 0xffffffffull)
     throw std::invalid_argument("parallel-in-time layer: fields beyond 32-bit offsets");
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
   auto go = [&](auto fuse, auto ul, auto h2) {
