@@ -129,14 +129,14 @@ def context(a, q, lattice, m, seed):
 def layer_bytes(V, E, Lt, nslot, first):
     """Algorithmic bytes of one network layer launch (binary16 fields, 8 B per point): the
     injection read and the output written (+ the input field read by a transport layer), the
-    spatial link phases of the temporal gauge as 16-bit angles (2 B per link and slice, each
-    link once), the wrap phase per site as a binary16 (cos, sin) pair (4 B) and the per-site
-    tables once (hop rows of both spin rows 2 x 8 B per slot, a 2 B local index per slot,
-    1/c and kappa_t)."""
+    spatial link phases of the temporal gauge as the 10-bit angles the layers read (10 bits
+    per link and slice, each link once), the wrap phase per site as a binary16 (cos, sin)
+    pair (4 B) and the per-site tables once (hop rows of both spin rows 2 x 8 B per slot, a
+    2 B local index per slot, 1/c and kappa_t)."""
     pts = V * Lt
     if first:
         return 2 * pts * 8 + V * 8
-    return 3 * pts * 8 + E * Lt * 2 + V * 4 + V * (nslot * (16 + 2) + 8)
+    return 3 * pts * 8 + (E * Lt * 10 + 7) // 8 + V * 4 + V * (nslot * (16 + 2) + 8)
 
 
 def main():

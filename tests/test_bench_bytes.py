@@ -50,10 +50,12 @@ def test_simplicial_counts():
 
 def test_preconditioner_layer_bytes():
     # a transport layer reads the injection and its input field and writes its output
-    # (binary16, 8 B per point each), reads the spatial link phases as 16-bit angles (2 B per
-    # link and slice) and one wrap phase per site as a binary16 pair (4 B), and the per-site
-    # tables once: 3 slots x (2 spin rows x 8 B hop coefficients + 2 B local index) + 1/c + kappa_t
+    # (binary16, 8 B per point each), reads the spatial link phases as 10-bit angles (10 bits
+    # per link and slice: 491,520 x 64 x 10 / 8 = 39,321,600 B) and one wrap phase per site as
+    # a binary16 pair (4 B), and the per-site tables once: 3 slots x (2 spin rows x 8 B hop
+    # coefficients + 2 B local index) + 1/c + kappa_t
     b = bench.layer_bytes(V_DUAL, E_DUAL, LT, 3, False)
-    assert b == 3 * PTS * 8 + E_DUAL * LT * 2 + V_DUAL * 4 + V_DUAL * (3 * 18 + 8) == 587_857_920
+    assert b == 503_316_480 + 39_321_600 + 1_310_720 + 20_316_160 == 564_264_960
+    assert PTS == 20_971_520 and V_DUAL * (3 * 18 + 8) == 20_316_160
     # a first layer: the injection read and the output written, 1/c and kappa_t
     assert bench.layer_bytes(V_DUAL, E_DUAL, LT, 3, True) == 2 * PTS * 8 + V_DUAL * 8 == 338_165_760
