@@ -385,17 +385,28 @@ constexpr int kTcDepth = 2;    // steps whose loads are in flight ahead of the o
 constexpr int kTcBlock = 256;  // most threads per block (S sites x C lanes)
 constexpr int kTcSites = 256;  // most sites per block
 
+// LDS written by threads of a wave and read back by other threads of the same wave: the wave
+// executes its LDS instructions in order, so only the compiler must keep them in order
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // FUSE (K^dagger's last layer only): the kernel then also applies the first layer of K to
 // the output it has just stored, from the same registers: gout2 = T^{-1}(cj2 gout) with
 // gout as stored (binary16), which saves K's first launch and its read of gout.
+// WS (S = the wave size): a lane is one wave, which stages its tile (its sites and their
+// whole halo) and alone reads it, so the per-step staging needs no block barrier.
 template <int NS, bool ADJ, bool FIRST, int T, class Src, class Epi, bool FUSE = false, bool UL = false,
-          bool H2 = false>
+          bool H2 = false, bool WS = false>
 __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc(HornerView hv, DiracView<float, phasepair_t> d,
                                                          HaloView halo, Src src, const SpinH* __restrict__ gin,
                                                          SpinH* __restrict__ gout, float2 cj, TileMap tm, Epi epi,
                                                          GridReduce gr, int C, SpinH* __restrict__ gout2,
                                                          float2 cj2) {
   static_assert(!FUSE || ADJ, "K's first layer fuses into a layer of K^dagger");
+  static_assert(!WS || UL, "wave-synchronous staging needs one wave per lane");
   constexpr bool kDots = SrcHasDots<Src>::value;
   static_assert(!kDots || Epi::kReduce, "the injection's products need the epilogue's reduction");
   if (epi.skip()) return;  // uniform
@@ -524,6 +535,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   Loads lb[NB];
 #pragma unroll
   for (int j = 0; j < kTcDepth && j < T; j++) load(j, lb[j]);
+  if constexpr (WS && !FIRST) __syncthreads();  // the hop rows lane 0 staged, before any lane reads them
 #pragma unroll
   for (int j = 0; j < T; j++) {
     if (j + kTcDepth < T) load(j + kTcDepth, lb[(j + kTcDepth) % NB]);
@@ -538,7 +550,10 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
           for (int q = sl + 2 * S; q < nh; q += S) tb[S + q] = gin[size_t(s0 + j) * V + halo.halo[h0 + q]];
         }
       }
-      __syncthreads();
+      if constexpr (WS)
+        wave_sync();
+      else
+        __syncthreads();
     }
     if (j < lenC) {
       if constexpr (kDots) {
@@ -729,29 +744,35 @@ inline void launch_horner_layer_tc(const HornerView& hv, const DiracView<float, 
       size_t(hv.V) * size_t((hv.nslot + 2) / 3) * size_t(hv.Lt) * sizeof(unsigned) > 0xffffffffull)
     throw std::invalid_argument("parallel-in-time layer: fields beyond 32-bit offsets");
   if (bracket) QED3_HIP(hipEventRecord(bracket[0], st));
-  auto go = [&](auto fuse, auto ul, auto h2) {
+  auto go = [&](auto fuse, auto ul, auto h2, auto ws) {
     constexpr bool F = decltype(fuse)::value && ADJ;
     constexpr bool U = decltype(ul)::value;
     constexpr bool H = decltype(h2)::value && !FIRST;
+    constexpr bool W = decltype(ws)::value && U && !FIRST;
     if (hv.nslot == 6)
-      hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H>), dim3(nblk),
+      hipLaunchKernelGGL((k_horner_layer_tc<6, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H, W>), dim3(nblk),
                          dim3(C * S), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
     else
-      hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H>), dim3(nblk),
+      hipLaunchKernelGGL((k_horner_layer_tc<3, ADJ, FIRST, kTcChunk, Src, Epi, F, U, H, W>), dim3(nblk),
                          dim3(C * S), smem, st, hv, tables, halo, src, gin, gout, cj, tm, epi, gr, C, gout2, cj2);
   };
-  // the wave-uniform form where a wave's 64 threads belong to one lane (S a multiple of 64);
-  // the second halo load only where some block's halo is longer than its range
-  const bool ul = S % 64 == 0, h2 = !FIRST && halo.hmax > S;
+  // the wave-uniform form where a wave's 64 threads belong to one lane (S a multiple of 64),
+  // wave-synchronous where a lane is exactly one wave (S = 64); the second halo load only
+  // where some block's halo is longer than its range
+  const bool ul = S % 64 == 0, ws = S == 64, h2 = !FIRST && halo.hmax > S;
   auto go2 = [&](auto fuse) {
-    if (ul && h2)
-      go(fuse, std::true_type{}, std::true_type{});
+    if (ws && h2)
+      go(fuse, std::true_type{}, std::true_type{}, std::true_type{});
+    else if (ws)
+      go(fuse, std::true_type{}, std::false_type{}, std::true_type{});
+    else if (ul && h2)
+      go(fuse, std::true_type{}, std::true_type{}, std::false_type{});
     else if (ul)
-      go(fuse, std::true_type{}, std::false_type{});
+      go(fuse, std::true_type{}, std::false_type{}, std::false_type{});
     else if (h2)
-      go(fuse, std::false_type{}, std::true_type{});
+      go(fuse, std::false_type{}, std::true_type{}, std::false_type{});
     else
-      go(fuse, std::false_type{}, std::false_type{});
+      go(fuse, std::false_type{}, std::false_type{}, std::false_type{});
   };
   if (gout2)
     go2(std::true_type{});
