@@ -490,12 +490,14 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
   constexpr int NW = (NS + 2) / 3;  // packed angle words per site and slice
   struct Loads {
     SpinH v, gc, gh0, gh1;
+    SpinH o;  // kDots: the side field
     unsigned u[NW];  // the slots' link angles, packed
   };
-  // slice s0 + j of this lane (own point, links) and slice c T + j of the halo share.  Every
-  // thread issues the same loads at every step (lanes past the column's end re-read its last
-  // slice, idle threads site 0; the values are not used), so that the loads of the next
-  // kTcDepth steps stay in flight while this one computes; 32-bit offsets (checked at launch).
+  // slice s0 + j of this lane (own point, links, kDots: the side field) and slice c T + j of
+  // the halo share.  Every thread issues the same loads at every step (lanes past the column's
+  // end re-read its last slice, idle threads site 0; the values are not used), so that the
+  // loads of the next kTcDepth steps stay in flight while this one computes; 32-bit offsets
+  // (checked at launch).
   // buffer loads: the slice's byte offset in the scalar offset when it is wave-uniform (UL),
   // the thread's site or link in the vector offset
   const unsigned fbytes = unsigned(V) * unsigned(Lt) * unsigned(sizeof(SpinH));
@@ -509,6 +511,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     const unsigned oU = s * unsigned(NW * V) * usz;  // ... and phases
     const unsigned sG = UL ? oG : 0u, sU = UL ? oU : 0u, vG = UL ? 0u : oG, vU = UL ? 0u : oU;
     Ld.v = in.bat(fbytes, vG + vx, sG);
+    if constexpr (kDots) Ld.o = in.bside(fbytes, vG + vx, sG);
     if constexpr (!FIRST) {
 #pragma unroll
       for (int w = 0; w < NW; w++) Ld.u[w] = bload_w(d.uSite10, ubytes, vU + vxu, sU + unsigned(w * V) * usz);
@@ -558,9 +561,7 @@ __global__ __launch_bounds__(kTcBlock, (NS == 3 ? 4 : 2)) void k_horner_layer_tc
     if (j < lenC) {
       if constexpr (kDots) {
         if (act) {
-          const unsigned oG = unsigned(s0 + j) * unsigned(V) * unsigned(sizeof(SpinH));
-          const Spin<float> v = from_half(cur.v),
-                            o = from_half(UL ? in.bside(fbytes, vx, oG) : in.bside(fbytes, oG + vx, 0u));
+          const Spin<float> v = from_half(cur.v), o = from_half(cur.o);
           dots.x += v.a.x * v.a.x + v.a.y * v.a.y + v.b.x * v.b.x + v.b.y * v.b.y;
           dots.y += o.a.x * v.a.x + o.a.y * v.a.y + o.b.x * v.b.x + o.b.y * v.b.y;
         }
