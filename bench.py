@@ -355,11 +355,8 @@ def learned_leg(a, q):
                                                   fuse="false"), kms[7], b_layer + pts * 8),
           "K1": ent(f"k_stencil_lds<float,{ns},false,EpiK1PH,SrcPUpdateH,phase16_t> (p = z + beta p_old, w = D p)",
                     kms[0], cgb["K1"]),
-          "K2": ent(f"k_stencil_lds<float,{ns},true,EpiK2F<0..6>,SrcHalf,phase16_t>", kms[1], cgb["K2"]),
+          "K2": ent(f"k_stencil_lds<float,{ns},true,EpiK2F,SrcHalf,phase16_t>", kms[1], cgb["K2"]),
           "iterations": rep.iterations, "layers": a.layers, "tile": tile}
-    if kms[2] > 0:
-        lk["K2x"] = ent(f"k_stencil_lds<float,{ns},true,EpiK2F<7>,SrcHalf,phase16_t> (x += the ring of 8 "
-                        "directions)", kms[2], cgb["K2x"])
     # launches per iteration: K^dagger's first layer, L - 2 transport layers of K^dagger and of
     # K each, K^dagger's last layer with K's first and the reductions, K's last layer, K1, K2
     per_it = {"first_layer": kms[3], "transport_layers": (a.layers - 2) * (kms[4] + kms[5]),

@@ -269,11 +269,12 @@ int qed3_precond_train(qed3_precond* pc, const void* d_v, const void* d_e, int n
 int qed3_pcg(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b, const qed3_cg_opts* opts,
              qed3_cg_report* rep);
 /* The same solve with direct launches and HIP events around its kernels, ms[8]:
- * ms[0] K1, ms[1] K2 with the x term pending, ms[2] K2 updating x (0 if no group of eight
- * iterations ran without a reliable update), ms[3] the first layer of K^dagger (recurrence
- * only), ms[4] a transport layer of K^dagger, ms[5] a transport layer of K, ms[6] the last
- * layer of K^dagger with K's first layer fused in, ms[7] the last layer of K with the FCG
- * reductions (means per launch over the iterations run; 0 where no such launch ran) */
+ * ms[0] K1, ms[1] K2 (its x term left pending), ms[2] 0 (kept for the layout: no K2 updates
+ * x; directions still pending when the ring is reused are folded by a pass of their own), ms[3]
+ * the first layer of K^dagger (recurrence only), ms[4] a transport layer of K^dagger, ms[5]
+ * a transport layer of K, ms[6] the last layer of K^dagger with K's first layer fused in,
+ * ms[7] the last layer of K with the FCG reductions (means per launch over the iterations
+ * run; 0 where no such launch ran) */
 int qed3_pcg_kernel_times(qed3_ctx* ctx, qed3_precond* pc, void* d_x, const void* d_b,
                           const qed3_cg_opts* opts, qed3_cg_report* rep, double* ms);
 

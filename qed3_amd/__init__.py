@@ -517,9 +517,9 @@ class Precond:
         return d_x.to_numpy(np.complex128), rep
 
     def solve_kernel_times(self, d_x, d_b, tol=1e-10, maxiter=20000, delta=0.01):
-        """One solve with HIP-event timing: (report, [K1, K2, K2 with x update, first layer,
-        K^dagger transport layer, K transport layer, K^dagger's last layer fused with K's
-        first, K's last layer with the reductions] ms per launch)."""
+        """One solve with HIP-event timing: (report, [K1, K2, 0 (no K2 updates x), first
+        layer, K^dagger transport layer, K transport layer, K^dagger's last layer fused with
+        K's first, K's last layer with the reductions] ms per launch)."""
         opts = CGOpts(tol, maxiter, CG_HALF, delta, 0)
         rep = CGReport()
         ms = (C.c_double * 8)()

@@ -171,7 +171,7 @@ struct EpiK1PH {
 // alpha p in one read-modify-write of x.  A reliable update or the end of the solve
 // flushes the pending terms (k_flush_h).
 constexpr int kXGroup = 4;
-constexpr int kMaxPend = 7;  // entries of CGScalars::alpha_pend
+constexpr int kMaxPend = 8;  // entries of CGScalars::alpha_pend
 template <int POS>
 struct EpiK2H {
   static constexpr bool kReduce = true;

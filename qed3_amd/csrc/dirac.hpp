@@ -82,8 +82,8 @@ struct CGScalars {
   double true_r2;  // last true FP64 |b - A x|^2
   double scale;    // half-storage CG: true field = scale * stored field (1 otherwise)
   double rfac;     // half-storage CG: factor from the kept residual to the new stored one
-  double alpha_pend[7];  // alphas of the group's iterations whose x update is pending (0 =
-                         // none): the half-storage CG uses 3, the preconditioned CG 7
+  double alpha_pend[8];  // alphas of the iterations whose x update is pending (0 = none):
+                         // the half-storage CG uses 3, the preconditioned CG 8 (its ring)
   double rho;      // preconditioned CG: <r, z> of the last iteration (stored units)
   int done;        // 0 running, 1 converged, 2 breakdown / NaN, 3 maxiter
   int need_ru;     // a reliable update is due in this iteration

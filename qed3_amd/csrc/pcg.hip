@@ -34,63 +34,32 @@ using namespace cgdrive;
 
 
 // The directions of the last kFGroup iterations stay in a ring of buffers and their x terms
-// pending (alpha_pend) until a reliable update flushes them, or until the ring is full, when
-// the group's last K2 adds them: with an update every three or four iterations K2 hardly
-// ever touches x.
+// pending (alpha_pend[slot]) until a reliable update flushes them; terms still pending when
+// a group of kFGroup iterations starts to reuse the ring (no update for up to kFGroup
+// iterations) are first added to x_inner by a streaming pass of their own (k_fold_f).  With
+// an update every three or four iterations few are, and K2 never touches x.
 constexpr int kFGroup = 8;
-static_assert(kFGroup - 1 <= kMaxPend, "pending alphas of a group");
+static_assert(kFGroup <= kMaxPend, "pending alphas of a group");
 
-// K2: q = D^dag w; r -= alpha q in place, |r|^2 and the reliable-update decision; at the
-// group's last position x += the group's alpha p terms
-template <int POS>
+// K2: q = D^dag w; r = rpend r - alpha q in place, |r|^2 and the reliable-update decision;
+// the iteration's x term is left pending in its ring slot `pos`
 struct EpiK2F {
   static constexpr bool kReduce = true;
-  static constexpr bool kUpdateX = POS == kFGroup - 1;
-  Spin<float>* x;
   SpinH* r;
-  const SpinH* ring[kFGroup];
   CGScalars* sc;
+  int pos;
   float alpha;
   float f;  // the factor the stored r still owes after a reliable update (else 1)
-  bool xset;  // x_inner holds earlier folds (else it is taken as 0)
-  float ap[kFGroup - 1];
   __device__ bool skip() const { return sc->done != 0 || sc->need_ru != 0; }
   __device__ void begin() {
     alpha = float(sc->alpha);
     f = float(sc->rpend);
-    xset = kUpdateX && sc->xin_set != 0;
-#pragma unroll
-    for (int j = 0; j < kFGroup - 1; j++) ap[j] = kUpdateX ? float(sc->alpha_pend[j]) : 0.f;
   }
-  // r is loaded with the slice's other loads (as EpiK2H) except where x is updated
-  template <int P = POS, std::enable_if_t<P != kFGroup - 1, int> = 0>
-  __device__ SpinH prefetch(size_t i) const {
-    return r[i];
-  }
+  // r is loaded with the slice's other loads (as EpiK2H)
+  __device__ SpinH prefetch(size_t i) const { return r[i]; }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc) const { (*this)(i, q, acc, r[i]); }
   __device__ void operator()(size_t i, const Spin<float>& q, double2& acc, const SpinH& rpre) const {
     Spin<float> rr = from_half(rpre);
-    if constexpr (kUpdateX) {
-      Spin<float> xx;
-      if (xset) {
-        xx = ld(x + i);
-      } else {
-        xx.a = {0.f, 0.f};
-        xx.b = {0.f, 0.f};
-      }
-#pragma unroll
-      for (int j = 0; j < kFGroup; j++) {
-        const float a = j == kFGroup - 1 ? alpha : ap[j];
-        if (a != 0.f) {  // uniform
-          const Spin<float> pp = ldh(ring[j] + i);
-          xx.a.x += a * pp.a.x;
-          xx.a.y += a * pp.a.y;
-          xx.b.x += a * pp.b.x;
-          xx.b.y += a * pp.b.y;
-        }
-      }
-      st(x + i, xx);
-    }
     rr.a.x = f * rr.a.x - alpha * q.a.x;
     rr.a.y = f * rr.a.y - alpha * q.a.y;
     rr.b.x = f * rr.b.x - alpha * q.b.x;
@@ -101,13 +70,8 @@ struct EpiK2F {
   }
   __device__ void operator()(double2 t) const {
     const double m = t.x;
-    if constexpr (kUpdateX) {
-      for (int j = 0; j < kFGroup - 1; j++) sc->alpha_pend[j] = 0.0;
-      sc->xin_set = 1;
-    } else {
-      sc->alpha_pend[POS] = sc->alpha;
-    }
-    sc->last_pos = POS;
+    sc->alpha_pend[pos] = sc->alpha;
+    sc->last_pos = pos;
     sc->iter += 1;
     sc->rpend = 1.0;  // r is re-stored at the current scale
     if (!(m >= 0.0 && m < 1e300)) {
@@ -123,7 +87,7 @@ struct EpiK2F {
 };
 
 struct PendRing8 {
-  const SpinH* p[kFGroup - 1];
+  const SpinH* p[kFGroup];
 };
 
 // x64 += scale exp(-i phi) (x_inner + sum_j alpha_pend[j] ring[j]) (before a reliable
@@ -133,9 +97,9 @@ struct PendRing8 {
 __global__ __launch_bounds__(256) void k_flush_f(Spin<double>* x64, Spin<float>* xin, PendRing8 ring, size_t n,
                                                  const CGScalars* sc, int always, const phase16_t* rot) {
   if (!always && (sc->done != 0 || sc->need_ru == 0)) return;
-  float ap[kFGroup - 1];
+  float ap[kFGroup];
 #pragma unroll
-  for (int j = 0; j < kFGroup - 1; j++) ap[j] = float(sc->alpha_pend[j]);
+  for (int j = 0; j < kFGroup; j++) ap[j] = float(sc->alpha_pend[j]);
   const double s = sc->scale;
   const bool xset = sc->x_set != 0, iset = sc->xin_set != 0;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
@@ -147,7 +111,7 @@ __global__ __launch_bounds__(256) void k_flush_f(Spin<double>* x64, Spin<float>*
       a.b = {0.f, 0.f};
     }
 #pragma unroll
-    for (int j = 0; j < kFGroup - 1; j++) {
+    for (int j = 0; j < kFGroup; j++) {
       if (ap[j] != 0.f) {  // uniform
         const Spin<float> pp = ldh(ring.p[j] + i);
         a.a.x += ap[j] * pp.a.x;
@@ -170,6 +134,49 @@ __global__ __launch_bounds__(256) void k_flush_f(Spin<double>* x64, Spin<float>*
     xx.b.y += s * a.b.y;
     x64[i] = xx;
   }
+}
+
+// Before a group reuses the ring: x_inner += sum_j alpha_pend[j] ring[j] over the slots still
+// pending (no reliable update since their directions were made); k_fold_done_f then marks
+// them folded (one thread, after every block of the fold has read the flags).  Both exit at
+// once when no slot is pending, or the solve is done (the final flush takes the terms).
+__device__ __forceinline__ bool fold_due(const CGScalars* sc) {
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < kFGroup; j++) any |= sc->alpha_pend[j] != 0.0;
+  return sc->done == 0 && any;
+}
+__global__ __launch_bounds__(256) void k_fold_f(Spin<float>* xin, PendRing8 ring, size_t n, const CGScalars* sc) {
+  if (!fold_due(sc)) return;
+  float ap[kFGroup];
+#pragma unroll
+  for (int j = 0; j < kFGroup; j++) ap[j] = float(sc->alpha_pend[j]);
+  const bool iset = sc->xin_set != 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    Spin<float> xx;
+    if (iset) {
+      xx = xin[i];
+    } else {
+      xx.a = {0.f, 0.f};
+      xx.b = {0.f, 0.f};
+    }
+#pragma unroll
+    for (int j = 0; j < kFGroup; j++) {
+      if (ap[j] != 0.f) {  // uniform
+        const Spin<float> pp = ldh(ring.p[j] + i);
+        xx.a.x += ap[j] * pp.a.x;
+        xx.a.y += ap[j] * pp.a.y;
+        xx.b.x += ap[j] * pp.b.x;
+        xx.b.y += ap[j] * pp.b.y;
+      }
+    }
+    xin[i] = xx;
+  }
+}
+__global__ void k_fold_done_f(CGScalars* sc) {
+  if (!fold_due(sc)) return;
+  for (int j = 0; j < kFGroup; j++) sc->alpha_pend[j] = 0.0;
+  sc->xin_set = 1;
 }
 
 // Start: |b|^2 and p_old = 0 (x and x_inner are not cleared: see k_flush_f)
@@ -460,7 +467,7 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
   const int L = K.layers();
   const double inv_ncomp = 1.0 / (4.0 * double(n));
   PendRing8 pend;
-  for (int j = 0; j < kFGroup - 1; j++) pend.p[j] = ring[j];
+  for (int j = 0; j < kFGroup; j++) pend.p[j] = ring[j];
   EventTimer timer(o.times), ltimer(layer_times);
   const hk::SrcFS src_r{R, &sc->rpend};
   const EpiSkip eskip{sc};
@@ -472,15 +479,12 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
     precond_layers(K, hv, vpairs, halo, src_r, h, G, Z, eskip, hk::SrcDots{h, H[(i & 1) ^ 1]}, efcg, gr, st,
                    &ltimer, k);
   };
-  auto k2 = [&](auto tag, int k) {
-    constexpr int POS = decltype(tag)::value;
-    EpiK2F<POS> e{};
-    e.x = xin;
+  auto k2 = [&](int pos, int k) {
+    EpiK2F e{};
     e.r = R;
-    for (int j = 0; j < kFGroup; j++) e.ring[j] = ring[j];
     e.sc = sc;
-    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt,
-                                    timer.pair(EpiK2F<POS>::kUpdateX ? 2 : 1, k));
+    e.pos = pos;
+    launch_stencil_src<float, true>(vin, nn, SrcHalf{w}, e, D.tile<float>(true), gr, st, hvt, timer.pair(1, k));
   };
   // The reliable update's kernels follow every iteration and exit at once unless it flagged
   // one: with an update every three to four iterations (delta = 0.001), waiting for a
@@ -495,20 +499,14 @@ CGResult pcg_solve(Horner& K, Spin<double>* x, const Spin<double>* b, const CGOp
     const int pos = i % kFGroup;
     SpinH* pold = ring[(pos + kFGroup - 1) % kFGroup];
     SpinH* pnew = ring[pos];
+    if (pos == 0) {  // the group starts to reuse the ring
+      hipLaunchKernelGGL(k_fold_f, dim3(ngate), dim3(256), 0, st, xin, pend, n, sc);
+      hipLaunchKernelGGL(k_fold_done_f, dim3(1), dim3(1), 0, st, sc);
+    }
     precond(i, k);
     launch_stencil_src<float, false>(vin, nn, SrcPUpdateH{Z, pold, sc}, EpiK1PH{w, pnew, sc},
                                      D.tile<float>(false), gr, st, hvf, timer.pair(0, k));
-    static_assert(kFGroup == 8, "one K2 instantiation per group position");
-    switch (pos) {
-      case 0: k2(std::integral_constant<int, 0>{}, k); break;
-      case 1: k2(std::integral_constant<int, 1>{}, k); break;
-      case 2: k2(std::integral_constant<int, 2>{}, k); break;
-      case 3: k2(std::integral_constant<int, 3>{}, k); break;
-      case 4: k2(std::integral_constant<int, 4>{}, k); break;
-      case 5: k2(std::integral_constant<int, 5>{}, k); break;
-      case 6: k2(std::integral_constant<int, 6>{}, k); break;
-      default: k2(std::integral_constant<int, 7>{}, k); break;
-    }
+    k2(pos, k);
     reliable_update();
   };
   // ring positions run 0 .. kFGroup-1 within every chunk

@@ -45,12 +45,13 @@ def test_precond_apply_matches_oracle(gpu, lattice, n, Lt, bc, layers):
 @pytest.mark.gpu
 @pytest.mark.parametrize("lattice,n,Lt,bc,layers", [(1, 2, 4, 0, 3), (0, 2, 16, 0, 4), (1, 2, 20, 1, 4),
                                                     (0, 2, 48, 0, 2), (1, 2, 64, 0, 4), (1, 3, 1, 0, 3),
-                                                    (0, 4, 33, 0, 3)])
+                                                    (0, 4, 33, 0, 3), (0, 3, 64, 1, 4)])
 def test_solver_kernels_match_oracle(gpu, lattice, n, Lt, bc, layers):
     """The solver's parallel-in-time layers (binary16 fields in the temporal gauge, time
     chunks of 16 slices per lane with carries between lanes) apply K and K^dagger as the
     oracle does, to binary16 accuracy: every chunking case (one lane, a partial last lane,
-    an empty padding lane, four lanes) and the 2D operator."""
+    an empty padding lane, four lanes of one wave each on both lattices) and the 2D
+    operator."""
     print(f"case lattice={lattice} n={n} Lt={Lt} bc={bc} layers={layers}", flush=True)
     Do, Dq = _pair(lattice, n, Lt, 0.05, bc=bc)
     K = q.Precond(Dq, layers=layers)
@@ -141,8 +142,9 @@ def test_pcg_edge_cases(gpu):
 @pytest.mark.gpu
 def test_pcg_x_update_of_a_full_ring(gpu):
     """With rare reliable updates (delta = 1e-6) more than eight iterations run between
-    updates, so K2 folds the full ring of directions into x; the solution still equals the
-    oracle's and reaches the FP64 true residual."""
+    updates, so directions are still pending when a group of eight starts to reuse the ring
+    and are first folded into x_inner; the solution still equals the oracle's and reaches
+    the FP64 true residual."""
     lattice, n, Lt = 1, 8, 16
     Do, Dq = _pair(lattice, n, Lt, 0.02)
     K = q.Precond(Dq, layers=2)
