@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), keyed by
 # kernel, lattice, n, Lt (and launch shape for the stencils)
-MEASURED_TRAFFIC = {("layer-tc", 1, 128, 64): 6.646e8,
+MEASURED_TRAFFIC = {("layer-tc", 1, 128, 64): 6.644e8,
                     ("f16", 1, 128, 64, 256, 64, 1, 1): 5.017e8,
                     ("f32", 1, 128, 64, 256, 64, 1, 1): 1.3096e9,
                     ("K1-half", 1, 128, 64, 256, 64, 1, 1): 8.836e8,
